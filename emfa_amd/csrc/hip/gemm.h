@@ -1,0 +1,38 @@
+// MFMA GEMM for complex updates written as real products.
+//
+// A complex product C -= A B (A: M x K, B: K x Nc) on interleaved storage is
+// the real product C' -= A' Bt'^T with
+//   A'  = A viewed as M x 2K reals (row-major, interleaved re/im),
+//   Bt' = 2Nc x 2K reals, Bt'[2j][2k] = Re B(k,j),  Bt'[2j][2k+1] = -Im B(k,j),
+//                         Bt'[2j+1][2k] = Im B(k,j), Bt'[2j+1][2k+1] = Re B(k,j),
+//   C'  = C viewed as M x 2Nc reals.
+// Both operands are K-contiguous, which is the order the MFMA operand fetch
+// wants from LDS.  FP32 uses V_MFMA_F32_32X32X2_F32, FP64 (and the mixed
+// FP32-matrix / FP64-accumulate residual) uses V_MFMA_F64_16X16X4_F64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace emfa {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+struct GemmArgs {
+    int M, Nr, Kr;          // rows, real columns, real inner dimension
+    const void *A;          // TA*, row-major, row stride lda (elements)
+    long long lda;
+    const void *Bt;         // TC*, row-major 2Nc x Kr, row stride ldb
+    long long ldb;
+    void *C;                // TC*, row-major, row stride ldc
+    long long ldc;
+    int alpha_neg;          // 1: C = beta C - A B ; 0: C = beta C + A B
+    int beta;               // 0 or 1
+};
+
+emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g);       // TA=float,  TC=float
+emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g);       // TA=double, TC=double
+emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g);   // TA=float,  TC=double
+
+}  // namespace emfa

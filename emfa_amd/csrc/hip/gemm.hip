@@ -1,0 +1,268 @@
+// MFMA GEMM kernels (see gemm.h for the complex-as-real formulation).
+#include "gemm.h"
+
+namespace emfa {
+
+// ---------------------------------------------------------------------------
+// FP32: workgroup tile 128 x 128 (reals), 4 waves in 2 x 2, wave tile 64 x 64 =
+// 2 x 2 MFMA 32x32 tiles.  K step 32: MFMA t (0..15) takes k = 16*(lane/32) + t,
+// so each lane reads 16 consecutive K values of its row from LDS.
+// Double-buffered LDS, next tile prefetched into registers during the MFMAs.
+// ---------------------------------------------------------------------------
+namespace f32k {
+constexpr int BM = 128, BN = 128, BK = 32, PAD = 4;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
+    using namespace f32k;
+    __shared__ float As[2][BM][BK + PAD];
+    __shared__ float Bs[2][BN][BK + PAD];
+    const float *__restrict__ A = (const float *)g.A;
+    const float *__restrict__ Bt = (const float *)g.Bt;
+    float *__restrict__ C = (float *)g.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long long bm = (long long)blockIdx.y * BM, bn = (long long)blockIdx.x * BN;
+    const int nk = (g.Kr + BK - 1) / BK;
+
+    float2 ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            int idx = tid + 256 * i;
+            int row = idx >> 4, kk = (idx & 15) * 2;
+            long long gr = bm + row;
+            int gk = k0 + kk;
+            ra[i] = (gr < g.M && gk < g.Kr) ? *(const float2 *)(A + gr * g.lda + gk) : make_float2(0.f, 0.f);
+            long long gc = bn + row;
+            rb[i] = (gc < g.Nr && gk < g.Kr) ? *(const float2 *)(Bt + gc * g.ldb + gk) : make_float2(0.f, 0.f);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            int idx = tid + 256 * i;
+            int row = idx >> 4, kk = (idx & 15) * 2;
+            As[buf][row][kk] = ra[i].x;
+            As[buf][row][kk + 1] = ra[i].y;
+            Bs[buf][row][kk] = rb[i].x;
+            Bs[buf][row][kk + 1] = rb[i].y;
+        }
+    };
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int r32 = lane & 31, kg = 16 * (lane >> 5);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+        float av[2][16], bv[2][16];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            const float4 *src = (const float4 *)&As[buf][wm * 64 + mt * 32 + r32][kg];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float4 v = src[q];
+                av[mt][4 * q] = v.x;
+                av[mt][4 * q + 1] = v.y;
+                av[mt][4 * q + 2] = v.z;
+                av[mt][4 * q + 3] = v.w;
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const float4 *src = (const float4 *)&Bs[buf][wn * 64 + nt * 32 + r32][kg];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float4 v = src[q];
+                bv[nt][4 * q] = v.x;
+                bv[nt][4 * q + 1] = v.y;
+                bv[nt][4 * q + 2] = v.z;
+                bv[nt][4 * q + 3] = v.w;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mt][t], bv[nt][t], acc[mt][nt], 0, 0, 0);
+        if (kt + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const float sgn = g.alpha_neg ? -1.f : 1.f;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                long long i = bm + wm * 64 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                long long j = bn + wn * 64 + nt * 32 + r32;
+                if (i < g.M && j < g.Nr) {
+                    float *c = C + i * g.ldc + j;
+                    float val = sgn * acc[mt][nt][v];
+                    *c = g.beta ? (*c + val) : val;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 accumulate: workgroup tile 128 x 128, wave tile 64 x 64 = 4 x 4 MFMA
+// 16x16 tiles, K step 16: MFMA t (0..3) takes k = 4*(lane/16) + t.
+// TA = float converts the matrix operand to double on its way into LDS.
+// ---------------------------------------------------------------------------
+namespace f64k {
+constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
+}
+
+template <typename TA>
+__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
+    using namespace f64k;
+    __shared__ double As[2][BM][BK + PAD];
+    __shared__ double Bs[2][BN][BK + PAD];
+    const TA *__restrict__ A = (const TA *)g.A;
+    const double *__restrict__ Bt = (const double *)g.Bt;
+    double *__restrict__ C = (double *)g.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long long bm = (long long)blockIdx.y * BM, bn = (long long)blockIdx.x * BN;
+    const int nk = (g.Kr + BK - 1) / BK;
+
+    double2 ra[4], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int idx = tid + 256 * i;
+            int row = idx >> 3, kk = (idx & 7) * 2;
+            long long gr = bm + row;
+            int gk = k0 + kk;
+            if (gr < g.M && gk < g.Kr) {
+                if constexpr (sizeof(TA) == 4) {
+                    float2 v = *(const float2 *)(A + gr * g.lda + gk);
+                    ra[i] = make_double2(v.x, v.y);
+                } else {
+                    ra[i] = *(const double2 *)(A + gr * g.lda + gk);
+                }
+            } else {
+                ra[i] = make_double2(0.0, 0.0);
+            }
+            long long gc = bn + row;
+            rb[i] = (gc < g.Nr && gk < g.Kr) ? *(const double2 *)(Bt + gc * g.ldb + gk) : make_double2(0.0, 0.0);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int idx = tid + 256 * i;
+            int row = idx >> 3, kk = (idx & 7) * 2;
+            As[buf][row][kk] = ra[i].x;
+            As[buf][row][kk + 1] = ra[i].y;
+            Bs[buf][row][kk] = rb[i].x;
+            Bs[buf][row][kk + 1] = rb[i].y;
+        }
+    };
+
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int r16 = lane & 15, kg = 4 * (lane >> 4);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+        double av[4][4], bv[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const double2 *src = (const double2 *)&As[buf][wm * 64 + mt * 16 + r16][kg];
+            double2 v0 = src[0], v1 = src[1];
+            av[mt][0] = v0.x;
+            av[mt][1] = v0.y;
+            av[mt][2] = v1.x;
+            av[mt][3] = v1.y;
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const double2 *src = (const double2 *)&Bs[buf][wn * 64 + nt * 16 + r16][kg];
+            double2 v0 = src[0], v1 = src[1];
+            bv[nt][0] = v0.x;
+            bv[nt][1] = v0.y;
+            bv[nt][2] = v1.x;
+            bv[nt][3] = v1.y;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt][t], bv[nt][t], acc[mt][nt], 0, 0, 0);
+        if (kt + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const double sgn = g.alpha_neg ? -1.0 : 1.0;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                long long i = bm + wm * 64 + mt * 16 + 4 * (lane >> 4) + v;
+                long long j = bn + wn * 64 + nt * 16 + r16;
+                if (i < g.M && j < g.Nr) {
+                    double *c = C + i * g.ldc + j;
+                    double val = sgn * acc[mt][nt][v];
+                    *c = g.beta ? (*c + val) : val;
+                }
+            }
+}
+
+static bool gemm_args_ok(const GemmArgs &g) {
+    return g.M > 0 && g.Nr > 0 && g.Kr > 0 && (g.Kr % 2) == 0 && (g.lda % 2) == 0 && (g.ldb % 2) == 0 &&
+           g.Nr <= 65535LL * 128 && g.M <= 65535LL * 128;
+}
+
+emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g) {
+    EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
+    dim3 grid((g.Nr + f32k::BN - 1) / f32k::BN, (g.M + f32k::BM - 1) / f32k::BM);
+    k_gemm_f32<<<grid, 256, 0, s>>>(g);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g) {
+    EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
+    dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
+    k_gemm_f64<double><<<grid, 256, 0, s>>>(g);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g) {
+    EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32/f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
+    dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
+    k_gemm_f64<float><<<grid, 256, 0, s>>>(g);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+}  // namespace emfa

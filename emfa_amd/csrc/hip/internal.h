@@ -1,0 +1,138 @@
+// Internal declarations of the gfx950 MoM library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/emfa_hip.h"
+
+namespace emfa {
+
+void set_error(const char *fmt, ...);
+
+#define EMFA_HIP_CHECK(expr)                                                                 \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess) {                                                              \
+            ::emfa::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(_e)); \
+            return EMFA_E_HIP;                                                               \
+        }                                                                                    \
+    } while (0)
+
+#define EMFA_REQUIRE(cond, code, ...)         \
+    do {                                      \
+        if (!(cond)) {                        \
+            ::emfa::set_error(__VA_ARGS__);   \
+            return (code);                    \
+        }                                     \
+    } while (0)
+
+// Per-family timing with HIP events on the context stream.
+struct ProfFamily {
+    double ms = 0;
+    long long launches = 0;
+    double work = 0;  // algorithmic flops or bytes attributed to the family
+};
+
+struct PendingEvent {
+    std::string family;
+    hipEvent_t a, b;
+    double work;
+};
+
+}  // namespace emfa
+
+struct emfa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t bytes = 0;
+    bool prof = false;
+    std::map<std::string, emfa::ProfFamily> fam;
+    std::vector<emfa::PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool;
+    // device scratch owned by the context (grown on demand)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int *flags = nullptr;  // [0] spin-timeout flag, [1] singular info
+};
+
+struct emfa_mesh {
+    emfa_ctx *ctx = nullptr;
+    int nv = 0, nt = 0, n = 0;
+    // per-triangle geometry, SoA [field][nt]; fields: 0-2 centroid, 3-11 vertices
+    // relative to the centroid (v0xyz, v1xyz, v2xyz), 12-14 unit normal, 15 area
+    double *tg64 = nullptr;
+    float *tg32 = nullptr;
+    // triangle -> RWG: [3][nt], sign*(m+1) for the RWG whose free vertex is local vertex a, 0 if none
+    int *t2r = nullptr;
+    // RWG tables [n]: tp, tm, ap, am (local index of the free vertex in tp / tm)
+    int *rtp = nullptr, *rtm = nullptr, *rap = nullptr, *ram = nullptr;
+    double *rlen64 = nullptr;
+    float *rlen32 = nullptr;
+    std::vector<int> t2r_host;
+    size_t bytes = 0;
+};
+
+struct emfa_mat {
+    emfa_ctx *ctx = nullptr;
+    int rows = 0, cols = 0, prec = EMFA_F64;
+    void *d = nullptr;  // interleaved complex, row-major
+    size_t bytes = 0;
+};
+
+struct emfa_lu {
+    emfa_ctx *ctx = nullptr;
+    emfa_mat *a = nullptr;  // factor (L unit-lower below diagonal, U on/above)
+    int *ipiv = nullptr;    // device, 0-based absolute row of each pivot
+    int info = 0;
+};
+
+namespace emfa {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kZ0 = 120.0 * kPi;
+
+inline size_t elem_bytes(int prec) { return prec == EMFA_F32 ? 8 : 16; }
+
+hipError_t ctx_alloc(emfa_ctx *ctx, void **p, size_t bytes);
+void ctx_free(emfa_ctx *ctx, void *p, size_t bytes);
+emfa_status ctx_scratch(emfa_ctx *ctx, size_t bytes, void **p);
+
+// profiling helpers (no-ops unless ctx->prof)
+void prof_begin(emfa_ctx *ctx, const char *family, double work, hipEvent_t *a, hipEvent_t *b);
+void prof_end(emfa_ctx *ctx, hipEvent_t a, hipEvent_t b);
+void prof_collect(emfa_ctx *ctx);
+
+struct ProfScope {
+    emfa_ctx *ctx;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(emfa_ctx *c, const char *family, double work) : ctx(c) { prof_begin(ctx, family, work, &a, &b); }
+    ~ProfScope() { prof_end(ctx, a, b); }
+};
+
+// complex scalar helpers usable on host and device
+template <typename R>
+struct cx {
+    R re, im;
+};
+
+template <typename R>
+__host__ __device__ inline cx<R> cmul(cx<R> a, cx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// LU driver entry (lu.hip)
+emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info);
+emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, emfa_mat *B);
+emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, emfa_mat *C, int negate,
+                          int accumulate);
+emfa_status convert_copy(emfa_ctx *ctx, const emfa_mat *src, emfa_mat *dst);
+emfa_status axpy_f64(emfa_ctx *ctx, emfa_mat *X, const emfa_mat *D);  // X += D (D any prec)
+
+}  // namespace emfa

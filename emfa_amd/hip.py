@@ -1,0 +1,324 @@
+"""ctypes binding of the C ABI in include/emfa_hip.h (libemfa_hip.so).
+
+The product path: every call goes to the gfx950 library; there is no CPU
+fallback.  If the library is missing, or no gfx950 device is present, the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemfa_hip.so")
+
+EMFA_OK, EMFA_E_INVALID, EMFA_E_OOM, EMFA_E_SINGULAR, EMFA_E_NOCONV, EMFA_E_HIP, EMFA_E_NODEVICE = range(7)
+F32, F64 = 0, 1
+EFIE, CFIE = 0, 1
+
+# every symbol include/emfa_hip.h declares
+EXPORTS = [
+    "emfa_last_error", "emfa_version", "emfa_ctx_create", "emfa_ctx_destroy", "emfa_ctx_sync", "emfa_ctx_stream",
+    "emfa_ctx_bytes_in_use", "emfa_geom_icosphere", "emfa_geom_plate", "emfa_rwg_build", "emfa_mesh_upload",
+    "emfa_mesh_destroy", "emfa_mesh_unknowns", "emfa_mat_create", "emfa_mat_destroy", "emfa_mat_upload",
+    "emfa_mat_download", "emfa_mat_copy", "emfa_mat_rows", "emfa_mat_cols", "emfa_mat_prec", "emfa_mat_device_ptr",
+    "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
+    "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
+    "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset",
+]
+
+_vp = ctypes.c_void_p
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+
+class FillParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("quad", ctypes.c_int), ("k", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("sigma_re", ctypes.c_double), ("sigma_im", ctypes.c_double), ("cfie_mode", ctypes.c_int)]
+
+
+class RefineReport(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("relres_first", ctypes.c_double), ("relres_final", ctypes.c_double)]
+
+
+class GmresReport(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("relres", ctypes.c_double)]
+
+
+class EmfaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[emfa status {code}] {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    """Load libemfa_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise EmfaError(EMFA_E_NODEVICE, f"{LIB_PATH} not built: run python -m emfa_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        L.emfa_last_error.restype = ctypes.c_char_p
+        L.emfa_ctx_stream.restype = _vp
+        L.emfa_ctx_bytes_in_use.restype = ctypes.c_size_t
+        L.emfa_mat_device_ptr.restype = _vp
+        L.emfa_lu_matrix.restype = _vp
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if f.restype is ctypes.c_int and name not in ("emfa_version",):
+                pass
+        _LIB = L
+    return _LIB
+
+
+def _check(code):
+    if code != EMFA_OK:
+        raise EmfaError(code, lib().emfa_last_error().decode())
+
+
+def _pd(a):
+    return a.ctypes.data_as(_d)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_i)
+
+
+# ------------------------------------------------------------------ geometry
+def icosphere(subdiv, ax=1.0, ay=None, az=None):
+    ay = ax if ay is None else ay
+    az = ax if az is None else az
+    nv, nt = ctypes.c_int(), ctypes.c_int()
+    _check(lib().emfa_geom_icosphere(subdiv, ctypes.c_double(ax), ctypes.c_double(ay), ctypes.c_double(az), None, 0,
+                                     None, 0, ctypes.byref(nv), ctypes.byref(nt)))
+    xyz = np.zeros((nv.value, 3))
+    tri = np.zeros((nt.value, 3), dtype=np.int32)
+    _check(lib().emfa_geom_icosphere(subdiv, ctypes.c_double(ax), ctypes.c_double(ay), ctypes.c_double(az), _pd(xyz),
+                                     nv.value, _pi(tri), nt.value, ctypes.byref(nv), ctypes.byref(nt)))
+    return xyz, tri
+
+
+def plate(cells, size=1.0):
+    nv, nt = ctypes.c_int(), ctypes.c_int()
+    _check(lib().emfa_geom_plate(cells, ctypes.c_double(size), None, 0, None, 0, ctypes.byref(nv), ctypes.byref(nt)))
+    xyz = np.zeros((nv.value, 3))
+    tri = np.zeros((nt.value, 3), dtype=np.int32)
+    _check(lib().emfa_geom_plate(cells, ctypes.c_double(size), _pd(xyz), nv.value, _pi(tri), nt.value,
+                                 ctypes.byref(nv), ctypes.byref(nt)))
+    return xyz, tri
+
+
+def rwg_build(tri):
+    tri = np.ascontiguousarray(tri, dtype=np.int32)
+    n, nb = ctypes.c_int(), ctypes.c_int()
+    _check(lib().emfa_rwg_build(tri.shape[0], _pi(tri), None, 0, ctypes.byref(n), ctypes.byref(nb)))
+    out = np.zeros((max(n.value, 1), 6), dtype=np.int32)
+    _check(lib().emfa_rwg_build(tri.shape[0], _pi(tri), _pi(out), n.value, ctypes.byref(n), ctypes.byref(nb)))
+    return out[:n.value].copy(), nb.value
+
+
+def wavenumber(freq):
+    """k = 2 pi f / c0 (EMSolver/EFIE.cpp:47)."""
+    return 2.0 * np.pi * float(freq) / 299792458.0
+
+
+def incidence(theta_deg, phi_deg, pol_deg):
+    """(k_hat, e_hat, h_hat) as EMSolver/EM.cpp:29-41 builds them."""
+    th, ph, po = np.radians([theta_deg, phi_deg, pol_deg])
+    k = np.array([-np.sin(th) * np.cos(ph), -np.sin(th) * np.sin(ph), -np.cos(th)])
+    e = np.array([-np.cos(po) * np.cos(th) * np.cos(ph) - np.sin(po) * np.sin(ph),
+                  -np.cos(po) * np.cos(th) * np.sin(ph) + np.sin(po) * np.cos(ph), np.cos(po) * np.sin(th)])
+    return k, e, np.cross(k, e)
+
+
+def inc_table(angles):
+    return np.array([np.concatenate(incidence(*a)) for a in angles]).reshape(-1, 9)
+
+
+# ------------------------------------------------------------------ handles
+class Context:
+    def __init__(self, device=0):
+        self.h = _vp()
+        _check(lib().emfa_ctx_create(device, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().emfa_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().emfa_ctx_sync(self.h))
+
+    def stream(self):
+        return lib().emfa_ctx_stream(self.h)
+
+    def bytes_in_use(self):
+        return lib().emfa_ctx_bytes_in_use(self.h)
+
+    def prof(self, on=True):
+        _check(lib().emfa_prof_enable(self.h, int(on)))
+
+    def prof_reset(self):
+        _check(lib().emfa_prof_reset(self.h))
+
+    def prof_get(self, family):
+        ms, n, w = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        _check(lib().emfa_prof_get(self.h, family.encode(), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w)))
+        return ms.value, n.value, w.value
+
+
+class Mesh:
+    def __init__(self, ctx, xyz, tri, rwg):
+        self.ctx = ctx
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        tri = np.ascontiguousarray(tri, dtype=np.int32)
+        rwg = np.ascontiguousarray(rwg, dtype=np.int32)
+        self.xyz, self.tri, self.rwg = xyz, tri, rwg
+        self.h = _vp()
+        _check(lib().emfa_mesh_upload(ctx.h, _pd(xyz), xyz.shape[0], _pi(tri), tri.shape[0], _pi(rwg), rwg.shape[0],
+                                      ctypes.byref(self.h)))
+        self.n = rwg.shape[0]
+
+    def close(self):
+        if self.h:
+            lib().emfa_mesh_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mat:
+    def __init__(self, ctx, rows, cols, prec=F64, handle=None, owned=True):
+        self.ctx = ctx
+        self.rows, self.cols, self.prec = rows, cols, prec
+        self.owned = owned
+        if handle is None:
+            self.h = _vp()
+            _check(lib().emfa_mat_create(ctx.h, rows, cols, prec, ctypes.byref(self.h)))
+        else:
+            self.h = _vp(handle)
+
+    @classmethod
+    def from_numpy(cls, ctx, a, prec=F64):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        m = cls(ctx, a.shape[0], a.shape[1], prec)
+        m.upload(a)
+        return m
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.rows, self.cols)
+        _check(lib().emfa_mat_upload(self.h, a.ctypes.data_as(_d)))
+
+    def numpy(self):
+        out = np.zeros((self.rows, self.cols), dtype=np.complex128)
+        _check(lib().emfa_mat_download(self.h, out.ctypes.data_as(_d)))
+        return out
+
+    def copy_from(self, other):
+        _check(lib().emfa_mat_copy(self.h, other.h))
+
+    def close(self):
+        if self.h and self.owned:
+            lib().emfa_mat_destroy(self.h)
+        self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LU:
+    def __init__(self, ctx, Z, prec):
+        self.ctx = ctx
+        self.h = _vp()
+        code = lib().emfa_lu_factor(ctx.h, Z.h, prec, ctypes.byref(self.h))
+        self.singular = code == EMFA_E_SINGULAR
+        if code not in (EMFA_OK, EMFA_E_SINGULAR):
+            _check(code)
+        self.n = Z.rows
+        self.prec = prec
+
+    def info(self):
+        return lib().emfa_lu_info(self.h)
+
+    def pivots(self):
+        p = np.zeros(self.n, dtype=np.int32)
+        _check(lib().emfa_lu_pivots(self.h, _pi(p)))
+        return p
+
+    def factor(self):
+        return Mat(self.ctx, self.n, self.n, self.prec, handle=lib().emfa_lu_matrix(self.h), owned=False)
+
+    def solve(self, B):
+        _check(lib().emfa_lu_solve(self.ctx.h, self.h, B.h))
+
+    def close(self):
+        if self.h:
+            lib().emfa_lu_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ operations
+def fill(ctx, mesh, Z, kind, k, quad=3, alpha=1.0, sigma=0j, cfie_mode=0):
+    p = FillParams(kind, quad, float(k), float(alpha), float(np.real(sigma)), float(np.imag(sigma)), int(cfie_mode))
+    _check(lib().emfa_fill(ctx.h, mesh.h, ctypes.byref(p), Z.h))
+
+
+def rhs_fill(ctx, mesh, B, kind, k, alpha, inc):
+    inc = np.ascontiguousarray(inc, dtype=np.float64).reshape(-1, 9)
+    _check(lib().emfa_rhs_fill(ctx.h, mesh.h, kind, ctypes.c_double(k), ctypes.c_double(alpha), inc.shape[0],
+                               _pd(inc), B.h))
+
+
+def refine(ctx, Z, lu, B, X, tol=1e-12, maxit=10):
+    rep = RefineReport()
+    code = lib().emfa_refine(ctx.h, Z.h, lu.h, B.h, X.h, ctypes.c_double(tol), maxit, ctypes.byref(rep))
+    if code not in (EMFA_OK, EMFA_E_NOCONV):
+        _check(code)
+    return rep, code == EMFA_OK
+
+
+def gemm_residual(ctx, A, X, C, negate=True, accumulate=True):
+    _check(lib().emfa_gemm_residual(ctx.h, A.h, X.h, C.h, int(negate), int(accumulate)))
+
+
+def gmres(ctx, Z, B, X, m=30, tol=1e-6, max_restart=100):
+    rep = GmresReport()
+    code = lib().emfa_gmres(ctx.h, Z.h, B.h, X.h, m, ctypes.c_double(tol), max_restart, ctypes.byref(rep))
+    if code not in (EMFA_OK, EMFA_E_NOCONV):
+        _check(code)
+    return rep, code == EMFA_OK
+
+
+def rcs(ctx, mesh, k, X, dirs, cols=None):
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(dirs.shape[0])
+    c = None
+    if cols is not None:
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+    _check(lib().emfa_rcs(ctx.h, mesh.h, ctypes.c_double(k), X.h, dirs.shape[0], _pd(dirs),
+                          _pi(c) if c is not None else None, _pd(out)))
+    return out

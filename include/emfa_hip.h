@@ -1,0 +1,185 @@
+/*
+ * emfa_hip.h -- C ABI of the MI355X (gfx950) method-of-moments library.
+ *
+ * Plain C: opaque handles, pointers and sizes; no exceptions cross this
+ * boundary.  Every call returns an emfa_status; emfa_last_error() gives
+ * the message of the last failure on the calling thread.
+ *
+ * Each entry point replaces a piece of the reference's in-process C++ path
+ * (cited as reference file:line).  The reference has no device boundary;
+ * these calls are what its solver classes (EMSolver/EFIE.cpp, CFIE.cpp,
+ * MFIE.cpp) bind instead of their CPU loops.
+ *
+ * Layouts (device side, owned by handles):
+ *   complex matrices: row-major (row = test function), interleaved re/im,
+ *                     complex<float> (EMFA_F32) or complex<double> (EMFA_F64)
+ *   mesh:             vertices double[nv][3], triangles int[nt][3] (0-based)
+ *   RWG list:         int[n][6] = {v1, v2, vxp, vxm, tp, tm}
+ *                     (EMComponent/CommonEdge.h:14-43, SCE fields)
+ */
+#ifndef EMFA_HIP_H
+#define EMFA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EMFA_OK = 0,
+    EMFA_E_INVALID = 1,  /* bad argument / shape                     */
+    EMFA_E_OOM = 2,      /* device allocation failed / over budget     */
+    EMFA_E_SINGULAR = 3, /* exactly zero pivot during factorisation    */
+    EMFA_E_NOCONV = 4,   /* iterative method did not converge          */
+    EMFA_E_HIP = 5,      /* HIP runtime error                          */
+    EMFA_E_NODEVICE = 6  /* no usable gfx950 device                    */
+} emfa_status;
+
+enum { EMFA_F32 = 0, EMFA_F64 = 1 };
+enum { EMFA_EFIE = 0, EMFA_CFIE = 1 };
+
+typedef struct emfa_ctx emfa_ctx;
+typedef struct emfa_mesh emfa_mesh;
+typedef struct emfa_mat emfa_mat;
+typedef struct emfa_lu emfa_lu;
+
+/* ---- library / context --------------------------------------------- */
+const char *emfa_last_error(void);
+int emfa_version(void);
+/* device: HIP ordinal.  Creates one HIP stream owned by the context.
+ * Replaces nothing in the reference (it has no device). */
+emfa_status emfa_ctx_create(int device, emfa_ctx **out);
+emfa_status emfa_ctx_destroy(emfa_ctx *ctx);
+emfa_status emfa_ctx_sync(emfa_ctx *ctx);
+/* the context's hipStream_t, as an opaque pointer */
+void *emfa_ctx_stream(emfa_ctx *ctx);
+/* bytes currently allocated by this context's handles */
+size_t emfa_ctx_bytes_in_use(const emfa_ctx *ctx);
+
+/* ---- host-side geometry (no device work) ---------------------------- */
+/* Synthetic meshes of the workloads: icosahedron subdivided `subdiv` times
+ * and projected on the sphere (T=20*4^s), scaled per axis for ellipsoids.
+ * Replaces the reference's file loaders EMComponent/Mesh.cpp:22-49 for the
+ * synthetic configs.  Sizes: nv = 10*4^s+2, nt = 20*4^s. */
+emfa_status emfa_geom_icosphere(int subdiv, double ax, double ay, double az, double *xyz, int nv_cap, int *tri,
+                                int nt_cap, int *nv, int *nt);
+/* Square plate in z=0, side `size`, n x n cells, 2 triangles per cell. */
+emfa_status emfa_geom_plate(int cells, double size, double *xyz, int nv_cap, int *tri, int nt_cap, int *nv, int *nt);
+/* RWG list with the reference's ordering (EMComponent/CommonEdge.cpp:74-106,
+ * 704-743: first-appearance order of shared edges, v1->v2 as in tp),
+ * built in O(E) with a hash map.  Returns the RWG count in *n. */
+emfa_status emfa_rwg_build(int nt, const int *tri, int *rwg, int cap, int *n, int *n_boundary);
+
+/* ---- mesh upload ---------------------------------------------------- */
+emfa_status emfa_mesh_upload(emfa_ctx *ctx, const double *xyz, int nv, const int *tri, int nt, const int *rwg, int n,
+                             emfa_mesh **out);
+emfa_status emfa_mesh_destroy(emfa_mesh *mesh);
+int emfa_mesh_unknowns(const emfa_mesh *mesh);
+
+/* ---- dense complex matrices ----------------------------------------- */
+emfa_status emfa_mat_create(emfa_ctx *ctx, int rows, int cols, int prec, emfa_mat **out);
+emfa_status emfa_mat_destroy(emfa_mat *m);
+/* host buffers are interleaved complex, row-major, double precision
+ * (converted to the matrix precision on upload) */
+emfa_status emfa_mat_upload(emfa_mat *m, const double *host);
+emfa_status emfa_mat_download(const emfa_mat *m, double *host);
+emfa_status emfa_mat_copy(emfa_mat *dst, const emfa_mat *src); /* converts precision */
+int emfa_mat_rows(const emfa_mat *m);
+int emfa_mat_cols(const emfa_mat *m);
+int emfa_mat_prec(const emfa_mat *m);
+/* raw device pointer of the matrix storage */
+void *emfa_mat_device_ptr(emfa_mat *m);
+
+/* ---- impedance fill (K1 regular, K2 singular, gather) --------------- */
+typedef struct {
+    int kind;        /* EMFA_EFIE or EMFA_CFIE                                  */
+    int quad;        /* 3 (reference rule) or 7 points per triangle             */
+    double k;        /* wave number 2 pi f / c                                  */
+    double alpha;    /* CFIE combination (EMSolver/Miscellaneous.cpp:29)        */
+    double sigma_re; /* EFIE surface impedance (EMSolver/EFIE.cpp:297-306)      */
+    double sigma_im;
+    int cfie_mode;   /* 0: reference self term (Miscellaneous.cpp:32-77), 1: EFIE-style extraction */
+} emfa_fill_params;
+
+/* Fills Z (n x n, precision of Z).  Replaces EFIE::fillZ
+ * (EMSolver/EFIE.cpp:603-696) and CFIE::fillZ (EMSolver/CFIE.cpp:485-568),
+ * EDM far-field shortcut (EFIE.cpp:647-651) not applied. */
+emfa_status emfa_fill(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_fill_params *p, emfa_mat *Z);
+
+/* ---- excitation (K3) ------------------------------------------------- */
+/* inc: nrhs x 9 doubles {k_hat(3), e_hat(3), h_hat(3)} (EMSolver/EM.cpp:33-41).
+ * B: n x nrhs.  Replaces EFIE::fillV (EFIE.cpp:568-601,698-722) and
+ * CFIE::fillV (CFIE.cpp:570-597, Miscellaneous.cpp:152-189). */
+emfa_status emfa_rhs_fill(emfa_ctx *ctx, const emfa_mesh *mesh, int kind, double k, double alpha, int nrhs,
+                          const double *inc, emfa_mat *B);
+
+/* ---- dense LU (K4 panel, K5 MFMA trailing update, K6 triangular solve) */
+/* Factor a copy of Z in precision `prec` with partial pivoting (LAPACK
+ * getrf semantics).  Replaces arma::solve(...,fast) -> cgesv
+ * (EMSolver/MFIE.cpp:64).  Returns EMFA_E_SINGULAR for an exactly zero
+ * pivot (the factor is still produced, like LAPACK's info > 0). */
+emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu **out);
+emfa_status emfa_lu_destroy(emfa_lu *lu);
+/* In place: B (n x nrhs, any precision) <- A^{-1} B using the factor. */
+emfa_status emfa_lu_solve(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B);
+/* pivots (0-based, like the oracle) and the factor itself */
+emfa_status emfa_lu_pivots(const emfa_lu *lu, int *ipiv_host);
+emfa_mat *emfa_lu_matrix(emfa_lu *lu);
+int emfa_lu_info(const emfa_lu *lu);
+
+/* ---- mixed-precision refinement -------------------------------------- */
+typedef struct {
+    int iterations;      /* refinement steps taken                       */
+    double relres_first; /* max over columns of ||b - A x|| / ||b||, before */
+    double relres_final; /* ... after the last step                       */
+} emfa_refine_report;
+/* X (F64, n x nrhs) <- solution of Z X = B, starting from the factor's
+ * solution and refining with FP64 residuals R = B - Z X (Z read in its own
+ * precision, products accumulated in FP64) until every column's relative
+ * residual <= tol or maxit steps.  No reference counterpart: it is the
+ * precision form built beside the reference's dense solve. */
+emfa_status emfa_refine(emfa_ctx *ctx, const emfa_mat *Z, const emfa_lu *lu, const emfa_mat *B, emfa_mat *X,
+                        double tol, int maxit, emfa_refine_report *rep);
+
+/* ---- products -------------------------------------------------------- */
+/* C (F64) = alpha_sign * (A * Xin) + beta * C, A any precision, Xin F64;
+ * accumulated in FP64.  alpha_sign is +1 or -1, beta 0 or 1. */
+emfa_status emfa_gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *Xin, emfa_mat *C, int negate,
+                               int accumulate);
+
+/* ---- GMRES(m) (K7 matvec, K8 FP64 orthogonalisation) ----------------- */
+typedef struct {
+    int iterations;
+    int restarts;
+    double relres;
+} emfa_gmres_report;
+/* Solves Z x = b column by column (b, x: n x nrhs F64 matrices) with
+ * restarted GMRES(m), FP64 Gram-Schmidt, relative tolerance.  Replaces
+ * iml::GMRES (EMIterative/iml.cpp:156-207, called at EMSolver/EFIE.cpp:79)
+ * with its conjugated Arnoldi coefficient and unbounded restarts fixed. */
+emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, int m, double tol,
+                       int max_restart, emfa_gmres_report *rep);
+
+/* ---- far field (K9) --------------------------------------------------- */
+/* RCS in dB for ndir directions dirs[ndir][3]; direction d uses column
+ * cols[d] of X (cols may be NULL: column 0).  Replaces getBiRCS
+ * (EMSolver/EFIE.cpp:788-832, CFIE.cpp:643-685). */
+emfa_status emfa_rcs(emfa_ctx *ctx, const emfa_mesh *mesh, double k, const emfa_mat *X, int ndir, const double *dirs,
+                     const int *cols, double *out_db);
+
+/* ---- instrumentation -------------------------------------------------- */
+/* When enabled, each launch of a named kernel family is bracketed by HIP
+ * events on the context stream; totals are read with emfa_prof_get.
+ * Families: "fill_pairs", "fill_gather", "lu_gemm", "lu_panel", "lu_trsm",
+ * "lu_laswp", "solve_trsm", "refine_gemm", "rhs", "rcs". */
+emfa_status emfa_prof_enable(emfa_ctx *ctx, int on);
+emfa_status emfa_prof_get(emfa_ctx *ctx, const char *family, double *total_ms, long long *launches,
+                          double *flops_or_bytes);
+emfa_status emfa_prof_reset(emfa_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMFA_HIP_H */

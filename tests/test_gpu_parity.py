@@ -1,0 +1,268 @@
+"""Parity of the gfx950 path (through the C ABI) against the FP64 oracle.
+
+Tolerances: the FP64 path must match the oracle to rounding (entry-wise
+relative 1e-10 on Z, 1e-9 on solutions); the FP32 fill is held to 2e-5
+relative (single-precision arithmetic on the same formulas); RCS in dB to
+0.05 dB (FP64) and 0.1 dB (FP32 + refinement), the north-star bounds.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from conftest import rel, sphere
+from emfa_amd import hip
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _cases_dirs(th):
+    th = np.radians(np.asarray(th, dtype=float))
+    return np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+
+
+# ---------------------------------------------------------------- products
+@pytest.mark.parametrize("aprec", [hip.F64, hip.F32])
+@pytest.mark.parametrize("shape", [(64, 48, 3), (300, 257, 17), (129, 1000, 181)])
+def test_gemm_residual(ctx, aprec, shape):
+    n, k, m = shape
+    _say(f"gemm_residual aprec={aprec} shape={shape}")
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    X = rng.standard_normal((k, m)) + 1j * rng.standard_normal((k, m))
+    C0 = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    if aprec == hip.F32:
+        A = A.astype(np.complex64).astype(np.complex128)
+    dA = hip.Mat.from_numpy(ctx, A, aprec)
+    dX = hip.Mat.from_numpy(ctx, X, hip.F64)
+    dC = hip.Mat.from_numpy(ctx, C0, hip.F64)
+    hip.gemm_residual(ctx, dA, dX, dC, negate=True, accumulate=True)
+    ref = C0 - A @ X
+    assert rel(dC.numpy(), ref) < 1e-13
+    hip.gemm_residual(ctx, dA, dX, dC, negate=False, accumulate=False)
+    assert rel(dC.numpy(), A @ X) < 1e-13
+
+
+# ---------------------------------------------------------------- fill
+@pytest.mark.parametrize("quad", [3, 7])
+@pytest.mark.parametrize("subdiv", [1, 2])
+def test_fill_efie_f64(ctx, quad, subdiv):
+    _say(f"fill efie f64 quad={quad} subdiv={subdiv}")
+    xyz, tri, rwg = sphere(subdiv, 0.5)
+    k = hip.wavenumber(3e8)
+    Zo = O.fill_efie(xyz, tri, rwg, k, quad)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z, hip.EFIE, k, quad)
+    Zg = Z.numpy()
+    err = np.abs(Zg - Zo) / np.abs(Zo).max()
+    assert err.max() < 1e-12, err.max()
+
+
+def test_fill_efie_sigma_f64(ctx):
+    _say("fill efie f64 sigma")
+    xyz, tri, rwg = sphere(1, 0.5)
+    k = hip.wavenumber(3e8)
+    sig = 0.0025 + 0.001j
+    Zo = O.fill_efie(xyz, tri, rwg, k, 3, sigma=sig)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z, hip.EFIE, k, 3, sigma=sig)
+    assert np.abs(Z.numpy() - Zo).max() / np.abs(Zo).max() < 1e-12
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("quad", [3, 7])
+def test_fill_cfie_f64(ctx, mode, quad):
+    _say(f"fill cfie f64 mode={mode} quad={quad}")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    Zo = O.fill_cfie(xyz, tri, rwg, k, 0.5, quad, mode)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z, hip.CFIE, k, quad, alpha=0.5, cfie_mode=mode)
+    assert np.abs(Z.numpy() - Zo).max() / np.abs(Zo).max() < 1e-12
+
+
+@pytest.mark.parametrize("kind", [hip.EFIE, hip.CFIE])
+def test_fill_f32(ctx, kind):
+    _say(f"fill f32 kind={kind}")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    if kind == hip.EFIE:
+        Zo = O.fill_efie(xyz, tri, rwg, k, 3)
+    else:
+        Zo = O.fill_cfie(xyz, tri, rwg, k, 0.5, 3, 0)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F32)
+    hip.fill(ctx, M, Z, kind, k, 3, alpha=0.5)
+    assert np.abs(Z.numpy() - Zo).max() / np.abs(Zo).max() < 2e-5
+
+
+def test_fill_chunked_equals_single(ctx, monkeypatch):
+    """A small scratch budget (many test-triangle chunks) gives bit-identical Z."""
+    _say("fill chunked")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z1 = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z1, hip.CFIE, k, 3, alpha=0.5)
+    monkeypatch.setenv("EMFA_FILL_SCRATCH_MB", "1")
+    Z2 = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z2, hip.CFIE, k, 3, alpha=0.5)
+    assert np.array_equal(Z1.numpy(), Z2.numpy())
+
+
+# ---------------------------------------------------------------- rhs / rcs
+@pytest.mark.parametrize("kind", [hip.EFIE, hip.CFIE])
+def test_rhs(ctx, kind):
+    _say(f"rhs kind={kind}")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    angles = [(th, ph, po) for th in (0, 37, 90, 180) for ph in (0, 45) for po in (0, 90)]
+    inc = hip.inc_table(angles)
+    Bo = O.rhs(xyz, tri, rwg, kind, k, 0.5, inc)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    B = hip.Mat(ctx, M.n, len(angles), hip.F64)
+    hip.rhs_fill(ctx, M, B, kind, k, 0.5, inc)
+    assert rel(B.numpy(), Bo) < 1e-13
+
+
+def test_rcs(ctx):
+    _say("rcs")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((rwg.shape[0], 3)) + 1j * rng.standard_normal((rwg.shape[0], 3))
+    dirs = _cases_dirs(np.arange(0, 361, 7.0))
+    cols = np.arange(dirs.shape[0]) % 3
+    ro = O.rcs(xyz, tri, rwg, k, X, dirs, cols)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    dX = hip.Mat.from_numpy(ctx, X, hip.F64)
+    rg = hip.rcs(ctx, M, k, dX, dirs, cols)
+    assert np.abs(rg - ro).max() < 1e-9
+
+
+# ---------------------------------------------------------------- LU
+@pytest.mark.parametrize("n", [5, 64, 333, 700])
+def test_lu_f64_matches_getrf(ctx, n):
+    _say(f"lu f64 n={n}")
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    dA = hip.Mat.from_numpy(ctx, A, hip.F64)
+    lu = hip.LU(ctx, dA, hip.F64)
+    LUo, pivo, info = O.lu(A)
+    assert info == 0 and lu.info() == 0
+    assert np.array_equal(lu.pivots(), pivo)
+    assert np.abs(lu.factor().numpy() - LUo).max() / np.abs(LUo).max() < 1e-10
+    B = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
+    dB = hip.Mat.from_numpy(ctx, B, hip.F64)
+    lu.solve(dB)
+    assert rel(dB.numpy(), np.linalg.solve(A, B)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [100, 517])
+def test_lu_f32(ctx, n):
+    _say(f"lu f32 n={n}")
+    rng = np.random.default_rng(n + 1)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    A32 = A.astype(np.complex64).astype(np.complex128)
+    dA = hip.Mat.from_numpy(ctx, A32, hip.F32)
+    lu = hip.LU(ctx, dA, hip.F32)
+    B = rng.standard_normal((n, 5)) + 1j * rng.standard_normal((n, 5))
+    dB = hip.Mat.from_numpy(ctx, B, hip.F32)
+    lu.solve(dB)
+    ref = np.linalg.solve(A32, B.astype(np.complex64).astype(np.complex128))
+    assert rel(dB.numpy(), ref) < 1e-4
+
+
+def test_lu_singular_reports(ctx):
+    _say("lu singular")
+    n = 40
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A[:, 17] = 0
+    dA = hip.Mat.from_numpy(ctx, A, hip.F64)
+    lu = hip.LU(ctx, dA, hip.F64)
+    assert lu.singular and lu.info() == 18
+
+
+def test_refine_f32_to_f64(ctx):
+    _say("refine")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F32)
+    hip.fill(ctx, M, Z, hip.CFIE, k, 3, alpha=0.5)
+    inc = hip.inc_table([(t, 0, 0) for t in range(0, 181, 30)])
+    B = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    hip.rhs_fill(ctx, M, B, hip.CFIE, k, 0.5, inc)
+    lu = hip.LU(ctx, Z, hip.F32)
+    X = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    rep, ok = hip.refine(ctx, Z, lu, B, X, tol=1e-12, maxit=10)
+    assert ok and rep.relres_final < 1e-12 and rep.relres_first > rep.relres_final
+    Zh = Z.numpy()
+    ref = np.linalg.solve(Zh, B.numpy())
+    assert rel(X.numpy(), ref) < 1e-10
+
+
+def test_gmres(ctx):
+    _say("gmres")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z, hip.CFIE, k, 3, alpha=0.5)
+    inc = hip.inc_table([(0, 0, 0), (90, 0, 90)])
+    B = hip.Mat(ctx, M.n, 2, hip.F64)
+    hip.rhs_fill(ctx, M, B, hip.CFIE, k, 0.5, inc)
+    X = hip.Mat(ctx, M.n, 2, hip.F64)
+    rep, ok = hip.gmres(ctx, Z, B, X, m=30, tol=1e-10, max_restart=50)
+    assert ok, rep.relres
+    ref = np.linalg.solve(Z.numpy(), B.numpy())
+    assert rel(X.numpy(), ref) < 1e-8
+
+
+# ---------------------------------------------------------------- end to end (C1 mesh)
+def test_end_to_end_efie_sphere(ctx):
+    """EFIE, PEC sphere r=0.5 m, 300 MHz, icosphere s=3 (1920 RWG), x-pol from +z."""
+    _say("end-to-end C1")
+    xyz, tri, rwg = sphere(3, 0.5)
+    k = hip.wavenumber(3e8)
+    inc = hip.inc_table([(0, 0, 0)])
+    Zo = O.fill_efie(xyz, tri, rwg, k, 3)
+    Bo = O.rhs(xyz, tri, rwg, 0, k, 1.0, inc)
+    Xo = sl.solve(Zo, Bo)
+    th = np.arange(0, 361, 1.0)
+    dirs = _cases_dirs(th)
+    ro = O.rcs(xyz, tri, rwg, k, Xo, dirs)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    # FP64 plain path
+    Z = hip.Mat(ctx, M.n, M.n, hip.F64)
+    hip.fill(ctx, M, Z, hip.EFIE, k, 3)
+    B = hip.Mat(ctx, M.n, 1, hip.F64)
+    hip.rhs_fill(ctx, M, B, hip.EFIE, k, 1.0, inc)
+    lu = hip.LU(ctx, Z, hip.F64)
+    lu.solve(B)
+    X = B.numpy()
+    assert rel(X, Xo) <= 1e-5
+    r64 = hip.rcs(ctx, M, k, B, dirs)
+    assert np.abs(r64 - ro).max() <= 0.05
+    # FP32 fill + FP32 LU + FP64 refinement
+    Z32 = hip.Mat(ctx, M.n, M.n, hip.F32)
+    hip.fill(ctx, M, Z32, hip.EFIE, k, 3)
+    B2 = hip.Mat(ctx, M.n, 1, hip.F64)
+    hip.rhs_fill(ctx, M, B2, hip.EFIE, k, 1.0, inc)
+    lu32 = hip.LU(ctx, Z32, hip.F32)
+    X2 = hip.Mat(ctx, M.n, 1, hip.F64)
+    rep, ok = hip.refine(ctx, Z32, lu32, B2, X2, tol=1e-10, maxit=10)
+    assert ok
+    r32 = hip.rcs(ctx, M, k, X2, dirs)
+    assert np.abs(r32 - ro).max() <= 0.1
