@@ -226,7 +226,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                long long i = bm + wm * 64 + mt * 16 + 4 * (lane >> 4) + v;
+                // V_MFMA_F64_16X16X4_F64 result: lane l, element v holds row l/16 + 4v, column l%16
+                long long i = bm + wm * 64 + mt * 16 + (lane >> 4) + 4 * v;
                 long long j = bn + wn * 64 + nt * 16 + r16;
                 if (i < g.M && j < g.Nr) {
                     double *c = C + i * g.ldc + j;

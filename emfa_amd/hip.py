@@ -189,9 +189,9 @@ class Mesh:
         self.n = rwg.shape[0]
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().emfa_mesh_destroy(self.h)
-            self.h = _vp()
+        self.h = _vp()
 
     def __del__(self):
         try:
@@ -233,7 +233,7 @@ class Mat:
         _check(lib().emfa_mat_copy(self.h, other.h))
 
     def close(self):
-        if self.h and self.owned:
+        if self.h and self.owned and self.ctx.h:
             lib().emfa_mat_destroy(self.h)
         self.h = _vp()
 
@@ -270,9 +270,9 @@ class LU:
         _check(lib().emfa_lu_solve(self.ctx.h, self.h, B.h))
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().emfa_lu_destroy(self.h)
-            self.h = _vp()
+        self.h = _vp()
 
     def __del__(self):
         try:
