@@ -1,0 +1,218 @@
+"""Benchmark of the MoM hot path on one MI355X.
+
+Workload (default): CFIE (alpha = 0.5) on a PEC sphere r = 1 m at 600 MHz,
+icosphere subdivided 5 times (20480 triangles, 30720 RWG unknowns), 181
+monostatic right-hand sides (theta 0..180 deg, 1 deg step, phi 0, pole 0).
+
+One step = one complete solve on the FP32+refine path:
+  complex-FP32 fill (3x3 Gauss, FP64 self terms) -> 181 RHS columns ->
+  FP32 LU (MFMA trailing updates) -> FP64 iterative refinement (FP64 MFMA
+  residual products) -> monostatic RCS of every column.
+`value` = solves per second over the timed steps.  The plain FP64 path
+(FP64 fill, FP64 LU, FP64 triangular solves) is timed once beside it, and
+the largest RCS difference between the two paths is reported.
+
+Roofline: the dominant kernel is the LU trailing update (MFMA GEMM); its
+algorithmic flops per launch are 8*M*Nc*K (complex update of an M x Nc block
+with rank K), timed with HIP events on the library's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f32_mfma": 157.3, "f64_mfma": 78.6, "hbm": 8000.0}
+
+CONFIGS = {
+    # name: (kind, subdiv, radius, freq, alpha, quad, nrhs-angles)
+    "c2": ("CFIE", 5, 1.0, 6e8, 0.5, 3, np.arange(0.0, 181.0, 1.0)),
+    "c1": ("EFIE", 3, 0.5, 3e8, 1.0, 7, np.array([0.0])),
+    "s4": ("CFIE", 4, 1.0, 6e8, 0.5, 3, np.arange(0.0, 181.0, 1.0)),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    return ap.parse_args()
+
+
+def monostatic_dirs(theta_deg):
+    th = np.radians(theta_deg)
+    return np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+
+
+def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
+    """Oracle fill on a row sample + LAPACK cgesv (the routine behind the
+    reference's arma::solve) on a smaller matrix, scaled to one full solve."""
+    from oracle import oracle as O
+    import scipy.linalg.lapack as lp
+    n = rwg.shape[0]
+    threads = O.max_threads()
+    # fill: grow the row sample until ~40% of the budget is spent
+    rows = 8
+    while True:
+        t0 = time.perf_counter()
+        if kind == "CFIE":
+            O.fill_cfie(xyz, tri, rwg, k, alpha, quad, 0, rows=(0, rows))
+        else:
+            O.fill_efie(xyz, tri, rwg, k, quad, rows=(0, rows))
+        tf = time.perf_counter() - t0
+        if tf > 0.4 * budget_s or rows >= n:
+            break
+        rows = min(n, rows * 2)
+    fill_est = tf * n / rows
+    # LU + solve: cgesv on ns x ns with the same RHS count, scaled by N^3
+    ns = 1024
+    while True:
+        rng = np.random.default_rng(0)
+        A = (rng.standard_normal((ns, ns)) + 1j * rng.standard_normal((ns, ns))).astype(np.complex64)
+        B = (rng.standard_normal((ns, n_rhs)) + 1j * rng.standard_normal((ns, n_rhs))).astype(np.complex64)
+        t0 = time.perf_counter()
+        lp.cgesv(A, B, overwrite_a=1, overwrite_b=1)
+        tl = time.perf_counter() - t0
+        if tl > 0.3 * budget_s or ns >= n:
+            break
+        ns = min(n, ns * 2)
+    lu_est = tl * (n / ns) ** 3
+    total = fill_est + lu_est
+    return {
+        "value": 1.0 / total,
+        "unit": "solves/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": (f"oracle {kind} fill (FP64, RWG-pair loop as EFIE.cpp:603-696/CFIE.cpp:485-568) on rows 0..{rows} "
+                   f"of {n} ({tf:.1f}s -> {fill_est:.0f}s for all rows) + LAPACK cgesv (arma::solve's routine) "
+                   f"{ns}x{ns} with {n_rhs} RHS ({tl:.1f}s -> {lu_est:.0f}s scaled by N^3); est. full solve "
+                   f"{total:.0f}s"),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    from emfa_amd import hip
+
+    kind_s, subdiv, radius, freq, alpha, quad, thetas = CONFIGS[a.config]
+    kind = hip.CFIE if kind_s == "CFIE" else hip.EFIE
+    xyz, tri = hip.icosphere(subdiv, radius)
+    rwg, _ = hip.rwg_build(tri)
+    n = rwg.shape[0]
+    k = hip.wavenumber(freq)
+    inc = hip.inc_table([(t, 0.0, 0.0) for t in thetas])
+    nrhs = inc.shape[0]
+    dirs = monostatic_dirs(thetas)
+    cols = np.arange(nrhs, dtype=np.int32)
+
+    ctx = hip.Context(0)
+    mesh = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, n, n, hip.F32)
+    B = hip.Mat(ctx, n, nrhs, hip.F64)
+    X = hip.Mat(ctx, n, nrhs, hip.F64)
+    reports = []
+
+    def step():
+        hip.fill(ctx, mesh, Z, kind, k, quad, alpha=alpha)
+        hip.rhs_fill(ctx, mesh, B, kind, k, alpha, inc)
+        lu = hip.LU(ctx, Z, hip.F32)
+        rep, ok = hip.refine(ctx, Z, lu, B, X, tol=1e-10, maxit=8)
+        reports.append((rep.iterations, rep.relres_first, rep.relres_final, ok))
+        r = hip.rcs(ctx, mesh, k, X, dirs, cols)
+        lu.close()
+        return r
+
+    for _ in range(a.warmup):
+        step()
+    ctx.sync()
+    ctx.prof_reset()
+    ctx.prof(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r32 = step()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / a.steps
+    fams = {}
+    for f in ["fill_pairs", "fill_gather", "rhs", "lu_panel", "lu_laswp", "lu_trsm", "lu_gemm", "solve_trsm",
+              "solve_gemm", "refine_gemm", "rcs"]:
+        ms, cnt, work = ctx.prof_get(f)
+        fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
+    ctx.prof(False)
+    g = fams["lu_gemm"]
+    gemm_tflops = g["work_per_step"] / (g["ms_per_step"] * 1e-3) / 1e12 if g["ms_per_step"] > 0 else 0.0
+    avg_launch_ms = g["ms_per_step"] / max(g["launches_per_step"], 1)
+
+    extra = {}
+    if not a.no_fp64:
+        Z.close()
+        Z64 = hip.Mat(ctx, n, n, hip.F64)
+        B64 = hip.Mat(ctx, n, nrhs, hip.F64)
+        ctx.sync()
+        t1 = time.perf_counter()
+        hip.fill(ctx, mesh, Z64, kind, k, quad, alpha=alpha)
+        hip.rhs_fill(ctx, mesh, B64, kind, k, alpha, inc)
+        lu64 = hip.LU(ctx, Z64, hip.F64)
+        lu64.solve(B64)
+        r64 = hip.rcs(ctx, mesh, k, B64, dirs, cols)
+        ctx.sync()
+        t64 = time.perf_counter() - t1
+        extra = {"fp64_path_solves_per_s": 1.0 / t64, "fp64_path_s": t64,
+                 "speedup_vs_fp64_path": t64 / dt,
+                 "rcs_max_abs_diff_db_vs_fp64_path": float(np.abs(r32 - r64).max())}
+        lu64.close()
+        Z64.close()
+        B64.close()
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(xyz, tri, rwg, kind_s, k, alpha, quad, nrhs, a.cpu_seconds)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}"}
+
+    out = {
+        "metric": "solves/sec (FP32+refine path) + p50/p99 latency",
+        "value": 1.0 / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 fill/LU + f64 refinement",
+        "data": "synthetic icosphere mesh",
+        "config": {"workload": f"{kind_s} PEC sphere r={radius} m f={freq / 1e6:.0f} MHz icosphere s={subdiv}",
+                   "unknowns": n, "triangles": int(tri.shape[0]), "rhs": nrhs, "quadrature": f"{quad}x{quad}",
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": dt * 1e3, "p99": dt * 1e3, "note": "one solve per step, sequential"},
+        "roofline": {"bound": "mfma", "kernel": "lu_gemm (k_gemm_f32, V_MFMA_F32_32X32X2_F32)",
+                     "achieved": gemm_tflops, "peak": PEAK["f32_mfma"], "unit": "TFLOP/s",
+                     "frac": gemm_tflops / PEAK["f32_mfma"], "traffic": None,
+                     "avg_launch_ms": avg_launch_ms, "launches_per_step": g["launches_per_step"]},
+        "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
+        "refine": {"iterations": reports[-1][0], "relres_first": reports[-1][1], "relres_final": reports[-1][2],
+                   "converged": reports[-1][3]},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -28,26 +28,63 @@ constexpr int kPanelMaxW = 64;
 constexpr unsigned kSpinLimit = 1u << 22;
 
 // --------------------------------------------------------------------------
-// grid barrier: monotonic counter; the spin is bounded and a timeout sets
-// flags[0] so that every later barrier falls through (the kernel always exits)
+// Grid barrier for the cooperative panel: monotonic counter.  Only wave 0 of
+// a workgroup writes the data exchanged through global memory, so only wave 0
+// needs the release fence; lane 0 arrives, spins (bounded: a timeout sets
+// flags[0] and every later barrier falls through, so the kernel always
+// exits) and acquires.  The caller continues in wave 0 and synchronises the
+// workgroup afterwards.
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *flags) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(bar, 1u);
+__device__ __forceinline__ void grid_arrive_wait_wave0(unsigned *bar, unsigned target, int *flags) {
+    // The exchanged data is written and read with agent-scope atomic
+    // accesses (coherent across the XCDs' L2s), so the arrival only has to
+    // wait for those stores to complete; no L2 write-back is needed.
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if ((threadIdx.x & 63) == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
             if (++spins > kSpinLimit) {
-                atomicExch(&flags[0], 1);
+                __hip_atomic_store(&flags[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(1);
         }
     }
-    __syncthreads();
-    __threadfence();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// coherent (agent-scope, relaxed) element stores / loads
+__device__ __forceinline__ void st_co(float *p, float v) {
+    __hip_atomic_store((unsigned *)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_co(double *p, double v) {
+    __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_co(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_co(const float *p) {
+    return __uint_as_float(__hip_atomic_load((unsigned *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ double ld_co(const double *p) {
+    return __longlong_as_double(
+        (long long)__hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ int ld_co(const int *p) {
+    return __hip_atomic_load((int *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename R>
+__device__ __forceinline__ void st_co(cx<R> *p, cx<R> v) {
+    st_co(&p->re, v.re);
+    st_co(&p->im, v.im);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> ld_co(const cx<R> *p) {
+    return {ld_co(&p->re), ld_co(&p->im)};
 }
 
 template <typename R>
@@ -66,39 +103,22 @@ __device__ __forceinline__ bool better(R v1, int r1, R v2, int r2) {
 }
 
 template <typename R>
-__device__ void block_argmax(R &v, int &r, R *sv, int *sr) {
-    // wave reduction
+__device__ __forceinline__ void wave_argmax(R &v, int &r) {
     for (int off = 32; off > 0; off >>= 1) {
-        R ov = __shfl_down(v, off, 64);
-        int orr = __shfl_down(r, off, 64);
+        R ov = __shfl_xor(v, off, 64);
+        int orr = __shfl_xor(r, off, 64);
         if (better(ov, orr, v, r)) {
             v = ov;
             r = orr;
         }
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        sv[wave] = v;
-        sr[wave] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-            if (better(sv[w], sr[w], v, r)) {
-                v = sv[w];
-                r = sr[w];
-            }
-        sv[0] = v;
-        sr[0] = r;
-    }
-    __syncthreads();
-    v = sv[0];
-    r = sr[0];
-    __syncthreads();
 }
 
 // Panel: rows [j0, j0+M) x columns [j0, j0+w) of A (row-major, lda complex).
-// Workgroup g owns rows [j0 + g*RB, j0 + min(M, (g+1)*RB)).
+// Workgroup g owns rows [j0 + g*RB, j0 + min(M, (g+1)*RB)), kept in LDS with a
+// padded row stride.  Per column: local i?amax -> wave 0 publishes (value,
+// row, row data) -> grid barrier -> wave 0 of every workgroup picks the same
+// winner, applies the interchange to its LDS rows -> scale + rank-1 update.
 template <typename R>
 __global__ __launch_bounds__(256) void k_panel(cx<R> *__restrict__ A, long long lda, int j0, int M, int w, int RB,
                                                int *__restrict__ ipiv, cx<R> *__restrict__ slot_rows,
@@ -106,81 +126,95 @@ __global__ __launch_bounds__(256) void k_panel(cx<R> *__restrict__ A, long long 
                                                cx<R> *__restrict__ rowc, unsigned *bar, unsigned bar_base,
                                                int *flags) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<R> *rows = (cx<R> *)smem_raw;  // [RB][w]
-    __shared__ cx<R> urow[kPanelMaxW], orow[kPanelMaxW];
+    cx<R> *rows = (cx<R> *)smem_raw;  // [RB][ws]
+    const int ws = w + 1;             // padded stride
+    __shared__ cx<R> urow[kPanelMaxW];
     __shared__ R sv[4];
     __shared__ int sr[4];
-    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+    __shared__ int s_piv;
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rbeg = j0 + g * RB;
     const int rend = min(j0 + M, rbeg + RB);
     const int nr = max(0, rend - rbeg);
     for (int i = tid; i < nr * w; i += 256) {
         int rr = i / w, cc = i % w;
-        rows[i] = A[(size_t)(rbeg + rr) * lda + j0 + cc];
+        rows[rr * ws + cc] = A[(size_t)(rbeg + rr) * lda + j0 + cc];
     }
     __syncthreads();
     for (int c = 0; c < w; ++c) {
         const int jc = j0 + c;
         const int par = c & 1;
-        // local candidate
+        // local candidate (all waves), combined in wave 0
         R bv = R(-1);
         int br = -1;
         for (int i = tid; i < nr; i += 256) {
             int r = rbeg + i;
             if (r < jc) continue;
-            R v = cabs1(rows[i * w + c]);
+            R v = cabs1(rows[i * ws + c]);
             if (better(v, r, bv, br)) {
                 bv = v;
                 br = r;
             }
         }
-        block_argmax(bv, br, sv, sr);
-        if (tid == 0) {
-            slot_val[par * G + g] = bv;
-            slot_idx[par * G + g] = br;
+        wave_argmax(bv, br);
+        if (lane == 0) {
+            sv[wave] = bv;
+            sr[wave] = br;
         }
-        if (br >= 0 && tid < w) slot_rows[((size_t)par * G + g) * kPanelMaxW + tid] = rows[(br - rbeg) * w + tid];
-        if (jc >= rbeg && jc < rend && tid < w) rowc[par * kPanelMaxW + tid] = rows[(jc - rbeg) * w + tid];
-        grid_barrier(bar, bar_base + (unsigned)(c + 1) * G, flags);
-        // pick the winner (every workgroup computes the same answer)
-        R wv = R(-1);
-        int wr = -1, wg = -1;
-        for (int q = tid; q < G; q += 256) {
-            R v = slot_val[par * G + q];
-            int r = slot_idx[par * G + q];
-            if (better(v, r, wv, wr)) {
-                wv = v;
-                wr = r;
-                wg = q;
+        __syncthreads();
+        if (wave == 0) {
+            bv = sv[0];
+            br = sr[0];
+            for (int q = 1; q < 4; ++q)
+                if (better(sv[q], sr[q], bv, br)) {
+                    bv = sv[q];
+                    br = sr[q];
+                }
+            if (lane == 0) {
+                st_co(&slot_val[par * G + g], bv);
+                st_co(&slot_idx[par * G + g], br);
+            }
+            if (br >= 0 && lane < w) st_co(&slot_rows[((size_t)par * G + g) * kPanelMaxW + lane], rows[(br - rbeg) * ws + lane]);
+            if (jc >= rbeg && jc < rend && lane < w) st_co(&rowc[par * kPanelMaxW + lane], rows[(jc - rbeg) * ws + lane]);
+            grid_arrive_wait_wave0(bar, bar_base + (unsigned)(c + 1) * G, flags);
+            // winner: every workgroup computes the same (value, row)
+            R wv = R(-1);
+            int wr = -1;
+            for (int q = lane; q < G; q += 64) {
+                R v = ld_co(&slot_val[par * G + q]);
+                int r = ld_co(&slot_idx[par * G + q]);
+                if (better(v, r, wv, wr)) {
+                    wv = v;
+                    wr = r;
+                }
+            }
+            wave_argmax(wv, wr);
+            const bool singular = !(wv > R(0));
+            const int p = singular ? jc : wr;
+            const int wg = (p - j0) / RB;
+            if (lane < w) {
+                cx<R> o = ld_co(&rowc[par * kPanelMaxW + lane]);
+                cx<R> u = singular ? o : ld_co(&slot_rows[((size_t)par * G + wg) * kPanelMaxW + lane]);
+                urow[lane] = u;
+                if (jc >= rbeg && jc < rend) rows[(jc - rbeg) * ws + lane] = u;
+                if (p != jc && p >= rbeg && p < rend) rows[(p - rbeg) * ws + lane] = o;
+            }
+            if (lane == 0) {
+                s_piv = p;
+                if (g == 0) {
+                    ipiv[jc] = p;
+                    if (singular && flags[1] == 0) flags[1] = jc + 1;
+                }
             }
         }
-        // reduce (value,row) then recover the group from the row
-        block_argmax(wv, wr, sv, sr);
-        const bool singular = !(wv > R(0));
-        const int p = singular ? jc : wr;
-        wg = (p - j0) / RB;
-        if (tid < w) {
-            orow[tid] = rowc[par * kPanelMaxW + tid];
-            urow[tid] = singular ? orow[tid] : slot_rows[((size_t)par * G + wg) * kPanelMaxW + tid];
-        }
         __syncthreads();
-        if (jc >= rbeg && jc < rend && tid < w) rows[(jc - rbeg) * w + tid] = urow[tid];
-        if (p != jc && p >= rbeg && p < rend && tid < w) rows[(p - rbeg) * w + tid] = orow[tid];
-        if (g == 0 && tid == 0) {
-            ipiv[jc] = p;
-            if (singular && flags[1] == 0) flags[1] = jc + 1;
-        }
-        __syncthreads();
-        // scale the column and update the remaining panel columns
+        // scale the column (one thread per row), then rank-1 update of the rest
         const cx<R> piv = urow[c];
-        R den = piv.re * piv.re + piv.im * piv.im;
-        cx<R> inv = {R(0), R(0)};
-        if (den > R(0)) inv = {piv.re / den, -piv.im / den};
-        const int first = max(jc + 1, rbeg) - rbeg;  // first local row to update
-        for (int i = first + tid; i < nr; i += 256) {
-            cx<R> l = rows[i * w + c];
-            if (den > R(0)) l = cmul(l, inv);
-            rows[i * w + c] = l;
+        const R den = piv.re * piv.re + piv.im * piv.im;
+        const int first = max(jc + 1, rbeg) - rbeg;
+        if (den > R(0)) {
+            const cx<R> inv = {piv.re / den, -piv.im / den};
+            for (int i = first + tid; i < nr; i += 256) rows[i * ws + c] = cmul(rows[i * ws + c], inv);
         }
         __syncthreads();
         const int ncol = w - c - 1;
@@ -188,9 +222,9 @@ __global__ __launch_bounds__(256) void k_panel(cx<R> *__restrict__ A, long long 
             const int nupd = (nr - first) * ncol;
             for (int e = tid; e < nupd; e += 256) {
                 int i = first + e / ncol, cc = c + 1 + e % ncol;
-                cx<R> l = rows[i * w + c];
+                cx<R> l = rows[i * ws + c];
                 cx<R> u = urow[cc];
-                cx<R> &a = rows[i * w + cc];
+                cx<R> &a = rows[i * ws + cc];
                 a.re -= l.re * u.re - l.im * u.im;
                 a.im -= l.re * u.im + l.im * u.re;
             }
@@ -199,7 +233,7 @@ __global__ __launch_bounds__(256) void k_panel(cx<R> *__restrict__ A, long long 
     }
     for (int i = tid; i < nr * w; i += 256) {
         int rr = i / w, cc = i % w;
-        A[(size_t)(rbeg + rr) * lda + j0 + cc] = rows[i];
+        A[(size_t)(rbeg + rr) * lda + j0 + cc] = rows[rr * ws + cc];
     }
 }
 
@@ -226,82 +260,211 @@ __global__ __launch_bounds__(256) void k_laswp(cx<R> *__restrict__ A, long long 
     }
 }
 
-// Triangular solve on a block of wb rows starting at row r0 of X (columns
-// [c0,c1)), with the wb x wb diagonal block of the factor at (r0, r0):
+// Triangular solve on a block of wb <= 64 rows starting at row r0 of X
+// (columns [c0,c1)), with the wb x wb diagonal block of the factor at (r0, r0):
 //   upper == 0:  X <- L^{-1} X, L unit lower
 //   upper == 1:  X <- U^{-1} X, U upper (non-unit)
-// Each workgroup solves 16 columns in LDS.  If Bt is given, the solved block
-// is also written in real-expanded transposed form (see gemm.h) for the
-// following update: Bt is (2*(c1-c0)) x (2*wb), row stride 2*wb.
-constexpr int kTrsmCols = 16;
+// The diagonal block is staged transposed in LDS; each wavefront solves whole
+// columns with one lane per row (the pivot value is broadcast by a shuffle),
+// so no workgroup barrier sits inside the recurrence.  Each workgroup covers
+// 32 columns.  If Bt is given, the solved rows are also written in
+// real-expanded transposed form (see gemm.h) at K offset kofs:
+//   Bt[2j][2(kofs+r)] = Re, [2j][..+1] = -Im, [2j+1][..] = Im, [2j+1][..+1] = Re.
+// Each wavefront keeps CPW columns in flight so that the dependent
+// broadcast/FMA chains of independent columns interleave.
+constexpr int kTrsmCPW = 2;
+constexpr int kTrsmColsPerPass = 4 * kTrsmCPW;
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_trsm(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
-                                              long long ldx, int r0, int wb, int c0, int c1, int upper,
-                                              R *__restrict__ Bt) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<R> *xs = (cx<R> *)smem_raw;  // [wb][16]
-    const int tid = threadIdx.x;
-    const int cb = c0 + blockIdx.x * kTrsmCols;
-    const int nc = min(kTrsmCols, c1 - cb);
-    for (int e = tid; e < wb * kTrsmCols; e += 256) {
-        int r = e / kTrsmCols, cc = e % kTrsmCols;
-        xs[e] = cc < nc ? X[(size_t)(r0 + r) * ldx + cb + cc] : cx<R>{R(0), R(0)};
+__device__ __forceinline__ R bcast(R v, int src);
+template <>
+__device__ __forceinline__ float bcast<float>(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+template <>
+__device__ __forceinline__ double bcast<double>(double v, int src) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// cols_per_wg = kTrsmColsPerPass * passes
+template <typename R>
+__global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
+                                                long long ldx, int r0, int wb, int c0, int c1, int upper,
+                                                R *__restrict__ Bt, long long ldbt, int kofs, int passes) {
+    __shared__ cx<R> Lt[64][65];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < wb * wb; e += 256) {
+        int r = e / wb, i = e % wb;
+        Lt[i][r] = F[(size_t)(r0 + r) * ldf + r0 + i];
     }
     __syncthreads();
-    if (!upper) {
-        for (int i = 0; i < wb - 1; ++i) {
-            const int nupd = (wb - 1 - i) * kTrsmCols;
-            for (int e = tid; e < nupd; e += 256) {
-                int r = i + 1 + e / kTrsmCols, cc = e % kTrsmCols;
-                cx<R> l = F[(size_t)(r0 + r) * ldf + r0 + i];
-                cx<R> xi = xs[i * kTrsmCols + cc];
-                cx<R> &a = xs[r * kTrsmCols + cc];
-                a.re -= l.re * xi.re - l.im * xi.im;
-                a.im -= l.re * xi.im + l.im * xi.re;
-            }
-            __syncthreads();
+    const int cols_per_wg = kTrsmColsPerPass * passes;
+    const int cb = c0 + blockIdx.x * cols_per_wg;
+    for (int ps = 0; ps < passes; ++ps) {
+        int col[kTrsmCPW];
+        bool ok[kTrsmCPW];
+        cx<R> x[kTrsmCPW];
+#pragma unroll
+        for (int q = 0; q < kTrsmCPW; ++q) {
+            col[q] = cb + ps * kTrsmColsPerPass + wave * kTrsmCPW + q;
+            ok[q] = col[q] < c1;
+            x[q] = (ok[q] && lane < wb) ? X[(size_t)(r0 + lane) * ldx + col[q]] : cx<R>{R(0), R(0)};
         }
-    } else {
-        for (int i = wb - 1; i >= 0; --i) {
-            if (tid < kTrsmCols) {
-                cx<R> d = F[(size_t)(r0 + i) * ldf + r0 + i];
+        if (!upper) {
+            for (int i = 0; i < wb - 1; ++i) {
+                cx<R> l = Lt[i][lane];
+                const bool act = lane > i && lane < wb;
+#pragma unroll
+                for (int q = 0; q < kTrsmCPW; ++q) {
+                    R xr = bcast(x[q].re, i), xi = bcast(x[q].im, i);
+                    if (act) {
+                        x[q].re -= l.re * xr - l.im * xi;
+                        x[q].im -= l.re * xi + l.im * xr;
+                    }
+                }
+            }
+        } else {
+            for (int i = wb - 1; i >= 0; --i) {
+                cx<R> d = Lt[i][i];
                 R den = d.re * d.re + d.im * d.im;
                 cx<R> inv = den > R(0) ? cx<R>{d.re / den, -d.im / den} : cx<R>{R(0), R(0)};
-                xs[i * kTrsmCols + tid] = cmul(xs[i * kTrsmCols + tid], inv);
+                cx<R> u = Lt[i][lane];
+#pragma unroll
+                for (int q = 0; q < kTrsmCPW; ++q) {
+                    if (lane == i) x[q] = cmul(x[q], inv);
+                    R xr = bcast(x[q].re, i), xi = bcast(x[q].im, i);
+                    if (lane < i) {
+                        x[q].re -= u.re * xr - u.im * xi;
+                        x[q].im -= u.re * xi + u.im * xr;
+                    }
+                }
             }
-            __syncthreads();
-            const int nupd = i * kTrsmCols;
-            for (int e = tid; e < nupd; e += 256) {
-                int r = e / kTrsmCols, cc = e % kTrsmCols;
-                cx<R> u = F[(size_t)(r0 + r) * ldf + r0 + i];
-                cx<R> xi = xs[i * kTrsmCols + cc];
-                cx<R> &a = xs[r * kTrsmCols + cc];
-                a.re -= u.re * xi.re - u.im * xi.im;
-                a.im -= u.re * xi.im + u.im * xi.re;
+        }
+#pragma unroll
+        for (int q = 0; q < kTrsmCPW; ++q) {
+            if (!ok[q] || lane >= wb) continue;
+            X[(size_t)(r0 + lane) * ldx + col[q]] = x[q];
+            if (Bt) {
+                long long j = (long long)(col[q] - c0);
+                R *b0 = Bt + (2 * j) * ldbt + 2 * (kofs + lane);
+                R *b1 = Bt + (2 * j + 1) * ldbt + 2 * (kofs + lane);
+                b0[0] = x[q].re;
+                b0[1] = -x[q].im;
+                b1[0] = x[q].im;
+                b1[1] = x[q].re;
             }
-            __syncthreads();
         }
     }
-    for (int e = tid; e < wb * kTrsmCols; e += 256) {
-        int r = e / kTrsmCols, cc = e % kTrsmCols;
-        if (cc < nc) X[(size_t)(r0 + r) * ldx + cb + cc] = xs[e];
+}
+
+// Row interchanges as a permutation gather.  k_perm_build replays the
+// interchanges k0..k1-1 (ipiv, LAPACK order) on the index set they touch with
+// one wavefront and emits (destination row, source row) pairs; k_perm_gather
+// copies the source rows of a column range to a staging buffer and
+// k_perm_scatter writes them to their destinations.  Every element moves
+// once, with no dependent chain of memory accesses.
+constexpr int kPermChunkMax = 256;
+
+__global__ __launch_bounds__(64) void k_perm_build(const int *__restrict__ ipiv, int k0, int k1, int *__restrict__ dst,
+                                                   int *__restrict__ src, int *__restrict__ count) {
+    __shared__ int in_src[512];     // origin of row k0 + i, i < k1 - k0 (<= 256)
+    __shared__ int out_row[256], out_src[256];
+    __shared__ int n_out;
+    __shared__ int piv[kPermChunkMax];
+    const int lane = threadIdx.x;
+    const int nb = k1 - k0;
+    for (int i = lane; i < nb; i += 64) {
+        in_src[i] = k0 + i;
+        piv[i] = ipiv[k0 + i];
     }
-    if (Bt) {
-        const long long ldb = 2LL * wb;
-        for (int e = tid; e < wb * kTrsmCols; e += 256) {
-            int r = e % wb, cc = e / wb;  // r fastest -> contiguous Bt writes
-            if (cc >= nc) continue;
-            cx<R> v = xs[r * kTrsmCols + cc];
-            long long j = (long long)(cb - c0 + cc);
-            R *b0 = Bt + (2 * j) * ldb + 2 * r;
-            R *b1 = Bt + (2 * j + 1) * ldb + 2 * r;
-            b0[0] = v.re;
-            b0[1] = -v.im;
-            b1[0] = v.im;
-            b1[1] = v.re;
+    if (lane == 0) n_out = 0;
+    __syncthreads();
+    for (int k = k0; k < k1; ++k) {
+        const int p = piv[k - k0];
+        if (p == k) continue;
+        if (p < k1) {
+            if (lane == 0) {
+                int t = in_src[k - k0];
+                in_src[k - k0] = in_src[p - k0];
+                in_src[p - k0] = t;
+            }
+        } else {
+            // find p among the rows outside the block touched so far
+            int found = -1;
+            for (int j = lane; j < n_out; j += 64)
+                if (out_row[j] == p) found = j;
+            unsigned long long m = __ballot(found >= 0);
+            int idx;
+            if (m) {
+                int srcl = __ffsll((long long)m) - 1;
+                idx = __shfl(found, srcl, 64);
+            } else {
+                idx = n_out;
+            }
+            if (lane == 0) {
+                if (!m) {
+                    out_row[idx] = p;
+                    out_src[idx] = p;
+                    n_out = idx + 1;
+                }
+                int t = in_src[k - k0];
+                in_src[k - k0] = out_src[idx];
+                out_src[idx] = t;
+            }
         }
+        __syncthreads();
     }
+    // emit non-identity moves
+    __shared__ int n_emit;
+    if (lane == 0) n_emit = 0;
+    __syncthreads();
+    if (lane == 0) {
+        int c = 0;
+        for (int i = 0; i < nb; ++i)
+            if (in_src[i] != k0 + i) {
+                dst[c] = k0 + i;
+                src[c] = in_src[i];
+                ++c;
+            }
+        for (int j = 0; j < n_out; ++j)
+            if (out_src[j] != out_row[j]) {
+                dst[c] = out_row[j];
+                src[c] = out_src[j];
+                ++c;
+            }
+        n_emit = c;
+        *count = c;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_perm_gather(const cx<R> *__restrict__ A, long long lda,
+                                                     const int *__restrict__ src, const int *__restrict__ count,
+                                                     int c0, int c1, int c2, int c3, cx<R> *__restrict__ stage) {
+    const int i = blockIdx.y;
+    if (i >= *count) return;
+    const int n1 = c1 - c0, ncols = n1 + (c3 - c2);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ncols) return;
+    const int col = t < n1 ? c0 + t : c2 + (t - n1);
+    stage[(size_t)i * ncols + t] = A[(size_t)src[i] * lda + col];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_perm_scatter(cx<R> *__restrict__ A, long long lda,
+                                                      const int *__restrict__ dst, const int *__restrict__ count,
+                                                      int c0, int c1, int c2, int c3,
+                                                      const cx<R> *__restrict__ stage) {
+    const int i = blockIdx.y;
+    if (i >= *count) return;
+    const int n1 = c1 - c0, ncols = n1 + (c3 - c2);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ncols) return;
+    const int col = t < n1 ? c0 + t : c2 + (t - n1);
+    A[(size_t)dst[i] * lda + col] = stage[(size_t)i * ncols + t];
 }
 
 // Real-expanded transposed copy of a whole complex matrix X (n x m) into
@@ -333,9 +496,11 @@ struct LuPlan {
     int w = 64;    // panel width
 };
 
+// C (complex, M x ncols, ldc) -= Aop (M x K, lda) * B, with B given as its
+// real-expanded transposed form Bt (ldb reals per row).
 template <typename R>
-static emfa_status gemm_update(emfa_ctx *ctx, int M, int ncols, int K, const cx<R> *Aop, long long lda,
-                               const R *Bt, cx<R> *C, long long ldc) {
+static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols, int K, const cx<R> *Aop,
+                               long long lda, const R *Bt, long long ldb, cx<R> *C, long long ldc) {
     if (M <= 0 || ncols <= 0 || K <= 0) return EMFA_OK;
     GemmArgs g;
     g.M = M;
@@ -344,39 +509,102 @@ static emfa_status gemm_update(emfa_ctx *ctx, int M, int ncols, int K, const cx<
     g.A = Aop;
     g.lda = 2 * lda;
     g.Bt = Bt;
-    g.ldb = 2LL * K;
+    g.ldb = ldb;
     g.C = C;
     g.ldc = 2 * ldc;
     g.alpha_neg = 1;
     g.beta = 1;
-    ProfScope ps(ctx, "lu_gemm", 8.0 * (double)M * ncols * K);
+    ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K);
     if constexpr (sizeof(R) == 4)
         return gemm_launch_f32(ctx->stream, g);
     else
         return gemm_launch_f64(ctx->stream, g);
 }
 
+// Blocked triangular solve of rows [r0, r0+wb) of X (columns [c0,c1)) with the
+// diagonal block of F: 64-row diagonal solves (k_trsm64) + MFMA updates of
+// the remaining rows of the block.  Writes the expanded operand of the whole
+// block into Bt (2*(c1-c0) x 2*wb reals, row stride 2*wb).
 template <typename R>
-static emfa_status trsm_launch(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx<R> *X, long long ldx, int r0, int wb,
-                               int c0, int c1, int upper, R *Bt, const char *fam) {
+static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx<R> *X, long long ldx, int r0, int wb,
+                                int c0, int c1, int upper, R *Bt, const char *fam) {
     if (c1 <= c0 || wb <= 0) return EMFA_OK;
-    int blocks = (c1 - c0 + kTrsmCols - 1) / kTrsmCols;
-    size_t lds = (size_t)wb * kTrsmCols * sizeof(cx<R>);
-    ProfScope ps(ctx, fam, 0);
-    k_trsm<R><<<blocks, 256, lds, ctx->stream>>>(F, ldf, X, ldx, r0, wb, c0, c1, upper, Bt);
-    EMFA_HIP_CHECK(hipGetLastError());
+    const long long ldbt = 2LL * wb;
+    const int ncols = c1 - c0;
+    // enough workgroups to fill the chip; more columns per workgroup when wide
+    int passes = std::max(1, std::min(8, ncols / (kTrsmColsPerPass * 1024)));
+    const int cpw = kTrsmColsPerPass * passes;
+    const int blocks = (ncols + cpw - 1) / cpw;
+    if (!upper) {
+        for (int s = 0; s < wb; s += 64) {
+            int b = std::min(64, wb - s);
+            {
+                ProfScope ps(ctx, fam, 0);
+                k_trsm64<R><<<blocks, 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0, c1, 0, Bt, ldbt, s, passes);
+                EMFA_HIP_CHECK(hipGetLastError());
+            }
+            if (s + b < wb) {
+                emfa_status st = gemm_update<R>(ctx, "trsm_gemm", wb - s - b, ncols, b,
+                                                F + (size_t)(r0 + s + b) * ldf + r0 + s, ldf, Bt + 2 * s, ldbt,
+                                                X + (size_t)(r0 + s + b) * ldx + c0, ldx);
+                if (st) return st;
+            }
+        }
+    } else {
+        int last = ((wb - 1) / 64) * 64;
+        for (int s = last; s >= 0; s -= 64) {
+            int b = std::min(64, wb - s);
+            {
+                ProfScope ps(ctx, fam, 0);
+                k_trsm64<R><<<blocks, 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0, c1, 1, Bt, ldbt, s, passes);
+                EMFA_HIP_CHECK(hipGetLastError());
+            }
+            if (s > 0) {
+                emfa_status st = gemm_update<R>(ctx, "trsm_gemm", s, ncols, b, F + (size_t)r0 * ldf + r0 + s, ldf,
+                                                Bt + 2 * s, ldbt, X + (size_t)r0 * ldx + c0, ldx);
+                if (st) return st;
+            }
+        }
+    }
     return EMFA_OK;
 }
 
+// Workspace of the row interchanges: pair lists and a staging buffer of
+// 2*kPermChunk rows x max_cols.
+constexpr int kPermChunk = 256;
+
+template <typename R>
+struct PermWs {
+    int *dst = nullptr, *src = nullptr, *count = nullptr;
+    cx<R> *stage = nullptr;
+    // layout: dst[512] ints | src[512] ints | count | ... | stage (from byte 8192)
+    static size_t bytes(int max_cols) { return 8192 + (size_t)2 * kPermChunk * max_cols * sizeof(cx<R>); }
+    void carve(void *p) {
+        char *q = (char *)p;
+        dst = (int *)q;
+        src = (int *)(q + 2048);
+        count = (int *)(q + 4096);
+        stage = (cx<R> *)(q + 8192);
+    }
+};
+
+// Apply the interchanges k0..k1-1 to columns [c0,c1) and [c2,c3).
 template <typename R>
 static emfa_status laswp_launch(emfa_ctx *ctx, cx<R> *A, long long lda, const int *ipiv, int k0, int k1, int c0,
-                                int c1, int c2, int c3) {
-    int ncols = std::max(0, c1 - c0) + std::max(0, c3 - c2);
+                                int c1, int c2, int c3, PermWs<R> &ws) {
+    c1 = std::max(c0, c1);
+    c3 = std::max(c2, c3);
+    const int ncols = (c1 - c0) + (c3 - c2);
     if (ncols <= 0 || k1 <= k0) return EMFA_OK;
     ProfScope ps(ctx, "lu_laswp", 0);
-    k_laswp<R><<<(ncols + 255) / 256, 256, 0, ctx->stream>>>(A, lda, ipiv, k0, k1, c0, std::max(c0, c1), c2,
-                                                              std::max(c2, c3));
-    EMFA_HIP_CHECK(hipGetLastError());
+    for (int kb = k0; kb < k1; kb += kPermChunk) {
+        int ke = std::min(k1, kb + kPermChunk);
+        k_perm_build<<<1, 64, 0, ctx->stream>>>(ipiv, kb, ke, ws.dst, ws.src, ws.count);
+        dim3 grid((ncols + 255) / 256, 2 * (ke - kb));
+        k_perm_gather<R><<<grid, 256, 0, ctx->stream>>>(A, lda, ws.src, ws.count, c0, c1, c2, c3, ws.stage);
+        k_perm_scatter<R><<<grid, 256, 0, ctx->stream>>>(A, lda, ws.dst, ws.count, c0, c1, c2, c3, ws.stage);
+        EMFA_HIP_CHECK(hipGetLastError());
+    }
     return EMFA_OK;
 }
 
@@ -391,7 +619,10 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
 
     // panel geometry: co-resident workgroups, rows per workgroup in LDS
     const size_t lds_cap = 150 * 1024;
-    const int G_max = std::max(1, ctx->num_cu);
+    int G_max = std::max(1, ctx->num_cu);
+    if (const char *s = getenv("EMFA_PANEL_G")) G_max = std::max(1, std::min(G_max, atoi(s)));
+    int panel_rows_min = 16;
+    if (const char *s = getenv("EMFA_PANEL_ROWS_MIN")) panel_rows_min = std::max(1, atoi(s));
     // scratch: slots + Bt workspace
     size_t slot_rows_b = 2ull * G_max * kPanelMaxW * sizeof(cx<R>);
     size_t slot_val_b = 2ull * G_max * sizeof(R);
@@ -399,7 +630,8 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     size_t rowc_b = 2ull * kPanelMaxW * sizeof(cx<R>);
     size_t bar_b = 256;
     size_t bt_b = (size_t)2 * N * 2 * nb * sizeof(R);
-    size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + bt_b + 1024;
+    size_t perm_b = PermWs<R>::bytes(N);
+    size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + bt_b + perm_b + 4096;
     void *scr = nullptr;
     emfa_status st = ctx_scratch(ctx, total, &scr);
     if (st != EMFA_OK) return st;
@@ -415,6 +647,8 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     cx<R> *rowc = (cx<R> *)take(rowc_b);
     unsigned *bar = (unsigned *)take(bar_b);
     R *Bt = (R *)take(bt_b);
+    PermWs<R> pws;
+    pws.carve(take(perm_b));
     k_zero_u32<<<1, 64, 0, ctx->stream>>>(bar, 64);
     EMFA_HIP_CHECK(hipGetLastError());
     k_zero_u32<<<1, 64, 0, ctx->stream>>>((unsigned *)ctx->flags, 16);
@@ -428,12 +662,12 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             const int M = N - js;
             // panel width and workgroup count that fit LDS
             int w = std::min(plan.w, j0 + jb - js);
-            int G = std::min(G_max, std::max(1, (M + 15) / 16));
+            int G = std::min(G_max, std::max(1, (M + panel_rows_min - 1) / panel_rows_min));
             int RB = (M + G - 1) / G;
-            while ((size_t)RB * w * sizeof(cx<R>) > lds_cap && w > 8) w /= 2;
-            EMFA_REQUIRE((size_t)RB * w * sizeof(cx<R>) <= lds_cap, EMFA_E_INVALID,
+            while ((size_t)RB * (w + 1) * sizeof(cx<R>) > lds_cap && w > 8) w /= 2;
+            EMFA_REQUIRE((size_t)RB * (w + 1) * sizeof(cx<R>) <= lds_cap, EMFA_E_INVALID,
                          "LU panel does not fit LDS (N=%d)", N);
-            size_t lds = (size_t)RB * w * sizeof(cx<R>);
+            size_t lds = (size_t)RB * (w + 1) * sizeof(cx<R>);
             {
                 ProfScope ps(ctx, "lu_panel", 0);
                 void *args[] = {(void *)&A,        (void *)&lda,      (void *)&js,   (void *)&M,
@@ -446,24 +680,24 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             }
             bar_base += (unsigned)w * G;
             // swaps on the rest of the outer panel
-            st = laswp_launch<R>(ctx, A, lda, ipiv, js, js + w, j0, js, js + w, j0 + jb);
+            st = laswp_launch<R>(ctx, A, lda, ipiv, js, js + w, j0, js, js + w, j0 + jb, pws);
             if (st) return st;
             if (js + w < j0 + jb) {
-                st = trsm_launch<R>(ctx, A, lda, A, lda, js, w, js + w, j0 + jb, 0, Bt, "lu_trsm");
+                st = trsm_blocked<R>(ctx, A, lda, A, lda, js, w, js + w, j0 + jb, 0, Bt, "lu_trsm");
                 if (st) return st;
-                st = gemm_update<R>(ctx, N - js - w, j0 + jb - js - w, w, A + (size_t)(js + w) * lda + js, lda, Bt,
-                                    A + (size_t)(js + w) * lda + js + w, lda);
+                st = gemm_update<R>(ctx, "lu_gemm", N - js - w, j0 + jb - js - w, w, A + (size_t)(js + w) * lda + js,
+                                    lda, Bt, 2LL * w, A + (size_t)(js + w) * lda + js + w, lda);
                 if (st) return st;
             }
             js += w;
         }
-        st = laswp_launch<R>(ctx, A, lda, ipiv, j0, j0 + jb, 0, j0, j0 + jb, N);
+        st = laswp_launch<R>(ctx, A, lda, ipiv, j0, j0 + jb, 0, j0, j0 + jb, N, pws);
         if (st) return st;
         if (j0 + jb < N) {
-            st = trsm_launch<R>(ctx, A, lda, A, lda, j0, jb, j0 + jb, N, 0, Bt, "lu_trsm");
+            st = trsm_blocked<R>(ctx, A, lda, A, lda, j0, jb, j0 + jb, N, 0, Bt, "lu_trsm");
             if (st) return st;
-            st = gemm_update<R>(ctx, N - j0 - jb, N - j0 - jb, jb, A + (size_t)(j0 + jb) * lda + j0, lda, Bt,
-                                A + (size_t)(j0 + jb) * lda + j0 + jb, lda);
+            st = gemm_update<R>(ctx, "lu_gemm", N - j0 - jb, N - j0 - jb, jb, A + (size_t)(j0 + jb) * lda + j0, lda,
+                                Bt, 2LL * jb, A + (size_t)(j0 + jb) * lda + j0 + jb, lda);
             if (st) return st;
         }
     }
@@ -492,57 +726,30 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
     const long long ldf = N, ldb = m;
     int nb = 256;
     if (const char *s = getenv("EMFA_SOLVE_NB")) nb = std::max(16, atoi(s));
-    size_t bt_b = (size_t)2 * m * 2 * nb * sizeof(R);
+    size_t bt_b = (((size_t)2 * m * 2 * nb * sizeof(R)) + 255) & ~(size_t)255;
     void *scr = nullptr;
-    emfa_status st = ctx_scratch(ctx, bt_b + 1024, &scr);
+    emfa_status st = ctx_scratch(ctx, bt_b + PermWs<R>::bytes(m) + 1024, &scr);
     if (st) return st;
     R *Bt = (R *)scr;
-    st = laswp_launch<R>(ctx, B, ldb, ipiv, 0, N, 0, m, 0, 0);
+    PermWs<R> pws;
+    pws.carve((char *)scr + bt_b);
+    st = laswp_launch<R>(ctx, B, ldb, ipiv, 0, N, 0, m, 0, 0, pws);
     if (st) return st;
     for (int r0 = 0; r0 < N; r0 += nb) {
         int wb = std::min(nb, N - r0);
-        st = trsm_launch<R>(ctx, F, ldf, B, ldb, r0, wb, 0, m, 0, Bt, "solve_trsm");
+        st = trsm_blocked<R>(ctx, F, ldf, B, ldb, r0, wb, 0, m, 0, Bt, "solve_trsm");
         if (st) return st;
-        if (r0 + wb < N) {
-            GemmArgs g;
-            g.M = N - r0 - wb;
-            g.Nr = 2 * m;
-            g.Kr = 2 * wb;
-            g.A = F + (size_t)(r0 + wb) * ldf + r0;
-            g.lda = 2 * ldf;
-            g.Bt = Bt;
-            g.ldb = 2LL * wb;
-            g.C = B + (size_t)(r0 + wb) * ldb;
-            g.ldc = 2 * ldb;
-            g.alpha_neg = 1;
-            g.beta = 1;
-            ProfScope ps(ctx, "solve_gemm", 8.0 * g.M * m * wb);
-            st = sizeof(R) == 4 ? gemm_launch_f32(ctx->stream, g) : gemm_launch_f64(ctx->stream, g);
-            if (st) return st;
-        }
+        st = gemm_update<R>(ctx, "solve_gemm", N - r0 - wb, m, wb, F + (size_t)(r0 + wb) * ldf + r0, ldf, Bt,
+                            2LL * wb, B + (size_t)(r0 + wb) * ldb, ldb);
+        if (st) return st;
     }
     int last = ((N - 1) / nb) * nb;
     for (int r0 = last; r0 >= 0; r0 -= nb) {
         int wb = std::min(nb, N - r0);
-        st = trsm_launch<R>(ctx, F, ldf, B, ldb, r0, wb, 0, m, 1, Bt, "solve_trsm");
+        st = trsm_blocked<R>(ctx, F, ldf, B, ldb, r0, wb, 0, m, 1, Bt, "solve_trsm");
         if (st) return st;
-        if (r0 > 0) {
-            GemmArgs g;
-            g.M = r0;
-            g.Nr = 2 * m;
-            g.Kr = 2 * wb;
-            g.A = F + r0;
-            g.lda = 2 * ldf;
-            g.Bt = Bt;
-            g.ldb = 2LL * wb;
-            g.C = B;
-            g.ldc = 2 * ldb;
-            g.alpha_neg = 1;
-            g.beta = 1;
-            ProfScope ps(ctx, "solve_gemm", 8.0 * g.M * m * wb);
-            st = sizeof(R) == 4 ? gemm_launch_f32(ctx->stream, g) : gemm_launch_f64(ctx->stream, g);
-            if (st) return st;
-        }
+        st = gemm_update<R>(ctx, "solve_gemm", r0, m, wb, F + r0, ldf, Bt, 2LL * wb, B, ldb);
+        if (st) return st;
     }
     return EMFA_OK;
 }
