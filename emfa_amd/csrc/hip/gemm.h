@@ -31,8 +31,8 @@ struct GemmArgs {
     int beta;               // 0 or 1
 };
 
-emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g);       // TA=float,  TC=float
-emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g);       // TA=double, TC=double
+emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=float,  TC=float
+emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=double, TC=double
 emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g);   // TA=float,  TC=double
 
 }  // namespace emfa

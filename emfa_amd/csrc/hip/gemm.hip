@@ -13,6 +13,7 @@ namespace f32k {
 constexpr int BM = 128, BN = 128, BK = 32, PAD = 4;
 }
 
+template <int TAG>
 __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
     using namespace f32k;
     __shared__ float As[2][BM][BK + PAD];
@@ -127,7 +128,7 @@ namespace f64k {
 constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
 }
 
-template <typename TA>
+template <typename TA, int TAG>
 __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
     using namespace f64k;
     __shared__ double As[2][BM][BK + PAD];
@@ -242,18 +243,25 @@ static bool gemm_args_ok(const GemmArgs &g) {
            g.Nr <= 65535LL * 128 && g.M <= 65535LL * 128;
 }
 
-emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g) {
+emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag) {
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f32k::BN - 1) / f32k::BN, (g.M + f32k::BM - 1) / f32k::BM);
-    k_gemm_f32<<<grid, 256, 0, s>>>(g);
+    // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart
+    if (tag == 1)
+        k_gemm_f32<1><<<grid, 256, 0, s>>>(g);
+    else
+        k_gemm_f32<0><<<grid, 256, 0, s>>>(g);
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
 }
 
-emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g) {
+emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g, int tag) {
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
-    k_gemm_f64<double><<<grid, 256, 0, s>>>(g);
+    if (tag == 1)
+        k_gemm_f64<double, 1><<<grid, 256, 0, s>>>(g);
+    else
+        k_gemm_f64<double, 0><<<grid, 256, 0, s>>>(g);
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
 }
@@ -261,7 +269,7 @@ emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g) {
 emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g) {
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32/f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
-    k_gemm_f64<float><<<grid, 256, 0, s>>>(g);
+    k_gemm_f64<float, 0><<<grid, 256, 0, s>>>(g);
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
 }

@@ -17,6 +17,7 @@
 // EMSolver/MFIE.cpp:64) up to rounding.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "gemm.h"
@@ -515,10 +516,11 @@ static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols,
     g.alpha_neg = 1;
     g.beta = 1;
     ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K);
+    const int tag = std::strcmp(fam, "lu_gemm") == 0 ? 1 : 0;
     if constexpr (sizeof(R) == 4)
-        return gemm_launch_f32(ctx->stream, g);
+        return gemm_launch_f32(ctx->stream, g, tag);
     else
-        return gemm_launch_f64(ctx->stream, g);
+        return gemm_launch_f64(ctx->stream, g, tag);
 }
 
 // Blocked triangular solve of rows [r0, r0+wb) of X (columns [c0,c1)) with the
