@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-sign-compare",
               "-I" + os.path.join(ROOT, "include"), "-I" + HOST_DIR]
 
 LIB_HIP = os.path.join(HERE, "libemfa_hip.so")

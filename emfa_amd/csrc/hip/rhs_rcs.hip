@@ -31,6 +31,8 @@ __device__ __forceinline__ void load_tri(const double *__restrict__ tg, int T, i
 // V(u, j) = 1/2 l_u sum_{+,-} sum_a w_a rho . [alpha e + (1-alpha) n x h] exp(-jk k.r)
 // EFIE (kind 0): 7-point rule, e only (EMSolver/EFIE.cpp:568-601,698-722).
 // CFIE (kind 1): 3-point rule (EMSolver/Miscellaneous.cpp:152-189).
+// MFIE (kind 2): 7-point rule, n x h part only (EMSolver/MFIE.cpp:289-322,
+// 435-436 without its 1/Z0, matching the Z0-scaled MFIE operator of the fill).
 // One thread per (u, j), j fastest so the row-major writes coalesce.
 template <typename R>
 __global__ __launch_bounds__(256) void k_rhs(const double *__restrict__ tg, const int *__restrict__ rtp,
@@ -52,7 +54,7 @@ __global__ __launch_bounds__(256) void k_rhs(const double *__restrict__ tg, cons
         double sg = side == 0 ? 1.0 : -1.0;
         double nxh[3] = {ta.n[1] * hv[2] - ta.n[2] * hv[1], ta.n[2] * hv[0] - ta.n[0] * hv[2],
                          ta.n[0] * hv[1] - ta.n[1] * hv[0]};
-        int Q = kind == EMFA_EFIE ? 7 : 3;
+        int Q = kind == EMFA_CFIE ? 3 : 7;
         for (int p = 0; p < Q; ++p) {
             const double *rw = Q == 7 ? c_r7[p] : c_r3[p];
             double r[3];
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(256) void k_rhs(const double *__restrict__ tg, cons
             double de = rho[0] * ev[0] + rho[1] * ev[1] + rho[2] * ev[2];
             vr += w * de * co;
             vi -= w * de * s;
-            if (kind == EMFA_CFIE) {
+            if (kind != EMFA_EFIE) {
                 double dm = rho[0] * nxh[0] + rho[1] * nxh[1] + rho[2] * nxh[2];
                 mr += w * dm * co;
                 mi -= w * dm * s;
@@ -77,6 +79,9 @@ __global__ __launch_bounds__(256) void k_rhs(const double *__restrict__ tg, cons
     if (kind == EMFA_EFIE) {
         outr = vr;
         outi = vi;
+    } else if (kind == EMFA_MFIE) {
+        outr = mr;
+        outi = mi;
     } else {
         outr = alpha * vr + (1.0 - alpha) * mr;
         outi = alpha * vi + (1.0 - alpha) * mi;
@@ -206,7 +211,8 @@ extern "C" emfa_status emfa_rhs_fill(emfa_ctx *ctx, const emfa_mesh *M, int kind
     EMFA_REQUIRE(ctx && M && inc && B, EMFA_E_INVALID, "emfa_rhs_fill: null argument");
     EMFA_REQUIRE(nrhs > 0 && B->rows == M->n && B->cols == nrhs, EMFA_E_INVALID,
                  "emfa_rhs_fill: B is %dx%d, expected %dx%d", B->rows, B->cols, M->n, nrhs);
-    EMFA_REQUIRE(kind == EMFA_EFIE || kind == EMFA_CFIE, EMFA_E_INVALID, "emfa_rhs_fill: bad kind");
+    EMFA_REQUIRE(kind == EMFA_EFIE || kind == EMFA_CFIE || kind == EMFA_MFIE, EMFA_E_INVALID,
+                 "emfa_rhs_fill: bad kind");
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     void *dinc = nullptr;
     EMFA_HIP_CHECK(ctx_alloc(ctx, &dinc, 9 * sizeof(double) * nrhs));

@@ -1,0 +1,464 @@
+#include "em.h"
+
+#include <cmath>
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+#include <mutex>
+
+namespace mom {
+
+static const double kPi = 3.14159265358979323846264338327950288;
+static const double kC0 = 299792458.0;
+
+// ------------------------------------------------------------------ EM
+EM::EM() {
+    logger_ << std::left;
+    time_logger_ << std::left;
+}
+
+EM::~EM() {
+    tool::totalTime(clock_, time_logger_);
+    std::string t = "Time Information\n" + time_logger_.str();
+    std::cout << t << std::flush;
+    if (logger_.is_open()) logger_ << t << std::flush;
+}
+
+void EM::timeLog(const std::string &what) { tool::elapsedTime(clock_, time_logger_, "Time of " + what); }
+
+// EMSolver/EM.cpp:24-61
+void EM::init(component::ConfigLoader *ploader) {
+    rtype_ = ploader->getResultType();
+    incidence_ = ploader->getInc();
+    const double d2r = kPi / 180.0;
+    double pol = incidence_.pole * d2r, theta = incidence_.theta * d2r, phi = incidence_.phi * d2r;
+    inc_k_ = {-std::sin(theta) * std::cos(phi), -std::sin(theta) * std::sin(phi), -std::cos(theta)};
+    inc_e_ = {-std::cos(pol) * std::cos(theta) * std::cos(phi) - std::sin(pol) * std::sin(phi),
+              -std::cos(pol) * std::cos(theta) * std::sin(phi) + std::sin(pol) * std::cos(phi),
+              std::cos(pol) * std::sin(theta)};
+    inc_h_ = {inc_k_[1] * inc_e_[2] - inc_k_[2] * inc_e_[1], inc_k_[2] * inc_e_[0] - inc_k_[0] * inc_e_[2],
+              inc_k_[0] * inc_e_[1] - inc_k_[1] * inc_e_[0]};
+    lamda = kC0 / incidence_.freq;
+    if (rtype_ == policy::Rad)
+        radiation_ = ploader->getRad();
+    else
+        scattering_ = ploader->getSca();
+    nearfield_ = ploader->getNearfield();
+    std::string mesh_file = ploader->getMeshPath();
+    auto pos_suffix = mesh_file.find_last_of('.');
+    if (pos_suffix == std::string::npos) throw component::ConfigInvalid("mesh path parameter cannot match");
+    auto pos1 = mesh_file.find_last_of('/');
+    auto pos2 = mesh_file.find_last_of('\\');
+    if (pos1 == std::string::npos && pos2 == std::string::npos) {
+        dir_ = ".";
+        folder_name_ = mesh_file.substr(0, pos_suffix);
+    } else {
+        auto pos_dir = (pos1 != std::string::npos && pos2 != std::string::npos) ? std::max(pos1, pos2)
+                                                                                : std::min(pos1, pos2);
+        dir_ = mesh_file.substr(0, pos_dir);
+        folder_name_ = mesh_file.substr(pos_dir + 1, pos_suffix - pos_dir - 1);
+    }
+}
+
+// ------------------------------------------------------------------ device
+emfa_ctx *device_context(int device) {
+    static std::mutex mu;
+    static emfa_ctx *ctx = nullptr;
+    static int dev = -1;
+    std::lock_guard<std::mutex> lock(mu);
+    if (ctx && dev == device) return ctx;
+    if (ctx) emfa_ctx_destroy(ctx);
+    ctx = nullptr;
+    emfa_status st = emfa_ctx_create(device, &ctx);
+    if (st != EMFA_OK) throw std::runtime_error(std::string("device: ") + emfa_last_error());
+    dev = device;
+    return ctx;
+}
+
+// ------------------------------------------------------------------ DenseMoM
+DenseMoM::~DenseMoM() { clear(); }
+
+void DenseMoM::check(emfa_status st, const char *what) const {
+    if (st == EMFA_OK) return;
+    std::string msg = std::string(what) + ": " + emfa_last_error();
+    if (st == EMFA_E_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+void DenseMoM::init(component::ConfigLoader *ploader) {
+    EM::init(ploader);
+    if (rtype_ == policy::Rad)
+        throw component::ConfigInvalid("result_type Radiation (delta-gap .rad excitation) is not part of this build");
+    dir_ = tool::creatFolder(dir_, folder_name_ + folderTag());
+    logger_.open(dir_ + "/sca_runtime.log");
+    mesh_.loadMeshFile(ploader->getMeshPath());
+    ce_.buildCommonEdge(mesh_);
+    unknowns_ = ce_.getCommonEdgeNum();
+    if (unknowns_ <= 0) throw component::FileError("mesh has no interior edges (no RWG unknowns)");
+    k_ = 2.0 * kPi * incidence_.freq / kC0;
+    if (fillKind() == EMFA_EFIE && ploader->has("Sigma:")) sigma_ = ploader->getSigma();
+    std::string p = ploader->getString("precision:", "FP64");
+    if (p == "FP64")
+        prec_ = EMFA_F64;
+    else if (p == "FP32")
+        prec_ = EMFA_F32;
+    else
+        throw component::ConfigInvalid("precision must be FP64 or FP32");
+    quad_ = ploader->getInt("quadrature_order:", 3);
+    if (quad_ != 3 && quad_ != 7) throw component::ConfigInvalid("quadrature_order must be 3 or 7");
+    std::string ls = ploader->getString("linear_solver:", "LU");
+    if (ls == "LU")
+        use_gmres_ = false;
+    else if (ls == "GMRES")
+        use_gmres_ = true;
+    else
+        throw component::ConfigInvalid("linear_solver must be LU or GMRES");
+    gmres_m_ = ploader->getInt("gmres_restart:", 30);
+    gmres_tol_ = ploader->getDouble("gmres_tolerance:", 1e-6);
+    std::string cm = ploader->getString("cfie_self_term:", "reference");
+    if (cm == "reference")
+        cfie_mode_ = 0;
+    else if (cm == "accurate")
+        cfie_mode_ = 1;
+    else
+        throw component::ConfigInvalid("cfie_self_term must be reference or accurate");
+    write_matrix_ = ploader->getString("write_matrix:", "false") == "true";
+    device_ = ploader->getInt("device:", 0);
+
+    // right-hand sides: one incidence (bistatic) or one per direction (monostatic)
+    rhs_angles_.clear();
+    rcs_dirs_.clear();
+    if (rtype_ == policy::BiRCS) {
+        rhs_angles_.push_back({incidence_.theta, incidence_.phi, incidence_.pole});
+    } else {
+        // angle order of EMComponent/Result.h:107-155
+        const auto &sca = scattering_;
+        int phi_num = static_cast<int>((sca.phi_to - sca.phi_from) / sca.phi_delta) + 1;
+        for (int i = 0; i < phi_num; ++i) {
+            float phiDegree = sca.phi_from + i * sca.phi_delta;
+            for (float thetaDegree = sca.theta_from; thetaDegree <= sca.theta_to; thetaDegree += sca.theta_delta)
+                rhs_angles_.push_back({thetaDegree, phiDegree, incidence_.pole});
+        }
+        for (float phiDegree = 0; phiDegree < 360; phiDegree += 1) rhs_angles_.push_back({90.0, phiDegree, incidence_.pole});
+    }
+    logger_ << ploader->getReport();
+    reportInfo(logger_);
+    timeLog("init");
+}
+
+void DenseMoM::reportInfo(std::ostream &strm) const {
+    mesh_.reportInfo(strm);
+    ce_.reportInfo(strm);
+    strm << "Device path\n"
+         << "  -> precision: " << (prec_ == EMFA_F64 ? "FP64" : "FP32 + FP64 refinement") << '\n'
+         << "  -> quadrature: " << quad_ << "x" << quad_ << '\n'
+         << "  -> linear solver: " << (use_gmres_ ? "GMRES" : "LU") << '\n'
+         << "  -> right-hand sides: " << rhs_angles_.size() << '\n';
+}
+
+void DenseMoM::solve() {
+    std::cout << "Solve Information\n";
+    ctx_ = device_context(device_);
+    const auto &xyz = mesh_.xyz();
+    const auto &tri = mesh_.tri();
+    check(emfa_mesh_upload(ctx_, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3),
+                           ce_.flat().data(), unknowns_, &dmesh_),
+          "mesh upload");
+    const int n = unknowns_;
+    ncols_ = (int)rhs_angles_.size();
+    emfa_mat *Z = nullptr, *B = nullptr;
+    emfa_lu *lu = nullptr;
+    auto release = [&]() {
+        if (lu) emfa_lu_destroy(lu);
+        if (Z) emfa_mat_destroy(Z);
+        if (B) emfa_mat_destroy(B);
+        lu = nullptr;
+        Z = B = nullptr;
+    };
+    try {
+        check(emfa_mat_create(ctx_, n, n, prec_, &Z), "impedance matrix");
+        emfa_fill_params fp{fillKind(), quad_, k_, alpha(), sigma_.real(), sigma_.imag(), cfie_mode_};
+        check(emfa_fill(ctx_, dmesh_, &fp, Z), "fill");
+        check(emfa_ctx_sync(ctx_), "fill");
+        timeLog("fillZ");
+
+        std::vector<double> inc(9 * (size_t)ncols_);
+        const double d2r = kPi / 180.0;
+        for (int j = 0; j < ncols_; ++j) {
+            double th = rhs_angles_[j][0] * d2r, ph = rhs_angles_[j][1] * d2r, po = rhs_angles_[j][2] * d2r;
+            double kv[3] = {-std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
+            double ev[3] = {-std::cos(po) * std::cos(th) * std::cos(ph) - std::sin(po) * std::sin(ph),
+                            -std::cos(po) * std::cos(th) * std::sin(ph) + std::sin(po) * std::cos(ph),
+                            std::cos(po) * std::sin(th)};
+            double hv[3] = {kv[1] * ev[2] - kv[2] * ev[1], kv[2] * ev[0] - kv[0] * ev[2],
+                            kv[0] * ev[1] - kv[1] * ev[0]};
+            for (int c = 0; c < 3; ++c) {
+                inc[9 * j + c] = kv[c];
+                inc[9 * j + 3 + c] = ev[c];
+                inc[9 * j + 6 + c] = hv[c];
+            }
+            rcs_dirs_.push_back({-kv[0], -kv[1], -kv[2]});
+        }
+        check(emfa_mat_create(ctx_, n, ncols_, EMFA_F64, &B), "excitation");
+        check(emfa_rhs_fill(ctx_, dmesh_, rhsKind(), k_, alpha(), ncols_, inc.data(), B), "fillV");
+        timeLog("fillV");
+
+        check(emfa_mat_create(ctx_, n, ncols_, EMFA_F64, &X_), "solution");
+        if (use_gmres_) {
+            emfa_gmres_report rep{};
+            emfa_status st = emfa_gmres(ctx_, Z, B, X_, gmres_m_, gmres_tol_, 1000, &rep);
+            logger_ << "  -> GMRES iterations: " << rep.iterations << ", restarts: " << rep.restarts
+                    << ", relative residual: " << rep.relres << '\n';
+            std::cout << "  -> GMRES iterations: " << rep.iterations << '\n';
+            if (st == EMFA_E_NOCONV)
+                logger_ << "  -> WARNING: Iteration cannot converge\n";
+            else
+                check(st, "GMRES");
+        } else {
+            emfa_status st = emfa_lu_factor(ctx_, Z, prec_, &lu);
+            if (st == EMFA_E_SINGULAR) throw std::runtime_error("fail solving matrix equation (singular matrix)");
+            check(st, "LU");
+            if (prec_ == EMFA_F64) {
+                check(emfa_mat_copy(X_, B), "solve");
+                check(emfa_lu_solve(ctx_, lu, X_), "solve");
+            } else {
+                emfa_refine_report rep{};
+                emfa_status rs = emfa_refine(ctx_, Z, lu, B, X_, 1e-12, 10, &rep);
+                logger_ << "  -> refinement steps: " << rep.iterations << ", relative residual " << rep.relres_first
+                        << " -> " << rep.relres_final << '\n';
+                if (rs != EMFA_E_NOCONV) check(rs, "refinement");
+            }
+        }
+        check(emfa_ctx_sync(ctx_), "solve");
+        I_.assign((size_t)n * ncols_, {0, 0});
+        check(emfa_mat_download(X_, reinterpret_cast<double *>(I_.data())), "solution download");
+        timeLog("solve");
+        if (write_matrix_) {
+            std::vector<std::complex<double>> Zh((size_t)n * n), Bh((size_t)n * ncols_);
+            check(emfa_mat_download(Z, reinterpret_cast<double *>(Zh.data())), "Z download");
+            check(emfa_mat_download(B, reinterpret_cast<double *>(Bh.data())), "V download");
+            auto save = [&](const std::string &name, const std::vector<std::complex<double>> &m, int rows, int cols) {
+                // arma_ascii as EFIE.cpp:932-950 (complex<float> matrices)
+                std::ofstream f(dir_ + "/" + name);
+                f << "ARMA_MAT_TXT_FC008\n" << rows << ' ' << cols << '\n';
+                for (int r = 0; r < rows; ++r) {
+                    for (int c = 0; c < cols; ++c) {
+                        std::ostringstream ss;
+                        ss << '(' << (float)m[(size_t)r * cols + c].real() << ',' << (float)m[(size_t)r * cols + c].imag()
+                           << ')';
+                        f.put(' ');
+                        f << ss.str();
+                    }
+                    f.put('\n');
+                }
+            };
+            save("matrix_Z.txt", Zh, n, n);
+            std::vector<std::complex<double>> v0(n), i0(n);
+            for (int r = 0; r < n; ++r) {
+                v0[r] = Bh[(size_t)r * ncols_];
+                i0[r] = I_[(size_t)r * ncols_];
+            }
+            save("matrix_V.txt", v0, n, 1);
+            save("martix_I.txt", i0, n, 1);
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    std::cout << "success" << std::endl;
+}
+
+void DenseMoM::output() {
+    std::cout << "Result Information\n";
+    if (rtype_ == policy::BiRCS) {
+        writeBistatic();
+        timeLog("getBistaticRCS");
+    } else {
+        writeMonostatic();
+        timeLog("getMonostaticRCS");
+    }
+    std::vector<component::CurrentData> cur;
+    calculateSurfaceCurrent(&cur, 0);
+    component::result::writeCurrent(dir_, cur);
+    timeLog("getSurfaceCurrent");
+}
+
+void DenseMoM::clear() {
+    if (X_) emfa_mat_destroy(X_);
+    if (dmesh_) emfa_mesh_destroy(dmesh_);
+    X_ = nullptr;
+    dmesh_ = nullptr;
+    mesh_.clear();
+    ce_.clear();
+}
+
+// EMComponent/Result.h:48-104: phi cuts over theta, then the theta = 90 cut.
+void DenseMoM::writeBistatic() {
+    const auto &sca = scattering_;
+    const float d2r = (float)(kPi / 180.0);
+    std::vector<double> dirs;
+    std::vector<std::pair<std::string, std::vector<float>>> files;  // name, angle column
+    int num = static_cast<int>((sca.phi_to - sca.phi_from) / sca.phi_delta + 1);
+    for (int i = 0; i < num; ++i) {
+        float phiDegree = sca.phi_from + i * sca.phi_delta;
+        float phi = phiDegree * d2r;
+        std::string str = std::to_string(phiDegree);
+        files.push_back({"/bistatic_rcs_phi_" + str.substr(0, str.find('.')) + ".dat", {}});
+        for (float thetaDegree = sca.theta_from; thetaDegree <= sca.theta_to; thetaDegree += sca.theta_delta) {
+            float theta = thetaDegree * d2r;
+            dirs.insert(dirs.end(), {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)});
+            files.back().second.push_back(thetaDegree);
+        }
+    }
+    files.push_back({"/bistatic_rcs_theta_90.dat", {}});
+    for (float phiDegree = 0; phiDegree < 360; phiDegree += 1) {
+        float phi = phiDegree * d2r;
+        dirs.insert(dirs.end(), {std::cos(phi), std::sin(phi), 0.0});
+        files.back().second.push_back(phiDegree);
+    }
+    int nd = (int)(dirs.size() / 3);
+    std::vector<double> db(nd);
+    check(emfa_rcs(ctx_, dmesh_, k_, X_, nd, dirs.data(), nullptr, db.data()), "RCS");
+    size_t o = 0;
+    for (auto &f : files) {
+        std::ofstream out(dir_ + f.first);
+        if (out.fail()) throw component::FileError("cannot write " + dir_ + f.first);
+        component::result::prefixRCSData(out);
+        for (float ang : f.second) out << std::setw(10) << ang << '\t' << std::setw(10) << (float)db[o++] << '\n';
+    }
+}
+
+// EMComponent/Result.h:107-155: one incidence per direction (backscatter).
+void DenseMoM::writeMonostatic() {
+    const auto &sca = scattering_;
+    int nd = ncols_;
+    std::vector<double> dirs(3 * (size_t)nd);
+    std::vector<int> cols(nd);
+    for (int j = 0; j < nd; ++j) {
+        for (int c = 0; c < 3; ++c) dirs[3 * j + c] = rcs_dirs_[j][c];
+        cols[j] = j;
+    }
+    std::vector<double> db(nd);
+    check(emfa_rcs(ctx_, dmesh_, k_, X_, nd, dirs.data(), cols.data(), db.data()), "RCS");
+    int phi_num = static_cast<int>((sca.phi_to - sca.phi_from) / sca.phi_delta) + 1;
+    size_t o = 0;
+    for (int i = 0; i < phi_num; ++i) {
+        float phiDegree = sca.phi_from + i * sca.phi_delta;
+        std::string str = std::to_string(phiDegree);
+        std::ofstream out(dir_ + "/monostatic_rcs_phi_" + str.substr(0, str.find('.')) + ".dat");
+        component::result::prefixRCSData(out);
+        for (float thetaDegree = sca.theta_from; thetaDegree <= sca.theta_to; thetaDegree += sca.theta_delta)
+            out << std::setw(10) << thetaDegree << '\t' << std::setw(10) << (float)db[o++] << '\n';
+    }
+    std::ofstream out(dir_ + "/monostatic_rcs_theta_90.dat");
+    component::result::prefixRCSData(out);
+    for (float phiDegree = 0; phiDegree < 360; phiDegree += 1)
+        out << std::setw(10) << phiDegree << '\t' << std::setw(10) << (float)db[o++] << '\n';
+}
+
+// Per-triangle current magnitudes at the corners and the centroid,
+// EMSolver/CFIE.cpp:727-794 (computed from the solved currents).
+void DenseMoM::calculateSurfaceCurrent(std::vector<component::CurrentData> *currents, int col) const {
+    const size_t nt = mesh_.getTriangleNum();
+    const auto &xyz = mesh_.xyz();
+    std::vector<std::vector<int>> tris(nt);
+    for (int u = 0; u < unknowns_; ++u) {
+        const auto &r = ce_.getRWGRef(u);
+        tris[r.tpid].push_back(u);
+        tris[r.tmid].push_back(u);
+    }
+    auto V = [&](int i) { return component::Vec3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}; };
+    auto sub = [](component::Vec3 a, component::Vec3 b) { return component::Vec3{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto dist = [&](component::Vec3 a, component::Vec3 b) {
+        auto d = sub(a, b);
+        return std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    };
+    currents->resize(nt);
+    for (size_t t = 0; t < nt; ++t) {
+        auto &unks = tris[t];
+        component::Vec3 vx[3];
+        double len[3] = {0, 0, 0};
+        for (size_t i = 0; i < unks.size() && i < 3; ++i) {
+            const auto &r = ce_.getRWGRef(unks[i]);
+            vx[i] = V(((int)t == r.tpid) ? r.vxp : r.vxm);
+            len[i] = r.length;
+        }
+        if (unks.size() == 1) {
+            const auto &r = ce_.getRWGRef(unks[0]);
+            vx[1] = V(r.v1);
+            vx[2] = V(r.v2);
+        }
+        if (unks.size() == 2) {
+            const auto &r1 = ce_.getRWGRef(unks[0]);
+            const auto &r2 = ce_.getRWGRef(unks[1]);
+            vx[2] = (r1.v1 != r2.v1 && r1.v1 != r2.v2) ? V(r1.v2) : V(r1.v1);
+        }
+        component::Vec3 cen{(vx[0].x + vx[1].x + vx[2].x) / 3, (vx[0].y + vx[1].y + vx[2].y) / 3,
+                            (vx[0].z + vx[1].z + vx[2].z) / 3};
+        double la = dist(vx[0], vx[1]), lb = dist(vx[1], vx[2]), lc = dist(vx[2], vx[0]);
+        double s = 0.5 * (la + lb + lc);
+        double dom = 2 * std::sqrt(std::max(0.0, s * (s - la) * (s - lb) * (s - lc)));
+        std::complex<double> cc[3] = {0, 0, 0};
+        double mg[3] = {0, 0, 0};
+        size_t m = std::min<size_t>(unks.size(), 3);
+        for (size_t i = 0; i < m; ++i) {
+            const auto &r = ce_.getRWGRef(unks[i]);
+            double sg = ((int)t == r.tpid) ? 1.0 : -1.0;
+            std::complex<double> f = sg * I_[(size_t)unks[i] * ncols_ + col] * (len[i] / dom);
+            auto dv = sub(cen, vx[i]);
+            cc[0] += f * dv.x;
+            cc[1] += f * dv.y;
+            cc[2] += f * dv.z;
+            std::complex<double> nc[3] = {0, 0, 0};
+            for (size_t j = 0; j < m; ++j) {
+                if (i == j) continue;
+                const auto &rj = ce_.getRWGRef(unks[j]);
+                double sj = ((int)t == rj.tpid) ? 1.0 : -1.0;
+                std::complex<double> fj = sj * I_[(size_t)unks[j] * ncols_ + col] * (len[j] / dom);
+                auto d2 = sub(vx[i], vx[j]);
+                nc[0] += fj * d2.x;
+                nc[1] += fj * d2.y;
+                nc[2] += fj * d2.z;
+            }
+            mg[i] = std::sqrt(std::norm(nc[0]) + std::norm(nc[1]) + std::norm(nc[2]));
+        }
+        auto &d = (*currents)[t];
+        d.v1 = vx[0];
+        d.v2 = vx[1];
+        d.v3 = vx[2];
+        d.magnc = std::sqrt(std::norm(cc[0]) + std::norm(cc[1]) + std::norm(cc[2]));
+        d.magn1 = mg[0];
+        d.magn2 = mg[1];
+        d.magn3 = mg[2];
+    }
+}
+
+// ------------------------------------------------------------------ CFIE
+void CFIE::init(component::ConfigLoader *ploader) {
+    alpha_ = ploader->getAlpha();
+    DenseMoM::init(ploader);
+}
+
+// EMSolver/CFIE.cpp:35: "_CFIE_" + to_string(alpha).substr(0, 3)
+std::string CFIE::folderTag() const { return "_CFIE_" + std::to_string((float)alpha_).substr(0, 3); }
+
+}  // namespace mom
+
+namespace solver {
+
+void loadSolver(const std::string &conf_file) {
+    component::ConfigLoader loader;
+    loader.load(conf_file);
+    auto solver_type = loader.getSolverType();
+    if (solver_type == "EFIE")
+        invoke_solver<mom::EFIE>(&loader);
+    else if (solver_type == "CFIE")
+        invoke_solver<mom::CFIE>(&loader);
+    else if (solver_type == "MFIE")
+        invoke_solver<mom::MFIE>(&loader);
+    else
+        throw component::ConfigInvalid("undefined EM solver type (" + solver_type + ')');
+}
+
+}  // namespace solver

@@ -1,0 +1,163 @@
+// Solver lifecycle of the drop-in: mom::EMBase / mom::EM with the five
+// virtuals of EMSolver/EM.h:28-32, and the dense MoM solvers EFIE, CFIE and
+// MFIE whose fill, excitation, factorisation, solve and far field run on the
+// gfx950 device through the C ABI of include/emfa_hip.h.
+#pragma once
+#include <array>
+#include <complex>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "core.h"
+#include "emfa_hip.h"
+
+namespace mom {
+
+// EMSolver/EMBase.h:10-17
+class EMBase {
+  public:
+    EMBase() {}
+    virtual ~EMBase() {}
+    EMBase(const EMBase &) = delete;
+    EMBase &operator=(const EMBase &) = delete;
+};
+
+// EMSolver/EM.h:21-61 and EM.cpp:24-61
+class EM : public EMBase {
+  public:
+    EM();
+    ~EM() override;
+    virtual void init(component::ConfigLoader *ploader);
+    virtual void solve() {}
+    virtual void output() {}
+    virtual void clear() {}
+    virtual void reportInfo(std::ostream &strm) const { (void)strm; }
+
+    const std::string &dir() const { return dir_; }
+
+  protected:
+    void timeLog(const std::string &what);
+    policy::ResultType rtype_ = policy::BiRCS;
+    std::string dir_, folder_name_;
+    std::ofstream logger_;
+    std::ostringstream time_logger_;
+    tool::Clock clock_;
+    component::Incidence incidence_;
+    component::Scattering scattering_;
+    component::Radiation radiation_;
+    component::Nearfield nearfield_;
+    std::array<double, 3> inc_k_{}, inc_e_{}, inc_h_{};
+    double lamda = 0;
+};
+
+// Device context shared by all solvers of the process (created lazily).
+emfa_ctx *device_context(int device);
+
+// Dense method of moments on the device.  Build-specific configuration keys
+// (absent from the reference's CONFIG.yml, defaults in brackets):
+//   precision:        FP64 | FP32          [FP64]  FP32 = complex-FP32 fill and
+//                                                  LU with FP64 iterative refinement
+//   quadrature_order: 3 | 7                [3]     triangle rule of the fill
+//   linear_solver:    LU | GMRES           [LU]
+//   gmres_restart:    m                    [30]
+//   gmres_tolerance:  relative residual    [1e-6]
+//   cfie_self_term:   reference | accurate [reference]
+//   write_matrix:     true | false         [false] ASCII Z/V/I as EFIE.cpp:932-950
+//   device:           HIP ordinal          [0]
+class DenseMoM : public EM {
+  public:
+    ~DenseMoM() override;
+    void init(component::ConfigLoader *ploader) override;
+    void solve() override;
+    void output() override;
+    void clear() override;
+    void reportInfo(std::ostream &strm) const override;
+
+    // results, for callers that drive the solver directly
+    int unknowns() const { return unknowns_; }
+    const std::vector<std::complex<double>> &current() const { return I_; }  // n x ncols, row-major
+    int currentColumns() const { return ncols_; }
+    double wavenumber() const { return k_; }
+
+  protected:
+    virtual int fillKind() const = 0;  // EMFA_EFIE or EMFA_CFIE
+    virtual int rhsKind() const = 0;   // EMFA_EFIE, EMFA_CFIE or EMFA_MFIE
+    virtual double alpha() const { return 1.0; }
+    virtual std::string folderTag() const = 0;
+
+    void check(emfa_status st, const char *what) const;
+    void calculateSurfaceCurrent(std::vector<component::CurrentData> *currents, int col) const;
+    void writeBistatic();
+    void writeMonostatic();
+    void writeMatrices();
+
+    component::Mesh mesh_;
+    component::CommonEdge ce_;
+    int unknowns_ = 0;
+    double k_ = 0;
+    std::complex<double> sigma_{0, 0};
+    int prec_ = EMFA_F64, quad_ = 3, cfie_mode_ = 0, gmres_m_ = 30, device_ = 0;
+    double gmres_tol_ = 1e-6;
+    bool use_gmres_ = false, write_matrix_ = false;
+    std::vector<std::array<double, 3>> rhs_angles_;  // (theta, phi, pole) per column
+    std::vector<std::array<double, 3>> rcs_dirs_;    // monostatic: direction per column
+    int ncols_ = 0;
+    std::vector<std::complex<double>> I_;
+
+    emfa_ctx *ctx_ = nullptr;
+    emfa_mesh *dmesh_ = nullptr;
+    emfa_mat *X_ = nullptr;  // solution (F64), kept for the far field
+};
+
+class EFIE : public DenseMoM {
+  protected:
+    int fillKind() const override { return EMFA_EFIE; }
+    int rhsKind() const override { return EMFA_EFIE; }
+    std::string folderTag() const override { return "_EFIE"; }
+};
+
+class CFIE : public DenseMoM {
+  public:
+    void init(component::ConfigLoader *ploader) override;
+
+  protected:
+    int fillKind() const override { return EMFA_CFIE; }
+    int rhsKind() const override { return EMFA_CFIE; }
+    double alpha() const override { return alpha_; }
+    std::string folderTag() const override;
+    double alpha_ = 0.5;
+};
+
+// MFIE: the magnetic half of CFIE (alpha = 0), solved with the dense LU as
+// EMSolver/MFIE.cpp:50-73 does (arma::solve).
+class MFIE : public DenseMoM {
+  protected:
+    int fillKind() const override { return EMFA_CFIE; }
+    int rhsKind() const override { return EMFA_MFIE; }
+    double alpha() const override { return 0.0; }
+    std::string folderTag() const override { return "_MFIE"; }
+};
+
+}  // namespace mom
+
+namespace solver {
+
+// EMSolver/Solver.h:21-29
+template <typename QSolver>
+void invoke_solver(component::ConfigLoader *ploader) {
+    auto s = std::make_unique<QSolver>();
+    s->init(ploader);
+    s->reportInfo(std::cout);
+    s->solve();
+    s->output();
+    s->clear();
+}
+
+// EMSolver/Solver.cpp:45-95: string registry on "EMSolver:"
+void loadSolver(const std::string &conf_file);
+
+}  // namespace solver

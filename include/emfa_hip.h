@@ -38,7 +38,7 @@ typedef enum {
 } emfa_status;
 
 enum { EMFA_F32 = 0, EMFA_F64 = 1 };
-enum { EMFA_EFIE = 0, EMFA_CFIE = 1 };
+enum { EMFA_EFIE = 0, EMFA_CFIE = 1, EMFA_MFIE = 2 /* excitation only: MFIE.cpp:289-322 */ };
 
 typedef struct emfa_ctx emfa_ctx;
 typedef struct emfa_mesh emfa_mesh;
@@ -110,8 +110,10 @@ emfa_status emfa_fill(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_fill_para
 
 /* ---- excitation (K3) ------------------------------------------------- */
 /* inc: nrhs x 9 doubles {k_hat(3), e_hat(3), h_hat(3)} (EMSolver/EM.cpp:33-41).
- * B: n x nrhs.  Replaces EFIE::fillV (EFIE.cpp:568-601,698-722) and
- * CFIE::fillV (CFIE.cpp:570-597, Miscellaneous.cpp:152-189). */
+ * B: n x nrhs.  Replaces EFIE::fillV (EFIE.cpp:568-601,698-722),
+ * CFIE::fillV (CFIE.cpp:570-597, Miscellaneous.cpp:152-189) and, for kind
+ * EMFA_MFIE, MFIE::fillV (MFIE.cpp:289-322,425-437) scaled by Z0 to match a
+ * CFIE fill with alpha = 0. */
 emfa_status emfa_rhs_fill(emfa_ctx *ctx, const emfa_mesh *mesh, int kind, double k, double alpha, int nrhs,
                           const double *inc, emfa_mat *B);
 

@@ -488,8 +488,11 @@ void orc_fill_cfie(const double *xyz, const int *tri, int n, const int *rwg, dou
 /* ------------------------------------------------------------------ */
 /* Excitation.  EFIE: EMSolver/EFIE.cpp:568-601 + 698-722 (7-pt rule). */
 /* CFIE: EMSolver/Miscellaneous.cpp:152-189 + CFIE.cpp:570-597 (3-pt). */
+/* MFIE: EMSolver/MFIE.cpp:289-322 + 425-437 (7-pt, n x h part), times  */
+/* Z0 (the reference divides V by Z0 and leaves Z unscaled; here both  */
+/* sides carry Z0 so the MFIE system is CFIE with alpha = 0).          */
 /* inc: nrhs x 9 doubles {k(3), e(3), h(3)}; B: n x nrhs complex.      */
-/* kind 0 = EFIE, 1 = CFIE (alpha used).                               */
+/* kind 0 = EFIE, 1 = CFIE (alpha used), 2 = MFIE.                     */
 /* ------------------------------------------------------------------ */
 void orc_rhs(const double *xyz, const int *tri, int n, const int *rwg, int kind, double k, double alpha, int nrhs,
              const double *inc, double *B)
@@ -502,7 +505,7 @@ void orc_rhs(const double *xyz, const int *tri, int n, const int *rwg, int kind,
         tri_vtx(xyz, tri, r[5], tm);
         v3 vp = vtx(xyz, r[2]), vm = vtx(xyz, r[3]);
         double l = norm3(sub(vtx(xyz, r[0]), vtx(xyz, r[1])));
-        int q = (kind == 0) ? 7 : 3;
+        int q = (kind == 1) ? 3 : 7;
         v3 pp[7], pm[7];
         const double *w = quad_pts(q, tp[0], tp[1], tp[2], pp);
         quad_pts(q, tm[0], tm[1], tm[2], pm);
@@ -516,13 +519,13 @@ void orc_rhs(const double *xyz, const int *tri, int n, const int *rwg, int kind,
                 v3 rp = sub(pp[a], vp);
                 cplx G = cexp(-I * k * dot(pp[a], kv));
                 Ve += w[a] * dot(rp, ev) * G;
-                if (kind == 1) Vm += w[a] * dot(rp, cross(np, hv)) * G;
+                if (kind != 0) Vm += w[a] * dot(rp, cross(np, hv)) * G;
                 v3 rm = sub(vm, pm[a]);
                 G = cexp(-I * k * dot(pm[a], kv));
                 Ve += w[a] * dot(rm, ev) * G;
-                if (kind == 1) Vm += w[a] * dot(rm, cross(nm, hv)) * G;
+                if (kind != 0) Vm += w[a] * dot(rm, cross(nm, hv)) * G;
             }
-            cplx v = (kind == 0) ? Ve : (alpha * Ve + (1.0 - alpha) * Vm);
+            cplx v = (kind == 0) ? Ve : (kind == 2) ? Vm : (alpha * Ve + (1.0 - alpha) * Vm);
             v *= 0.5 * l;
             B[2 * ((size_t)u * nrhs + j)] = creal(v);
             B[2 * ((size_t)u * nrhs + j) + 1] = cimag(v);
@@ -871,3 +874,15 @@ void orc_current(const double *xyz, int nt, int n, const int *rwg, const double 
 
 /* Wave number for a frequency, EMSolver/EFIE.cpp:47 (k = 2 pi f / c). */
 double orc_wavenumber(double freq) { return 2.0 * ORC_PI * freq / ORC_C0; }
+
+/* Exported for the tests: the analytic potentials Is1 (vector) and Is2 of
+ * triangle t (9 doubles) seen from point r (EMSolver/EFIE.cpp:259-291). */
+void orc_potentials(const double *t9, const double *r3, double *is1, double *is2)
+{
+    v3 t[3] = {mk(t9[0], t9[1], t9[2]), mk(t9[3], t9[4], t9[5]), mk(t9[6], t9[7], t9[8])};
+    v3 s1;
+    efie_potentials(t, mk(r3[0], r3[1], r3[2]), &s1, is2);
+    is1[0] = s1.x;
+    is1[1] = s1.y;
+    is1[2] = s1.z;
+}

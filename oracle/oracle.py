@@ -50,6 +50,7 @@ def lib():
         L.orc_quad_rule.argtypes = [ctypes.c_int, _d, _d]
         L.orc_wavenumber.restype = ctypes.c_double
         L.orc_wavenumber.argtypes = [ctypes.c_double]
+        L.orc_potentials.argtypes = [_d, _d, _d, _d]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         _LIB = L
@@ -232,3 +233,13 @@ def current(xyz, nt, rwg, X, col=0):
     lib().orc_current(_pd(xyz), nt, rwg.shape[0], _pi(rwg), X.ctypes.data_as(_d), X.shape[1], int(col), _pd(out),
                       _pd(vout))
     return out, vout
+
+
+def potentials(tri9, r):
+    """Analytic (Is1, Is2) of EMSolver/EFIE.cpp:259-291 for triangle tri9 (3x3) seen from r."""
+    t = _c(tri9, np.float64).reshape(9)
+    rr = _c(r, np.float64).reshape(3)
+    is1 = np.zeros(3)
+    is2 = np.zeros(1)
+    lib().orc_potentials(_pd(t), _pd(rr), _pd(is1), _pd(is2))
+    return is1, float(is2[0])
