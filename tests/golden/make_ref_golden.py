@@ -1,0 +1,72 @@
+"""Writes the golden vectors of tests/golden/ref_*.npz by running the
+reference's own EFIE/CFIE code (oracle/_ref/emfa_ref, built by
+`make -C oracle -f ref.mk` from the reference sources) on small synthetic
+meshes.  Each fixture holds the inputs (mesh as the reference read it, RWG
+list, frequency, incidence) and the reference's outputs (complex<float> Z
+rows, V, the LU solution I, the RCS cut).
+
+  python tests/golden/make_ref_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(HERE))
+import meshgen  # noqa: E402
+from test_cpu_host import write_nas  # noqa: E402
+
+EXE = os.path.join(ROOT, "oracle", "_ref", "emfa_ref")
+
+# name: (kind, mesh builder, freq, alpha, sigma, (theta, phi, pole), rows of Z kept)
+CASES = {
+    "efie_sphere_s1": ("efie", lambda: meshgen.icosphere(1, 0.5), 3e8, 1.0, 0j, (0, 0, 0), None),
+    "efie_sphere_s2_sigma": ("efie", lambda: meshgen.icosphere(2, 0.5), 3e8, 1.0, 0.0025 + 0.001j, (20, 10, 30), 64),
+    "efie_plate4": ("efie", lambda: meshgen.plate(4, 1.0), 3e8, 1.0, 0j, (30, 45, 0), None),
+    "cfie_sphere_s2": ("cfie", lambda: meshgen.icosphere(2, 1.0), 6e8, 0.5, 0j, (0, 0, 0), 64),
+    "cfie_ellipsoid_s1": ("cfie", lambda: meshgen.ellipsoid(1, 1.0, 0.6, 0.4), 4e8, 0.3, 0j, (40, 20, 90), None),
+}
+
+
+def read_out(path):
+    with open(path, "rb") as f:
+        nv, nt, n, na = np.frombuffer(f.read(16), dtype=np.int32)
+        xyz = np.frombuffer(f.read(12 * nv), dtype=np.float32).reshape(nv, 3)
+        tri = np.frombuffer(f.read(12 * nt), dtype=np.int32).reshape(nt, 3)
+        rwg = np.frombuffer(f.read(24 * n), dtype=np.int32).reshape(n, 6)
+        Z = np.frombuffer(f.read(8 * n * n), dtype=np.float32).reshape(n, n, 2)
+        V = np.frombuffer(f.read(8 * n), dtype=np.float32).reshape(n, 2)
+        I = np.frombuffer(f.read(8 * n), dtype=np.float32).reshape(n, 2)
+        th = np.frombuffer(f.read(4 * na), dtype=np.float32)
+        rcs = np.frombuffer(f.read(4 * na), dtype=np.float32)
+    c = lambda a: a[..., 0].astype(np.float64) + 1j * a[..., 1].astype(np.float64)
+    return xyz, tri, rwg, c(Z), c(V), c(I), th, rcs
+
+
+def main():
+    if not os.path.exists(EXE):
+        sys.exit("oracle/_ref/emfa_ref is not built (make -C oracle -f ref.mk)")
+    for name, (kind, build, freq, alpha, sigma, inc, rows) in CASES.items():
+        xyz, tri = build()
+        with tempfile.TemporaryDirectory() as d:
+            nas = os.path.join(d, "mesh.nas")
+            write_nas(nas, xyz, tri)
+            out = os.path.join(d, "out.bin")
+            subprocess.run([EXE, kind, nas, repr(freq), repr(alpha), repr(sigma.real), repr(sigma.imag),
+                            str(inc[0]), str(inc[1]), str(inc[2]), out], cwd=d, check=True,
+                           stdout=subprocess.DEVNULL)
+            rx, rt, rr, Z, V, I, th, rcs = read_out(out)
+        keep = Z if rows is None else Z[:rows]
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), kind=kind, freq=freq, alpha=alpha,
+                            sigma=sigma, inc=np.array(inc, dtype=np.float64), xyz=rx, tri=rt, rwg=rr,
+                            Z=keep.astype(np.complex64), V=V.astype(np.complex64), I=I.astype(np.complex64),
+                            theta=th, rcs=rcs)
+        print(name, "N =", rr.shape[0], "Z rows kept:", keep.shape[0])
+
+
+if __name__ == "__main__":
+    main()
