@@ -52,7 +52,7 @@ emfa_status ctx_scratch(emfa_ctx *ctx, size_t bytes, void **p) {
     return EMFA_OK;
 }
 
-void prof_begin(emfa_ctx *ctx, const char *family, double work, hipEvent_t *a, hipEvent_t *b) {
+void prof_begin(emfa_ctx *ctx, const char *family, double work, hipEvent_t *a, hipEvent_t *b, hipStream_t s) {
     *a = *b = nullptr;
     if (!ctx->prof) return;
     auto get = [&]() {
@@ -67,19 +67,20 @@ void prof_begin(emfa_ctx *ctx, const char *family, double work, hipEvent_t *a, h
     };
     *a = get();
     *b = get();
-    (void)hipEventRecord(*a, ctx->stream);
+    (void)hipEventRecord(*a, s ? s : ctx->stream);
     ctx->pending.push_back({family, *a, *b, work});
 }
 
-void prof_end(emfa_ctx *ctx, hipEvent_t a, hipEvent_t b) {
+void prof_end(emfa_ctx *ctx, hipEvent_t a, hipEvent_t b, hipStream_t s) {
     (void)a;
     if (!ctx->prof || !b) return;
-    (void)hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, s ? s : ctx->stream);
 }
 
 void prof_collect(emfa_ctx *ctx) {
     if (ctx->pending.empty()) return;
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->aux);
     for (auto &p : ctx->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
@@ -124,11 +125,13 @@ emfa_status emfa_ctx_create(int device, emfa_ctx **out) {
     emfa_ctx *c = new emfa_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         set_error("hipStreamCreate failed");
         return EMFA_E_HIP;
     }
+    for (auto &e : c->ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     void *f = nullptr;
     if (ctx_alloc(c, &f, 64) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -150,6 +153,9 @@ emfa_status emfa_ctx_destroy(emfa_ctx *ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->scratch) ctx_free(ctx, ctx->scratch, ctx->scratch_bytes);
     ctx_free(ctx, ctx->flags, 64);
+    (void)hipStreamSynchronize(ctx->aux);
+    for (auto e : ctx->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->aux);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return EMFA_OK;

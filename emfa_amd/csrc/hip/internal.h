@@ -50,6 +50,8 @@ struct PendingEvent {
 struct emfa_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // second stream (LU look-ahead), default priority
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cu = 0;
     size_t bytes = 0;
     bool prof = false;
@@ -105,15 +107,19 @@ void ctx_free(emfa_ctx *ctx, void *p, size_t bytes);
 emfa_status ctx_scratch(emfa_ctx *ctx, size_t bytes, void **p);
 
 // profiling helpers (no-ops unless ctx->prof)
-void prof_begin(emfa_ctx *ctx, const char *family, double work, hipEvent_t *a, hipEvent_t *b);
-void prof_end(emfa_ctx *ctx, hipEvent_t a, hipEvent_t b);
+void prof_begin(emfa_ctx *ctx, const char *family, double work, hipEvent_t *a, hipEvent_t *b,
+                hipStream_t s = nullptr);
+void prof_end(emfa_ctx *ctx, hipEvent_t a, hipEvent_t b, hipStream_t s = nullptr);
 void prof_collect(emfa_ctx *ctx);
 
 struct ProfScope {
     emfa_ctx *ctx;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(emfa_ctx *c, const char *family, double work) : ctx(c) { prof_begin(ctx, family, work, &a, &b); }
-    ~ProfScope() { prof_end(ctx, a, b); }
+    hipStream_t s = nullptr;
+    ProfScope(emfa_ctx *c, const char *family, double work, hipStream_t st = nullptr) : ctx(c), s(st) {
+        prof_begin(ctx, family, work, &a, &b, s);
+    }
+    ~ProfScope() { prof_end(ctx, a, b, s); }
 };
 
 // complex scalar helpers usable on host and device

@@ -501,7 +501,9 @@ struct LuPlan {
 // real-expanded transposed form Bt (ldb reals per row).
 template <typename R>
 static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols, int K, const cx<R> *Aop,
-                               long long lda, const R *Bt, long long ldb, cx<R> *C, long long ldc) {
+                               long long lda, const R *Bt, long long ldb, cx<R> *C, long long ldc,
+                               hipStream_t stream = nullptr) {
+    hipStream_t s = stream ? stream : ctx->stream;
     if (M <= 0 || ncols <= 0 || K <= 0) return EMFA_OK;
     GemmArgs g;
     g.M = M;
@@ -515,12 +517,12 @@ static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols,
     g.ldc = 2 * ldc;
     g.alpha_neg = 1;
     g.beta = 1;
-    ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K);
+    ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K, s);
     const int tag = std::strcmp(fam, "lu_gemm") == 0 ? 1 : 0;
     if constexpr (sizeof(R) == 4)
-        return gemm_launch_f32(ctx->stream, g, tag);
+        return gemm_launch_f32(s, g, tag);
     else
-        return gemm_launch_f64(ctx->stream, g, tag);
+        return gemm_launch_f64(s, g, tag);
 }
 
 // Blocked triangular solve of rows [r0, r0+wb) of X (columns [c0,c1)) with the
@@ -633,7 +635,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     size_t bar_b = 256;
     size_t bt_b = (size_t)2 * N * 2 * nb * sizeof(R);
     size_t perm_b = PermWs<R>::bytes(N);
-    size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + bt_b + perm_b + 4096;
+    size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + 2 * bt_b + perm_b + 4096;
     void *scr = nullptr;
     emfa_status st = ctx_scratch(ctx, total, &scr);
     if (st != EMFA_OK) return st;
@@ -648,7 +650,8 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     int *slot_idx = (int *)take(slot_idx_b);
     cx<R> *rowc = (cx<R> *)take(rowc_b);
     unsigned *bar = (unsigned *)take(bar_b);
-    R *Bt = (R *)take(bt_b);
+    R *Bt_in = (R *)take(bt_b);   // U rows of the inner panels (read by the in-panel updates)
+    R *Bt_out = (R *)take(bt_b);  // U rows of the outer panel (read by the trailing update)
     PermWs<R> pws;
     pws.carve(take(perm_b));
     k_zero_u32<<<1, 64, 0, ctx->stream>>>(bar, 64);
@@ -656,13 +659,16 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     k_zero_u32<<<1, 64, 0, ctx->stream>>>((unsigned *)ctx->flags, 16);
     EMFA_HIP_CHECK(hipGetLastError());
     unsigned bar_base = 0;
+    bool coop = true, lookahead = true;
+    if (const char *s = getenv("EMFA_PANEL_COOP")) coop = atoi(s) != 0;
+    if (const char *s = getenv("EMFA_LU_LOOKAHEAD")) lookahead = atoi(s) != 0;
 
-    for (int j0 = 0; j0 < N; j0 += nb) {
-        const int jb = std::min(nb, N - j0);
+    // Factor the outer panel [j0, j0+jb): inner panels of width w with the
+    // in-panel row interchanges, triangular solves and updates (main stream).
+    auto factor_outer = [&](int j0, int jb) -> emfa_status {
         int js = j0;
         while (js < j0 + jb) {
             const int M = N - js;
-            // panel width and workgroup count that fit LDS
             int w = std::min(plan.w, j0 + jb - js);
             int G = std::min(G_max, std::max(1, (M + panel_rows_min - 1) / panel_rows_min));
             int RB = (M + G - 1) / G;
@@ -672,36 +678,76 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             size_t lds = (size_t)RB * (w + 1) * sizeof(cx<R>);
             {
                 ProfScope ps(ctx, "lu_panel", 0);
-                void *args[] = {(void *)&A,        (void *)&lda,      (void *)&js,   (void *)&M,
-                                (void *)&w,        (void *)&RB,       (void *)&ipiv, (void *)&slot_rows,
-                                (void *)&slot_val, (void *)&slot_idx, (void *)&rowc, (void *)&bar,
-                                (void *)&bar_base, (void *)&ctx->flags};
-                hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<R>, dim3(G), dim3(256), args, lds,
-                                                          ctx->stream);
-                EMFA_HIP_CHECK(e);
+                if (coop) {
+                    void *args[] = {(void *)&A,        (void *)&lda,      (void *)&js,   (void *)&M,
+                                    (void *)&w,        (void *)&RB,       (void *)&ipiv, (void *)&slot_rows,
+                                    (void *)&slot_val, (void *)&slot_idx, (void *)&rowc, (void *)&bar,
+                                    (void *)&bar_base, (void *)&ctx->flags};
+                    EMFA_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_panel<R>, dim3(G), dim3(256), args,
+                                                              lds, ctx->stream));
+                } else {
+                    // G workgroups of 256 threads with <= 160 KB LDS each always fit one per CU, and
+                    // the concurrent update never waits on the panel, so all of them become
+                    // resident; the barrier's bounded spin reports it if not.
+                    k_panel<R><<<G, 256, lds, ctx->stream>>>(A, lda, js, M, w, RB, ipiv, slot_rows, slot_val,
+                                                             slot_idx, rowc, bar, bar_base, ctx->flags);
+                    EMFA_HIP_CHECK(hipGetLastError());
+                }
             }
             bar_base += (unsigned)w * G;
-            // swaps on the rest of the outer panel
-            st = laswp_launch<R>(ctx, A, lda, ipiv, js, js + w, j0, js, js + w, j0 + jb, pws);
-            if (st) return st;
+            emfa_status s2 = laswp_launch<R>(ctx, A, lda, ipiv, js, js + w, j0, js, js + w, j0 + jb, pws);
+            if (s2) return s2;
             if (js + w < j0 + jb) {
-                st = trsm_blocked<R>(ctx, A, lda, A, lda, js, w, js + w, j0 + jb, 0, Bt, "lu_trsm");
-                if (st) return st;
-                st = gemm_update<R>(ctx, "lu_gemm", N - js - w, j0 + jb - js - w, w, A + (size_t)(js + w) * lda + js,
-                                    lda, Bt, 2LL * w, A + (size_t)(js + w) * lda + js + w, lda);
-                if (st) return st;
+                s2 = trsm_blocked<R>(ctx, A, lda, A, lda, js, w, js + w, j0 + jb, 0, Bt_in, "lu_trsm");
+                if (s2) return s2;
+                s2 = gemm_update<R>(ctx, "lu_gemm", N - js - w, j0 + jb - js - w, w,
+                                    A + (size_t)(js + w) * lda + js, lda, Bt_in, 2LL * w,
+                                    A + (size_t)(js + w) * lda + js + w, lda);
+                if (s2) return s2;
             }
             js += w;
         }
-        st = laswp_launch<R>(ctx, A, lda, ipiv, j0, j0 + jb, 0, j0, j0 + jb, N, pws);
-        if (st) return st;
-        if (j0 + jb < N) {
-            st = trsm_blocked<R>(ctx, A, lda, A, lda, j0, jb, j0 + jb, N, 0, Bt, "lu_trsm");
+        return EMFA_OK;
+    };
+    // Apply the outer panel's interchanges outside it and solve for its U rows.
+    auto outer_swap_trsm = [&](int j0, int jb) -> emfa_status {
+        emfa_status s2 = laswp_launch<R>(ctx, A, lda, ipiv, j0, j0 + jb, 0, j0, j0 + jb, N, pws);
+        if (s2) return s2;
+        if (j0 + jb < N) return trsm_blocked<R>(ctx, A, lda, A, lda, j0, jb, j0 + jb, N, 0, Bt_out, "lu_trsm");
+        return EMFA_OK;
+    };
+
+    int jb0 = std::min(nb, N);
+    if ((st = factor_outer(0, jb0))) return st;
+    if ((st = outer_swap_trsm(0, jb0))) return st;
+    for (int j0 = 0; j0 < N; j0 += nb) {
+        const int jb = std::min(nb, N - j0);
+        const int next = j0 + jb;
+        if (next >= N) break;
+        const int nbn = std::min(nb, N - next);
+        const int M = N - next;
+        const long long ldbt = 2LL * jb;
+        const cx<R> *Aop = A + (size_t)next * lda + j0;
+        if (lookahead && N - next - nbn > 0) {
+            // G2 (all columns but the next panel's) on the second stream, G1 and the next
+            // panel's factorisation on the main stream meanwhile.
+            EMFA_HIP_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
+            EMFA_HIP_CHECK(hipStreamWaitEvent(ctx->aux, ctx->ev[0], 0));
+            st = gemm_update<R>(ctx, "lu_gemm", M, N - next - nbn, jb, Aop, lda, Bt_out + 2LL * nbn * ldbt, ldbt,
+                                A + (size_t)next * lda + next + nbn, lda, ctx->aux);
             if (st) return st;
-            st = gemm_update<R>(ctx, "lu_gemm", N - j0 - jb, N - j0 - jb, jb, A + (size_t)(j0 + jb) * lda + j0, lda,
-                                Bt, 2LL * jb, A + (size_t)(j0 + jb) * lda + j0 + jb, lda);
+            EMFA_HIP_CHECK(hipEventRecord(ctx->ev[1], ctx->aux));
+            st = gemm_update<R>(ctx, "lu_gemm", M, nbn, jb, Aop, lda, Bt_out, ldbt, A + (size_t)next * lda + next,
+                                lda);
             if (st) return st;
+            if ((st = factor_outer(next, nbn))) return st;
+            EMFA_HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->ev[1], 0));
+        } else {
+            st = gemm_update<R>(ctx, "lu_gemm", M, M, jb, Aop, lda, Bt_out, ldbt, A + (size_t)next * lda + next, lda);
+            if (st) return st;
+            if ((st = factor_outer(next, nbn))) return st;
         }
+        if ((st = outer_swap_trsm(next, nbn))) return st;
     }
     int flags[2] = {0, 0};
     EMFA_HIP_CHECK(hipMemcpyAsync(flags, ctx->flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
