@@ -26,6 +26,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The solve server runs two streams per worker context; HIP maps streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 unless set), and streams that share a
+# queue execute in order.  Read once when HIP initialises, so set it here.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK = {"f32_mfma": 157.3, "f64_mfma": 78.6, "hbm": 8000.0}
 
@@ -34,18 +38,24 @@ CONFIGS = {
     "c2": ("CFIE", 5, 1.0, 6e8, 0.5, 3, np.arange(0.0, 181.0, 1.0)),
     "c1": ("EFIE", 3, 0.5, 3e8, 1.0, 7, np.array([0.0])),
     "s4": ("CFIE", 4, 1.0, 6e8, 0.5, 3, np.arange(0.0, 181.0, 1.0)),
+    "c3": None,  # 256 uneven EFIE ellipsoid problems (emfa_amd.workload.batch_problems)
+    "c4": None,  # mixed serving stream (emfa_amd.workload.mixed_stream)
 }
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3; c4: requests, default 64)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-stream summary of the default line")
+    ap.add_argument("--mixed-requests", type=int, default=32)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=96.0)
     return ap.parse_args()
 
 
@@ -100,6 +110,47 @@ def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
     }
 
 
+def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None):
+    """Answer `reqs` on the plain FP64 path (sequential, nothing cached) and
+    through the solve server (FP32+refine, `workers` contexts, resident
+    factors): once with every request queued at t=0 (capacity), once on
+    Poisson arrivals at `latency_rate` x that capacity (latency), each on a
+    fresh server.  Returns a summary dict."""
+    from emfa_amd import hip, serve, workload
+    precision = hip.F32 if precision is None else precision
+    ctx = hip.Context(0)
+    workload.solve_plain(ctx, reqs[0])  # loads the kernels
+    plain, plain_lat, plain_wall = workload.run_plain(ctx, reqs)
+    ctx.close()
+    with serve.Server(workers=workers, hbm_budget=budget) as srv:
+        srv.wait(srv.submit(reqs[0]["angles"][:1], kind=serve.ICOSPHERE, subdiv=1, freq=3e8, precision=precision))
+        sat, st = workload.run_served(srv, reqs, precision)
+    cap = st["solves_per_s"]
+    with serve.Server(workers=workers, hbm_budget=budget) as srv:
+        srv.wait(srv.submit(reqs[0]["angles"][:1], kind=serve.ICOSPHERE, subdiv=1, freq=3e8, precision=precision))
+        lat, st2 = workload.run_served(srv, reqs, precision, rate=latency_rate * cap)
+    err = max(float(np.abs(p - r["rcs_db"]).max()) for p, r in zip(plain, sat))
+    err2 = max(float(np.abs(p - r["rcs_db"]).max()) for p, r in zip(plain, lat))
+    ns = [workload.unknowns(r) for r in reqs]
+    pl = np.sort(plain_lat)
+    return {
+        "requests": len(reqs), "unknowns_min": int(min(ns)), "unknowns_max": int(max(ns)),
+        "rhs_total": int(sum(len(r["angles"]) for r in reqs)),
+        "workers": workers, "hbm_budget_gb": budget / 2**30,
+        "served_solves_per_s": cap, "served_rhs_per_s": st["rhs_per_s"], "served_wall_s": st["wall_s"],
+        "served_cache_hits": st["cache_hits"], "served_failed": st["failed"],
+        "served_fp64_fallbacks": int(sum(r["fp64_fallback"] for r in sat)),
+        "served_peak_gb": st["peak_bytes"] / 2**30,
+        "plain_fp64_solves_per_s": len(reqs) / plain_wall, "plain_fp64_wall_s": plain_wall,
+        "plain_fp64_p50_ms": float(np.percentile(pl, 50)), "plain_fp64_p99_ms": float(np.percentile(pl, 99)),
+        "speedup_vs_plain_fp64": cap / (len(reqs) / plain_wall),
+        "latency_run": {"arrival_rate_per_s": latency_rate * cap, "p50_ms": st2["p50_ms"], "p99_ms": st2["p99_ms"],
+                        "mean_ms": st2["mean_ms"], "max_ms": st2["max_ms"], "solves_per_s": st2["solves_per_s"]},
+        "saturated_p50_ms": st["p50_ms"], "saturated_p99_ms": st["p99_ms"],
+        "rcs_max_abs_diff_db_vs_fp64": max(err, err2),
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -107,6 +158,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from emfa_amd import hip
+    if a.config in ("c3", "c4"):
+        if a.steps is None:
+            a.steps = 64
+        return main_stream(a)
+    if a.steps is None:
+        a.steps = 3
 
     kind_s, subdiv, radius, freq, alpha, quad, thetas = CONFIGS[a.config]
     kind = hip.CFIE if kind_s == "CFIE" else hip.EFIE
@@ -211,6 +268,44 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    if not a.no_mixed:
+        from emfa_amd import workload
+        out["mixed_stream"] = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers,
+                                         int(a.budget_gb * 2**30))
+    print(json.dumps(out))
+
+
+def main_stream(a):
+    """c3: the 256 uneven problems; c4: the mixed serving stream with
+    `--steps` requests.  A step is one request; value = served solves/s."""
+    from emfa_amd import workload
+    if a.config == "c3":
+        reqs = workload.batch_problems()
+        name = "256 PEC ellipsoids, semi-axes U[0.3,1.2] m, icosphere s=2..4, 100-600 MHz, EFIE, 1-64 RHS"
+    else:
+        reqs = workload.mixed_stream(a.steps)
+        name = ("mixed stream: spheres/ellipsoids (s=3,4) and 1x1 m plates, 500-15000 RWG, EFIE/CFIE, 1-64 RHS, "
+                "30% repeat meshes")
+    leg = stream_leg(reqs, a.workers, int(a.budget_gb * 2**30))
+    lr = leg["latency_run"]
+    out = {
+        "metric": "solves/sec (served FP32+refine) + p50/p99 latency",
+        "value": leg["served_solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 1,
+        "ms_per_step": 1e3 / leg["served_solves_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 fill/LU + f64 refinement",
+        "data": f"synthetic meshes, seed {workload.SEED}",
+        "config": {"workload": name, "parallelism": "single GPU", "workers": a.workers},
+        "latency_ms": {"p50": lr["p50_ms"], "p99": lr["p99_ms"],
+                       "note": f"Poisson arrivals at {lr['arrival_rate_per_s']:.2f}/s (0.5x capacity)"},
+        "stream": leg,
+    }
     print(json.dumps(out))
 
 

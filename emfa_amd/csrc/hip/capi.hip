@@ -2,6 +2,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <iterator>
+#include <map>
+#include <mutex>
 
 #include "internal.h"
 
@@ -16,18 +19,135 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
+// Device memory pool, one per device, shared by every context on it.
+// hipFree waits for the whole device, which would serialise contexts that
+// work concurrently (a solve server's workers), so freed blocks are kept and
+// handed out again.  A freed block carries an event recorded on the freeing
+// context's stream; a context that takes it first makes both its streams
+// wait on that event, so reuse across contexts stays in stream order.  The
+// cache is trimmed when it exceeds EMFA_POOL_CACHE_MB (default 32768), when
+// hipMalloc fails, and when the device's last context is destroyed.
+struct PoolBlock {
+    void *p;
+    size_t bytes;
+    hipEvent_t ev;
+};
+
+struct DevicePool {
+    std::mutex mu;
+    std::multimap<size_t, PoolBlock> free_blocks;
+    std::map<void *, size_t> live;  // handed-out block -> its size
+    std::vector<hipEvent_t> events;
+    size_t cached = 0;
+    int contexts = 0;
+};
+
+static DevicePool &pool(int device) {
+    static DevicePool pools[64];
+    return pools[device & 63];
+}
+
+static size_t pool_limit() {
+    static const size_t lim = [] {
+        const char *e = getenv("EMFA_POOL_CACHE_MB");
+        return (size_t)(e ? atoll(e) : 32768) << 20;
+    }();
+    return lim;
+}
+
+static size_t pool_round(size_t bytes) {
+    const size_t g = bytes >= (1u << 20) ? (2u << 20) : 4096;
+    return (bytes + g - 1) / g * g;
+}
+
+// frees cached blocks (largest first) until at most `keep` bytes stay cached; call with mu held
+static void pool_trim_locked(DevicePool &P, size_t keep) {
+    while (P.cached > keep && !P.free_blocks.empty()) {
+        auto it = std::prev(P.free_blocks.end());
+        (void)hipEventSynchronize(it->second.ev);
+        (void)hipFree(it->second.p);
+        P.events.push_back(it->second.ev);
+        P.cached -= it->second.bytes;
+        P.free_blocks.erase(it);
+    }
+}
+
 hipError_t ctx_alloc(emfa_ctx *ctx, void **p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) return hipSuccess;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e == hipSuccess) ctx->bytes += bytes;
+    const size_t rb = pool_round(bytes);
+    DevicePool &P = pool(ctx->device);
+    {
+        std::lock_guard<std::mutex> l(P.mu);
+        auto it = P.free_blocks.lower_bound(rb);
+        if (it != P.free_blocks.end() && it->first <= rb + rb / 4 + (2u << 20)) {
+            PoolBlock b = it->second;
+            P.free_blocks.erase(it);
+            P.cached -= b.bytes;
+            (void)hipStreamWaitEvent(ctx->stream, b.ev, 0);
+            (void)hipStreamWaitEvent(ctx->aux, b.ev, 0);
+            P.events.push_back(b.ev);
+            P.live[b.p] = b.bytes;
+            *p = b.p;
+            ctx->bytes += b.bytes;
+            return hipSuccess;
+        }
+    }
+    hipError_t e = hipMalloc(p, rb);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> l(P.mu);
+        pool_trim_locked(P, 0);
+        e = hipMalloc(p, rb);
+    }
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> l(P.mu);
+        P.live[*p] = rb;
+        ctx->bytes += rb;
+    }
     return e;
 }
 
 void ctx_free(emfa_ctx *ctx, void *p, size_t bytes) {
     if (!p) return;
-    (void)hipFree(p);
-    ctx->bytes -= bytes;
+    DevicePool &P = pool(ctx->device);
+    std::lock_guard<std::mutex> l(P.mu);
+    size_t rb = pool_round(bytes);
+    auto lv = P.live.find(p);
+    if (lv != P.live.end()) {
+        rb = lv->second;
+        P.live.erase(lv);
+    }
+    hipEvent_t ev = nullptr;
+    if (!P.events.empty()) {
+        ev = P.events.back();
+        P.events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(p);
+        ctx->bytes -= rb;
+        return;
+    }
+    (void)hipEventRecord(ev, ctx->stream);
+    P.free_blocks.emplace(rb, PoolBlock{p, rb, ev});
+    P.cached += rb;
+    ctx->bytes -= rb;
+    if (P.cached > pool_limit()) pool_trim_locked(P, pool_limit() / 2);
+}
+
+static void pool_attach(int device) {
+    DevicePool &P = pool(device);
+    std::lock_guard<std::mutex> l(P.mu);
+    ++P.contexts;
+}
+
+static void pool_detach(int device) {
+    DevicePool &P = pool(device);
+    std::lock_guard<std::mutex> l(P.mu);
+    if (--P.contexts > 0) return;
+    pool_trim_locked(P, 0);
+    for (auto e : P.events) (void)hipEventDestroy(e);
+    P.events.clear();
 }
 
 emfa_status ctx_scratch(emfa_ctx *ctx, size_t bytes, void **p) {
@@ -132,9 +252,12 @@ emfa_status emfa_ctx_create(int device, emfa_ctx **out) {
         return EMFA_E_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    pool_attach(device);
     void *f = nullptr;
     if (ctx_alloc(c, &f, 64) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->aux);
+        pool_detach(device);
         delete c;
         set_error("flag allocation failed");
         return EMFA_E_OOM;
@@ -155,8 +278,10 @@ emfa_status emfa_ctx_destroy(emfa_ctx *ctx) {
     ctx_free(ctx, ctx->flags, 64);
     (void)hipStreamSynchronize(ctx->aux);
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->aux);
     (void)hipStreamDestroy(ctx->stream);
+    pool_detach(ctx->device);
     delete ctx;
     return EMFA_OK;
 }
@@ -170,6 +295,14 @@ emfa_status emfa_ctx_sync(emfa_ctx *ctx) {
 void *emfa_ctx_stream(emfa_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 size_t emfa_ctx_bytes_in_use(const emfa_ctx *ctx) { return ctx ? ctx->bytes : 0; }
+
+int emfa_ctx_cu_count(const emfa_ctx *ctx) { return ctx ? ctx->num_cu : 0; }
+
+emfa_status emfa_ctx_set_cu_share(emfa_ctx *ctx, int cus) {
+    EMFA_REQUIRE(ctx && cus >= 0, EMFA_E_INVALID, "emfa_ctx_set_cu_share: bad argument");
+    ctx->cu_share = cus;
+    return EMFA_OK;
+}
 
 // ---------------------------------------------------------------- mesh
 emfa_status emfa_mesh_upload(emfa_ctx *ctx, const double *xyz, int nv, const int *tri, int nt, const int *rwg, int n,

@@ -357,11 +357,10 @@ static bool gemm_args_ok(const GemmArgs &g) {
 emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag) {
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f32k::BN - 1) / f32k::BN, (g.M + f32k::BM - 1) / f32k::BM);
-    static int variant = -1;
-    if (variant < 0) {
+    static const int variant = [] {
         const char *e = getenv("EMFA_GEMM_VARIANT");
-        variant = e ? atoi(e) : 0;
-    }
+        return e ? atoi(e) : 0;
+    }();
     const bool vec4 = (g.lda % 4) == 0 && (g.ldb % 4) == 0 && (g.Kr % 4) == 0 &&
                       ((uintptr_t)g.A % 16) == 0 && ((uintptr_t)g.Bt % 16) == 0;
     // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart

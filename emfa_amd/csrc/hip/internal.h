@@ -53,6 +53,7 @@ struct emfa_ctx {
     hipStream_t aux = nullptr;  // second stream (LU look-ahead), default priority
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cu = 0;
+    int cu_share = 0;  // cap on the cooperative panel's workgroups (0: num_cu)
     size_t bytes = 0;
     bool prof = false;
     std::map<std::string, emfa::ProfFamily> fam;

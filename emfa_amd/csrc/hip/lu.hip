@@ -238,6 +238,271 @@ __global__ __launch_bounds__(256) void k_panel(cx<R> *__restrict__ A, long long 
     }
 }
 
+// --------------------------------------------------------------------------
+// Panel with a packet exchange (default).  Same arithmetic and pivot choice
+// as k_panel; what changes is how workgroups agree on the pivot of a column:
+// instead of an arrival counter plus separate value/row reads, every
+// workgroup writes one 64-bit packet per column into its own slot (double
+// buffered by column parity) and every workgroup polls the G packets, so the
+// barrier and the candidate values travel together.
+//   complex FP32: packet = value bits | row << 32 | (epoch & 0xFFF) << 52;
+//     candidate rows are 64-bit words {float bits, epoch}, two per element,
+//     so a reader can tell a fresh word from a stale one and the writer never
+//     waits for its stores.
+//   complex FP64: packet = row | epoch << 32; the value and the row are
+//     written first and ordered before the packet by s_waitcnt.
+// The epoch of column jc is jc + 1; the slots are zeroed before each
+// factorisation.  Rows live in LDS column-major (stride RB + 1) and one
+// thread owns a row in the update, which also yields the next column's local
+// candidate, so a column costs two workgroup barriers.
+// --------------------------------------------------------------------------
+constexpr int kPanelMaxG = 256;
+
+__device__ __forceinline__ void st64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld64(const unsigned long long *p) {
+    return __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long tag32(float x, unsigned ep) {
+    return (unsigned long long)__float_as_uint(x) | ((unsigned long long)ep << 32);
+}
+
+// FP32 self-validating pair load; false if the bounded spin ran out
+__device__ __forceinline__ bool ld_tagged(const unsigned long long *p, unsigned ep, cx<float> *out, bool active,
+                                          int *flags) {
+    unsigned spins = 0;
+    bool ok = !active;
+    for (;;) {
+        if (!ok) {
+            unsigned long long a = ld64(p), b = ld64(p + 1);
+            if ((unsigned)(a >> 32) == ep && (unsigned)(b >> 32) == ep) {
+                *out = {__uint_as_float((unsigned)a), __uint_as_float((unsigned)b)};
+                ok = true;
+            }
+        }
+        if (__ballot(!ok) == 0) return true;
+        if (++spins > kSpinLimit) {
+            __hip_atomic_store(&flags[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_x(cx<R> *__restrict__ A, long long lda, int j0, int M, int w, int RB,
+                                                 int *__restrict__ ipiv, unsigned long long *pk,
+                                                 unsigned long long *rw, unsigned long long *cw, R *sval,
+                                                 int *flags) {
+    constexpr bool kF32 = sizeof(R) == 4;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<R> *cols = (cx<R> *)smem_raw;  // [w][RBp]
+    const int RBp = RB + 1;
+    __shared__ cx<R> urow[kPanelMaxW];
+    __shared__ R sv[4];
+    __shared__ int sr[4];
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rbeg = j0 + g * RB;
+    const int rend = min(j0 + M, rbeg + RB);
+    const int nr = max(0, rend - rbeg);
+    for (int e = tid; e < nr * w; e += 256) {
+        int rr = e / w, cc = e - rr * w;
+        cols[cc * RBp + rr] = A[(size_t)(rbeg + rr) * lda + j0 + cc];
+    }
+    __syncthreads();
+    R bv = R(-1);
+    int br = -1;
+    for (int i = tid; i < nr; i += 256) {
+        R v = cabs1(cols[i]);
+        if (better(v, rbeg + i, bv, br)) {
+            bv = v;
+            br = rbeg + i;
+        }
+    }
+    bool dead = false;  // a bounded wait ran out: finish without exchanging
+    for (int c = 0; c < w; ++c) {
+        const int jc = j0 + c, par = c & 1;
+        const unsigned ep = (unsigned)jc + 1u;
+        wave_argmax(bv, br);
+        if (lane == 0) {
+            sv[wave] = bv;
+            sr[wave] = br;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            bv = sv[0];
+            br = sr[0];
+            for (int q = 1; q < 4; ++q)
+                if (better(sv[q], sr[q], bv, br)) {
+                    bv = sv[q];
+                    br = sr[q];
+                }
+            const bool own_jc = jc >= rbeg && jc < rend;
+            // publish this workgroup's candidate
+            if (kF32) {
+                if (br >= 0 && lane < w) {
+                    cx<R> x = cols[lane * RBp + (br - rbeg)];
+                    unsigned long long *d = rw + ((size_t)par * G + g) * (2 * kPanelMaxW) + 2 * lane;
+                    st64(d, tag32((float)x.re, ep));
+                    st64(d + 1, tag32((float)x.im, ep));
+                }
+                if (own_jc && lane < w) {
+                    cx<R> x = cols[lane * RBp + (jc - rbeg)];
+                    unsigned long long *d = cw + (size_t)par * (2 * kPanelMaxW) + 2 * lane;
+                    st64(d, tag32((float)x.re, ep));
+                    st64(d + 1, tag32((float)x.im, ep));
+                }
+                if (lane == 0) {
+                    unsigned idx = br >= 0 ? (unsigned)br : 0xFFFFFu;
+                    float v = br >= 0 ? (float)bv : 0.f;
+                    st64(&pk[par * kPanelMaxG + g], (unsigned long long)__float_as_uint(v) |
+                                                        ((unsigned long long)idx << 32) |
+                                                        ((unsigned long long)(ep & 0xFFFu) << 52));
+                }
+            } else {
+                cx<R> *rwc = (cx<R> *)rw, *cwc = (cx<R> *)cw;
+                if (br >= 0 && lane < w)
+                    st_co(&rwc[((size_t)par * G + g) * kPanelMaxW + lane], cols[lane * RBp + (br - rbeg)]);
+                if (own_jc && lane < w) st_co(&cwc[(size_t)par * kPanelMaxW + lane], cols[lane * RBp + (jc - rbeg)]);
+                if (lane == 0) st_co(&sval[par * kPanelMaxG + g], bv);
+                __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                if (lane == 0)
+                    st64(&pk[par * kPanelMaxG + g],
+                         (unsigned long long)(br >= 0 ? (unsigned)br : 0xFFFFFFFFu) | ((unsigned long long)ep << 32));
+            }
+            // gather the G packets (up to four per lane)
+            unsigned long long pkv[4] = {0, 0, 0, 0};
+            int okm = 0;
+            unsigned spins = 0;
+            while (!dead) {
+                bool all = true;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int q = lane + 64 * k;
+                    if (q < G && !((okm >> k) & 1)) {
+                        unsigned long long v = ld64(&pk[par * kPanelMaxG + q]);
+                        bool ok = kF32 ? (unsigned)(v >> 52) == (ep & 0xFFFu) : (unsigned)(v >> 32) == ep;
+                        if (ok) {
+                            pkv[k] = v;
+                            okm |= 1 << k;
+                        } else {
+                            all = false;
+                        }
+                    }
+                }
+                if (__ballot(!all) == 0) break;
+                if (++spins > kSpinLimit ||
+                    __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0) {
+                    __hip_atomic_store(&flags[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    dead = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            R wv = R(-1);
+            int wr = -1;
+            if (!dead) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int q = lane + 64 * k;
+                    if (q < G) {
+                        R v;
+                        int r;
+                        if (kF32) {
+                            unsigned idx = (unsigned)(pkv[k] >> 32) & 0xFFFFFu;
+                            r = idx == 0xFFFFFu ? -1 : (int)idx;
+                            v = (R)__uint_as_float((unsigned)pkv[k]);
+                        } else {
+                            unsigned idx = (unsigned)pkv[k];
+                            r = idx == 0xFFFFFFFFu ? -1 : (int)idx;
+                            v = r >= 0 ? ld_co(&sval[par * kPanelMaxG + q]) : R(-1);
+                        }
+                        if (better(v, r, wv, wr)) {
+                            wv = v;
+                            wr = r;
+                        }
+                    }
+                }
+            }
+            wave_argmax(wv, wr);
+            const bool singular = !(wv > R(0));
+            int p = singular ? jc : wr;
+            if (p < j0 || p >= j0 + M) p = jc;  // only after a failed wait
+            const int wg = (p - j0) / RB;
+            cx<R> u = {R(0), R(0)}, o = {R(0), R(0)};
+            if (!dead) {
+                if (kF32) {
+                    cx<float> t;
+                    bool ok = ld_tagged(cw + (size_t)par * (2 * kPanelMaxW) + 2 * lane, ep, &t, lane < w, flags);
+                    o = {(R)t.re, (R)t.im};
+                    if (ok && p != jc) {
+                        ok = ld_tagged(rw + ((size_t)par * G + wg) * (2 * kPanelMaxW) + 2 * lane, ep, &t, lane < w,
+                                       flags);
+                        u = {(R)t.re, (R)t.im};
+                    } else {
+                        u = o;
+                    }
+                    if (!ok) dead = true;
+                } else if (lane < w) {
+                    o = ld_co(&((cx<R> *)cw)[(size_t)par * kPanelMaxW + lane]);
+                    u = p != jc ? ld_co(&((cx<R> *)rw)[((size_t)par * G + wg) * kPanelMaxW + lane]) : o;
+                }
+            }
+            if (lane < w) {
+                urow[lane] = u;
+                if (jc >= rbeg && jc < rend) cols[lane * RBp + (jc - rbeg)] = u;
+                if (p != jc && p >= rbeg && p < rend) cols[lane * RBp + (p - rbeg)] = o;
+            }
+            if (lane == 0 && g == 0) {
+                ipiv[jc] = p;
+                if (singular && !dead && flags[1] == 0) flags[1] = jc + 1;
+            }
+        }
+        __syncthreads();
+        // scale column c, rank-1 update, next column's local candidate: one thread per row
+        const cx<R> piv = urow[c];
+        const R den = piv.re * piv.re + piv.im * piv.im;
+        const bool scale = den > R(0);
+        cx<R> inv = {R(0), R(0)};
+        if (scale) inv = {piv.re / den, -piv.im / den};
+        const int first = max(jc + 1, rbeg) - rbeg;
+        // tpr threads share a row (a power of two <= 64, so a row's threads sit
+        // in one wave and read the multiplier before its owner writes it back)
+        int tpr = 1;
+        while (tpr < 64 && tpr * 2 * (nr - first) <= 256) tpr *= 2;
+        const int jr = tid & (tpr - 1);
+        bv = R(-1);
+        br = -1;
+        for (int i = first + tid / tpr; i < nr; i += 256 / tpr) {
+            cx<R> l = cols[c * RBp + i];
+            if (scale) l = cmul(l, inv);
+            for (int cc = c + 1 + jr; cc < w; cc += tpr) {
+                const cx<R> uu = urow[cc];
+                cx<R> &a = cols[cc * RBp + i];
+                a.re -= l.re * uu.re - l.im * uu.im;
+                a.im -= l.re * uu.im + l.im * uu.re;
+            }
+            if (jr == 0) {
+                if (scale) cols[c * RBp + i] = l;
+                if (c + 1 < w) {
+                    R v = cabs1(cols[(c + 1) * RBp + i]);
+                    if (better(v, rbeg + i, bv, br)) {
+                        bv = v;
+                        br = rbeg + i;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nr * w; e += 256) {
+        int rr = e / w, cc = e - rr * w;
+        A[(size_t)(rbeg + rr) * lda + j0 + cc] = cols[cc * RBp + rr];
+    }
+}
+
 // Row interchanges k0..k1-1 (sequential, in order) on columns [c0,c1) and [c2,c3).
 template <typename R>
 __global__ __launch_bounds__(256) void k_laswp(cx<R> *__restrict__ A, long long lda, const int *__restrict__ ipiv,
@@ -623,7 +888,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
 
     // panel geometry: co-resident workgroups, rows per workgroup in LDS
     const size_t lds_cap = 150 * 1024;
-    int G_max = std::max(1, ctx->num_cu);
+    int G_max = std::max(1, ctx->cu_share > 0 ? std::min(ctx->cu_share, ctx->num_cu) : ctx->num_cu);
     if (const char *s = getenv("EMFA_PANEL_G")) G_max = std::max(1, std::min(G_max, atoi(s)));
     int panel_rows_min = 16;
     if (const char *s = getenv("EMFA_PANEL_ROWS_MIN")) panel_rows_min = std::max(1, atoi(s));
@@ -635,7 +900,11 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     size_t bar_b = 256;
     size_t bt_b = (size_t)2 * N * 2 * nb * sizeof(R);
     size_t perm_b = PermWs<R>::bytes(N);
-    size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + 2 * bt_b + perm_b + 4096;
+    // packet-exchange slots: packets, candidate rows, current row (zeroed below), values
+    size_t x_pk_b = 2ull * kPanelMaxG * 8, x_rw_b = 2ull * kPanelMaxG * kPanelMaxW * 16,
+           x_cw_b = 2ull * kPanelMaxW * 16, x_val_b = 2ull * kPanelMaxG * sizeof(R);
+    size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + 2 * bt_b + perm_b + x_pk_b + x_rw_b +
+                   x_cw_b + x_val_b + 8192;
     void *scr = nullptr;
     emfa_status st = ctx_scratch(ctx, total, &scr);
     if (st != EMFA_OK) return st;
@@ -654,13 +923,27 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     R *Bt_out = (R *)take(bt_b);  // U rows of the outer panel (read by the trailing update)
     PermWs<R> pws;
     pws.carve(take(perm_b));
+    unsigned long long *x_pk = (unsigned long long *)take(x_pk_b + x_rw_b + x_cw_b);
+    unsigned long long *x_rw = x_pk + x_pk_b / 8, *x_cw = x_rw + x_rw_b / 8;
+    R *x_val = (R *)take(x_val_b);
+    {
+        const int nw = (int)((x_pk_b + x_rw_b + x_cw_b) / 4);
+        k_zero_u32<<<(nw + 255) / 256, 256, 0, ctx->stream>>>((unsigned *)x_pk, nw);
+    }
     k_zero_u32<<<1, 64, 0, ctx->stream>>>(bar, 64);
     EMFA_HIP_CHECK(hipGetLastError());
     k_zero_u32<<<1, 64, 0, ctx->stream>>>((unsigned *)ctx->flags, 16);
     EMFA_HIP_CHECK(hipGetLastError());
     unsigned bar_base = 0;
-    bool coop = true, lookahead = true;
+    // Cooperative launches of concurrent contexts are serialised by the
+    // runtime, so contexts sharing the device (cu_share > 0) launch the
+    // panel normally; its workgroups still become co-resident because the
+    // shares sum to at most the CU count and nothing else waits on them.
+    bool coop = ctx->cu_share <= 0, lookahead = true, xchg = true;
     if (const char *s = getenv("EMFA_PANEL_COOP")) coop = atoi(s) != 0;
+    if (const char *s = getenv("EMFA_PANEL_X")) xchg = atoi(s) != 0;
+    EMFA_REQUIRE(G_max <= kPanelMaxG, EMFA_E_INVALID, "panel grid %d exceeds %d", G_max, kPanelMaxG);
+    if (sizeof(R) == 4 && N >= 0xFFFFF) xchg = false;  // the FP32 packet holds a 20-bit row index
     if (const char *s = getenv("EMFA_LU_LOOKAHEAD")) lookahead = atoi(s) != 0;
 
     // Factor the outer panel [j0, j0+jb): inner panels of width w with the
@@ -672,13 +955,26 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             int w = std::min(plan.w, j0 + jb - js);
             int G = std::min(G_max, std::max(1, (M + panel_rows_min - 1) / panel_rows_min));
             int RB = (M + G - 1) / G;
-            while ((size_t)RB * (w + 1) * sizeof(cx<R>) > lds_cap && w > 8) w /= 2;
-            EMFA_REQUIRE((size_t)RB * (w + 1) * sizeof(cx<R>) <= lds_cap, EMFA_E_INVALID,
+            while ((size_t)(RB + 1) * (w + 1) * sizeof(cx<R>) > lds_cap && w > 8) w /= 2;
+            EMFA_REQUIRE((size_t)(RB + 1) * (w + 1) * sizeof(cx<R>) <= lds_cap, EMFA_E_INVALID,
                          "LU panel does not fit LDS (N=%d)", N);
             size_t lds = (size_t)RB * (w + 1) * sizeof(cx<R>);
+            size_t lds_x = (size_t)(RB + 1) * w * sizeof(cx<R>);
             {
                 ProfScope ps(ctx, "lu_panel", 0);
-                if (coop) {
+                if (xchg) {
+                    if (coop) {
+                        void *args[] = {(void *)&A,    (void *)&lda,  (void *)&js,    (void *)&M,
+                                        (void *)&w,    (void *)&RB,   (void *)&ipiv,  (void *)&x_pk,
+                                        (void *)&x_rw, (void *)&x_cw, (void *)&x_val, (void *)&ctx->flags};
+                        EMFA_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_panel_x<R>, dim3(G), dim3(256),
+                                                                  args, lds_x, ctx->stream));
+                    } else {
+                        k_panel_x<R><<<G, 256, lds_x, ctx->stream>>>(A, lda, js, M, w, RB, ipiv, x_pk, x_rw, x_cw,
+                                                                     x_val, ctx->flags);
+                        EMFA_HIP_CHECK(hipGetLastError());
+                    }
+                } else if (coop) {
                     void *args[] = {(void *)&A,        (void *)&lda,      (void *)&js,   (void *)&M,
                                     (void *)&w,        (void *)&RB,       (void *)&ipiv, (void *)&slot_rows,
                                     (void *)&slot_val, (void *)&slot_idx, (void *)&rowc, (void *)&bar,

@@ -1,12 +1,22 @@
-// Small C entry points of the host core, for bindings and for the tests:
-// configuration lookup with the loader's semantics, the NAS reader, and a
-// whole CONFIG.yml run in-process.
+// C entry points of the host core (declared in include/emfa_core.h):
+// configuration lookup with the loader's semantics, the NAS reader, a whole
+// CONFIG.yml run in-process, and the solve server.
 #include <cstring>
 #include <iostream>
+#include <map>
+#include <mutex>
 #include <string>
 
+#include "../../../include/emfa_core.h"
 #include "core.h"
 #include "em.h"
+#include "serve.h"
+
+struct emfa_server {
+    serve::Server *srv = nullptr;
+    std::mutex mu;
+    std::map<long long, serve::Result> taken;  // waited for, not yet fetched
+};
 
 namespace {
 thread_local std::string g_core_err;
@@ -76,6 +86,128 @@ int emfa_core_run(const char *config_path) {
     } catch (const std::exception &e) {
         return fail(6, e.what());
     }
+}
+
+// ---------------------------------------------------------------- server
+int emfa_serve_create(int device, int workers, size_t hbm_budget, int keep_current, emfa_server **out) {
+    if (!out || workers < 1 || hbm_budget == 0) return fail(1, "emfa_serve_create: bad argument");
+    *out = nullptr;
+    try {
+        auto *s = new emfa_server;
+        try {
+            s->srv = new serve::Server(device, workers, hbm_budget, keep_current != 0);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(5, e.what());
+    }
+}
+
+int emfa_serve_destroy(emfa_server *s) {
+    if (!s) return 0;
+    delete s->srv;
+    delete s;
+    return 0;
+}
+
+long long emfa_serve_submit(emfa_server *s, const emfa_serve_request *r) {
+    if (!s || !r || r->nrhs < 1 || !r->angles) return fail(-1, "emfa_serve_submit: bad argument");
+    if (r->mesh_kind != EMFA_MESH_ICOSPHERE && r->mesh_kind != EMFA_MESH_PLATE)
+        return fail(-1, "emfa_serve_submit: unknown mesh kind");
+    if (r->formulation != EMFA_EFIE && r->formulation != EMFA_CFIE)
+        return fail(-1, "emfa_serve_submit: formulation must be EFIE or CFIE");
+    if (r->precision != EMFA_F32 && r->precision != EMFA_F64) return fail(-1, "emfa_serve_submit: bad precision");
+    if (r->quad != 3 && r->quad != 7) return fail(-1, "emfa_serve_submit: quadrature must be 3 or 7");
+    if (!(r->freq > 0)) return fail(-1, "emfa_serve_submit: frequency must be positive");
+    serve::Request q;
+    q.mesh.kind = r->mesh_kind;
+    q.mesh.subdiv = r->subdiv;
+    q.mesh.ax = r->ax;
+    q.mesh.ay = r->ay;
+    q.mesh.az = r->az;
+    q.formulation = r->formulation;
+    q.alpha = r->alpha;
+    q.freq = r->freq;
+    q.precision = r->precision;
+    q.quad = r->quad;
+    q.angles.assign(r->angles, r->angles + 3 * (size_t)r->nrhs);
+    return s->srv->submit(q);
+}
+
+int emfa_serve_wait(emfa_server *s, long long id, emfa_serve_result *res) {
+    if (!s || !res) return fail(1, "emfa_serve_wait: bad argument");
+    serve::Result r;
+    if (!s->srv->wait(id, &r)) return fail(1, "emfa_serve_wait: no such pending request");
+    res->id = r.id;
+    res->status = r.status;
+    res->unknowns = r.unknowns;
+    res->nrhs = r.nrhs;
+    res->cache_hit = r.cache_hit ? 1 : 0;
+    res->refine_iterations = r.refine_iterations;
+    res->relres = r.relres;
+    res->fp64_fallback = r.fp64_fallback ? 1 : 0;
+    res->queue_ms = r.queue_ms;
+    res->service_ms = r.service_ms;
+    res->latency_ms = r.latency_ms;
+    if (r.status != 0) g_core_err = r.error;
+    std::lock_guard<std::mutex> l(s->mu);
+    s->taken[r.id] = std::move(r);
+    return 0;
+}
+
+int emfa_serve_fetch(emfa_server *s, long long id, double *rcs_db, int rcs_cap, double *current,
+                     long long current_cap) {
+    if (!s) return fail(1, "emfa_serve_fetch: bad argument");
+    std::lock_guard<std::mutex> l(s->mu);
+    auto it = s->taken.find(id);
+    if (it == s->taken.end()) return fail(1, "emfa_serve_fetch: unknown id");
+    const serve::Result &r = it->second;
+    if (rcs_db && rcs_cap < (int)r.rcs_db.size()) return fail(1, "emfa_serve_fetch: rcs buffer too small");
+    if (current && current_cap < (long long)r.current.size())
+        return fail(1, "emfa_serve_fetch: current buffer too small");
+    if (rcs_db) std::memcpy(rcs_db, r.rcs_db.data(), sizeof(double) * r.rcs_db.size());
+    if (current) std::memcpy(current, r.current.data(), sizeof(double) * r.current.size());
+    s->taken.erase(it);
+    return 0;
+}
+
+int emfa_serve_drain(emfa_server *s) {
+    if (!s) return fail(1, "emfa_serve_drain: bad argument");
+    s->srv->drain();
+    return 0;
+}
+
+int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out) {
+    if (!s || !out) return fail(1, "emfa_serve_get_stats: bad argument");
+    serve::Stats t = s->srv->stats();
+    out->completed = t.completed;
+    out->failed = t.failed;
+    out->cache_hits = t.cache_hits;
+    out->p50_ms = t.p50_ms;
+    out->p99_ms = t.p99_ms;
+    out->mean_ms = t.mean_ms;
+    out->max_ms = t.max_ms;
+    out->wall_s = t.wall_s;
+    out->solves_per_s = t.solves_per_s;
+    out->rhs_per_s = t.rhs_per_s;
+    out->peak_bytes = t.peak_bytes;
+    return 0;
+}
+
+int emfa_serve_reset_stats(emfa_server *s) {
+    if (!s) return fail(1, "emfa_serve_reset_stats: bad argument");
+    s->srv->reset_stats();
+    return 0;
+}
+
+int emfa_serve_cached_factors(emfa_server *s) { return s ? (int)s->srv->cached_factors() : 0; }
+
+size_t emfa_serve_footprint(int unknowns, int nrhs, int precision, int factor_cached) {
+    return serve::Server::footprint(unknowns, nrhs, precision, factor_cached != 0);
 }
 
 }  // extern "C"

@@ -1,0 +1,449 @@
+#include "serve.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+#include <stdexcept>
+
+namespace serve {
+
+using clock_t_ = std::chrono::steady_clock;
+
+static double ms_between(clock_t_::time_point a, clock_t_::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+std::string MeshSpec::key() const {
+    std::ostringstream o;
+    o.precision(17);
+    o << kind << ':' << subdiv << ':' << ax << ':' << ay << ':' << az;
+    return o.str();
+}
+
+struct Server::Factor {
+    emfa_mesh *mesh = nullptr;
+    emfa_mat *Z = nullptr;
+    emfa_lu *lu = nullptr;
+    int n = 0, prec = EMFA_F64;
+    size_t bytes = 0;
+    double k = 0;
+    ~Factor() {
+        if (lu) emfa_lu_destroy(lu);
+        if (Z) emfa_mat_destroy(Z);
+        if (mesh) emfa_mesh_destroy(mesh);
+    }
+};
+
+struct StatusError : std::runtime_error {
+    int code;
+    StatusError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+static void check(emfa_status st, const char *what) {
+    if (st != EMFA_OK) throw StatusError(st, std::string(what) + ": " + emfa_last_error());
+}
+
+static void build_mesh(const MeshSpec &m, std::vector<double> &xyz, std::vector<int> &tri, std::vector<int> &rwg,
+                       int &n) {
+    int nv = 0, nt = 0;
+    emfa_status st;
+    if (m.kind == PLATE) {
+        st = emfa_geom_plate(m.subdiv, m.ax, nullptr, 0, nullptr, 0, &nv, &nt);
+        if (st) throw StatusError(st, emfa_last_error());
+        xyz.resize(3 * (size_t)nv);
+        tri.resize(3 * (size_t)nt);
+        st = emfa_geom_plate(m.subdiv, m.ax, xyz.data(), nv, tri.data(), nt, &nv, &nt);
+    } else {
+        st = emfa_geom_icosphere(m.subdiv, m.ax, m.ay, m.az, nullptr, 0, nullptr, 0, &nv, &nt);
+        if (st) throw StatusError(st, emfa_last_error());
+        xyz.resize(3 * (size_t)nv);
+        tri.resize(3 * (size_t)nt);
+        st = emfa_geom_icosphere(m.subdiv, m.ax, m.ay, m.az, xyz.data(), nv, tri.data(), nt, &nv, &nt);
+    }
+    if (st) throw StatusError(st, emfa_last_error());
+    int nb = 0;
+    st = emfa_rwg_build(nt, tri.data(), nullptr, 0, &n, &nb);
+    if (st) throw StatusError(st, emfa_last_error());
+    rwg.resize(6 * (size_t)std::max(n, 1));
+    st = emfa_rwg_build(nt, tri.data(), rwg.data(), n, &n, &nb);
+    if (st) throw StatusError(st, emfa_last_error());
+}
+
+size_t Server::footprint(int n, int nrhs, int precision, bool factor_cached) {
+    const size_t e = precision == EMFA_F32 ? 8 : 16;
+    size_t vecs = (size_t)n * nrhs * 16 * 4 + (size_t)n * nrhs * e;  // B, X, R, Bd, D
+    if (factor_cached) return vecs;
+    return 2 * (size_t)n * n * e + vecs + (size_t)n * 64 * 16;  // Z + factor + LU workspace share
+}
+
+Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
+    : device_(device), budget_(hbm_budget), keep_current_(keep_current) {
+    workers = std::max(1, workers);
+    for (int w = 0; w < workers; ++w) {
+        emfa_ctx *c = nullptr;
+        emfa_status st = emfa_ctx_create(device, &c);
+        if (st) {
+            for (auto x : ctxs_) emfa_ctx_destroy(x);
+            throw std::runtime_error(std::string("serve: ") + emfa_last_error());
+        }
+        // concurrent factorisations split the compute units between them
+        if (workers > 1) emfa_ctx_set_cu_share(c, std::max(1, emfa_ctx_cu_count(c) / workers));
+        ctxs_.push_back(c);
+    }
+    for (int w = 0; w < workers; ++w) threads_.emplace_back(&Server::worker, this, w);
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> l(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    cv_mem_.notify_all();
+    for (auto &t : threads_) t.join();
+    cache_.clear();  // factors are released before their contexts
+    lru_.clear();
+    for (auto c : ctxs_) emfa_ctx_destroy(c);
+}
+
+int64_t Server::submit(const Request &r) {
+    std::lock_guard<std::mutex> l(mu_);
+    Job j{next_id_++, r, clock_t_::now()};
+    if (!have_t0_) {
+        t0_ = j.arrival;
+        have_t0_ = true;
+    }
+    queue_.push_back(std::move(j));
+    cv_work_.notify_one();
+    return next_id_ - 1;
+}
+
+bool Server::wait(int64_t id, Result *out) {
+    std::unique_lock<std::mutex> l(mu_);
+    for (;;) {
+        if (id >= 0) {
+            auto it = done_.find(id);
+            if (it != done_.end()) {
+                *out = std::move(it->second);
+                done_.erase(it);
+                return true;
+            }
+            if (id >= next_id_) return false;
+        } else {
+            if (!done_.empty()) {
+                auto it = done_.begin();
+                *out = std::move(it->second);
+                done_.erase(it);
+                return true;
+            }
+            if (queue_.empty() && busy_ == 0) return false;
+        }
+        cv_done_.wait(l);
+    }
+}
+
+void Server::drain() {
+    std::unique_lock<std::mutex> l(mu_);
+    cv_done_.wait(l, [&] { return queue_.empty() && busy_ == 0; });
+}
+
+size_t Server::cached_factors() {
+    std::lock_guard<std::mutex> l(mu_);
+    return cache_.size();
+}
+
+// Admission: reserve `bytes` within the budget, evicting idle cached factors
+// (least recently used first) when needed; waits while other requests hold
+// the memory.  `pinned` is what the caller already holds (reservation and
+// factors in use); false when bytes + pinned can never fit.  Called with mu_
+// locked; returns with it locked.
+bool Server::reserve(size_t bytes, size_t pinned) {
+    if (bytes + pinned > budget_) return false;
+    std::unique_lock<std::mutex> l(mu_, std::adopt_lock);
+    for (;;) {
+        while (reserved_ + cache_bytes_ + bytes > budget_ && !lru_.empty()) {
+            // evict the least recently used factor nobody is using
+            bool evicted = false;
+            for (auto it = lru_.begin(); it != lru_.end(); ++it) {
+                auto c = cache_.find(*it);
+                if (c != cache_.end() && c->second.use_count() == 1) {
+                    cache_bytes_ -= c->second->bytes;
+                    cache_.erase(c);
+                    lru_.erase(it);
+                    evicted = true;
+                    break;
+                }
+            }
+            if (!evicted) break;
+        }
+        if (reserved_ + cache_bytes_ + bytes <= budget_ || stop_) break;
+        cv_mem_.wait(l);
+    }
+    reserved_ += bytes;
+    peak_ = std::max(peak_, reserved_ + cache_bytes_);
+    l.release();
+    return true;
+}
+
+void Server::release(size_t bytes) {
+    reserved_ -= bytes;
+    cv_mem_.notify_all();
+}
+
+std::shared_ptr<Server::Factor> Server::cache_get(const std::string &key) {
+    auto it = cache_.find(key);
+    if (it == cache_.end()) return nullptr;
+    lru_.remove(key);
+    lru_.push_back(key);
+    return it->second;
+}
+
+void Server::worker(int w) {
+    emfa_ctx *ctx = ctxs_[w];
+    for (;;) {
+        Job job;
+        {
+            std::unique_lock<std::mutex> l(mu_);
+            cv_work_.wait(l, [&] { return stop_ || !queue_.empty(); });
+            if (stop_) return;
+            job = std::move(queue_.front());
+            queue_.pop_front();
+            ++busy_;
+        }
+        Result res;
+        res.id = job.id;
+        auto start = clock_t_::now();
+        try {
+            run(ctx, job, res);
+        } catch (const StatusError &e) {
+            res.status = e.code;
+            res.error = e.what();
+        } catch (const std::exception &e) {
+            res.status = EMFA_E_INVALID;
+            res.error = e.what();
+        }
+        auto end = clock_t_::now();
+        res.queue_ms = ms_between(job.arrival, start);
+        res.service_ms = ms_between(start, end);
+        res.latency_ms = ms_between(job.arrival, end);
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            --busy_;
+            t1_ = end;
+            if (res.status == EMFA_OK) {
+                lat_.push_back(res.latency_ms);
+                rhs_done_ += res.nrhs;
+                if (res.cache_hit) ++hits_;
+            } else {
+                ++failed_;
+            }
+            done_[res.id] = std::move(res);
+        }
+        cv_done_.notify_all();
+    }
+}
+
+static std::string factor_key(const Request &r, int prec) {
+    std::ostringstream ks;
+    ks.precision(17);
+    ks << r.mesh.key() << '|' << r.freq << '|' << r.formulation << '|' << r.alpha << '|' << prec << '|' << r.quad;
+    return ks.str();
+}
+
+// The factor of (request, prec) from the cache, or computed and published.
+// A factor another worker is computing right now is waited for, not computed
+// twice.  The request's vectors are reserved here too (`vec_bytes`); the
+// bytes this call adds to the request's reservation are added to *held.
+std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r, int prec, int nrhs, bool *hit,
+                                                size_t *held, size_t pinned) {
+    const std::string key = factor_key(r, prec);
+    std::shared_ptr<Factor> f;
+    {
+        std::unique_lock<std::mutex> l(mu_);
+        for (;;) {
+            f = cache_get(key);
+            if (f || !building_.count(key)) break;
+            cv_build_.wait(l);
+        }
+        if (!f) building_.insert(key);
+    }
+    struct BuildGuard {
+        Server *s;
+        const std::string &key;
+        bool on;
+        ~BuildGuard() {
+            if (!on) return;
+            {
+                std::lock_guard<std::mutex> l(s->mu_);
+                s->building_.erase(key);
+            }
+            s->cv_build_.notify_all();
+        }
+    } bguard{this, key, !f};
+    *hit = (bool)f;
+    std::vector<double> xyz;
+    std::vector<int> tri, rwg;
+    int n = 0;
+    if (f)
+        n = f->n;
+    else
+        build_mesh(r.mesh, xyz, tri, rwg, n);
+    const size_t need = footprint(n, nrhs, prec, (bool)f);
+    {
+        mu_.lock();
+        bool ok = reserve(need, *held + pinned);
+        mu_.unlock();
+        if (!ok) throw StatusError(EMFA_E_OOM, "request footprint exceeds the HBM budget");
+    }
+    *held += need;
+    if (f) return f;
+
+    const double k = 2.0 * 3.14159265358979323846 * r.freq / 299792458.0;
+    auto nf = std::make_shared<Factor>();
+    nf->n = n;
+    nf->prec = prec;
+    nf->k = k;
+    check(emfa_mesh_upload(ctx, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3), rwg.data(), n,
+                           &nf->mesh),
+          "mesh");
+    check(emfa_mat_create(ctx, n, n, prec, &nf->Z), "Z");
+    emfa_fill_params fp{r.formulation, r.quad, k, r.alpha, 0.0, 0.0, 0};
+    check(emfa_fill(ctx, nf->mesh, &fp, nf->Z), "fill");
+    check(emfa_lu_factor(ctx, nf->Z, prec, &nf->lu), "LU");
+    nf->bytes = 2 * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16);
+    // other workers use the factor on their own streams: finish it first
+    check(emfa_ctx_sync(ctx), "sync");
+    {
+        std::lock_guard<std::mutex> l(mu_);
+        cache_[key] = nf;
+        lru_.push_back(key);
+        // the factor's bytes move from this request's reservation to the cache
+        cache_bytes_ += nf->bytes;
+        reserved_ -= nf->bytes;
+        *held -= nf->bytes;
+        building_.erase(key);
+        bguard.on = false;
+    }
+    cv_build_.notify_all();
+    return nf;
+}
+
+void Server::run(emfa_ctx *ctx, const Job &job, Result &res) {
+    const Request &r = job.req;
+    const int nrhs = (int)(r.angles.size() / 3);
+    if (nrhs <= 0) throw StatusError(EMFA_E_INVALID, "request without right-hand sides");
+    const double k = 2.0 * 3.14159265358979323846 * r.freq / 299792458.0;
+
+    struct Guard {
+        Server *s;
+        size_t b = 0;
+        ~Guard() {
+            std::lock_guard<std::mutex> l(s->mu_);
+            s->release(b);
+        }
+    } guard{this};
+    bool hit = false;
+    std::shared_ptr<Factor> f = acquire(ctx, r, r.precision, nrhs, &hit, &guard.b, 0);
+    res.cache_hit = hit;
+    const int n = f->n;
+    res.unknowns = n;
+    res.nrhs = nrhs;
+
+    // excitation (EMSolver/EM.cpp:29-41 per incidence), monostatic directions -k
+    std::vector<double> inc(9 * (size_t)nrhs), dirs(3 * (size_t)nrhs);
+    std::vector<int> cols(nrhs);
+    const double d2r = 3.14159265358979323846 / 180.0;
+    for (int j = 0; j < nrhs; ++j) {
+        double th = r.angles[3 * j] * d2r, ph = r.angles[3 * j + 1] * d2r, po = r.angles[3 * j + 2] * d2r;
+        double kv[3] = {-std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
+        double ev[3] = {-std::cos(po) * std::cos(th) * std::cos(ph) - std::sin(po) * std::sin(ph),
+                        -std::cos(po) * std::cos(th) * std::sin(ph) + std::sin(po) * std::cos(ph),
+                        std::cos(po) * std::sin(th)};
+        double hv[3] = {kv[1] * ev[2] - kv[2] * ev[1], kv[2] * ev[0] - kv[0] * ev[2], kv[0] * ev[1] - kv[1] * ev[0]};
+        for (int c = 0; c < 3; ++c) {
+            inc[9 * j + c] = kv[c];
+            inc[9 * j + 3 + c] = ev[c];
+            inc[9 * j + 6 + c] = hv[c];
+            dirs[3 * j + c] = -kv[c];
+        }
+        cols[j] = j;
+    }
+    emfa_mat *B = nullptr, *X = nullptr;
+    struct MatGuard {
+        emfa_mat **a, **b;
+        ~MatGuard() {
+            if (*a) emfa_mat_destroy(*a);
+            if (*b) emfa_mat_destroy(*b);
+        }
+    } mg{&B, &X};
+    check(emfa_mat_create(ctx, n, nrhs, EMFA_F64, &B), "B");
+    check(emfa_rhs_fill(ctx, f->mesh, r.formulation, k, r.alpha, nrhs, inc.data(), B), "rhs");
+    check(emfa_mat_create(ctx, n, nrhs, EMFA_F64, &X), "X");
+    if (r.precision == EMFA_F64) {
+        check(emfa_mat_copy(X, B), "copy");
+        check(emfa_lu_solve(ctx, f->lu, X), "solve");
+    } else {
+        emfa_refine_report rep{};
+        emfa_status st = emfa_refine(ctx, f->Z, f->lu, B, X, refine_tol_, refine_maxit_, &rep);
+        res.refine_iterations = rep.iterations;
+        res.relres = rep.relres_final;
+        if (st == EMFA_E_NOCONV) {
+            // the FP32 factor is too inaccurate for this matrix: solve with the FP64 factor
+            bool hit64 = false;
+            std::shared_ptr<Factor> f64 = acquire(ctx, r, EMFA_F64, nrhs, &hit64, &guard.b, f->bytes);
+            check(emfa_mat_copy(X, B), "copy");
+            check(emfa_lu_solve(ctx, f64->lu, X), "solve");
+            res.fp64_fallback = true;
+        } else {
+            check(st, "refine");
+        }
+    }
+    res.rcs_db.resize(nrhs);
+    check(emfa_rcs(ctx, f->mesh, k, X, nrhs, dirs.data(), cols.data(), res.rcs_db.data()), "rcs");
+    if (keep_current_) {
+        res.current.resize(2 * (size_t)n * nrhs);
+        check(emfa_mat_download(X, res.current.data()), "download");
+    }
+    check(emfa_ctx_sync(ctx), "sync");
+}
+
+Stats Server::stats() {
+    std::lock_guard<std::mutex> l(mu_);
+    Stats s;
+    s.completed = (int64_t)lat_.size();
+    s.failed = failed_;
+    s.cache_hits = hits_;
+    s.peak_bytes = peak_;
+    if (!lat_.empty()) {
+        std::vector<double> v = lat_;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            double idx = p * (v.size() - 1);
+            size_t lo = (size_t)std::floor(idx), hi = std::min(v.size() - 1, lo + 1);
+            return v[lo] + (idx - lo) * (v[hi] - v[lo]);
+        };
+        s.p50_ms = pct(0.50);
+        s.p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        s.mean_ms = sum / v.size();
+        s.max_ms = v.back();
+        s.wall_s = std::chrono::duration<double>(t1_ - t0_).count();
+        if (s.wall_s > 0) {
+            s.solves_per_s = s.completed / s.wall_s;
+            s.rhs_per_s = rhs_done_ / s.wall_s;
+        }
+    }
+    return s;
+}
+
+void Server::reset_stats() {
+    std::lock_guard<std::mutex> l(mu_);
+    lat_.clear();
+    failed_ = hits_ = rhs_done_ = 0;
+    have_t0_ = false;
+    peak_ = reserved_ + cache_bytes_;
+}
+
+}  // namespace serve

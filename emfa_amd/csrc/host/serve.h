@@ -1,0 +1,131 @@
+// Continuous-batching solve server on one device.
+//
+// Requests (mesh, frequency, formulation, precision, right-hand sides) are
+// queued on arrival; worker threads, each with its own device context and
+// stream, take them as HBM budget and a worker slot free up, so the work of
+// several requests overlaps on the device.  Factors (Z and its LU) stay
+// resident in a cache keyed by (mesh, frequency, formulation, precision,
+// quadrature); a repeat request only fills its excitation, refines/solves and
+// evaluates the far field.  Every request records its queueing and service
+// time; the server reports latency percentiles and throughput.
+#pragma once
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "emfa_hip.h"
+
+namespace serve {
+
+enum MeshKind { ICOSPHERE = 0, PLATE = 1 };
+
+struct MeshSpec {
+    int kind = ICOSPHERE;
+    int subdiv = 2;               // icosphere subdivisions, or plate cells per side
+    double ax = 1, ay = 1, az = 1;  // semi-axes (icosphere) / side length in ax (plate)
+    std::string key() const;
+};
+
+struct Request {
+    MeshSpec mesh;
+    int formulation = EMFA_EFIE;  // EMFA_EFIE or EMFA_CFIE
+    double alpha = 1.0;
+    double freq = 3e8;
+    int precision = EMFA_F64;     // EMFA_F64: FP64 LU; EMFA_F32: FP32 LU + FP64 refinement
+    int quad = 3;
+    std::vector<double> angles;   // nrhs x 3 (theta, phi, pole) degrees; monostatic RCS per column
+};
+
+struct Result {
+    int64_t id = -1;
+    int status = EMFA_OK;
+    std::string error;
+    int unknowns = 0, nrhs = 0;
+    bool cache_hit = false;
+    int refine_iterations = 0;
+    double relres = 0;
+    bool fp64_fallback = false;  // refinement did not converge; solved with the FP64 factor
+    double queue_ms = 0, service_ms = 0, latency_ms = 0;
+    std::vector<double> rcs_db;   // monostatic RCS per right-hand side
+    std::vector<double> current;  // optional: solution (interleaved complex, n x nrhs)
+};
+
+struct Stats {
+    int64_t completed = 0, failed = 0, cache_hits = 0;
+    double p50_ms = 0, p99_ms = 0, mean_ms = 0, max_ms = 0;
+    double wall_s = 0;  // first arrival to last completion
+    double solves_per_s = 0, rhs_per_s = 0;
+    size_t peak_bytes = 0;
+};
+
+class Server {
+  public:
+    Server(int device, int workers, size_t hbm_budget, bool keep_current = false);
+    ~Server();
+    Server(const Server &) = delete;
+    Server &operator=(const Server &) = delete;
+
+    int64_t submit(const Request &r);
+    // blocks until request `id` is done (id < 0: any); false if none pending
+    bool wait(int64_t id, Result *out);
+    void drain();  // wait until the queue is empty and all workers are idle
+    Stats stats();
+    void reset_stats();
+    size_t cached_factors();
+
+    // footprint model used for admission (bytes a request needs while it runs)
+    static size_t footprint(int n, int nrhs, int precision, bool factor_cached);
+
+  private:
+    struct Factor;
+    struct Job {
+        int64_t id;
+        Request req;
+        std::chrono::steady_clock::time_point arrival;
+    };
+    void worker(int w);
+    void run(emfa_ctx *ctx, const Job &job, Result &res);
+    std::shared_ptr<Factor> cache_get(const std::string &key);
+    std::shared_ptr<Factor> acquire(emfa_ctx *ctx, const Request &r, int prec, int nrhs, bool *hit, size_t *held,
+                                    size_t pinned);
+    bool reserve(size_t bytes, size_t pinned);  // admission; false when the request can never fit
+    void release(size_t bytes);
+
+    int device_;
+    size_t budget_;
+    double refine_tol_ = 1e-10;
+    int refine_maxit_ = 8;
+    bool keep_current_;
+    std::vector<std::thread> threads_;
+    std::vector<emfa_ctx *> ctxs_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_, cv_mem_, cv_build_;
+    std::deque<Job> queue_;
+    std::map<int64_t, Result> done_;
+    int64_t next_id_ = 0;
+    int busy_ = 0;
+    bool stop_ = false;
+    size_t reserved_ = 0, peak_ = 0;
+    // factor cache (LRU)
+    std::map<std::string, std::shared_ptr<Factor>> cache_;
+    std::list<std::string> lru_;
+    size_t cache_bytes_ = 0;
+    std::set<std::string> building_;  // factors being computed by some worker
+    // stats
+    std::vector<double> lat_;
+    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0;
+    bool have_t0_ = false;
+    std::chrono::steady_clock::time_point t0_, t1_;
+};
+
+}  // namespace serve

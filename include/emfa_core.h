@@ -1,0 +1,101 @@
+/* C ABI of the host core (libemfa_core.so): the configuration loader, the
+ * NAS mesh reader, a whole CONFIG.yml run, and the continuous-batching solve
+ * server.  Plain C types only.  Errors: a non-zero return, with the message
+ * in emfa_core_last_error() (per thread). */
+#ifndef EMFA_CORE_H
+#define EMFA_CORE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *emfa_core_last_error(void);
+
+/* Value of `key` (with its colon, e.g. "freq:") as ConfigLoader stores it
+ * (EMComponent/ConfigLoader.cpp:23-50 parse rules).  `src` is a file path,
+ * or the text itself when is_text != 0.  Returns 0, 1 (missing key) or 2. */
+int emfa_core_config_get(const char *src, int is_text, const char *key, char *out, int cap);
+
+/* Mesh::loadMeshFile for .nas files (EMComponent/Mesh.cpp:261-309): 0-based
+ * triangles, vertex coordinates as the reader stores them.  With null
+ * buffers only nv/nt are returned.  Returns 0, 1 (buffers too small), 3. */
+int emfa_core_read_mesh(const char *path, double *xyz, int nv_cap, int *tri, int nt_cap, int *nv, int *nt);
+
+/* solver::loadSolver(config_path) (EMSolver/Solver.cpp:45-95): reads the
+ * config, runs the named solver, writes its result files.  Returns 0 or the
+ * launcher's exit codes: 2 config, 3 file, 4 invalid argument, 5 runtime. */
+int emfa_core_run(const char *config_path);
+
+/* ---------------------------------------------------------------------
+ * Solve server.  No counterpart in the reference (it runs one CONFIG.yml
+ * per process).  Requests are queued on submit and taken by worker
+ * threads, each with its own device context; a request is admitted when
+ * its device footprint fits the HBM budget beside the resident factors,
+ * idle factors are evicted least-recently-used first, and a request whose
+ * (mesh, frequency, formulation, precision, quadrature) factor is resident
+ * only fills its excitation, solves and evaluates the far field.  Results
+ * are bitwise the same whatever else runs: every kernel is deterministic
+ * and a request runs on one context's streams only. */
+typedef struct emfa_server emfa_server;
+
+enum { EMFA_MESH_ICOSPHERE = 0, EMFA_MESH_PLATE = 1 };
+
+typedef struct {
+    int mesh_kind;      /* EMFA_MESH_ICOSPHERE (ellipsoid) or EMFA_MESH_PLATE */
+    int subdiv;         /* icosphere subdivisions, or plate cells per side */
+    double ax, ay, az;  /* semi-axes (m); plate: side length in ax */
+    int formulation;    /* EMFA_EFIE or EMFA_CFIE */
+    double alpha;       /* CFIE combination factor */
+    double freq;        /* Hz */
+    int precision;      /* EMFA_F64: FP64 LU; EMFA_F32: FP32 fill + LU, FP64 refinement */
+    int quad;           /* 3 or 7 */
+    int nrhs;           /* monostatic incidences */
+    const double *angles; /* nrhs x (theta, phi, pole), degrees */
+} emfa_serve_request;
+
+typedef struct {
+    long long id;
+    int status;         /* EMFA_OK or an emfa_status code */
+    int unknowns, nrhs;
+    int cache_hit;
+    int refine_iterations;
+    double relres;      /* final relative residual of the refinement (FP32 requests) */
+    int fp64_fallback;  /* refinement did not converge: solved with the FP64 factor */
+    double queue_ms, service_ms, latency_ms;
+} emfa_serve_result;
+
+typedef struct {
+    long long completed, failed, cache_hits;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* arrival-to-completion latency */
+    double wall_s;                          /* first arrival to last completion */
+    double solves_per_s, rhs_per_s;
+    size_t peak_bytes;                      /* admitted footprint + resident factors */
+} emfa_serve_stats;
+
+/* workers >= 1 contexts on `device`; hbm_budget in bytes; keep_current != 0
+ * keeps each request's solution for emfa_serve_fetch. */
+int emfa_serve_create(int device, int workers, size_t hbm_budget, int keep_current, emfa_server **out);
+int emfa_serve_destroy(emfa_server *s);
+/* Returns the request id (>= 0), or -1 with the message set. */
+long long emfa_serve_submit(emfa_server *s, const emfa_serve_request *r);
+/* Blocks until request `id` (id < 0: any) is done and fills `res`.
+ * Returns 0, or 1 when no such request is pending. */
+int emfa_serve_wait(emfa_server *s, long long id, emfa_serve_result *res);
+/* Copies a waited-for result's monostatic RCS (dB, nrhs values) and, when
+ * kept, its solution (interleaved complex, unknowns x nrhs, row-major) and
+ * releases it.  Either buffer may be null.  Returns 0, or 1 (unknown id or
+ * buffer too small). */
+int emfa_serve_fetch(emfa_server *s, long long id, double *rcs_db, int rcs_cap, double *current, long long current_cap);
+int emfa_serve_drain(emfa_server *s);
+int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out);
+int emfa_serve_reset_stats(emfa_server *s);
+int emfa_serve_cached_factors(emfa_server *s);
+/* Footprint model used for admission, in bytes. */
+size_t emfa_serve_footprint(int unknowns, int nrhs, int precision, int factor_cached);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -57,6 +57,14 @@ emfa_status emfa_ctx_sync(emfa_ctx *ctx);
 void *emfa_ctx_stream(emfa_ctx *ctx);
 /* bytes currently allocated by this context's handles */
 size_t emfa_ctx_bytes_in_use(const emfa_ctx *ctx);
+/* number of compute units of the context's device */
+int emfa_ctx_cu_count(const emfa_ctx *ctx);
+/* Caps the workgroups of this context's cooperative LU panel at `cus`
+ * (0: all compute units).  Contexts that factor concurrently on one device
+ * keep the sum of their caps within emfa_ctx_cu_count so every panel's
+ * workgroups can be resident together.  No counterpart in the reference
+ * (it has no device). */
+emfa_status emfa_ctx_set_cu_share(emfa_ctx *ctx, int cus);
 
 /* ---- host-side geometry (no device work) ---------------------------- */
 /* Synthetic meshes of the workloads: icosahedron subdivided `subdiv` times

@@ -36,10 +36,24 @@ def test_hip_library_exports_every_declared_symbol():
     assert declared <= exported
 
 
-def test_core_library_exports():
-    L = ctypes.CDLL(CORE)
-    for name in ["emfa_core_last_error", "emfa_core_config_get", "emfa_core_read_mesh", "emfa_core_run"]:
-        assert hasattr(L, name)
+def test_core_library_exports_every_declared_symbol():
+    from emfa_amd import serve
+    L = serve.core()
+    declared = _declared("emfa_core.h")
+    assert declared == set(serve.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", CORE], stdout=subprocess.PIPE, text=True).stdout
+    assert declared <= set(re.findall(r" T (emfa_[a-z0-9_]+)", nm))
+
+
+def test_server_footprint_model():
+    from emfa_amd import serve
+    n, m = 1920, 16
+    cached = serve.footprint(n, m, hip.F32, True)
+    assert serve.footprint(n, m, hip.F32) - cached >= 2 * n * n * 8  # Z and its factor
+    assert serve.footprint(n, m, hip.F64) - serve.footprint(n, m, hip.F64, True) >= 2 * n * n * 16
+    assert serve.unknowns(serve.PLATE, 14) == 560 and serve.unknowns(serve.ICOSPHERE, 3) == 1920
 
 
 def _has_gpu():
@@ -195,3 +209,29 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the behaviour without a device")
+def test_server_fails_loudly_without_device():
+    from emfa_amd import serve
+    with pytest.raises(hip.EmfaError):
+        serve.Server(workers=2)
+
+
+def test_workload_generators_are_deterministic_and_in_range():
+    from emfa_amd import serve, workload
+    a = workload.mixed_stream(200)
+    b = workload.mixed_stream(200)
+    assert all(x["subdiv"] == y["subdiv"] and x["freq"] == y["freq"] and np.array_equal(x["angles"], y["angles"])
+               for x, y in zip(a, b))
+    ns = [workload.unknowns(r) for r in a]
+    assert min(ns) >= 500 and max(ns) <= 15000
+    assert all(r["formulation"] == hip.EFIE for r in a if r["kind"] == serve.PLATE)
+    assert all(1 <= len(r["angles"]) <= 64 for r in a)
+    keys = [(r["kind"], r["subdiv"], r["axes"], r["freq"], r["formulation"]) for r in a]
+    repeats = len(keys) - len(set(keys))
+    assert 0.2 * len(a) <= repeats <= 0.4 * len(a)
+    c = workload.batch_problems()
+    assert len(c) == 256 and {r["subdiv"] for r in c} == {2, 3, 4}
+    assert all(0.3 <= x <= 1.2 for r in c for x in r["axes"])
+    assert all(100e6 <= r["freq"] <= 600e6 and r["formulation"] == hip.EFIE for r in c)
