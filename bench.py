@@ -39,6 +39,7 @@ CONFIGS = {
     "c1": ("EFIE", 3, 0.5, 3e8, 1.0, 7, np.array([0.0])),
     "s4": ("CFIE", 4, 1.0, 6e8, 0.5, 3, np.arange(0.0, 181.0, 1.0)),
     "c3": None,  # 256 uneven EFIE ellipsoid problems (emfa_amd.workload.batch_problems)
+    "c5": None,  # MLFMA CFIE, icosphere s=6 r=1 m at 1.5 GHz, GMRES(30) to 1e-4
     "c4": None,  # mixed serving stream (emfa_amd.workload.mixed_stream)
 }
 
@@ -162,6 +163,10 @@ def main():
         if a.steps is None:
             a.steps = 64
         return main_stream(a)
+    if a.config == "c5":
+        if a.steps is None:
+            a.steps = 2
+        return main_mlfma(a)
     if a.steps is None:
         a.steps = 3
 
@@ -273,6 +278,89 @@ def main():
         out["mixed_stream"] = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers,
                                          int(a.budget_gb * 2**30))
     print(json.dumps(out))
+
+
+def main_mlfma(a, subdiv=6, radius=1.0, freq=1.5e9, alpha=0.5, threshold=0.25, tol=1e-4, restart=30):
+    """c5: one step = MLFMA setup (tree, tables, near matrix, patterns) +
+    CFIE excitation + GMRES(30) to 1e-4 + bistatic RCS at 361 angles."""
+    from emfa_amd import hip
+    xyz, tri = hip.icosphere(subdiv, radius)
+    rwg, _ = hip.rwg_build(tri)
+    n = rwg.shape[0]
+    k = hip.wavenumber(freq)
+    inc = hip.inc_table([(0.0, 0.0, 0.0)])
+    th = np.radians(np.arange(0.0, 361.0, 1.0))
+    dirs = np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+    ctx = hip.Context(0)
+    mesh = hip.Mesh(ctx, xyz, tri, rwg)
+    B = hip.Mat(ctx, n, 1, hip.F64)
+    X = hip.Mat(ctx, n, 1, hip.F64)
+    Z0 = hip.Mat(ctx, n, 1, hip.F64)
+    out = {}
+
+    def step():
+        t0 = time.perf_counter()
+        op = hip.MLFMA(ctx, mesh, k, alpha, threshold)
+        ctx.sync()
+        t1 = time.perf_counter()
+        hip.rhs_fill(ctx, mesh, B, hip.CFIE, k, alpha, inc)
+        X.copy_from(Z0)
+        rep, ok = op.gmres(B, X, m=restart, tol=tol, max_restart=40)
+        r = hip.rcs(ctx, mesh, k, X, dirs, np.zeros(len(th), dtype=np.int32))
+        ctx.sync()
+        t2 = time.perf_counter()
+        out.update(setup_s=t1 - t0, solve_s=t2 - t1, iterations=rep.iterations, restarts=rep.restarts,
+                   relres=rep.relres, converged=ok,
+                   info=op.info(), rcs=r)
+        op.close()
+
+    for _ in range(a.warmup):
+        step()
+    ctx.prof_reset()
+    ctx.prof(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / a.steps
+    fams = {}
+    for f in ["mlfma_setup", "mlfma_near", "mlfma_agg", "mlfma_up", "mlfma_down", "mlfma_out"]:
+        ms, cnt, work = ctx.prof_get(f)
+        fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
+    ctx.prof(False)
+    info = out["info"]
+    nf = fams["mlfma_near"]
+    near_gbs = nf["work_per_step"] / (nf["ms_per_step"] * 1e-3) / 1e9 if nf["ms_per_step"] > 0 else 0.0
+    matvecs = out["iterations"] + out["restarts"] + 1  # Arnoldi steps + restart residuals + final residual
+    mv_ms = sum(fams[f]["ms_per_step"] for f in ["mlfma_near", "mlfma_agg", "mlfma_up", "mlfma_down", "mlfma_out"])
+    res = {
+        "metric": "solves/sec (MLFMA CFIE, GMRES to 1e-4)",
+        "value": 1.0 / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 near/far, f64 self terms and GMRES",
+        "data": "synthetic icosphere mesh",
+        "config": {"workload": f"MLFMA CFIE alpha={alpha} PEC sphere r={radius} m f={freq / 1e9:.1f} GHz "
+                               f"icosphere s={subdiv}, box threshold {threshold} lambda, GMRES({restart}) to {tol}",
+                   "unknowns": n, "triangles": int(tri.shape[0]), "parallelism": "single GPU"},
+        "mlfma": {"levels": info["levels"], "boxes": info["boxes"], "multipoles": info["multipoles"],
+                  "near_nnz": info["near_nnz"], "device_gb": info["bytes"] / 2**30,
+                  "setup_s": out["setup_s"], "solve_s": out["solve_s"], "gmres_iterations": out["iterations"],
+                  "relres": out["relres"], "converged": out["converged"],
+                  "matvec_ms": mv_ms / max(matvecs, 1)},
+        "roofline": {"bound": "hbm", "kernel": "mlfma_near (k_ml_near, CSR near-field SpMV)",
+                     "achieved": near_gbs, "peak": PEAK["hbm"], "unit": "GB/s", "frac": near_gbs / PEAK["hbm"],
+                     "traffic": None},
+        "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res))
 
 
 def main_stream(a):

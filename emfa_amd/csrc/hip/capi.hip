@@ -385,6 +385,9 @@ emfa_status emfa_mesh_upload(emfa_ctx *ctx, const double *xyz, int nv, const int
     M->nt = nt;
     M->n = n;
     M->t2r_host = t2r;
+    M->xyz_host.assign(xyz, xyz + 3 * (size_t)nv);
+    M->tri_host.assign(tri, tri + 3 * (size_t)nt);
+    M->rwg_host.assign(rwg, rwg + 6 * (size_t)n);
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     struct A {
         void **p;

@@ -8,6 +8,7 @@
 // number of restarts is bounded.
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <vector>
 
 #include "internal.h"
@@ -98,18 +99,13 @@ __global__ __launch_bounds__(256) void k_sub_from(const double2 *__restrict__ b,
     if (t < n) r[t] = make_double2(b[t].x - r[t].x, b[t].y - r[t].y);
 }
 
-}  // namespace emfa
-
-using namespace emfa;
-
-extern "C" emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, int m, double tol,
-                                  int max_restart, emfa_gmres_report *rep) {
-    EMFA_REQUIRE(ctx && Z && B && X, EMFA_E_INVALID, "emfa_gmres: null argument");
-    const int n = Z->rows, nr = B->cols;
-    EMFA_REQUIRE(Z->cols == n && B->rows == n && X->rows == n && X->cols == nr && B->prec == EMFA_F64 &&
-                     X->prec == EMFA_F64,
-                 EMFA_E_INVALID, "emfa_gmres: B and X must be F64 n x nrhs");
-    EMFA_REQUIRE(m >= 1 && m <= 512 && max_restart >= 1, EMFA_E_INVALID, "emfa_gmres: bad m/max_restart");
+// GMRES core over any operator y = A x on device vectors of n complex FP64.
+emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(const double2 *, double2 *)> &matvec,
+                       const emfa_mat *B, emfa_mat *X, int m, double tol, int max_restart, emfa_gmres_report *rep) {
+    const int nr = B->cols;
+    EMFA_REQUIRE(B->rows == n && X->rows == n && X->cols == nr && B->prec == EMFA_F64 && X->prec == EMFA_F64,
+                 EMFA_E_INVALID, "gmres: B and X must be F64 n x nrhs");
+    EMFA_REQUIRE(m >= 1 && m <= 512 && max_restart >= 1, EMFA_E_INVALID, "gmres: bad m/max_restart");
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     size_t vb = (size_t)(m + 1) * n * sizeof(double2);
     size_t total = vb + 4 * (size_t)n * sizeof(double2) + 2 * (size_t)(m + 2) * sizeof(double2) + 4096;
@@ -131,15 +127,6 @@ extern "C" emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_m
     const int nb = (n + 255) / 256;
     hipStream_t s = ctx->stream;
     using cd = std::complex<double>;
-    auto matvec = [&](const double2 *in, double2 *out) -> emfa_status {
-        ProfScope ps(ctx, "gmres_matvec", (double)n * n * elem_bytes(Z->prec));
-        if (Z->prec == EMFA_F32)
-            k_gemv<float><<<(n + 3) / 4, 256, 0, s>>>((const cx<float> *)Z->d, n, in, out);
-        else
-            k_gemv<double><<<(n + 3) / 4, 256, 0, s>>>((const cx<double> *)Z->d, n, in, out);
-        EMFA_HIP_CHECK(hipGetLastError());
-        return EMFA_OK;
-    };
     auto norm = [&](const double2 *v, double *out) -> emfa_status {
         k_dots<<<1, 256, 0, s>>>(v, n, v, h2);
         EMFA_HIP_CHECK(hipGetLastError());
@@ -247,8 +234,30 @@ extern "C" emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_m
         rep->relres = worst;
     }
     if (!all_conv) {
-        set_error("emfa_gmres: relative residual %.3e above tolerance %.3e", worst, tol);
+        set_error("gmres: relative residual %.3e above tolerance %.3e", worst, tol);
         return EMFA_E_NOCONV;
     }
     return EMFA_OK;
+}
+
+}  // namespace emfa
+
+using namespace emfa;
+
+extern "C" emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, int m, double tol,
+                                  int max_restart, emfa_gmres_report *rep) {
+    EMFA_REQUIRE(ctx && Z && B && X, EMFA_E_INVALID, "emfa_gmres: null argument");
+    const int n = Z->rows;
+    EMFA_REQUIRE(Z->cols == n, EMFA_E_INVALID, "emfa_gmres: Z must be square");
+    hipStream_t s = ctx->stream;
+    auto matvec = [&](const double2 *in, double2 *out) -> emfa_status {
+        ProfScope ps(ctx, "gmres_matvec", (double)n * n * elem_bytes(Z->prec));
+        if (Z->prec == EMFA_F32)
+            k_gemv<float><<<(n + 3) / 4, 256, 0, s>>>((const cx<float> *)Z->d, n, in, out);
+        else
+            k_gemv<double><<<(n + 3) / 4, 256, 0, s>>>((const cx<double> *)Z->d, n, in, out);
+        EMFA_HIP_CHECK(hipGetLastError());
+        return EMFA_OK;
+    };
+    return gmres_impl(ctx, n, matvec, B, X, m, tol, max_restart, rep);
 }

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -79,6 +80,9 @@ struct emfa_mesh {
     double *rlen64 = nullptr;
     float *rlen32 = nullptr;
     std::vector<int> t2r_host;
+    std::vector<double> xyz_host;  // vertices (nv x 3)
+    std::vector<int> tri_host;     // triangles (nt x 3)
+    std::vector<int> rwg_host;     // RWG list (n x 6: v1 v2 vp vm tp tm)
     size_t bytes = 0;
 };
 
@@ -141,5 +145,8 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
                           int accumulate);
 emfa_status convert_copy(emfa_ctx *ctx, const emfa_mat *src, emfa_mat *dst);
 emfa_status axpy_f64(emfa_ctx *ctx, emfa_mat *X, const emfa_mat *D);  // X += D (D any prec)
+// restarted GMRES over an operator on device vectors of n complex FP64 (gmres.hip)
+emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(const double2 *, double2 *)> &matvec,
+                       const emfa_mat *B, emfa_mat *X, int m, double tol, int max_restart, emfa_gmres_report *rep);
 
 }  // namespace emfa

@@ -1,0 +1,943 @@
+// Multilevel fast multipole matvec (CFIE/EFIE) on gfx950.
+//
+// Same algorithm as EMSolver/MLFMA.cpp (setup :688-1422, matvec :454-686):
+//   octree        cube = extent + 0.1 lambda (EMComponent/MLFMA_Boundary.cpp:15-35),
+//                 halved until the box edge is <= threshold * lambda; RWGs
+//                 binned by edge midpoint into octants (x*2+y)*2+z; near =
+//                 children of the parent's near boxes closer than 1.75 box
+//                 edges (= Chebyshev offset <= 1 on the box grid), far = rest
+//   samples       L = int(kd + 6 cbrt(kd)), kd = 1.732051 k a; L Gauss-Legendre
+//                 theta nodes, 2L Gauss-Legendre phi nodes, sample k = p*L + t
+//   interpolation 3-point Lagrange per angle, 9 weights per coarse sample
+//   translation   sum_{l <= min(L, kD)} (2l+1) (-j)^l h_l^(2)(kD) P_l(khat.Dhat),
+//                 tabulated per box-grid offset (at most 343 per level) instead
+//                 of per box pair
+//   rad / rec     3-point radiation / receive patterns of each RWG about its
+//                 leaf-box centre; the receive pattern carries the quadrature
+//                 coefficient w sin(theta) k^2 Z0 / 32
+//   near          CFIE (or EFIE) entries of all RWG pairs in neighbouring leaf
+//                 boxes, FP32 with FP64 self terms, stored CSR in box order
+// Kernels (FP32 complex, one writer per output, no atomics, so repeated
+// matvecs are bitwise identical):
+//   k_ml_near     one wavefront per row of the near matrix
+//   k_ml_agg      leaf aggregation: one workgroup per (leaf box, 256 samples)
+//   k_ml_up       interpolation + shift to the parent box, per (parent, samples)
+//   k_ml_down     translation from far boxes + anterpolation from the parent
+//   k_ml_out      receive (dot with the leaf down-going pattern) + near part,
+//                 scattered back to RWG order in FP64
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "em_device.h"
+#include "internal.h"
+
+namespace emfa {
+
+using cf = cx<float>;
+
+struct MlLevel {
+    int nbox = 0, L = 0, K = 0;
+    double a = 0;
+    std::vector<double> cen;  // nbox x 3
+    std::vector<int> ix;      // nbox x 3 integer box coordinates
+    std::vector<int> parent, octant, ubeg, uend;
+    std::vector<int> near_ptr, near_idx, far_ptr, far_idx, far_code, ch_ptr, ch_idx;
+    std::vector<double> xt, wt, xp, wp, W, kh, theta;  // samples
+    // device
+    cf *up = nullptr, *down = nullptr;
+    cf *trans = nullptr;  // 343 x K
+    int *far_ptr_d = nullptr, *far_idx_d = nullptr, *far_code_d = nullptr;
+    int *ch_ptr_d = nullptr, *ch_idx_d = nullptr, *oct_d = nullptr, *parent_d = nullptr;
+    int *ip_col = nullptr;  // interpolation from the child level: K x 9
+    float *ip_w = nullptr;
+    cf *ph = nullptr;       // 8 x K: exp(-j k khat.(c_parent - c_child)) per child octant
+    int *an_ptr = nullptr, *an_kp = nullptr;  // anterpolation from the parent level
+    float *an_w = nullptr;
+    int *ubeg_d = nullptr, *uend_d = nullptr;  // leaf level: unknown ranges (box order)
+};
+
+}  // namespace emfa
+
+struct emfa_mlfma {
+    emfa_ctx *ctx = nullptr;
+    int n = 0, levels = 0, kind = EMFA_CFIE;
+    double k = 0, alpha = 1;
+    std::vector<emfa::MlLevel> lv;
+    int *perm = nullptr, *leaf_of = nullptr;
+    long long *row_ptr = nullptr;
+    int *col = nullptr;
+    emfa::cf *val = nullptr;
+    emfa::cf *rad = nullptr, *rec = nullptr;
+    emfa::cf *xb = nullptr, *yn = nullptr;
+    double *leaf_cen = nullptr, *kh_leaf = nullptr;
+    long long nnz = 0;
+    std::vector<std::pair<void *, size_t>> allocs;
+    size_t bytes = 0;
+};
+
+namespace emfa {
+
+// ------------------------------------------------------------------ host math
+// Gauss-Legendre nodes (ascending) and weights on [-1, 1] by Newton iteration.
+static void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n, 0);
+    w.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(kPi * (i + 0.75) / (n + 0.5)), dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1, p1 = z;
+            for (int l = 2; l <= n; ++l) {
+                double p2 = ((2 * l - 1) * z * p1 - (l - 1) * p0) / l;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) {
+                p1 = z;
+                p0 = 1;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1);
+            double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-15) break;
+        }
+        double p0 = 1, p1 = z;
+        for (int l = 2; l <= n; ++l) {
+            double p2 = ((2 * l - 1) * z * p1 - (l - 1) * p0) / l;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1);
+        x[n - 1 - i] = z;
+        w[n - 1 - i] = 2 / ((1 - z * z) * dp * dp);
+    }
+}
+
+// Three-point Lagrange weights of every `cur` abscissa on the sorted `nxt`
+// grid (EMSolver/MLFMA.cpp:957-1002, same index walk).
+static void lagrange3(const std::vector<double> &nxt, const std::vector<double> &cur, std::vector<int> &loc,
+                      std::vector<double> &el) {
+    const int n = (int)cur.size(), nlast = (int)nxt.size() - 1;
+    loc.assign(3 * n, 0);
+    el.assign(3 * n, 0);
+    int nid = 2;
+    const double low = nxt[1], high = nxt[nlast - 1];
+    for (int c = 0; c < n; ++c) {
+        const double xx = cur[c];
+        int p0, p1, p2;
+        if (xx < low) {
+            p0 = 0, p1 = 1, p2 = 2;
+        } else if (xx > high) {
+            p0 = nlast - 2, p1 = nlast - 1, p2 = nlast;
+        } else {
+            nid += nxt[nid] < xx ? 1 : 0;
+            int off = (nxt[nid] - xx < xx - nxt[nid - 1]) ? 1 : 0;
+            p0 = nid + off - 2, p1 = nid + off - 1, p2 = nid + off;
+        }
+        const double x0 = nxt[p0], x1 = nxt[p1], x2 = nxt[p2];
+        loc[3 * c] = p0, loc[3 * c + 1] = p1, loc[3 * c + 2] = p2;
+        el[3 * c] = (xx - x1) * (xx - x2) / ((x0 - x1) * (x0 - x2));
+        el[3 * c + 1] = (xx - x0) * (xx - x2) / ((x1 - x0) * (x1 - x2));
+        el[3 * c + 2] = (xx - x0) * (xx - x1) / ((x2 - x0) * (x2 - x1));
+    }
+}
+
+static int multipoles(double k, double a) {
+    double kd = 1.732051 * k * a;
+    return (int)(kd + 6.0 * std::cbrt(kd));
+}
+
+// Translation operator for box offset D (receiving minus source centre).
+static void translation(double k, const double *D, const std::vector<double> &kh, int Lt, cf *out) {
+    using cd = std::complex<double>;
+    const double d = std::sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    const double x = k * d;
+    const int lmax = (int)std::min((double)Lt, x);
+    std::vector<cd> h(lmax + 1);
+    h[0] = cd(std::sin(x), std::cos(x)) / x;
+    if (lmax >= 1) h[1] = cd(std::sin(x) / x - std::cos(x), std::cos(x) / x + std::sin(x)) / x;
+    for (int l = 2; l <= lmax; ++l) h[l] = ((2 * l - 1) / x) * h[l - 1] - h[l - 2];
+    const cd jl[4] = {cd(1, 0), cd(0, -1), cd(-1, 0), cd(0, 1)};
+    std::vector<cd> c(lmax + 1);
+    for (int l = 0; l <= lmax; ++l) c[l] = double(2 * l + 1) * jl[l % 4] * h[l];
+    const int K = (int)kh.size() / 3;
+    for (int q = 0; q < K; ++q) {
+        double ct = (kh[3 * q] * D[0] + kh[3 * q + 1] * D[1] + kh[3 * q + 2] * D[2]) / d;
+        double p0 = 1, p1 = ct;
+        cd acc = c[0];
+        if (lmax >= 1) acc += c[1] * p1;
+        for (int l = 2; l <= lmax; ++l) {
+            double p2 = ((2 * l - 1) * ct * p1 - (l - 1) * p0) / l;
+            p0 = p1;
+            p1 = p2;
+            acc += c[l] * p2;
+        }
+        out[q] = {(float)acc.real(), (float)acc.imag()};
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ cf cfma(cf acc, cf a, cf b) {
+    return {acc.re + a.re * b.re - a.im * b.im, acc.im + a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cf cscale(cf a, float s) { return {a.re * s, a.im * s}; }
+
+// Regular CFIE/EFIE triangle-pair integral for free vertices a (test) and b
+// (source), 3 x 3 points, FP32 (same integrand as the dense fill's K1).
+template <int KIND>
+__device__ cf pair_regular(const float *__restrict__ tg, int T, int tf, int a, int ts, int b, float k, float ik2,
+                           float cE, float cM) {
+    float cfv[3], csv[3], Af[3], Bs[3], nf[3], vf[3][3], vs[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cfv[c] = tg[(size_t)c * T + tf];
+        csv[c] = tg[(size_t)c * T + ts];
+        nf[c] = tg[(size_t)(12 + c) * T + tf];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            vf[i][c] = tg[(size_t)(3 + 3 * i + c) * T + tf];
+            vs[i][c] = tg[(size_t)(3 + 3 * i + c) * T + ts];
+        }
+        Af[c] = vf[a][c];
+        Bs[c] = vs[b][c];
+    }
+    float er = 0, ei = 0, mr = 0, mi = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        float x[3], rf[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[c] = (float)rule<3>(p, 0) * vf[0][c] + (float)rule<3>(p, 1) * vf[1][c] + (float)rule<3>(p, 2) * vf[2][c];
+            rf[c] = x[c] - Af[c];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            float y[3], rs[3], u[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                y[c] = (float)rule<3>(q, 0) * vs[0][c] + (float)rule<3>(q, 1) * vs[1][c] + (float)rule<3>(q, 2) * vs[2][c];
+                rs[c] = y[c] - Bs[c];
+                u[c] = cfv[c] - csv[c] + x[c] - y[c];
+            }
+            float r = sqrtf(dot3(u, u));
+            float ir = 1.0f / r;
+            float sn, cs;
+            sincos_r(k * r, &sn, &cs);
+            const float w = (1.0f / 9.0f);
+            float gr = w * cs * ir, gi = -w * sn * ir;
+            float f = 0.25f * dot3(rf, rs) - ik2;
+            er += f * gr;
+            ei += f * gi;
+            if (KIND == EMFA_CFIE) {
+                float ir2 = ir * ir, kr = k * r;
+                float kcr = (gr - kr * gi) * ir2, kci = (gi + kr * gr) * ir2;
+                float m = dot3(rf, u) * dot3(nf, rs) - dot3(rf, rs) * dot3(nf, u);
+                mr += kcr * m;
+                mi += kci * m;
+            }
+        }
+    }
+    // cE is the imaginary coefficient j * cE
+    return {-cE * ei + cM * mr, cE * er + cM * mi};
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_ml_near_fill(const float *__restrict__ tg, const double *__restrict__ tg64,
+                                                      int T, const int *__restrict__ rtp, const int *__restrict__ rtm,
+                                                      const int *__restrict__ rap, const int *__restrict__ ram,
+                                                      const float *__restrict__ rlen, const int *__restrict__ perm,
+                                                      const long long *__restrict__ row_ptr,
+                                                      const int *__restrict__ col, int n, float k, float ik2, float cE,
+                                                      float cM, double kd, double alpha, int cfie_mode,
+                                                      cf *__restrict__ val) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const int f = perm[row];
+    const int tfs[2] = {rtp[f], rtm[f]}, afs[2] = {rap[f], ram[f]};
+    const float lf = rlen[f];
+    for (long long j = row_ptr[row] + lane; j < row_ptr[row + 1]; j += 64) {
+        const int s = perm[col[j]];
+        const int tss[2] = {rtp[s], rtm[s]}, bss[2] = {rap[s], ram[s]};
+        float zr = 0, zi = 0;
+        for (int i = 0; i < 2; ++i) {
+            for (int q = 0; q < 2; ++q) {
+                const float sg = (i == q) ? 1.0f : -1.0f;
+                const int tf = tfs[i], ts = tss[q];
+                if (tf == ts) {
+                    d3 t[3];
+                    for (int v = 0; v < 3; ++v)
+                        t[v] = mk3(tg64[(size_t)(3 + 3 * v) * T + tf], tg64[(size_t)(4 + 3 * v) * T + tf],
+                                   tg64[(size_t)(5 + 3 * v) * T + tf]);
+                    double sr, si, gr, gi;
+                    self_term<3>(KIND, kd, alpha, 0.0, 0.0, cfie_mode, t, tg64[(size_t)15 * T + tf], afs[i], bss[q],
+                                 &sr, &si, &gr, &gi);
+                    zr += sg * (float)sr;
+                    zi += sg * (float)si;
+                } else {
+                    cf z = pair_regular<KIND>(tg, T, tf, afs[i], ts, bss[q], k, ik2, cE, cM);
+                    zr += sg * z.re;
+                    zi += sg * z.im;
+                }
+            }
+        }
+        const float ll = lf * rlen[s];
+        val[j] = {ll * zr, ll * zi};
+    }
+}
+
+// Radiation and receive patterns of RWG perm[i] about its leaf-box centre
+// (EMSolver/MLFMA.cpp:1244-1292,1393-1422), evaluated in FP64, stored FP32.
+__global__ __launch_bounds__(256) void k_ml_radrec(const double *__restrict__ tg, int T, const int *__restrict__ rtp,
+                                                   const int *__restrict__ rtm, const int *__restrict__ rap,
+                                                   const int *__restrict__ ram, const double *__restrict__ rlen,
+                                                   const int *__restrict__ perm, const int *__restrict__ leaf_of,
+                                                   const double *__restrict__ leaf_cen,
+                                                   const double *__restrict__ kh, const double *__restrict__ coeff,
+                                                   int n, int K, double k, double alpha, cf *__restrict__ rad,
+                                                   cf *__restrict__ rec) {
+    const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= (long long)n * K) return;
+    const int i = (int)(id / K), q = (int)(id % K);
+    const int u = perm[i], b = leaf_of[i];
+    const double kx = kh[3 * q], ky = kh[3 * q + 1], kz = kh[3 * q + 2];
+    double vr[3] = {0, 0, 0}, vi[3] = {0, 0, 0}, hr[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int side = 0; side < 2; ++side) {
+        const int t = side == 0 ? rtp[u] : rtm[u];
+        const int a = side == 0 ? rap[u] : ram[u];
+        const double sg = side == 0 ? 1.0 : -1.0;
+        double c[3], v[3][3], nn[3];
+        for (int cc = 0; cc < 3; ++cc) {
+            c[cc] = tg[(size_t)cc * T + t] - leaf_cen[3 * b + cc];
+            nn[cc] = tg[(size_t)(12 + cc) * T + t];
+            for (int j = 0; j < 3; ++j) v[j][cc] = tg[(size_t)(3 + 3 * j + cc) * T + t];
+        }
+        for (int p = 0; p < 3; ++p) {
+            double x[3], rho[3];
+            for (int cc = 0; cc < 3; ++cc) {
+                x[cc] = rule<3>(p, 0) * v[0][cc] + rule<3>(p, 1) * v[1][cc] + rule<3>(p, 2) * v[2][cc];
+                rho[cc] = sg * (x[cc] - v[a][cc]);
+            }
+            const double ph = k * (kx * (c[0] + x[0]) + ky * (c[1] + x[1]) + kz * (c[2] + x[2]));
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const double w = 1.0 / 3.0;
+            double nr[3] = {nn[1] * rho[2] - nn[2] * rho[1], nn[2] * rho[0] - nn[0] * rho[2],
+                            nn[0] * rho[1] - nn[1] * rho[0]};
+            for (int cc = 0; cc < 3; ++cc) {
+                vr[cc] += w * rho[cc] * cs;  // exp(+j ph)
+                vi[cc] += w * rho[cc] * sn;
+                hr[cc] += w * nr[cc] * cs;   // exp(-j ph)
+                hi[cc] -= w * nr[cc] * sn;
+            }
+        }
+    }
+    // transverse projection
+    const double pr = kx * vr[0] + ky * vr[1] + kz * vr[2], pi = kx * vi[0] + ky * vi[1] + kz * vi[2];
+    const double kv[3] = {kx, ky, kz};
+    for (int cc = 0; cc < 3; ++cc) {
+        vr[cc] -= pr * kv[cc];
+        vi[cc] -= pi * kv[cc];
+    }
+    // k x h
+    double cr[3] = {ky * hr[2] - kz * hr[1], kz * hr[0] - kx * hr[2], kx * hr[1] - ky * hr[0]};
+    double ci[3] = {ky * hi[2] - kz * hi[1], kz * hi[0] - kx * hi[2], kx * hi[1] - ky * hi[0]};
+    const double hl = 0.5 * rlen[u], cq = coeff[q];
+    cf *ro = rad + ((size_t)i * K + q) * 3;
+    cf *co = rec + ((size_t)i * K + q) * 3;
+    for (int cc = 0; cc < 3; ++cc) {
+        ro[cc] = {(float)(hl * vr[cc]), (float)(hl * vi[cc])};
+        double er = alpha * vr[cc] + (1 - alpha) * cr[cc];
+        double ei = -alpha * vi[cc] + (1 - alpha) * ci[cc];
+        co[cc] = {(float)(cq * hl * er), (float)(cq * hl * ei)};
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ml_gather(const double2 *__restrict__ x, const int *__restrict__ perm, int n,
+                                                   cf *__restrict__ xb) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        double2 v = x[perm[i]];
+        xb[i] = {(float)v.x, (float)v.y};
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ml_near(const long long *__restrict__ row_ptr, const int *__restrict__ col,
+                                                 const cf *__restrict__ val, const cf *__restrict__ xb, int n,
+                                                 cf *__restrict__ yn) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    float sr = 0, si = 0;
+    for (long long j = row_ptr[row] + lane; j < row_ptr[row + 1]; j += 64) {
+        cf a = val[j], b = xb[col[j]];
+        sr += a.re * b.re - a.im * b.im;
+        si += a.re * b.im + a.im * b.re;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    if (lane == 0) yn[row] = {sr, si};
+}
+
+// leaf aggregation: up[b][q] = sum_{i in b} x_i rad[i][q]
+__global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, const int *__restrict__ uend,
+                                                const cf *__restrict__ rad, const cf *__restrict__ xb, int K,
+                                                cf *__restrict__ up) {
+    const int b = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
+    if (q >= K) return;
+    cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    for (int i = ubeg[b]; i < uend[b]; ++i) {
+        const cf x = xb[i];
+        const cf *r = rad + ((size_t)i * K + q) * 3;
+        a0 = cfma(a0, x, r[0]);
+        a1 = cfma(a1, x, r[1]);
+        a2 = cfma(a2, x, r[2]);
+    }
+    cf *o = up + ((size_t)b * K + q) * 3;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+}
+
+// upward: up_p[q] = sum_children ph[oct][q] * sum_9 w * up_c[col]
+__global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, const int *__restrict__ ch_idx,
+                                               const int *__restrict__ ch_oct, const int *__restrict__ ip_col,
+                                               const float *__restrict__ ip_w, const cf *__restrict__ ph, int K,
+                                               int Kc, const cf *__restrict__ upc, cf *__restrict__ upp) {
+    const int p = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
+    if (q >= K) return;
+    int cols[9];
+    float ws[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        cols[j] = ip_col[q * 9 + j];
+        ws[j] = ip_w[q * 9 + j];
+    }
+    cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    for (int e = ch_ptr[p]; e < ch_ptr[p + 1]; ++e) {
+        const int c = ch_idx[e];
+        const cf *src = upc + (size_t)c * Kc * 3;
+        cf s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0};
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const cf *v = src + (size_t)cols[j] * 3;
+            s0 = {s0.re + ws[j] * v[0].re, s0.im + ws[j] * v[0].im};
+            s1 = {s1.re + ws[j] * v[1].re, s1.im + ws[j] * v[1].im};
+            s2 = {s2.re + ws[j] * v[2].re, s2.im + ws[j] * v[2].im};
+        }
+        const cf f = ph[ch_oct[c] * K + q];
+        a0 = cfma(a0, f, s0);
+        a1 = cfma(a1, f, s1);
+        a2 = cfma(a2, f, s2);
+    }
+    cf *o = upp + ((size_t)p * K + q) * 3;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+}
+
+// downward at one level: translation from far boxes + anterpolation from the parent
+__global__ __launch_bounds__(256) void k_ml_down(const int *__restrict__ far_ptr, const int *__restrict__ far_idx,
+                                                 const int *__restrict__ far_code, const cf *__restrict__ trans,
+                                                 const cf *__restrict__ up, int K, const int *__restrict__ parent,
+                                                 const int *__restrict__ oct, const int *__restrict__ an_ptr,
+                                                 const int *__restrict__ an_kp, const float *__restrict__ an_w,
+                                                 const cf *__restrict__ php, int Kp, const cf *__restrict__ downp,
+                                                 cf *__restrict__ down) {
+    const int b = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
+    if (q >= K) return;
+    cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    for (int e = far_ptr[b]; e < far_ptr[b + 1]; ++e) {
+        const cf t = trans[(size_t)far_code[e] * K + q];
+        const cf *u = up + ((size_t)far_idx[e] * K + q) * 3;
+        a0 = cfma(a0, t, u[0]);
+        a1 = cfma(a1, t, u[1]);
+        a2 = cfma(a2, t, u[2]);
+    }
+    if (downp) {
+        const int p = parent[b], o = oct[b];
+        const cf *dp = downp + (size_t)p * Kp * 3;
+        for (int e = an_ptr[q]; e < an_ptr[q + 1]; ++e) {
+            const int kp = an_kp[e];
+            const cf f0 = php[o * Kp + kp];
+            const cf f = {an_w[e] * f0.re, -an_w[e] * f0.im};  // conj: exp(+j k khat.(c_p - c_b))
+            const cf *v = dp + (size_t)kp * 3;
+            a0 = cfma(a0, f, v[0]);
+            a1 = cfma(a1, f, v[1]);
+            a2 = cfma(a2, f, v[2]);
+        }
+    }
+    cf *o = down + ((size_t)b * K + q) * 3;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+}
+
+// receive + near part, back to RWG order: y[perm[i]] = yn[i] + sum_q rec[i][q] . down[leaf(i)][q]
+__global__ __launch_bounds__(256) void k_ml_out(const cf *__restrict__ rec, const cf *__restrict__ down,
+                                                const int *__restrict__ leaf_of, const int *__restrict__ perm,
+                                                const cf *__restrict__ yn, int n, int K, double2 *__restrict__ y) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const cf *d = down + (size_t)leaf_of[i] * K * 3;
+    const cf *r = rec + (size_t)i * K * 3;
+    float sr = 0, si = 0;
+    for (int q = lane; q < K; q += 64) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cf a = r[3 * q + c], b = d[3 * q + c];
+            sr += a.re * b.re - a.im * b.im;
+            si += a.re * b.im + a.im * b.re;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    if (lane == 0) {
+        cf v = yn[i];
+        y[perm[i]] = make_double2((double)v.re + (double)sr, (double)v.im + (double)si);
+    }
+}
+
+// ------------------------------------------------------------------ host
+template <typename T>
+static emfa_status ml_upload(emfa_mlfma *M, T **dst, const std::vector<T> &src) {
+    *dst = nullptr;
+    if (src.empty()) return EMFA_OK;
+    size_t b = src.size() * sizeof(T);
+    void *p = nullptr;
+    if (ctx_alloc(M->ctx, &p, b) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("mlfma: cannot allocate %zu bytes", b);
+        return EMFA_E_OOM;
+    }
+    M->allocs.push_back({p, b});
+    M->bytes += b;
+    EMFA_HIP_CHECK(hipMemcpyAsync(p, src.data(), b, hipMemcpyHostToDevice, M->ctx->stream));
+    EMFA_HIP_CHECK(hipStreamSynchronize(M->ctx->stream));  // src may be a temporary
+    *dst = (T *)p;
+    return EMFA_OK;
+}
+
+template <typename T>
+static emfa_status ml_alloc(emfa_mlfma *M, T **dst, size_t count) {
+    *dst = nullptr;
+    size_t b = std::max<size_t>(count, 1) * sizeof(T);
+    void *p = nullptr;
+    if (ctx_alloc(M->ctx, &p, b) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("mlfma: cannot allocate %zu bytes", b);
+        return EMFA_E_OOM;
+    }
+    M->allocs.push_back({p, b});
+    M->bytes += b;
+    *dst = (T *)p;
+    return EMFA_OK;
+}
+
+static void ml_free(emfa_mlfma *M) {
+    if (!M) return;
+    if (M->ctx) (void)hipStreamSynchronize(M->ctx->stream);
+    for (auto &a : M->allocs) ctx_free(M->ctx, a.first, a.second);
+    M->allocs.clear();
+}
+
+#define ML_TRY(x)                        \
+    do {                                 \
+        emfa_status _s = (x);            \
+        if (_s != EMFA_OK) {             \
+            ml_free(M);                  \
+            delete M;                    \
+            return _s;                   \
+        }                                \
+    } while (0)
+
+static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *prm, emfa_mlfma **out) {
+    const int n = mesh->n, T = mesh->nt, nv = mesh->nv;
+    const double k = prm->k, lam = 2 * kPi / k;
+    const std::vector<double> &xyz = mesh->xyz_host;
+    const std::vector<int> &rwg = mesh->rwg_host;
+    // octree (MLFMA_Boundary.cpp:15-35)
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int v = 0; v < nv; ++v)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = std::min(mn[c], xyz[3 * v + c]);
+            mx[c] = std::max(mx[c], xyz[3 * v + c]);
+        }
+    double center[3], ext = 0;
+    for (int c = 0; c < 3; ++c) {
+        center[c] = (mn[c] + mx[c]) / 2;
+        ext = std::max(ext, mx[c] - mn[c]);
+    }
+    const double L0 = ext + 0.1 * lam, delta = prm->box_threshold * lam;
+    int levels = 0;
+    for (double ml = L0; ml > delta; ml /= 2) ++levels;
+    EMFA_REQUIRE(levels >= 2, EMFA_E_INVALID,
+                 "mlfma: %d level(s) only; the box threshold (%.3g wavelengths) is too large for this mesh", levels,
+                 prm->box_threshold);
+    EMFA_REQUIRE(levels <= 14, EMFA_E_INVALID, "mlfma: %d levels (box threshold too small)", levels);
+    std::vector<double> mid(3 * (size_t)n);
+    for (int u = 0; u < n; ++u)
+        for (int c = 0; c < 3; ++c) mid[3 * u + c] = 0.5 * (xyz[3 * rwg[6 * u] + c] + xyz[3 * rwg[6 * u + 1] + c]);
+
+    emfa_mlfma *M = new emfa_mlfma();
+    M->ctx = ctx;
+    M->n = n;
+    M->levels = levels;
+    M->k = k;
+    M->alpha = prm->kind == EMFA_EFIE ? 1.0 : prm->alpha;
+    M->kind = prm->kind;
+    M->lv.resize(levels + 1);
+    std::vector<int> order(n);
+    for (int u = 0; u < n; ++u) order[u] = u;
+    {
+        MlLevel &r = M->lv[0];
+        r.nbox = 1;
+        r.a = L0;
+        r.cen = {center[0], center[1], center[2]};
+        r.ix = {0, 0, 0};
+        r.parent = {-1};
+        r.octant = {0};
+        r.ubeg = {0};
+        r.uend = {n};
+    }
+    for (int l = 1; l <= levels; ++l) {
+        MlLevel &P = M->lv[l - 1], &C = M->lv[l];
+        C.a = L0 / std::pow(2.0, l);
+        P.ch_ptr.assign(P.nbox + 1, 0);
+        std::vector<int> tmp;
+        for (int p = 0; p < P.nbox; ++p) {
+            const double *pc = &P.cen[3 * p];
+            std::vector<int> bucket[8];
+            for (int i = P.ubeg[p]; i < P.uend[p]; ++i) {
+                int u = order[i];
+                int o = ((mid[3 * u] < pc[0] ? 0 : 1) * 2 + (mid[3 * u + 1] < pc[1] ? 0 : 1)) * 2 +
+                        (mid[3 * u + 2] < pc[2] ? 0 : 1);
+                bucket[o].push_back(u);
+            }
+            int pos = P.ubeg[p];
+            for (int o = 0; o < 8; ++o) {
+                if (bucket[o].empty()) continue;
+                int bx = o >> 2, by = (o >> 1) & 1, bz = o & 1;
+                C.cen.push_back(pc[0] + C.a * (bx - 0.5));
+                C.cen.push_back(pc[1] + C.a * (by - 0.5));
+                C.cen.push_back(pc[2] + C.a * (bz - 0.5));
+                C.ix.push_back(2 * P.ix[3 * p] + bx);
+                C.ix.push_back(2 * P.ix[3 * p + 1] + by);
+                C.ix.push_back(2 * P.ix[3 * p + 2] + bz);
+                C.parent.push_back(p);
+                C.octant.push_back(o);
+                C.ubeg.push_back(pos);
+                for (int u : bucket[o]) order[pos++] = u;
+                C.uend.push_back(pos);
+                P.ch_idx.push_back(C.nbox);
+                ++C.nbox;
+            }
+            P.ch_ptr[p + 1] = (int)P.ch_idx.size();
+        }
+    }
+    // near / far lists (MLFMA.cpp:836-875)
+    {
+        MlLevel &l1 = M->lv[1];
+        l1.near_ptr.assign(l1.nbox + 1, 0);
+        for (int b = 0; b < l1.nbox; ++b) {
+            for (int c = 0; c < l1.nbox; ++c) l1.near_idx.push_back(c);
+            l1.near_ptr[b + 1] = (int)l1.near_idx.size();
+        }
+        l1.far_ptr.assign(l1.nbox + 1, 0);
+    }
+    for (int l = 2; l <= levels; ++l) {
+        MlLevel &P = M->lv[l - 1], &C = M->lv[l];
+        C.near_ptr.assign(C.nbox + 1, 0);
+        C.far_ptr.assign(C.nbox + 1, 0);
+        for (int b = 0; b < C.nbox; ++b) {
+            const int p = C.parent[b];
+            for (int e = P.near_ptr[p]; e < P.near_ptr[p + 1]; ++e) {
+                const int pn = P.near_idx[e];
+                for (int f = P.ch_ptr[pn]; f < P.ch_ptr[pn + 1]; ++f) {
+                    const int c = P.ch_idx[f];
+                    int dx = C.ix[3 * b] - C.ix[3 * c], dy = C.ix[3 * b + 1] - C.ix[3 * c + 1],
+                        dz = C.ix[3 * b + 2] - C.ix[3 * c + 2];
+                    if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) <= 1) {
+                        C.near_idx.push_back(c);
+                    } else {
+                        C.far_idx.push_back(c);
+                        C.far_code.push_back((dx + 3) * 49 + (dy + 3) * 7 + (dz + 3));
+                    }
+                }
+            }
+            C.near_ptr[b + 1] = (int)C.near_idx.size();
+            C.far_ptr[b + 1] = (int)C.far_idx.size();
+        }
+    }
+    // samples (MLFMA.cpp:877-915,1133-1166)
+    for (int l = 2; l <= levels; ++l) {
+        MlLevel &C = M->lv[l];
+        C.L = std::max(3, multipoles(k, C.a));
+        C.K = 2 * C.L * C.L;
+        gauss_legendre(C.L, C.xt, C.wt);
+        gauss_legendre(2 * C.L, C.xp, C.wp);
+        C.kh.resize(3 * (size_t)C.K);
+        C.W.resize(C.K);
+        C.theta.resize(C.K);
+        for (int p = 0; p < 2 * C.L; ++p)
+            for (int t = 0; t < C.L; ++t) {
+                const int q = p * C.L + t;
+                const double th = kPi / 2 * (C.xt[t] + 1), ph = kPi * (C.xp[p] + 1);
+                C.kh[3 * q] = std::sin(th) * std::cos(ph);
+                C.kh[3 * q + 1] = std::sin(th) * std::sin(ph);
+                C.kh[3 * q + 2] = std::cos(th);
+                C.W[q] = C.wp[p] * C.wt[t];
+                C.theta[q] = th;
+            }
+    }
+    // device tables per level
+    for (int l = 2; l <= levels; ++l) {
+        MlLevel &C = M->lv[l];
+        ML_TRY(ml_alloc(M, &C.up, (size_t)C.nbox * C.K * 3));
+        ML_TRY(ml_alloc(M, &C.down, (size_t)C.nbox * C.K * 3));
+        ML_TRY(ml_upload(M, &C.far_ptr_d, C.far_ptr));
+        ML_TRY(ml_upload(M, &C.far_idx_d, C.far_idx.empty() ? std::vector<int>{0} : C.far_idx));
+        ML_TRY(ml_upload(M, &C.far_code_d, C.far_code.empty() ? std::vector<int>{0} : C.far_code));
+        ML_TRY(ml_upload(M, &C.parent_d, C.parent));
+        ML_TRY(ml_upload(M, &C.oct_d, C.octant));
+        // translation per offset code in use
+        std::vector<char> used(343, 0);
+        for (int c : C.far_code) used[c] = 1;
+        std::vector<cf> tr((size_t)343 * C.K, cf{0, 0});
+        for (int code = 0; code < 343; ++code) {
+            if (!used[code]) continue;
+            double D[3] = {C.a * (code / 49 - 3), C.a * ((code / 7) % 7 - 3), C.a * (code % 7 - 3)};
+            translation(k, D, C.kh, C.L, &tr[(size_t)code * C.K]);
+        }
+        ML_TRY(ml_upload(M, &C.trans, tr));
+        if (l < levels) {
+            // interpolation from level l+1 and the child-octant phases
+            MlLevel &N = M->lv[l + 1];
+            std::vector<int> tloc, ploc;
+            std::vector<double> tel, pel;
+            lagrange3(N.xt, C.xt, tloc, tel);
+            lagrange3(N.xp, C.xp, ploc, pel);
+            std::vector<int> ipc((size_t)C.K * 9);
+            std::vector<float> ipw((size_t)C.K * 9);
+            std::vector<std::vector<std::pair<int, double>>> tr_list(N.K);
+            for (int p = 0; p < 2 * C.L; ++p)
+                for (int t = 0; t < C.L; ++t) {
+                    const int q = p * C.L + t;
+                    int j = 0;
+                    for (int ia = 0; ia < 3; ++ia)
+                        for (int ib = 0; ib < 3; ++ib, ++j) {
+                            const int colc = ploc[3 * p + ia] * N.L + tloc[3 * t + ib];
+                            const double w = pel[3 * p + ia] * tel[3 * t + ib];
+                            ipc[(size_t)q * 9 + j] = colc;
+                            ipw[(size_t)q * 9 + j] = (float)w;
+                            tr_list[colc].push_back({q, w * C.W[q] / N.W[colc]});
+                        }
+                }
+            ML_TRY(ml_upload(M, &C.ip_col, ipc));
+            ML_TRY(ml_upload(M, &C.ip_w, ipw));
+            std::vector<cf> ph((size_t)8 * C.K);
+            for (int o = 0; o < 8; ++o) {
+                const double r[3] = {-N.a * ((o >> 2) - 0.5), -N.a * (((o >> 1) & 1) - 0.5), -N.a * ((o & 1) - 0.5)};
+                for (int q = 0; q < C.K; ++q) {
+                    double arg = -k * (C.kh[3 * q] * r[0] + C.kh[3 * q + 1] * r[1] + C.kh[3 * q + 2] * r[2]);
+                    ph[(size_t)o * C.K + q] = {(float)std::cos(arg), (float)std::sin(arg)};
+                }
+            }
+            ML_TRY(ml_upload(M, &C.ph, ph));
+            std::vector<int> chp(C.ch_ptr), chi(C.ch_idx);
+            ML_TRY(ml_upload(M, &C.ch_ptr_d, chp));
+            ML_TRY(ml_upload(M, &C.ch_idx_d, chi.empty() ? std::vector<int>{0} : chi));
+            // anterpolation into level l+1 (transpose, weighted by the quadrature weights)
+            std::vector<int> anp(N.K + 1, 0), ank;
+            std::vector<float> anw;
+            for (int qc = 0; qc < N.K; ++qc) {
+                for (auto &e : tr_list[qc]) {
+                    ank.push_back(e.first);
+                    anw.push_back((float)e.second);
+                }
+                anp[qc + 1] = (int)ank.size();
+            }
+            ML_TRY(ml_upload(M, &N.an_ptr, anp));
+            ML_TRY(ml_upload(M, &N.an_kp, ank));
+            ML_TRY(ml_upload(M, &N.an_w, anw));
+        }
+    }
+    // leaf data: permutation, leaf of each position, centres
+    MlLevel &F = M->lv[levels];
+    std::vector<int> leaf_of(n);
+    for (int b = 0; b < F.nbox; ++b)
+        for (int i = F.ubeg[b]; i < F.uend[b]; ++i) leaf_of[i] = b;
+    ML_TRY(ml_upload(M, &M->perm, order));
+    ML_TRY(ml_upload(M, &M->leaf_of, leaf_of));
+    ML_TRY(ml_upload(M, &M->leaf_cen, F.cen));
+    ML_TRY(ml_upload(M, &M->kh_leaf, F.kh));
+    ML_TRY(ml_upload(M, &F.ubeg_d, F.ubeg));
+    ML_TRY(ml_upload(M, &F.uend_d, F.uend));
+    // near matrix structure (box order)
+    std::vector<long long> rp(n + 1, 0);
+    std::vector<int> cols;
+    for (int b = 0; b < F.nbox; ++b) {
+        for (int i = F.ubeg[b]; i < F.uend[b]; ++i) {
+            for (int e = F.near_ptr[b]; e < F.near_ptr[b + 1]; ++e) {
+                const int c = F.near_idx[e];
+                for (int j = F.ubeg[c]; j < F.uend[c]; ++j) cols.push_back(j);
+            }
+            rp[i + 1] = (long long)cols.size();
+        }
+    }
+    M->nnz = (long long)cols.size();
+    ML_TRY(ml_upload(M, &M->row_ptr, rp));
+    ML_TRY(ml_upload(M, &M->col, cols));
+    ML_TRY(ml_alloc(M, &M->val, cols.size()));
+    ML_TRY(ml_alloc(M, &M->rad, (size_t)n * F.K * 3));
+    ML_TRY(ml_alloc(M, &M->rec, (size_t)n * F.K * 3));
+    ML_TRY(ml_alloc(M, &M->xb, n));
+    ML_TRY(ml_alloc(M, &M->yn, n));
+    // coefficients (MLFMA.cpp:382-400)
+    std::vector<double> coeff(F.K);
+    for (int q = 0; q < F.K; ++q) coeff[q] = F.W[q] * std::sin(F.theta[q]) * k * k * kZ0 / 32.0;
+    double *coeff_d = nullptr;
+    ML_TRY(ml_upload(M, &coeff_d, coeff));
+    hipStream_t s = ctx->stream;
+    {
+        ProfScope ps(ctx, "mlfma_setup", 0);
+        const long long tot = (long long)n * F.K;
+        k_ml_radrec<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap,
+                                                                   mesh->ram, mesh->rlen64, M->perm, M->leaf_of,
+                                                                   M->leaf_cen, M->kh_leaf, coeff_d, n, F.K, k,
+                                                                   M->alpha, M->rad, M->rec);
+        const double ce = k * kZ0 / (4.0 * kPi);
+        const float cE = (float)(M->alpha * ce), cM = (float)((1.0 - M->alpha) * kZ0 / (16.0 * kPi));
+        if (M->kind == EMFA_EFIE)
+            k_ml_near_fill<EMFA_EFIE><<<(n + 3) / 4, 256, 0, s>>>(
+                mesh->tg32, mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap, mesh->ram, mesh->rlen32, M->perm,
+                M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), (float)ce, 0.0f, k, 1.0, 0, M->val);
+        else
+            k_ml_near_fill<EMFA_CFIE><<<(n + 3) / 4, 256, 0, s>>>(
+                mesh->tg32, mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap, mesh->ram, mesh->rlen32, M->perm,
+                M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), cE, cM, k, M->alpha, prm->cfie_mode,
+                M->val);
+        EMFA_HIP_CHECK(hipGetLastError());
+    }
+    EMFA_HIP_CHECK(hipStreamSynchronize(s));
+    *out = M;
+    return EMFA_OK;
+}
+#undef ML_TRY
+
+static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, double2 *y) {
+    hipStream_t s = ctx->stream;
+    const int n = M->n, lvl = M->levels;
+    MlLevel &F = M->lv[lvl];
+    k_ml_gather<<<(n + 255) / 256, 256, 0, s>>>(x, M->perm, n, M->xb);
+    {
+        ProfScope ps(ctx, "mlfma_near", (double)M->nnz * 12);
+        k_ml_near<<<(n + 3) / 4, 256, 0, s>>>(M->row_ptr, M->col, M->val, M->xb, n, M->yn);
+    }
+    {
+        ProfScope ps(ctx, "mlfma_agg", 0);
+        k_ml_agg<<<dim3(F.nbox, (F.K + 255) / 256), 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.up);
+    }
+    for (int l = lvl - 1; l >= 2; --l) {
+        MlLevel &C = M->lv[l], &N = M->lv[l + 1];
+        ProfScope ps(ctx, "mlfma_up", 0);
+        k_ml_up<<<dim3(C.nbox, (C.K + 255) / 256), 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w,
+                                                                 C.ph, C.K, N.K, N.up, C.up);
+    }
+    for (int l = 2; l <= lvl; ++l) {
+        MlLevel &C = M->lv[l];
+        MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;
+        ProfScope ps(ctx, "mlfma_down", 0);
+        k_ml_down<<<dim3(C.nbox, (C.K + 255) / 256), 256, 0, s>>>(
+            C.far_ptr_d, C.far_idx_d, C.far_code_d, C.trans, C.up, C.K, C.parent_d, C.oct_d, C.an_ptr, C.an_kp,
+            C.an_w, P ? P->ph : nullptr, P ? P->K : 0, P ? P->down : nullptr, C.down);
+    }
+    {
+        ProfScope ps(ctx, "mlfma_out", 0);
+        k_ml_out<<<(n + 3) / 4, 256, 0, s>>>(M->rec, F.down, M->leaf_of, M->perm, M->yn, n, F.K, y);
+    }
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+}  // namespace emfa
+
+using namespace emfa;
+
+extern "C" {
+
+emfa_status emfa_mlfma_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out) {
+    EMFA_REQUIRE(ctx && mesh && p && out, EMFA_E_INVALID, "emfa_mlfma_create: null argument");
+    *out = nullptr;
+    EMFA_REQUIRE(p->kind == EMFA_EFIE || p->kind == EMFA_CFIE, EMFA_E_INVALID, "emfa_mlfma_create: kind");
+    EMFA_REQUIRE(p->k > 0 && p->box_threshold > 0, EMFA_E_INVALID, "emfa_mlfma_create: k and threshold must be > 0");
+    EMFA_REQUIRE(!mesh->rwg_host.empty(), EMFA_E_INVALID, "emfa_mlfma_create: mesh without host copy");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    return ml_build(ctx, mesh, p, out);
+}
+
+emfa_status emfa_mlfma_destroy(emfa_mlfma *m) {
+    if (!m) return EMFA_OK;
+    ml_free(m);
+    delete m;
+    return EMFA_OK;
+}
+
+emfa_status emfa_mlfma_get_info(const emfa_mlfma *m, emfa_mlfma_info *info) {
+    EMFA_REQUIRE(m && info, EMFA_E_INVALID, "emfa_mlfma_get_info: null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->levels = m->levels;
+    info->near_nnz = m->nnz;
+    info->bytes = m->bytes;
+    for (int l = 0; l <= m->levels && l < 16; ++l) {
+        info->boxes[l] = m->lv[l].nbox;
+        info->multipoles[l] = m->lv[l].L;
+        info->box_edge[l] = m->lv[l].a;
+    }
+    return EMFA_OK;
+}
+
+emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, emfa_mat *Y) {
+    EMFA_REQUIRE(ctx && m && X && Y, EMFA_E_INVALID, "emfa_mlfma_matvec: null argument");
+    EMFA_REQUIRE(X->rows == m->n && Y->rows == m->n && X->cols == Y->cols && X->prec == EMFA_F64 &&
+                     Y->prec == EMFA_F64 && X != Y,
+                 EMFA_E_INVALID, "emfa_mlfma_matvec: X, Y must be distinct F64 n x m");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    const int n = m->n, nc = X->cols;
+    if (nc == 1) {
+        emfa_status st = ml_apply(ctx, m, (const double2 *)X->d, (double2 *)Y->d);
+        if (st) return st;
+        EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return EMFA_OK;
+    }
+    void *scr = nullptr;
+    emfa_status st = ctx_scratch(ctx, 2 * (size_t)n * sizeof(double2), &scr);
+    if (st) return st;
+    double2 *xin = (double2 *)scr, *yout = xin + n;
+    for (int c = 0; c < nc; ++c) {
+        EMFA_HIP_CHECK(hipMemcpy2DAsync(xin, sizeof(double2), (const double2 *)X->d + c, nc * sizeof(double2),
+                                        sizeof(double2), n, hipMemcpyDeviceToDevice, ctx->stream));
+        if ((st = ml_apply(ctx, m, xin, yout))) return st;
+        EMFA_HIP_CHECK(hipMemcpy2DAsync((double2 *)Y->d + c, nc * sizeof(double2), yout, sizeof(double2),
+                                        sizeof(double2), n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return EMFA_OK;
+}
+
+emfa_status emfa_mlfma_gmres(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, int restart, double tol,
+                             int max_restart, emfa_gmres_report *rep) {
+    EMFA_REQUIRE(ctx && m && B && X, EMFA_E_INVALID, "emfa_mlfma_gmres: null argument");
+    auto op = [&](const double2 *in, double2 *out) -> emfa_status { return ml_apply(ctx, m, in, out); };
+    return gmres_impl(ctx, m->n, op, B, X, restart, tol, max_restart, rep);
+}
+
+}  // extern "C"

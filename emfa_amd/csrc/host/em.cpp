@@ -176,12 +176,6 @@ void DenseMoM::solve() {
         Z = B = nullptr;
     };
     try {
-        check(emfa_mat_create(ctx_, n, n, prec_, &Z), "impedance matrix");
-        emfa_fill_params fp{fillKind(), quad_, k_, alpha(), sigma_.real(), sigma_.imag(), cfie_mode_};
-        check(emfa_fill(ctx_, dmesh_, &fp, Z), "fill");
-        check(emfa_ctx_sync(ctx_), "fill");
-        timeLog("fillZ");
-
         std::vector<double> inc(9 * (size_t)ncols_);
         const double d2r = kPi / 180.0;
         for (int j = 0; j < ncols_; ++j) {
@@ -204,7 +198,16 @@ void DenseMoM::solve() {
         timeLog("fillV");
 
         check(emfa_mat_create(ctx_, n, ncols_, EMFA_F64, &X_), "solution");
-        if (use_gmres_) {
+        const bool matrix_free = solveMatrixFree(B);
+        if (!matrix_free) {
+            check(emfa_mat_create(ctx_, n, n, prec_, &Z), "impedance matrix");
+            emfa_fill_params fp{fillKind(), quad_, k_, alpha(), sigma_.real(), sigma_.imag(), cfie_mode_};
+            check(emfa_fill(ctx_, dmesh_, &fp, Z), "fill");
+            check(emfa_ctx_sync(ctx_), "fill");
+            timeLog("fillZ");
+        }
+        if (matrix_free) {
+        } else if (use_gmres_) {
             emfa_gmres_report rep{};
             emfa_status st = emfa_gmres(ctx_, Z, B, X_, gmres_m_, gmres_tol_, 1000, &rep);
             logger_ << "  -> GMRES iterations: " << rep.iterations << ", restarts: " << rep.restarts
@@ -233,7 +236,7 @@ void DenseMoM::solve() {
         I_.assign((size_t)n * ncols_, {0, 0});
         check(emfa_mat_download(X_, reinterpret_cast<double *>(I_.data())), "solution download");
         timeLog("solve");
-        if (write_matrix_) {
+        if (write_matrix_ && Z) {
             std::vector<std::complex<double>> Zh((size_t)n * n), Bh((size_t)n * ncols_);
             check(emfa_mat_download(Z, reinterpret_cast<double *>(Zh.data())), "Z download");
             check(emfa_mat_download(B, reinterpret_cast<double *>(Bh.data())), "V download");
@@ -443,6 +446,57 @@ void CFIE::init(component::ConfigLoader *ploader) {
 // EMSolver/CFIE.cpp:35: "_CFIE_" + to_string(alpha).substr(0, 3)
 std::string CFIE::folderTag() const { return "_CFIE_" + std::to_string((float)alpha_).substr(0, 3); }
 
+// ------------------------------------------------------------------ MLFMA
+void MLFMA::init(component::ConfigLoader *ploader) {
+    alpha_ = ploader->getAlpha();
+    iter_threshold_ = ploader->getDouble("iteration_threshold:", 1e-3);
+    max_iter_ = ploader->getInt("max_iteration_number:", 1000);
+    box_threshold_ = ploader->getDouble("mlfma_box_threshold:", 0.5);
+    precond_requested_ = ploader->getString("preconditioning:", "false") == "true";
+    if (!(iter_threshold_ > 0) || max_iter_ < 1 || !(box_threshold_ > 0))
+        throw component::ConfigInvalid("MLFMA: iteration_threshold, max_iteration_number and mlfma_box_threshold "
+                                       "must be positive");
+    DenseMoM::init(ploader);
+}
+
+// EMSolver/MLFMA.cpp:32: "_MLFMA_CFIE_" + to_string(alpha).substr(0, 3)
+std::string MLFMA::folderTag() const { return "_MLFMA_CFIE_" + std::to_string((float)alpha_).substr(0, 3); }
+
+void MLFMA::reportInfo(std::ostream &strm) const {
+    DenseMoM::reportInfo(strm);
+    strm << "MLFMA\n"
+         << "  -> box threshold: " << box_threshold_ << " wavelengths\n"
+         << "  -> GMRES(" << gmres_m_ << ") to " << iter_threshold_ << ", at most " << max_iter_ << " iterations\n"
+         << "  -> preconditioning: " << (precond_requested_ ? "requested, not built (unpreconditioned)" : "off")
+         << '\n';
+}
+
+bool MLFMA::solveMatrixFree(emfa_mat *B) {
+    emfa_mlfma_params mp{EMFA_CFIE, k_, alpha_, box_threshold_, cfie_mode_};
+    emfa_mlfma *op = nullptr;
+    emfa_status st = emfa_mlfma_create(ctx_, dmesh_, &mp, &op);
+    if (st == EMFA_E_INVALID) throw std::runtime_error(std::string("MLFMA: ") + emfa_last_error());
+    check(st, "MLFMA setup");
+    emfa_mlfma_info info{};
+    emfa_mlfma_get_info(op, &info);
+    logger_ << "MLFMA\n  -> levels: " << info.levels << ", near entries: " << info.near_nnz
+            << ", device bytes: " << info.bytes << '\n';
+    for (int l = 2; l <= info.levels; ++l)
+        logger_ << "  -> level " << l << ": " << info.boxes[l] << " boxes, L = " << info.multipoles[l] << '\n';
+    timeLog("MLFMA setup");
+    const int m = std::max(1, gmres_m_);
+    emfa_gmres_report rep{};
+    st = emfa_mlfma_gmres(ctx_, op, B, X_, m, iter_threshold_, (max_iter_ + m - 1) / m, &rep);
+    emfa_mlfma_destroy(op);
+    logger_ << "  -> Iterative number: " << rep.iterations << ", relative residual: " << rep.relres << '\n';
+    std::cout << "  -> Iterative number: " << rep.iterations << std::endl;
+    if (st == EMFA_E_NOCONV)
+        logger_ << "  -> WARNING: Iteration cannot converge\n";
+    else
+        check(st, "MLFMA GMRES");
+    return true;
+}
+
 }  // namespace mom
 
 namespace solver {
@@ -457,6 +511,8 @@ void loadSolver(const std::string &conf_file) {
         invoke_solver<mom::CFIE>(&loader);
     else if (solver_type == "MFIE")
         invoke_solver<mom::MFIE>(&loader);
+    else if (solver_type == "MLFMA")
+        invoke_solver<mom::MLFMA>(&loader);
     else
         throw component::ConfigInvalid("undefined EM solver type (" + solver_type + ')');
 }

@@ -89,6 +89,12 @@ class DenseMoM : public EM {
     virtual double alpha() const { return 1.0; }
     virtual std::string folderTag() const = 0;
 
+    // Matrix-free solvers override this: solve Z X_ = B without a dense Z
+    // and return true.  The dense path (fill, then LU or GMRES) runs otherwise.
+    virtual bool solveMatrixFree(emfa_mat *B) {
+        (void)B;
+        return false;
+    }
     void check(emfa_status st, const char *what) const;
     void calculateSurfaceCurrent(std::vector<component::CurrentData> *currents, int col) const;
     void writeBistatic();
@@ -140,6 +146,29 @@ class MFIE : public DenseMoM {
     int rhsKind() const override { return EMFA_MFIE; }
     double alpha() const override { return 0.0; }
     std::string folderTag() const override { return "_MFIE"; }
+};
+
+// EMSolver/MLFMA.cpp: the CFIE operator applied by the multilevel fast
+// multipole method (near matrix + aggregation / translation /
+// disaggregation on the device) and solved by GMRES(m) to
+// iteration_threshold (the reference iterates with BiCGSTAB,
+// MLFMA.cpp:402-452).  Keys: alpha:, iteration_threshold:,
+// max_iteration_number:, mlfma_box_threshold:; preconditioning: is read and
+// reported but the sparse approximate inverse is not part of this build.
+class MLFMA : public DenseMoM {
+  public:
+    void init(component::ConfigLoader *ploader) override;
+    void reportInfo(std::ostream &strm) const override;
+
+  protected:
+    int fillKind() const override { return EMFA_CFIE; }
+    int rhsKind() const override { return EMFA_CFIE; }
+    double alpha() const override { return alpha_; }
+    std::string folderTag() const override;
+    bool solveMatrixFree(emfa_mat *B) override;
+    double alpha_ = 0.5, box_threshold_ = 0.5, iter_threshold_ = 1e-3;
+    int max_iter_ = 1000;
+    bool precond_requested_ = false;
 };
 
 }  // namespace mom

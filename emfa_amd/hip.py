@@ -24,7 +24,8 @@ EXPORTS = [
     "emfa_mat_download", "emfa_mat_copy", "emfa_mat_rows", "emfa_mat_cols", "emfa_mat_prec", "emfa_mat_device_ptr",
     "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
-    "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset",
+    "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
+    "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres",
 ]
 
 _vp = ctypes.c_void_p
@@ -43,6 +44,16 @@ class RefineReport(ctypes.Structure):
 
 class GmresReport(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("relres", ctypes.c_double)]
+
+
+class MlfmaParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("k", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("box_threshold", ctypes.c_double), ("cfie_mode", ctypes.c_int)]
+
+
+class MlfmaInfo(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("near_nnz", ctypes.c_longlong), ("bytes", ctypes.c_size_t),
+                ("boxes", ctypes.c_int * 16), ("multipoles", ctypes.c_int * 16), ("box_edge", ctypes.c_double * 16)]
 
 
 class EmfaError(RuntimeError):
@@ -317,6 +328,40 @@ def gmres(ctx, Z, B, X, m=30, tol=1e-6, max_restart=100):
     if code not in (EMFA_OK, EMFA_E_NOCONV):
         _check(code)
     return rep, code == EMFA_OK
+
+
+class MLFMA:
+    """Multilevel fast multipole operator of a mesh (emfa_mlfma_*)."""
+
+    def __init__(self, ctx, mesh, k, alpha=0.5, box_threshold=0.25, kind=CFIE, cfie_mode=0):
+        self.ctx = ctx
+        self.h = _vp()
+        p = MlfmaParams(kind, float(k), float(alpha), float(box_threshold), int(cfie_mode))
+        _check(lib().emfa_mlfma_create(ctx.h, mesh.h, ctypes.byref(p), ctypes.byref(self.h)))
+        self.n = mesh.rwg.shape[0]
+
+    def info(self):
+        i = MlfmaInfo()
+        _check(lib().emfa_mlfma_get_info(self.h, ctypes.byref(i)))
+        lv = i.levels
+        return {"levels": lv, "near_nnz": i.near_nnz, "bytes": i.bytes, "boxes": list(i.boxes[:lv + 1]),
+                "multipoles": list(i.multipoles[:lv + 1]), "box_edge": list(i.box_edge[:lv + 1])}
+
+    def matvec(self, X, Y):
+        _check(lib().emfa_mlfma_matvec(self.ctx.h, self.h, X.h, Y.h))
+
+    def gmres(self, B, X, m=30, tol=1e-4, max_restart=50):
+        rep = GmresReport()
+        code = lib().emfa_mlfma_gmres(self.ctx.h, self.h, B.h, X.h, m, ctypes.c_double(tol), max_restart,
+                                      ctypes.byref(rep))
+        if code not in (EMFA_OK, EMFA_E_NOCONV):
+            _check(code)
+        return rep, code == EMFA_OK
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().emfa_mlfma_destroy(self.h)
+        self.h = _vp()
 
 
 def rcs(ctx, mesh, k, X, dirs, cols=None):

@@ -180,6 +180,41 @@ emfa_status emfa_rcs(emfa_ctx *ctx, const emfa_mesh *mesh, double k, const emfa_
                      const int *cols, double *out_db);
 
 /* ---- instrumentation -------------------------------------------------- */
+/* ---------------------------------------------------------------------
+ * MLFMA operator (EMSolver/MLFMA.cpp).  emfa_mlfma_create replaces
+ * MLFMA::buildBox, levelRootsAndWeights, calculateInterpolationMtr,
+ * calculateLevelK, calculateLevelTrans, calculateRadAndRecv and
+ * calculateNearZ (MLFMA.cpp:688-1422); emfa_mlfma_matvec replaces
+ * MLFMA::matrixVectorMultiply (MLFMA.cpp:454-509) and emfa_mlfma_gmres the
+ * iterative solve (MLFMA.cpp:402-452, GMRES(m) here).  Vectors are in RWG
+ * order (the reference keeps them in box order internally, as does this
+ * operator).  Near and far parts are complex FP32; self terms FP64. */
+typedef struct emfa_mlfma emfa_mlfma;
+typedef struct {
+    int kind;             /* EMFA_CFIE (MLFMA.cpp's formulation) or EMFA_EFIE */
+    double k;             /* wavenumber 2 pi f / c0 */
+    double alpha;         /* CFIE combination factor (alpha:) */
+    double box_threshold; /* bound on the finest box edge, in wavelengths (mlfma_box_threshold:) */
+    int cfie_mode;        /* CFIE self terms: 0 as Miscellaneous.cpp:32-77, 1 accurate */
+} emfa_mlfma_params;
+typedef struct {
+    int levels;           /* finest level index; boxes are at levels 1..levels */
+    long long near_nnz;   /* entries of the near matrix */
+    size_t bytes;         /* device bytes held by the operator */
+    int boxes[16];        /* non-empty boxes per level */
+    int multipoles[16];   /* L per level (levels >= 2) */
+    double box_edge[16];  /* box edge per level (m) */
+} emfa_mlfma_info;
+emfa_status emfa_mlfma_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out);
+emfa_status emfa_mlfma_destroy(emfa_mlfma *m);
+emfa_status emfa_mlfma_get_info(const emfa_mlfma *m, emfa_mlfma_info *info);
+/* Y = Z_mlfma X for F64 n x ncols matrices (column by column). */
+emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, emfa_mat *Y);
+/* GMRES(restart) with FP64 orthogonalisation on the MLFMA operator; X holds
+ * the initial guess.  EMFA_E_NOCONV when a column misses tol. */
+emfa_status emfa_mlfma_gmres(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, int restart, double tol,
+                             int max_restart, emfa_gmres_report *rep);
+
 /* When enabled, each launch of a named kernel family is bracketed by HIP
  * events on the context stream; totals are read with emfa_prof_get.
  * Families: "fill_pairs", "fill_gather", "lu_gemm", "lu_panel", "lu_trsm",
