@@ -97,6 +97,7 @@ struct emfa_lu {
     emfa_ctx *ctx = nullptr;
     emfa_mat *a = nullptr;  // factor (L unit-lower below diagonal, U on/above)
     int *ipiv = nullptr;    // device, 0-based absolute row of each pivot
+    int *perm = nullptr;    // device: row i of P A is row perm[i] of A (the interchanges composed)
     int info = 0;
 };
 
@@ -140,7 +141,7 @@ __host__ __device__ inline cx<R> cmul(cx<R> a, cx<R> b) {
 
 // LU driver entry (lu.hip)
 emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info);
-emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, emfa_mat *B);
+emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, const int *perm, emfa_mat *B);
 emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, emfa_mat *C, int negate,
                           int accumulate);
 emfa_status convert_copy(emfa_ctx *ctx, const emfa_mat *src, emfa_mat *dst);

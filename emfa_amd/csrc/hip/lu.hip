@@ -626,6 +626,129 @@ __global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, lon
     }
 }
 
+// --------------------------------------------------------------------------
+// Triangular solves in one launch per direction (look-back), for up to 64
+// right-hand sides.  Workgroup w owns 64-row block b (b = w forward, the
+// mirror backward) and all columns of the chunk: it folds in the solved
+// blocks in dependency order as their flags appear (L tile from the factor,
+// x tile through coherent loads), then solves its diagonal block with one
+// lane per row (v_readlane broadcast), writes x with coherent stores and
+// raises its flag.  Blocks depend only on lower workgroup ids, which the
+// dispatcher starts first, so every wait ends; the spin is bounded anyway
+// (flags[0] reports a timeout).  Forward reads the right-hand side through
+// the factor's row permutation, so no separate interchange pass is needed.
+// --------------------------------------------------------------------------
+constexpr int kLbCols = 64;
+
+template <typename R, bool UPPER>
+__global__ __launch_bounds__(256) void k_trsv_lb(const cx<R> *__restrict__ F, long long ldf, int N,
+                                                 const cx<R> *__restrict__ in, long long ldi,
+                                                 const int *__restrict__ perm, cx<R> *out, long long ldo, int c0,
+                                                 int mc, int *bflags, int flagval, int *flags) {
+    __shared__ cx<R> Lt[64][65];  // [row][k]
+    __shared__ cx<R> Xs[64][kLbCols];
+    __shared__ int s_dead;
+    const int nblk = (N + 63) / 64;
+    const int wg = blockIdx.x;
+    const int blk = UPPER ? nblk - 1 - wg : wg;
+    const int r0 = blk * 64, nr = min(64, N - r0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int CPT = kLbCols / 4;  // columns per thread: wave + 4 i
+    cx<R> acc[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int c = wave + 4 * i;
+        acc[i] = {R(0), R(0)};
+        if (lane < nr && c < mc) {
+            const int row = perm ? perm[r0 + lane] : r0 + lane;
+            acc[i] = in[(size_t)row * ldi + c0 + c];
+        }
+    }
+    if (tid == 0) s_dead = 0;
+    for (int jj = 0; jj < wg; ++jj) {
+        const int jb = UPPER ? nblk - 1 - jj : jj;
+        const int k0 = jb * 64, nk = min(64, N - k0);
+        // stage the factor tile while the producer may still be running
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int rr = e >> 6, cc = e & 63;
+            Lt[rr][cc] = (rr < nr && cc < nk) ? F[(size_t)(r0 + rr) * ldf + k0 + cc] : cx<R>{R(0), R(0)};
+        }
+        if (tid == 0) {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&bflags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != flagval) {
+                if (__hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                    ++spins > kSpinLimit) {
+                    __hip_atomic_store(&flags[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_dead = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        if (s_dead) break;
+        for (int e = tid; e < 64 * kLbCols; e += 256) {
+            const int kk = e / kLbCols, cc = e % kLbCols;
+            Xs[kk][cc] = (kk < nk && cc < mc) ? ld_co(&out[(size_t)(k0 + kk) * ldo + c0 + cc]) : cx<R>{R(0), R(0)};
+        }
+        __syncthreads();
+        for (int k = 0; k < nk; ++k) {
+            const cx<R> l = Lt[lane][k];
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                const cx<R> x = Xs[k][wave + 4 * i];
+                acc[i].re -= l.re * x.re - l.im * x.im;
+                acc[i].im -= l.re * x.im + l.im * x.re;
+            }
+        }
+        __syncthreads();
+    }
+    // diagonal block
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int rr = e >> 6, cc = e & 63;
+        Lt[rr][cc] = (rr < nr && cc < nr) ? F[(size_t)(r0 + rr) * ldf + r0 + cc] : cx<R>{R(0), R(0)};
+    }
+    __syncthreads();
+    if (!UPPER) {
+        for (int k = 0; k < nr - 1; ++k) {
+            const cx<R> l = Lt[lane][k];
+            const bool act = lane > k && lane < nr;
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                const R xr = bcast(acc[i].re, k), xi = bcast(acc[i].im, k);
+                if (act) {
+                    acc[i].re -= l.re * xr - l.im * xi;
+                    acc[i].im -= l.re * xi + l.im * xr;
+                }
+            }
+        }
+    } else {
+        for (int k = nr - 1; k >= 0; --k) {
+            const cx<R> d = Lt[k][k];
+            const R den = d.re * d.re + d.im * d.im;
+            const cx<R> inv = den > R(0) ? cx<R>{d.re / den, -d.im / den} : cx<R>{R(0), R(0)};
+            const cx<R> u = Lt[lane][k];
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                if (lane == k) acc[i] = cmul(acc[i], inv);
+                const R xr = bcast(acc[i].re, k), xi = bcast(acc[i].im, k);
+                if (lane < k) {
+                    acc[i].re -= u.re * xr - u.im * xi;
+                    acc[i].im -= u.re * xi + u.im * xr;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int c = wave + 4 * i;
+        if (lane < nr && c < mc) st_co(&out[(size_t)(r0 + lane) * ldo + c0 + c], acc[i]);
+    }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&bflags[blk], flagval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Row interchanges as a permutation gather.  k_perm_build replays the
 // interchanges k0..k1-1 (ipiv, LAPACK order) on the index set they touch with
 // one wavefront and emits (destination row, source row) pairs; k_perm_gather
@@ -887,7 +1010,8 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     const int nb = plan.nb;
 
     // panel geometry: co-resident workgroups, rows per workgroup in LDS
-    const size_t lds_cap = 150 * 1024;
+    size_t lds_cap = 150 * 1024;
+    if (const char *s = getenv("EMFA_PANEL_LDS_KB")) lds_cap = (size_t)std::max(16, std::min(150, atoi(s))) * 1024;
     int G_max = std::max(1, ctx->cu_share > 0 ? std::min(ctx->cu_share, ctx->num_cu) : ctx->num_cu);
     if (const char *s = getenv("EMFA_PANEL_G")) G_max = std::max(1, std::min(G_max, atoi(s)));
     int panel_rows_min = 16;
@@ -1063,11 +1187,36 @@ emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info) 
 
 // Solve with the factor (same precision as B).
 template <typename R>
-static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipiv, emfa_mat *Bm) {
+static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipiv, const int *perm, emfa_mat *Bm) {
     const int N = LUm->rows, m = Bm->cols;
     const cx<R> *F = (const cx<R> *)LUm->d;
     cx<R> *B = (cx<R> *)Bm->d;
     const long long ldf = N, ldb = m;
+    // few right-hand sides: one look-back launch per direction and column chunk
+    // (opt-in: the coherent tile loads make it slower than the blocked path)
+    int lb_max = 0;
+    if (const char *s = getenv("EMFA_SOLVE_LB_MAX")) lb_max = std::min(atoi(s), 1 << 20);
+    if (perm && m <= lb_max) {
+        const int nblk = (N + 63) / 64;
+        size_t yb = (((size_t)N * m * sizeof(cx<R>)) + 255) & ~(size_t)255;
+        void *scr = nullptr;
+        emfa_status st = ctx_scratch(ctx, yb + (size_t)nblk * sizeof(int) + 256, &scr);
+        if (st) return st;
+        cx<R> *Y = (cx<R> *)scr;
+        int *bflags = (int *)((char *)scr + yb);
+        ProfScope ps(ctx, "solve_trsm", 8.0 * (double)N * N * m);
+        k_zero_u32<<<(nblk + 255) / 256, 256, 0, ctx->stream>>>((unsigned *)bflags, nblk);
+        for (int c0 = 0, q = 0; c0 < m; c0 += kLbCols, ++q)
+            k_trsv_lb<R, false><<<nblk, 256, 0, ctx->stream>>>(F, ldf, N, B, ldb, perm, Y, ldb, c0,
+                                                                std::min(kLbCols, m - c0), bflags, 2 * q + 1,
+                                                                ctx->flags);
+        for (int c0 = 0, q = 0; c0 < m; c0 += kLbCols, ++q)
+            k_trsv_lb<R, true><<<nblk, 256, 0, ctx->stream>>>(F, ldf, N, Y, ldb, nullptr, B, ldb, c0,
+                                                               std::min(kLbCols, m - c0), bflags, 2 * q + 2,
+                                                               ctx->flags);
+        EMFA_HIP_CHECK(hipGetLastError());
+        return EMFA_OK;
+    }
     int nb = 256;
     if (const char *s = getenv("EMFA_SOLVE_NB")) nb = std::max(16, atoi(s));
     size_t bt_b = (((size_t)2 * m * 2 * nb * sizeof(R)) + 255) & ~(size_t)255;
@@ -1098,9 +1247,19 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
     return EMFA_OK;
 }
 
-emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, emfa_mat *B) {
-    if (LU->prec == EMFA_F32) return solve_impl<float>(ctx, LU, ipiv, B);
-    return solve_impl<double>(ctx, LU, ipiv, B);
+emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, const int *perm, emfa_mat *B) {
+    if (LU->prec == EMFA_F32) return solve_impl<float>(ctx, LU, ipiv, perm, B);
+    return solve_impl<double>(ctx, LU, ipiv, perm, B);
+}
+
+// C = (acc ? C : 0) + sign * sum_s P_s, slices summed in index order
+__global__ __launch_bounds__(256) void k_sum_slices(const double *__restrict__ P, int S, long long n,
+                                                    double *__restrict__ C, double sign, int acc) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double v = 0;
+    for (int s = 0; s < S; ++s) v += P[(size_t)s * n + i];
+    C[i] = (acc ? C[i] : 0.0) + sign * v;
 }
 
 // C = beta C + sign * A X in FP64 (A any precision, X and C FP64).
@@ -1109,11 +1268,23 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     const int N = A->rows, K = A->cols, m = X->cols;
     EMFA_REQUIRE(X->rows == K && C->rows == N && C->cols == m && X->prec == EMFA_F64 && C->prec == EMFA_F64,
                  EMFA_E_INVALID, "gemm_residual: shape/precision mismatch");
-    size_t bt_b = (size_t)2 * m * 2 * K * sizeof(double);
+    size_t bt_b = (((size_t)2 * m * 2 * K * sizeof(double)) + 255) & ~(size_t)255;
+    // Few right-hand sides leave most CUs idle: split the inner dimension
+    // into S slices (multiples of 64 reals) with partial results summed in a
+    // fixed order afterwards.
+    const long long tiles = (long long)((N + 63) / 64) * ((2 * m + 63) / 64);
+    int S = 1;  // measured no faster on gfx950 for 32 right-hand sides; opt-in
+    if (const char *e = getenv("EMFA_RESID_SPLIT"))
+        S = e[0] == 'a' ? (int)std::min<long long>(16, std::max<long long>(1, (2LL * ctx->num_cu + tiles - 1) / tiles))
+                        : std::max(1, std::min(64, atoi(e)));
+    long long kc = ((2LL * K + S - 1) / S + 63) / 64 * 64;
+    S = (int)((2LL * K + kc - 1) / kc);
+    const size_t p_b = S > 1 ? (size_t)S * N * 2 * m * sizeof(double) : 0;
     void *scr = nullptr;
-    emfa_status st = ctx_scratch(ctx, bt_b + 1024, &scr);
+    emfa_status st = ctx_scratch(ctx, bt_b + p_b + 1024, &scr);
     if (st) return st;
     double *Bt = (double *)scr;
+    double *P = (double *)((char *)scr + bt_b);
     long long tot = (long long)K * m;
     k_expand<double><<<(int)((tot + 255) / 256), 256, 0, ctx->stream>>>((const cx<double> *)X->d, K, m, Bt);
     EMFA_HIP_CHECK(hipGetLastError());
@@ -1130,7 +1301,26 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     g.alpha_neg = negate ? 1 : 0;
     g.beta = accumulate ? 1 : 0;
     ProfScope ps(ctx, "refine_gemm", 8.0 * (double)N * m * K);
-    return A->prec == EMFA_F32 ? gemm_launch_f32_f64(ctx->stream, g) : gemm_launch_f64(ctx->stream, g);
+    if (S == 1) return A->prec == EMFA_F32 ? gemm_launch_f32_f64(ctx->stream, g) : gemm_launch_f64(ctx->stream, g);
+    const size_t es = A->prec == EMFA_F32 ? sizeof(float) : sizeof(double);
+    for (int sl = 0; sl < S; ++sl) {
+        const long long k0 = (long long)sl * kc;
+        GemmArgs gs = g;
+        gs.Kr = (int)std::min<long long>(kc, 2LL * K - k0);
+        gs.A = (const char *)A->d + k0 * es;
+        gs.Bt = Bt + k0;
+        gs.C = P + (size_t)sl * N * 2 * m;
+        gs.ldc = 2LL * m;
+        gs.alpha_neg = 0;
+        gs.beta = 0;
+        emfa_status s2 = A->prec == EMFA_F32 ? gemm_launch_f32_f64(ctx->stream, gs) : gemm_launch_f64(ctx->stream, gs);
+        if (s2) return s2;
+    }
+    const long long tot2 = (long long)N * 2 * m;
+    k_sum_slices<<<(unsigned)((tot2 + 255) / 256), 256, 0, ctx->stream>>>(P, S, tot2, (double *)C->d,
+                                                                          negate ? -1.0 : 1.0, accumulate);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
 }
 
 }  // namespace emfa
@@ -1162,6 +1352,32 @@ emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu *
     }
     st = convert_copy(ctx, Z, lu->a);
     if (!st) st = lu_factor_inplace(ctx, lu->a, lu->ipiv, &lu->info);
+    if (!st) {
+        // compose the interchanges into one row permutation (used by the solves)
+        const int n = Z->rows;
+        std::vector<int> piv(n), perm(n);
+        if (hipMemcpy(piv.data(), lu->ipiv, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+            ctx_alloc(ctx, (void **)&lu->perm, sizeof(int) * n) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("emfa_lu_factor: permutation setup failed");
+            st = EMFA_E_HIP;
+        } else {
+            for (int i = 0; i < n; ++i) perm[i] = i;
+            for (int i = 0; i < n; ++i) {
+                const int p = piv[i];
+                if (p < 0 || p >= n) {
+                    set_error("emfa_lu_factor: pivot %d out of range at column %d", p, i);
+                    st = EMFA_E_HIP;
+                    break;
+                }
+                std::swap(perm[i], perm[p]);
+            }
+            if (!st && hipMemcpy(lu->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+                set_error("emfa_lu_factor: permutation upload failed");
+                st = EMFA_E_HIP;
+            }
+        }
+    }
     if (st) {
         emfa_lu_destroy(lu);
         return st;
@@ -1179,6 +1395,7 @@ emfa_status emfa_lu_destroy(emfa_lu *lu) {
     emfa_ctx *c = lu->ctx;
     (void)hipStreamSynchronize(c->stream);
     ctx_free(c, lu->ipiv, sizeof(int) * lu->a->rows);
+    if (lu->perm) ctx_free(c, lu->perm, sizeof(int) * lu->a->rows);
     emfa_mat_destroy(lu->a);
     delete lu;
     return EMFA_OK;
@@ -1189,16 +1406,28 @@ emfa_status emfa_lu_solve(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B) {
     EMFA_REQUIRE(B->rows == lu->a->rows, EMFA_E_INVALID, "emfa_lu_solve: B has %d rows, factor %d", B->rows,
                  lu->a->rows);
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
-    if (B->prec == lu->a->prec) return lu_solve_inplace(ctx, lu->a, lu->ipiv, B);
-    emfa_mat *T = nullptr;
-    emfa_status st = emfa_mat_create(ctx, B->rows, B->cols, lu->a->prec, &T);
+    emfa_status st = EMFA_OK;
+    if (B->prec == lu->a->prec) {
+        st = lu_solve_inplace(ctx, lu->a, lu->ipiv, lu->perm, B);
+    } else {
+        emfa_mat *T = nullptr;
+        st = emfa_mat_create(ctx, B->rows, B->cols, lu->a->prec, &T);
+        if (st) return st;
+        st = convert_copy(ctx, B, T);
+        if (!st) st = lu_solve_inplace(ctx, lu->a, lu->ipiv, lu->perm, T);
+        if (!st) st = convert_copy(ctx, T, B);
+        (void)hipStreamSynchronize(ctx->stream);
+        emfa_mat_destroy(T);
+    }
     if (st) return st;
-    st = convert_copy(ctx, B, T);
-    if (!st) st = lu_solve_inplace(ctx, lu->a, lu->ipiv, T);
-    if (!st) st = convert_copy(ctx, T, B);
-    (void)hipStreamSynchronize(ctx->stream);
-    emfa_mat_destroy(T);
-    return st;
+    int fl = 0;
+    EMFA_HIP_CHECK(hipMemcpyAsync(&fl, ctx->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (fl != 0) {
+        set_error("emfa_lu_solve: a bounded wait of the triangular solve timed out");
+        return EMFA_E_HIP;
+    }
+    return EMFA_OK;
 }
 
 emfa_status emfa_lu_pivots(const emfa_lu *lu, int *ipiv_host) {
