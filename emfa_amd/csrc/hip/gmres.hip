@@ -37,13 +37,19 @@ __global__ __launch_bounds__(256) void k_gemv(const cx<TA> *__restrict__ Z, int 
     if (lane == 0) y[row] = make_double2(sr, si);
 }
 
-// h[i] = V_i^H w for i < k (one workgroup per i)
+// Dot products h[i] = V_i^H w for i < k in two deterministic passes:
+// k_dots writes partial sums part[i][c] over chunk c of the rows (grid
+// chunks x k), k_dots_sum adds the chunks of each i in order.
+constexpr int kDotChunks = 64;
+
 __global__ __launch_bounds__(256) void k_dots(const double2 *__restrict__ V, int n, const double2 *__restrict__ w,
-                                              double2 *__restrict__ h) {
-    const int i = blockIdx.x;
+                                              double2 *__restrict__ part) {
+    const int i = blockIdx.y, c = blockIdx.x, nc = gridDim.x;
     const double2 *v = V + (size_t)i * n;
+    const int per = (n + nc - 1) / nc;
+    const int t0 = c * per, t1 = min(n, t0 + per);
     double sr = 0, si = 0;
-    for (int t = threadIdx.x; t < n; t += 256) {
+    for (int t = t0 + threadIdx.x; t < t1; t += 256) {
         double2 a = v[t], b = w[t];
         sr += a.x * b.x + a.y * b.y;  // conj(a) * b
         si += a.x * b.y - a.y * b.x;
@@ -59,7 +65,19 @@ __global__ __launch_bounds__(256) void k_dots(const double2 *__restrict__ V, int
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) h[i] = make_double2(rr[0], ri[0]);
+    if (threadIdx.x == 0) part[(size_t)i * nc + c] = make_double2(rr[0], ri[0]);
+}
+
+__global__ __launch_bounds__(64) void k_dots_sum(const double2 *__restrict__ part, int k, int nc,
+                                                 double2 *__restrict__ h) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= k) return;
+    double sr = 0, si = 0;
+    for (int c = 0; c < nc; ++c) {
+        sr += part[(size_t)i * nc + c].x;
+        si += part[(size_t)i * nc + c].y;
+    }
+    h[i] = make_double2(sr, si);
 }
 
 // w -= sum_{i<k} h_i V_i   (sign = -1)  or  x += sum y_i V_i (sign = +1)
@@ -108,7 +126,9 @@ emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(con
     EMFA_REQUIRE(m >= 1 && m <= 512 && max_restart >= 1, EMFA_E_INVALID, "gmres: bad m/max_restart");
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     size_t vb = (size_t)(m + 1) * n * sizeof(double2);
-    size_t total = vb + 4 * (size_t)n * sizeof(double2) + 2 * (size_t)(m + 2) * sizeof(double2) + 4096;
+    const int nchunk = std::max(1, std::min(kDotChunks, (n + 2047) / 2048));
+    size_t total = vb + 4 * (size_t)n * sizeof(double2) + 2 * (size_t)(m + 2) * sizeof(double2) +
+                   (size_t)(m + 2) * nchunk * sizeof(double2) + 8192;
     void *scr = nullptr;
     emfa_status st = ctx_scratch(ctx, total, &scr);
     if (st) return st;
@@ -124,11 +144,16 @@ emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(con
     double2 *xv = take(n * sizeof(double2));
     double2 *h = take((m + 2) * sizeof(double2));
     double2 *h2 = take((m + 2) * sizeof(double2));
+    double2 *part = take((size_t)(m + 2) * nchunk * sizeof(double2));
+    auto dots = [&](const double2 *Vb, int k, const double2 *wv, double2 *out) {
+        k_dots<<<dim3(nchunk, k), 256, 0, ctx->stream>>>(Vb, n, wv, part);
+        k_dots_sum<<<(k + 63) / 64, 64, 0, ctx->stream>>>(part, k, nchunk, out);
+    };
     const int nb = (n + 255) / 256;
     hipStream_t s = ctx->stream;
     using cd = std::complex<double>;
     auto norm = [&](const double2 *v, double *out) -> emfa_status {
-        k_dots<<<1, 256, 0, s>>>(v, n, v, h2);
+        dots(v, 1, v, h2);
         EMFA_HIP_CHECK(hipGetLastError());
         double2 r;
         EMFA_HIP_CHECK(hipMemcpyAsync(&r, h2, sizeof(double2), hipMemcpyDeviceToHost, s));
@@ -169,9 +194,9 @@ emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(con
                 if ((st = matvec(V + (size_t)j * n, w))) return st;
                 ++total_it;
                 // classical Gram-Schmidt, applied twice
-                k_dots<<<j + 1, 256, 0, s>>>(V, n, w, h);
+                dots(V, j + 1, w, h);
                 k_combine<<<nb, 256, 0, s>>>(V, n, j + 1, h, -1.0, w);
-                k_dots<<<j + 1, 256, 0, s>>>(V, n, w, h2);
+                dots(V, j + 1, w, h2);
                 k_combine<<<nb, 256, 0, s>>>(V, n, j + 1, h2, -1.0, w);
                 EMFA_HIP_CHECK(hipGetLastError());
                 EMFA_HIP_CHECK(hipMemcpyAsync(hh.data(), h, sizeof(double2) * (j + 1), hipMemcpyDeviceToHost, s));

@@ -15,6 +15,8 @@
 //   rad / rec     3-point radiation / receive patterns of each RWG about its
 //                 leaf-box centre; the receive pattern carries the quadrature
 //                 coefficient w sin(theta) k^2 Z0 / 32
+// Patterns are stored [box or RWG][K][3]: the three components of a sample
+// sit together (a component-major layout measured slower on gfx950).
 //   near          CFIE (or EFIE) entries of all RWG pairs in neighbouring leaf
 //                 boxes, FP32 with FP64 self terms, stored CSR in box order
 // Kernels (FP32 complex, one writer per output, no atomics, so repeated
@@ -30,6 +32,7 @@
 #include <complex>
 #include <cstring>
 #include <functional>
+#include <thread>
 #include <vector>
 
 #include "em_device.h"
@@ -58,6 +61,7 @@ struct MlLevel {
     int *an_ptr = nullptr, *an_kp = nullptr;  // anterpolation from the parent level
     float *an_w = nullptr;
     int *ubeg_d = nullptr, *uend_d = nullptr;  // leaf level: unknown ranges (box order)
+    int *near_ptr_d = nullptr, *near_idx_d = nullptr, *ix_d = nullptr;
 };
 
 }  // namespace emfa
@@ -387,9 +391,10 @@ __global__ __launch_bounds__(256) void k_ml_near(const long long *__restrict__ r
 // leaf aggregation: up[b][q] = sum_{i in b} x_i rad[i][q]
 __global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, const int *__restrict__ uend,
                                                 const cf *__restrict__ rad, const cf *__restrict__ xb, int K,
-                                                cf *__restrict__ up) {
-    const int b = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
-    if (q >= K) return;
+                                                int nbox, cf *__restrict__ up) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int b = (int)(idx / K), q = (int)(idx % K);
+    if (b >= nbox) return;
     cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
     for (int i = ubeg[b]; i < uend[b]; ++i) {
         const cf x = xb[i];
@@ -408,9 +413,10 @@ __global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, co
 __global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, const int *__restrict__ ch_idx,
                                                const int *__restrict__ ch_oct, const int *__restrict__ ip_col,
                                                const float *__restrict__ ip_w, const cf *__restrict__ ph, int K,
-                                               int Kc, const cf *__restrict__ upc, cf *__restrict__ upp) {
-    const int p = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
-    if (q >= K) return;
+                                               int Kc, int nbox, const cf *__restrict__ upc, cf *__restrict__ upp) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int p = (int)(idx / K), q = (int)(idx % K);
+    if (p >= nbox) return;
     int cols[9];
     float ws[9];
 #pragma unroll
@@ -426,9 +432,10 @@ __global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, c
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
             const cf *v = src + (size_t)cols[j] * 3;
-            s0 = {s0.re + ws[j] * v[0].re, s0.im + ws[j] * v[0].im};
-            s1 = {s1.re + ws[j] * v[1].re, s1.im + ws[j] * v[1].im};
-            s2 = {s2.re + ws[j] * v[2].re, s2.im + ws[j] * v[2].im};
+            const cf v0 = v[0], v1 = v[1], v2 = v[2];
+            s0 = {s0.re + ws[j] * v0.re, s0.im + ws[j] * v0.im};
+            s1 = {s1.re + ws[j] * v1.re, s1.im + ws[j] * v1.im};
+            s2 = {s2.re + ws[j] * v2.re, s2.im + ws[j] * v2.im};
         }
         const cf f = ph[ch_oct[c] * K + q];
         a0 = cfma(a0, f, s0);
@@ -448,9 +455,10 @@ __global__ __launch_bounds__(256) void k_ml_down(const int *__restrict__ far_ptr
                                                  const int *__restrict__ oct, const int *__restrict__ an_ptr,
                                                  const int *__restrict__ an_kp, const float *__restrict__ an_w,
                                                  const cf *__restrict__ php, int Kp, const cf *__restrict__ downp,
-                                                 cf *__restrict__ down) {
-    const int b = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
-    if (q >= K) return;
+                                                 int nbox, cf *__restrict__ down) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int b = (int)(idx / K), q = (int)(idx % K);
+    if (b >= nbox) return;
     cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
     for (int e = far_ptr[b]; e < far_ptr[b + 1]; ++e) {
         const cf t = trans[(size_t)far_code[e] * K + q];
@@ -476,6 +484,73 @@ __global__ __launch_bounds__(256) void k_ml_down(const int *__restrict__ far_ptr
     o[0] = a0;
     o[1] = a1;
     o[2] = a2;
+}
+
+// Downward pass at level l >= 3 grouped by parent: the children of parent P
+// share their candidate source boxes (children of P's near boxes), so each
+// candidate's pattern is read once and applied to every sibling for which it
+// is far.  Per child the far boxes are visited in the same order as its far
+// list, so the sums equal k_ml_down's.  Then the anterpolation from P.
+__global__ __launch_bounds__(256) void k_ml_down_sib(
+    const int *__restrict__ pnear_ptr, const int *__restrict__ pnear_idx, const int *__restrict__ pch_ptr,
+    const int *__restrict__ pch_idx, int nparent, const int *__restrict__ ix, const int *__restrict__ oct,
+    const cf *__restrict__ trans, const cf *__restrict__ up, int K, const int *__restrict__ an_ptr,
+    const int *__restrict__ an_kp, const float *__restrict__ an_w, const cf *__restrict__ php, int Kp,
+    const cf *__restrict__ downp, cf *__restrict__ down) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int P = (int)(idx / K), q = (int)(idx % K);
+    if (P >= nparent) return;
+    const int c0 = pch_ptr[P], nch = min(8, pch_ptr[P + 1] - c0);
+    int bx[8], by[8], bz[8];
+    cf a[8][3];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int b = s < nch ? pch_idx[c0 + s] : 0;
+        bx[s] = ix[3 * b];
+        by[s] = ix[3 * b + 1];
+        bz[s] = ix[3 * b + 2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[s][c] = {0, 0};
+    }
+    for (int e = pnear_ptr[P]; e < pnear_ptr[P + 1]; ++e) {
+        const int pn = pnear_idx[e];
+        for (int f = pch_ptr[pn]; f < pch_ptr[pn + 1]; ++f) {
+            const int c = pch_idx[f];
+            const int cx = ix[3 * c], cy = ix[3 * c + 1], cz = ix[3 * c + 2];
+            const cf *u = up + ((size_t)c * K + q) * 3;
+            const cf u0 = u[0], u1 = u[1], u2 = u[2];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                if (s >= nch) break;
+                const int dx = bx[s] - cx, dy = by[s] - cy, dz = bz[s] - cz;
+                if (max(abs(dx), max(abs(dy), abs(dz))) <= 1) continue;
+                const cf t = trans[(size_t)((dx + 3) * 49 + (dy + 3) * 7 + (dz + 3)) * K + q];
+                a[s][0] = cfma(a[s][0], t, u0);
+                a[s][1] = cfma(a[s][1], t, u1);
+                a[s][2] = cfma(a[s][2], t, u2);
+            }
+        }
+    }
+    const cf *dp = downp + (size_t)P * Kp * 3;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        if (s >= nch) break;
+        const int b = pch_idx[c0 + s], o = oct[b];
+        cf a0 = a[s][0], a1 = a[s][1], a2 = a[s][2];
+        for (int e = an_ptr[q]; e < an_ptr[q + 1]; ++e) {
+            const int kp = an_kp[e];
+            const cf f0 = php[o * Kp + kp];
+            const cf f = {an_w[e] * f0.re, -an_w[e] * f0.im};
+            const cf *v = dp + (size_t)kp * 3;
+            a0 = cfma(a0, f, v[0]);
+            a1 = cfma(a1, f, v[1]);
+            a2 = cfma(a2, f, v[2]);
+        }
+        cf *o3 = down + ((size_t)b * K + q) * 3;
+        o3[0] = a0;
+        o3[1] = a1;
+        o3[2] = a2;
+    }
 }
 
 // receive + near part, back to RWG order: y[perm[i]] = yn[i] + sum_q rec[i][q] . down[leaf(i)][q]
@@ -708,14 +783,28 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
         ML_TRY(ml_upload(M, &C.far_code_d, C.far_code.empty() ? std::vector<int>{0} : C.far_code));
         ML_TRY(ml_upload(M, &C.parent_d, C.parent));
         ML_TRY(ml_upload(M, &C.oct_d, C.octant));
+        ML_TRY(ml_upload(M, &C.ix_d, C.ix));
+        ML_TRY(ml_upload(M, &C.near_ptr_d, C.near_ptr));
+        ML_TRY(ml_upload(M, &C.near_idx_d, C.near_idx.empty() ? std::vector<int>{0} : C.near_idx));
         // translation per offset code in use
         std::vector<char> used(343, 0);
         for (int c : C.far_code) used[c] = 1;
+        std::vector<int> codes;
+        for (int code = 0; code < 343; ++code)
+            if (used[code]) codes.push_back(code);
         std::vector<cf> tr((size_t)343 * C.K, cf{0, 0});
-        for (int code = 0; code < 343; ++code) {
-            if (!used[code]) continue;
-            double D[3] = {C.a * (code / 49 - 3), C.a * ((code / 7) % 7 - 3), C.a * (code % 7 - 3)};
-            translation(k, D, C.kh, C.L, &tr[(size_t)code * C.K]);
+        {
+            const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nth; ++t)
+                pool.emplace_back([&, t] {
+                    for (size_t i = t; i < codes.size(); i += nth) {
+                        const int code = codes[i];
+                        double D[3] = {C.a * (code / 49 - 3), C.a * ((code / 7) % 7 - 3), C.a * (code % 7 - 3)};
+                        translation(k, D, C.kh, C.L, &tr[(size_t)code * C.K]);
+                    }
+                });
+            for (auto &th : pool) th.join();
         }
         ML_TRY(ml_upload(M, &C.trans, tr));
         if (l < levels) {
@@ -784,6 +873,15 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
     // near matrix structure (box order)
     std::vector<long long> rp(n + 1, 0);
     std::vector<int> cols;
+    {
+        size_t total = 0;
+        for (int b = 0; b < F.nbox; ++b) {
+            size_t w = 0;
+            for (int e = F.near_ptr[b]; e < F.near_ptr[b + 1]; ++e) w += F.uend[F.near_idx[e]] - F.ubeg[F.near_idx[e]];
+            total += w * (size_t)(F.uend[b] - F.ubeg[b]);
+        }
+        cols.reserve(total);
+    }
     for (int b = 0; b < F.nbox; ++b) {
         for (int i = F.ubeg[b]; i < F.uend[b]; ++i) {
             for (int e = F.near_ptr[b]; e < F.near_ptr[b + 1]; ++e) {
@@ -844,21 +942,32 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
     }
     {
         ProfScope ps(ctx, "mlfma_agg", 0);
-        k_ml_agg<<<dim3(F.nbox, (F.K + 255) / 256), 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.up);
+        k_ml_agg<<<(unsigned)(((long long)F.nbox * F.K + 255) / 256), 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb,
+                                                                                   F.K, F.nbox, F.up);
     }
     for (int l = lvl - 1; l >= 2; --l) {
         MlLevel &C = M->lv[l], &N = M->lv[l + 1];
         ProfScope ps(ctx, "mlfma_up", 0);
-        k_ml_up<<<dim3(C.nbox, (C.K + 255) / 256), 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w,
-                                                                 C.ph, C.K, N.K, N.up, C.up);
+        k_ml_up<<<(unsigned)(((long long)C.nbox * C.K + 255) / 256), 256, 0, s>>>(
+            C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox, N.up, C.up);
     }
+    static const bool sib = [] {
+        const char *e = getenv("EMFA_MLFMA_SIBLING");
+        return e && atoi(e) != 0;  // measured slower than one thread per (box, sample); opt-in
+    }();
     for (int l = 2; l <= lvl; ++l) {
         MlLevel &C = M->lv[l];
         MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;
         ProfScope ps(ctx, "mlfma_down", 0);
-        k_ml_down<<<dim3(C.nbox, (C.K + 255) / 256), 256, 0, s>>>(
+        if (P && sib) {
+            k_ml_down_sib<<<(unsigned)(((long long)P->nbox * C.K + 255) / 256), 256, 0, s>>>(
+                P->near_ptr_d, P->near_idx_d, P->ch_ptr_d, P->ch_idx_d, P->nbox, C.ix_d, C.oct_d, C.trans, C.up,
+                C.K, C.an_ptr, C.an_kp, C.an_w, P->ph, P->K, P->down, C.down);
+            continue;
+        }
+        k_ml_down<<<(unsigned)(((long long)C.nbox * C.K + 255) / 256), 256, 0, s>>>(
             C.far_ptr_d, C.far_idx_d, C.far_code_d, C.trans, C.up, C.K, C.parent_d, C.oct_d, C.an_ptr, C.an_kp,
-            C.an_w, P ? P->ph : nullptr, P ? P->K : 0, P ? P->down : nullptr, C.down);
+            C.an_w, P ? P->ph : nullptr, P ? P->K : 0, P ? P->down : nullptr, C.nbox, C.down);
     }
     {
         ProfScope ps(ctx, "mlfma_out", 0);
