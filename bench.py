@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-stream summary of the default line")
+    ap.add_argument("--no-tune", action="store_true", help="skip the LU autotuner (built-in nb / panel choice)")
     ap.add_argument("--mixed-requests", type=int, default=32)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=96.0)
@@ -182,6 +183,7 @@ def main():
     cols = np.arange(nrhs, dtype=np.int32)
 
     ctx = hip.Context(0)
+    tuning = None if a.no_tune else ctx.autotune_lu(n, hip.F32)  # before the timed region
     mesh = hip.Mesh(ctx, xyz, tri, rwg)
     Z = hip.Mat(ctx, n, n, hip.F32)
     B = hip.Mat(ctx, n, nrhs, hip.F64)
@@ -221,6 +223,7 @@ def main():
     extra = {}
     if not a.no_fp64:
         Z.close()
+        tuning64 = None if a.no_tune else ctx.autotune_lu(n, hip.F64)
         Z64 = hip.Mat(ctx, n, n, hip.F64)
         B64 = hip.Mat(ctx, n, nrhs, hip.F64)
         ctx.sync()
@@ -232,7 +235,7 @@ def main():
         r64 = hip.rcs(ctx, mesh, k, B64, dirs, cols)
         ctx.sync()
         t64 = time.perf_counter() - t1
-        extra = {"fp64_path_solves_per_s": 1.0 / t64, "fp64_path_s": t64,
+        extra = {"fp64_path_solves_per_s": 1.0 / t64, "fp64_path_s": t64, "fp64_lu_tuning": tuning64,
                  "speedup_vs_fp64_path": t64 / dt,
                  "rcs_max_abs_diff_db_vs_fp64_path": float(np.abs(r32 - r64).max())}
         lu64.close()
@@ -268,6 +271,7 @@ def main():
                      "frac": gemm_tflops / PEAK["f32_mfma"], "traffic": None,
                      "avg_launch_ms": avg_launch_ms, "launches_per_step": g["launches_per_step"]},
         "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
+        "lu_tuning": tuning,
         "refine": {"iterations": reports[-1][0], "relres_first": reports[-1][1], "relres_final": reports[-1][2],
                    "converged": reports[-1][3]},
         "cpu_baseline": cpu,

@@ -55,6 +55,8 @@ struct emfa_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cu = 0;
     int cu_share = 0;  // cap on the cooperative panel's workgroups (0: num_cu)
+    // LU parameters (0 / -1: built-in choice); set by emfa_lu_set_params or the autotuner
+    int lu_nb = 0, panel_g = 0, panel_x = -1;
     size_t bytes = 0;
     bool prof = false;
     std::map<std::string, emfa::ProfFamily> fam;

@@ -878,6 +878,22 @@ __global__ void k_zero_u32(unsigned *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// Deterministic test matrix for the autotuner: entries uniform in
+// [-0.5, 0.5] (integer hash of the index) plus n on the diagonal.
+template <typename R>
+__global__ __launch_bounds__(256) void k_tune_matrix(cx<R> *__restrict__ A, int n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)n * n) return;
+    unsigned long long h = (unsigned long long)i * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    const R re = (R)((double)(h & 0xFFFFFF) / 16777216.0 - 0.5);
+    const R im = (R)((double)((h >> 24) & 0xFFFFFF) / 16777216.0 - 0.5);
+    const bool diag = (i / n) == (i % n);
+    A[i] = {re + (diag ? (R)n : R(0)), im};
+}
+
 // ------------------------------------------------------------------ host
 template <typename R>
 struct LuPlan {
@@ -1006,6 +1022,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     cx<R> *A = (cx<R> *)Am->d;
     const long long lda = N;
     LuPlan<R> plan;
+    if (ctx->lu_nb > 0) plan.nb = std::max(16, ctx->lu_nb);
     if (const char *s = getenv("EMFA_LU_NB")) plan.nb = std::max(16, atoi(s));
     const int nb = plan.nb;
 
@@ -1013,6 +1030,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     size_t lds_cap = 150 * 1024;
     if (const char *s = getenv("EMFA_PANEL_LDS_KB")) lds_cap = (size_t)std::max(16, std::min(150, atoi(s))) * 1024;
     int G_max = std::max(1, ctx->cu_share > 0 ? std::min(ctx->cu_share, ctx->num_cu) : ctx->num_cu);
+    if (ctx->panel_g > 0) G_max = std::max(1, std::min(G_max, ctx->panel_g));
     if (const char *s = getenv("EMFA_PANEL_G")) G_max = std::max(1, std::min(G_max, atoi(s)));
     int panel_rows_min = 16;
     if (const char *s = getenv("EMFA_PANEL_ROWS_MIN")) panel_rows_min = std::max(1, atoi(s));
@@ -1065,6 +1083,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     // shares sum to at most the CU count and nothing else waits on them.
     bool coop = ctx->cu_share <= 0, lookahead = true, xchg = true;
     if (const char *s = getenv("EMFA_PANEL_COOP")) coop = atoi(s) != 0;
+    if (ctx->panel_x >= 0) xchg = ctx->panel_x != 0;
     if (const char *s = getenv("EMFA_PANEL_X")) xchg = atoi(s) != 0;
     EMFA_REQUIRE(G_max <= kPanelMaxG, EMFA_E_INVALID, "panel grid %d exceeds %d", G_max, kPanelMaxG);
     if (sizeof(R) == 4 && N >= 0xFFFFF) xchg = false;  // the FP32 packet holds a 20-bit row index
@@ -1438,6 +1457,100 @@ emfa_status emfa_lu_pivots(const emfa_lu *lu, int *ipiv_host) {
 
 emfa_mat *emfa_lu_matrix(emfa_lu *lu) { return lu ? lu->a : nullptr; }
 int emfa_lu_info(const emfa_lu *lu) { return lu ? lu->info : -1; }
+
+emfa_status emfa_lu_set_params(emfa_ctx *ctx, int nb, int panel_workgroups, int panel_variant) {
+    EMFA_REQUIRE(ctx && nb >= 0 && panel_workgroups >= 0 && panel_variant >= -1 && panel_variant <= 1,
+                 EMFA_E_INVALID, "emfa_lu_set_params: bad argument");
+    ctx->lu_nb = nb;
+    ctx->panel_g = panel_workgroups;
+    ctx->panel_x = panel_variant;
+    return EMFA_OK;
+}
+
+emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out) {
+    EMFA_REQUIRE(ctx && n > 0 && (prec == EMFA_F32 || prec == EMFA_F64), EMFA_E_INVALID,
+                 "emfa_lu_autotune: bad argument");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    // a well-conditioned test matrix of the requested size, generated on the device
+    emfa_mat *A = nullptr;
+    emfa_status st = emfa_mat_create(ctx, n, n, prec, &A);
+    if (st) return st;
+    {
+        const long long tot = (long long)n * n;
+        if (prec == EMFA_F32)
+            k_tune_matrix<float><<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>((cx<float> *)A->d, n);
+        else
+            k_tune_matrix<double><<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>((cx<double> *)A->d, n);
+        EMFA_HIP_CHECK(hipGetLastError());
+    }
+    const int save_nb = ctx->lu_nb, save_g = ctx->panel_g, save_x = ctx->panel_x;
+    struct Cand {
+        int nb, x, g;
+    };
+    std::vector<Cand> cands;
+    for (int nb : {256, 512})
+        if (nb <= std::max(256, n))
+            for (int x : {1, 0})
+                for (int g : {0, 128}) cands.push_back({nb, x, g});
+    hipEvent_t e0, e1;
+    EMFA_HIP_CHECK(hipEventCreate(&e0));
+    EMFA_HIP_CHECK(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    Cand best = cands[0];
+    int tried = 0;
+    emfa_lu *lu = nullptr;
+    for (size_t i = 0; i < cands.size() && !st; ++i) {
+        ctx->lu_nb = cands[i].nb;
+        ctx->panel_g = cands[i].g;
+        ctx->panel_x = cands[i].x;
+        if (i == 0) {  // warm-up (module load, pool)
+            st = emfa_lu_factor(ctx, A, prec, &lu);
+            emfa_lu_destroy(lu);
+            lu = nullptr;
+            if (st) break;
+        }
+        (void)hipEventRecord(e0, ctx->stream);
+        st = emfa_lu_factor(ctx, A, prec, &lu);
+        (void)hipEventRecord(e1, ctx->stream);
+        (void)hipEventSynchronize(e1);
+        emfa_lu_destroy(lu);
+        lu = nullptr;
+        if (st) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (out && i < 8) {
+            out->cand_nb[i] = cands[i].nb;
+            out->cand_variant[i] = cands[i].x;
+            out->cand_workgroups[i] = cands[i].g;
+            out->cand_ms[i] = ms;
+        }
+        ++tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cands[i];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    emfa_mat_destroy(A);
+    if (st) {
+        ctx->lu_nb = save_nb;
+        ctx->panel_g = save_g;
+        ctx->panel_x = save_x;
+        return st;
+    }
+    ctx->lu_nb = best.nb;
+    ctx->panel_g = best.g;
+    ctx->panel_x = best.x;
+    if (out) {
+        out->nb = best.nb;
+        out->panel_variant = best.x;
+        out->panel_workgroups = best.g;
+        out->best_ms = best_ms;
+        out->candidates = tried;
+    }
+    return EMFA_OK;
+}
 
 emfa_status emfa_gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *Xin, emfa_mat *C, int negate,
                                int accumulate) {

@@ -951,10 +951,9 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
         k_ml_up<<<(unsigned)(((long long)C.nbox * C.K + 255) / 256), 256, 0, s>>>(
             C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox, N.up, C.up);
     }
-    static const bool sib = [] {
-        const char *e = getenv("EMFA_MLFMA_SIBLING");
-        return e && atoi(e) != 0;  // measured slower than one thread per (box, sample); opt-in
-    }();
+    // sibling-grouped translation measured slower than one thread per (box, sample): opt-in
+    const char *sib_env = getenv("EMFA_MLFMA_SIBLING");
+    const bool sib = sib_env && atoi(sib_env) != 0;
     for (int l = 2; l <= lvl; ++l) {
         MlLevel &C = M->lv[l];
         MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;

@@ -25,7 +25,7 @@ EXPORTS = [
     "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
-    "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres",
+    "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_autotune",
 ]
 
 _vp = ctypes.c_void_p
@@ -44,6 +44,13 @@ class RefineReport(ctypes.Structure):
 
 class GmresReport(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("relres", ctypes.c_double)]
+
+
+class LuTuning(ctypes.Structure):
+    _fields_ = [("nb", ctypes.c_int), ("panel_variant", ctypes.c_int), ("panel_workgroups", ctypes.c_int),
+                ("best_ms", ctypes.c_float), ("candidates", ctypes.c_int), ("cand_nb", ctypes.c_int * 8),
+                ("cand_variant", ctypes.c_int * 8), ("cand_workgroups", ctypes.c_int * 8),
+                ("cand_ms", ctypes.c_float * 8)]
 
 
 class MlfmaParams(ctypes.Structure):
@@ -177,6 +184,20 @@ class Context:
 
     def cu_count(self):
         return lib().emfa_ctx_cu_count(self.h)
+
+    def lu_params(self, nb=0, panel_workgroups=0, panel_variant=-1):
+        _check(lib().emfa_lu_set_params(self.h, int(nb), int(panel_workgroups), int(panel_variant)))
+
+    def autotune_lu(self, n, prec):
+        """Time the LU candidates at size n, keep the fastest; returns the report."""
+        t = LuTuning()
+        _check(lib().emfa_lu_autotune(self.h, int(n), int(prec), ctypes.byref(t)))
+        c = t.candidates
+        return {"nb": t.nb, "panel_variant": t.panel_variant, "panel_workgroups": t.panel_workgroups,
+                "best_ms": t.best_ms,
+                "candidates": [{"nb": t.cand_nb[i], "panel_variant": t.cand_variant[i],
+                                "panel_workgroups": t.cand_workgroups[i], "ms": round(t.cand_ms[i], 2)}
+                               for i in range(min(c, 8))]}
 
     def set_cu_share(self, cus):
         _check(lib().emfa_ctx_set_cu_share(self.h, int(cus)))

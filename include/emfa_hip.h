@@ -139,6 +139,23 @@ emfa_status emfa_lu_pivots(const emfa_lu *lu, int *ipiv_host);
 emfa_mat *emfa_lu_matrix(emfa_lu *lu);
 int emfa_lu_info(const emfa_lu *lu);
 
+/* LU parameters of a context: outer block nb (0: 256), panel workgroup cap
+ * (0: all CUs or the context's CU share), panel variant (1: packet exchange,
+ * 0: arrival counter, -1: built-in choice).  No counterpart in the reference
+ * (LAPACK chooses its own blocking). */
+emfa_status emfa_lu_set_params(emfa_ctx *ctx, int nb, int panel_workgroups, int panel_variant);
+typedef struct {
+    int nb, panel_variant, panel_workgroups; /* chosen */
+    float best_ms;                           /* its factorisation time */
+    int candidates;                          /* candidates timed */
+    int cand_nb[8], cand_variant[8], cand_workgroups[8];
+    float cand_ms[8];
+} emfa_lu_tuning;
+/* Autotuner: times the LU of a generated n x n matrix in precision prec for
+ * each candidate (nb 256/512 x panel variant x panel workgroup cap all/128),
+ * keeps the fastest in the context and reports all timings. */
+emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out);
+
 /* ---- mixed-precision refinement -------------------------------------- */
 typedef struct {
     int iterations;      /* refinement steps taken                       */

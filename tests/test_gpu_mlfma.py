@@ -136,3 +136,13 @@ def test_gmres_solution_and_rcs_near_dense_solve(ctx, built):
     _assert_rcs_close(r_ml, r_d)
     for m in (B, X, Xd):
         m.close()
+
+
+def test_sibling_grouped_translation_gives_the_same_matvec(ctx, built, monkeypatch):
+    # the opt-in kernel visits each child's far boxes in far-list order: same sums
+    v = built["s4_800MHz"]
+    x = _vec(v["rwg"].shape[0], 9)
+    y0 = _apply(ctx, v["op"], x)
+    monkeypatch.setenv("EMFA_MLFMA_SIBLING", "1")
+    y1 = _apply(ctx, v["op"], x)
+    assert np.array_equal(y0, y1)

@@ -183,6 +183,45 @@ def test_lu_f32(ctx, n):
     assert rel(dB.numpy(), ref) < 1e-4
 
 
+@pytest.mark.parametrize("n,m", [(70, 3), (333, 64), (700, 90)])
+def test_lookback_solve_matches_blocked_solve(ctx, monkeypatch, n, m):
+    # the opt-in one-launch triangular solve (look-back flags) against the
+    # default blocked solve and LAPACK, FP64 factor
+    _say(f"lookback solve n={n} m={m}")
+    rng = np.random.default_rng(n + m)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    dA = hip.Mat.from_numpy(ctx, A, hip.F64)
+    lu = hip.LU(ctx, dA, hip.F64)
+    d0 = hip.Mat.from_numpy(ctx, B, hip.F64)
+    lu.solve(d0)
+    monkeypatch.setenv("EMFA_SOLVE_LB_MAX", "128")
+    d1 = hip.Mat.from_numpy(ctx, B, hip.F64)
+    lu.solve(d1)
+    ref = np.linalg.solve(A, B)
+    assert rel(d1.numpy(), ref) < 1e-10
+    assert rel(d1.numpy(), d0.numpy()) < 1e-11
+
+
+def test_autotuner_and_every_lu_parameter_set_keep_getrf_pivots(ctx):
+    _say("autotune")
+    n = 600
+    rep = ctx.autotune_lu(n, hip.F64)
+    assert rep["candidates"] and rep["nb"] in (256, 384, 512) and rep["best_ms"] > 0
+    rng = np.random.default_rng(77)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    LUo, pivo, _ = O.lu(A)
+    for nb, g, x in [(256, 0, 1), (384, 0, 0), (512, 16, 1), (0, 0, -1)]:
+        ctx.lu_params(nb, g, x)
+        dA = hip.Mat.from_numpy(ctx, A, hip.F64)
+        lu = hip.LU(ctx, dA, hip.F64)
+        assert np.array_equal(lu.pivots(), pivo), (nb, g, x)
+        assert np.abs(lu.factor().numpy() - LUo).max() / np.abs(LUo).max() < 1e-10
+        lu.close()
+        dA.close()
+    ctx.lu_params(0, 0, -1)
+
+
 def test_lu_singular_reports(ctx):
     _say("lu singular")
     n = 40
