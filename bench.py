@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-stream summary of the default line")
     ap.add_argument("--no-tune", action="store_true", help="skip the LU autotuner (built-in nb / panel choice)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="copy a buffer of this many MiB once (a plain copy kernel of known size, to scale "
+                         "hardware traffic counters)")
     ap.add_argument("--mixed-requests", type=int, default=32)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=96.0)
@@ -183,6 +186,14 @@ def main():
     cols = np.arange(nrhs, dtype=np.int32)
 
     ctx = hip.Context(0)
+    if a.calib_copy_mb > 0:
+        rows = a.calib_copy_mb * 2**20 // 16
+        Ca, Cb = hip.Mat(ctx, rows, 1, hip.F64), hip.Mat(ctx, rows, 1, hip.F64)
+        ctx.sync()
+        Cb.copy_from(Ca)
+        ctx.sync()
+        Ca.close()
+        Cb.close()
     tuning = None if a.no_tune else ctx.autotune_lu(n, hip.F32)  # before the timed region
     mesh = hip.Mesh(ctx, xyz, tri, rwg)
     Z = hip.Mat(ctx, n, n, hip.F32)

@@ -235,3 +235,52 @@ def test_workload_generators_are_deterministic_and_in_range():
     assert len(c) == 256 and {r["subdiv"] for r in c} == {2, 3, 4}
     assert all(0.3 <= x <= 1.2 for r in c for x in r["axes"])
     assert all(100e6 <= r["freq"] <= 600e6 and r["formulation"] == hip.EFIE for r in c)
+
+
+# ---------------------------------------------------------------- edge cases
+def test_geometry_argument_errors_and_size_queries():
+    for bad in [(-1, 1.0), (10, 1.0), (2, 0.0)]:
+        with pytest.raises(hip.EmfaError) as e:
+            hip.icosphere(*bad)
+        assert e.value.code == hip.EMFA_E_INVALID
+    nv, nt = ctypes.c_int(), ctypes.c_int()
+    assert hip.lib().emfa_geom_icosphere(3, ctypes.c_double(1), ctypes.c_double(1), ctypes.c_double(1), None, 0,
+                                         None, 0, ctypes.byref(nv), ctypes.byref(nt)) == hip.EMFA_OK
+    assert (nv.value, nt.value) == (642, 1280)
+    xyz = np.zeros((10, 3))
+    tri = np.zeros((10, 3), dtype=np.int32)
+    rc = hip.lib().emfa_geom_icosphere(3, ctypes.c_double(1), ctypes.c_double(1), ctypes.c_double(1),
+                                       xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 10,
+                                       tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 10, ctypes.byref(nv),
+                                       ctypes.byref(nt))
+    assert rc == hip.EMFA_E_INVALID and b"too small" in hip.lib().emfa_last_error()
+
+
+def test_rwg_builder_degenerate_and_ragged_meshes():
+    # one triangle: no interior edge, three boundary edges
+    rwg, nb = hip.rwg_build(np.array([[0, 1, 2]], dtype=np.int32))
+    assert rwg.shape[0] == 0 and nb == 3
+    # two triangles sharing an edge: one RWG, plus/minus triangles in order
+    rwg, nb = hip.rwg_build(np.array([[0, 1, 2], [1, 0, 3]], dtype=np.int32))
+    assert rwg.tolist() == [[0, 1, 2, 3, 0, 1]] and nb == 4
+    # open plates: 3c^2 - 2c unknowns, 4c boundary edges
+    for c in (1, 2, 7):
+        xyz, tri = hip.plate(c, 1.0)
+        rwg, nb = hip.rwg_build(tri)
+        assert rwg.shape[0] == 3 * c * c - 2 * c and nb == 4 * c
+    # a strip with a triangle removed (ragged boundary) keeps the reference order
+    xyz, tri = hip.plate(3, 1.0)
+    keep = np.delete(tri, 7, axis=0)
+    rwg, nb = hip.rwg_build(keep)
+    ref = O.build_rwg(keep)[0]
+    assert np.array_equal(rwg, ref)
+    with pytest.raises(hip.EmfaError):
+        hip.rwg_build(np.zeros((0, 3), dtype=np.int32))
+
+
+def test_mlfma_restatement_rejects_single_level():
+    from oracle import mlfma as OM
+    xyz, tri = hip.icosphere(1, 0.1)
+    rwg, _ = hip.rwg_build(tri)
+    with pytest.raises(ValueError):
+        OM.Tree(xyz, tri, rwg, 1e8, 0.5)  # the whole object fits one box of the threshold
