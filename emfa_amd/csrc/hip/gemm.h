@@ -29,6 +29,7 @@ struct GemmArgs {
     long long ldc;
     int alpha_neg;          // 1: C = beta C - A B ; 0: C = beta C + A B
     int beta;               // 0 or 1
+    int swizzle = 1;        // XCD-aware tile order (set by the launchers)
 };
 
 emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=float,  TC=float

@@ -6,6 +6,34 @@
 
 namespace emfa {
 
+// XCD-aware tile order.  The dispatcher hands workgroup L (linear id) to XCD
+// L % 8, and each XCD has its own L2.  The tile sequence is cut into eight
+// contiguous chunks, one per XCD, and walked in groups of kGroupRows tile
+// rows, so the workgroups resident on one XCD share A row panels and B
+// column panels in that XCD's L2.  A bijection for any grid, so only speed
+// depends on the dispatch order.
+constexpr int kGroupRows = 8;
+
+__device__ __forceinline__ void tile_of(int swizzle, int *ty, int *tx) {
+    const int TX = gridDim.x, TY = gridDim.y;
+    if (!swizzle) {
+        *ty = blockIdx.y;
+        *tx = blockIdx.x;
+        return;
+    }
+    const int nT = TX * TY;
+    const int L = blockIdx.y * TX + blockIdx.x;
+    const int xcd = L & 7, slot = L >> 3;
+    const int q = nT >> 3, r = nT & 7;
+    const int t = xcd * q + min(xcd, r) + slot;
+    const int group = t / (kGroupRows * TX);
+    const int first = group * kGroupRows;
+    const int rows = min(kGroupRows, TY - first);
+    const int local = t - group * kGroupRows * TX;
+    *ty = first + local % rows;
+    *tx = local / rows;
+}
+
 // ---------------------------------------------------------------------------
 // FP32: workgroup tile 128 x 128 (reals), 4 waves in 2 x 2, wave tile 64 x 64 =
 // 2 x 2 MFMA 32x32 tiles.  K step 32: MFMA t (0..15) takes k = 16*(lane/32) + t,
@@ -26,7 +54,9 @@ __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
     float *__restrict__ C = (float *)g.C;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const long long bm = (long long)blockIdx.y * BM, bn = (long long)blockIdx.x * BN;
+    int ty, tx;
+    tile_of(g.swizzle, &ty, &tx);
+    const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
     const int nk = (g.Kr + BK - 1) / BK;
 
     float2 ra[8], rb[8];
@@ -135,7 +165,9 @@ __global__ __launch_bounds__(256) void k_gemm_f32v(GemmArgs g) {
     float *__restrict__ C = (float *)g.C;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const long long bm = (long long)blockIdx.y * BM, bn = (long long)blockIdx.x * BN;
+    int ty, tx;
+    tile_of(g.swizzle, &ty, &tx);
+    const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
     const int nk = (g.Kr + BKV - 1) / BKV;
     float4 ra[LOADS], rb[LOADS];
     auto gload = [&](int k0) {
@@ -249,7 +281,9 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
     double *__restrict__ C = (double *)g.C;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const long long bm = (long long)blockIdx.y * BM, bn = (long long)blockIdx.x * BN;
+    int ty, tx;
+    tile_of(g.swizzle, &ty, &tx);
+    const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
     const int nk = (g.Kr + BK - 1) / BK;
 
     double2 ra[4], rb[4];
@@ -354,7 +388,17 @@ static bool gemm_args_ok(const GemmArgs &g) {
            g.Nr <= 65535LL * 128 && g.M <= 65535LL * 128;
 }
 
-emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag) {
+static int swizzle_default() {
+    static const int v = [] {
+        const char *e = getenv("EMFA_GEMM_SWIZZLE");
+        return e ? atoi(e) : 1;
+    }();
+    return v;
+}
+
+emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
+    GemmArgs g = g0;
+    g.swizzle = swizzle_default();
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f32k::BN - 1) / f32k::BN, (g.M + f32k::BM - 1) / f32k::BM);
     static const int variant = [] {
@@ -383,7 +427,9 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag) {
     return EMFA_OK;
 }
 
-emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g, int tag) {
+emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g0, int tag) {
+    GemmArgs g = g0;
+    g.swizzle = swizzle_default();
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
     if (tag == 1)
@@ -394,7 +440,9 @@ emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g, int tag) {
     return EMFA_OK;
 }
 
-emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g) {
+emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g0) {
+    GemmArgs g = g0;
+    g.swizzle = swizzle_default();
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32/f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
     k_gemm_f64<float, 0><<<grid, 256, 0, s>>>(g);

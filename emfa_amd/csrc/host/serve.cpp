@@ -114,6 +114,7 @@ int64_t Server::submit(const Request &r) {
         t0_ = j.arrival;
         have_t0_ = true;
     }
+    pending_.insert(j.id);
     queue_.push_back(std::move(j));
     cv_work_.notify_one();
     return next_id_ - 1;
@@ -123,21 +124,23 @@ bool Server::wait(int64_t id, Result *out) {
     std::unique_lock<std::mutex> l(mu_);
     for (;;) {
         if (id >= 0) {
+            if (!pending_.count(id)) return false;  // unknown, or already returned
             auto it = done_.find(id);
             if (it != done_.end()) {
                 *out = std::move(it->second);
                 done_.erase(it);
+                pending_.erase(id);
                 return true;
             }
-            if (id >= next_id_) return false;
         } else {
             if (!done_.empty()) {
                 auto it = done_.begin();
                 *out = std::move(it->second);
+                pending_.erase(it->first);
                 done_.erase(it);
                 return true;
             }
-            if (queue_.empty() && busy_ == 0) return false;
+            if (pending_.empty()) return false;
         }
         cv_done_.wait(l);
     }

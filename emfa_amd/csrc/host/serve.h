@@ -112,6 +112,7 @@ class Server {
     std::condition_variable cv_work_, cv_done_, cv_mem_, cv_build_;
     std::deque<Job> queue_;
     std::map<int64_t, Result> done_;
+    std::set<int64_t> pending_;  // submitted and not yet returned by wait()
     int64_t next_id_ = 0;
     int busy_ = 0;
     bool stop_ = false;

@@ -101,6 +101,29 @@ def test_matvec_matches_oracle_restatement(ctx, built, name):
     assert rel(y - ynear, yfar) < 2e-4
 
 
+def test_efie_operator_matches_oracle_restatement(ctx):
+    # EMFA_EFIE: receive pattern = conjugated radiation pattern, EFIE near entries
+    xyz, tri, rwg = sphere(3, 0.5)
+    f = 1.2e9
+    k = hip.wavenumber(f)
+    mesh = hip.Mesh(ctx, xyz, tri, rwg)
+    op = hip.MLFMA(ctx, mesh, k, 1.0, 0.25, kind=hip.EFIE)
+    T = OM.Tree(xyz, tri, rwg, f, 0.25)
+    n = rwg.shape[0]
+    x = _vec(n, 11)
+    y = _apply(ctx, op, x)
+    Z = hip.Mat(ctx, n, n, hip.F64)
+    hip.fill(ctx, mesh, Z, hip.EFIE, k, 3)
+    rows, cols = T.near_pairs()
+    mask = np.zeros((n, n), dtype=bool)
+    mask[rows, cols] = True
+    rad, rec = T.rad_rec(xyz, tri, rwg, 1.0)
+    assert rel(y, (Z.numpy() * mask) @ x + T.far_matvec(x, rad, rec)) < 2e-5
+    op.close()
+    Z.close()
+    mesh.close()
+
+
 def test_matvec_close_to_dense(ctx, built):
     v = built["s4_800MHz"]
     n = v["rwg"].shape[0]
