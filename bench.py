@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-stream summary of the default line")
     ap.add_argument("--no-tune", action="store_true", help="skip the LU autotuner (built-in nb / panel choice)")
+    ap.add_argument("--fp32-update-only", action="store_true",
+                    help="keep the FP32-MFMA trailing update (do not let the tuner pick the bf16-split update)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="copy a buffer of this many MiB once (a plain copy kernel of known size, to scale "
                          "hardware traffic counters)")
@@ -194,7 +196,8 @@ def main():
         ctx.sync()
         Ca.close()
         Cb.close()
-    tuning = None if a.no_tune else ctx.autotune_lu(n, hip.F32)  # before the timed region
+    # before the timed region; the FP32 path may pick the bf16-split update
+    tuning = None if a.no_tune else ctx.autotune_lu(n, hip.F32, try_split_update=not a.fp32_update_only)
     mesh = hip.Mesh(ctx, xyz, tri, rwg)
     Z = hip.Mat(ctx, n, n, hip.F32)
     B = hip.Mat(ctx, n, nrhs, hip.F64)
@@ -234,6 +237,7 @@ def main():
     extra = {}
     if not a.no_fp64:
         Z.close()
+        ctx.lu_update_mode(0)
         tuning64 = None if a.no_tune else ctx.autotune_lu(n, hip.F64)
         Z64 = hip.Mat(ctx, n, n, hip.F64)
         B64 = hip.Mat(ctx, n, nrhs, hip.F64)

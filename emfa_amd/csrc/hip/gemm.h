@@ -30,6 +30,10 @@ struct GemmArgs {
     int alpha_neg;          // 1: C = beta C - A B ; 0: C = beta C + A B
     int beta;               // 0 or 1
     int swizzle = 1;        // XCD-aware tile order (set by the launchers)
+    int bf16x3 = 0;         // FP32 launcher, tag 1: three-term bf16 split on bf16 MFMA
+    int kslices = 1;        // FP64 launchers: split K over blockIdx.z (C += z * cslice)
+    int kc = 0;             // reals of K per slice (multiple of 64)
+    long long cslice = 0;   // C elements between slice planes
 };
 
 emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=float,  TC=float

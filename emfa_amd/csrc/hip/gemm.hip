@@ -1,4 +1,5 @@
 // MFMA GEMM kernels (see gemm.h for the complex-as-real formulation).
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 
@@ -267,6 +268,176 @@ __global__ __launch_bounds__(256) void k_gemm_f32v(GemmArgs g) {
 // 16x16 tiles, K step 16: MFMA t (0..3) takes k = 4*(lane/16) + t.
 // TA = float converts the matrix operand to double on its way into LDS.
 // ---------------------------------------------------------------------------
+// FP32 update emulated on bf16 MFMA (opt-in per context, LU trailing update
+// only).  Each FP32 operand x is split on its way to LDS into
+// hi = bf16(x), lo = bf16(x - hi); the product uses hi*hi + hi*lo + lo*hi
+// (three V_MFMA_F32_32X32X16_BF16 per block and 16-deep K step) with FP32
+// accumulation, which keeps about 16 significant bits of every product;
+// the FP64 refinement on the FP32 matrix removes the difference.
+// Workgroup tile 256 x 256, 8 waves in 4 x 2, wave tile 64 x 128 = 2 x 4
+// MFMA 32x32 blocks.  K step 32 (two MFMA steps), LDS double-buffered (hi
+// and lo planes of A and B, 128 KB), next tile prefetched into registers
+// during the MFMAs.
+// MFMA lane layout: lane l holds row (A) / column (B) l % 32 and the eight
+// K values 8 * (l / 32) .. + 7; result element v of lane l is
+// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+namespace bf3k {
+constexpr int BM = 256, BN = 256, BK = 32;  // LDS rows of BK bf16 = 64 B, 16-byte chunks XOR-swizzled
+}
+
+// 16-byte chunk c (0..3) of LDS row r lives at chunk c ^ ((r >> 2) & 3): the 16
+// rows read together by a ds_read_b128 then hit 16 distinct 4-bank groups.
+__device__ __forceinline__ int bf3_off(int r, int k) {
+    return r * bf3k::BK + ((((k >> 3) ^ (r >> 2)) & 3) << 3) + (k & 7);
+}
+
+template <int TAG>
+__global__ __launch_bounds__(512) void k_gemm_bf3(GemmArgs g) {
+    using namespace bf3k;
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][BM * BK], Al[2][BM * BK], Bh[2][BN * BK], Bl[2][BN * BK];
+    const float *__restrict__ A = (const float *)g.A;
+    const float *__restrict__ Bt = (const float *)g.Bt;
+    float *__restrict__ C = (float *)g.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    int ty, tx;
+    tile_of(g.swizzle, &ty, &tx);
+    const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
+    const int nk = (g.Kr + BK - 1) / BK;
+
+    // per k tile: 256 rows x 32 reals of A and of B = 2048 float4 each, 4 per thread
+    float4 ra[4], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 512 * i;
+            const int row = idx >> 3, kq = (idx & 7) * 4;
+            const int gk = k0 + kq;
+            const long long gr = bm + row, gc = bn + row;
+            float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gr < g.M) {
+                const float *p = A + gr * g.lda + gk;
+                if (gk + 3 < g.Kr && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+                    va = *(const float4 *)p;
+                } else {
+                    va.x = gk < g.Kr ? p[0] : 0.f;
+                    va.y = gk + 1 < g.Kr ? p[1] : 0.f;
+                    va.z = gk + 2 < g.Kr ? p[2] : 0.f;
+                    va.w = gk + 3 < g.Kr ? p[3] : 0.f;
+                }
+            }
+            if (gc < g.Nr) {
+                const float *p = Bt + gc * g.ldb + gk;
+                if (gk + 3 < g.Kr && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+                    vb = *(const float4 *)p;
+                } else {
+                    vb.x = gk < g.Kr ? p[0] : 0.f;
+                    vb.y = gk + 1 < g.Kr ? p[1] : 0.f;
+                    vb.z = gk + 2 < g.Kr ? p[2] : 0.f;
+                    vb.w = gk + 3 < g.Kr ? p[3] : 0.f;
+                }
+            }
+            ra[i] = va;
+            rb[i] = vb;
+        }
+    };
+    auto split = [](float4 v, bf16x4 &h, bf16x4 &l) {
+        const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const __bf16 hj = (__bf16)x[j];
+            h[j] = hj;
+            l[j] = (__bf16)(x[j] - (float)hj);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 512 * i;
+            const int row = idx >> 3, kq = (idx & 7) * 4;
+            const int o = bf3_off(row, kq);
+            bf16x4 h, l;
+            split(ra[i], h, l);
+            *(bf16x4 *)&Ah[buf][o] = h;
+            *(bf16x4 *)&Al[buf][o] = l;
+            split(rb[i], h, l);
+            *(bf16x4 *)&Bh[buf][o] = h;
+            *(bf16x4 *)&Bl[buf][o] = l;
+        }
+    };
+
+    floatx16 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int r32 = lane & 31, kg = 8 * (lane >> 5);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 16) {
+            bf16x8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int o = bf3_off(wm * 64 + mt * 32 + r32, ks + kg);
+                ah[mt] = *(const bf16x8 *)&Ah[buf][o];
+                al[mt] = *(const bf16x8 *)&Al[buf][o];
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int o = bf3_off(wn * 128 + nt * 32 + r32, ks + kg);
+                bh[nt] = *(const bf16x8 *)&Bh[buf][o];
+                bl[nt] = *(const bf16x8 *)&Bl[buf][o];
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bl[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+        }
+        if (kt + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    const float sgn = g.alpha_neg ? -1.f : 1.f;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const long long i = bm + wm * 64 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const long long j = bn + wn * 128 + nt * 32 + r32;
+                if (i < g.M && j < g.Nr) {
+                    float *c = C + i * g.ldc + j;
+                    const float val = sgn * acc[mt][nt][v];
+                    *c = g.beta ? (*c + val) : val;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
 namespace f64k {
 constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
 }
@@ -278,13 +449,17 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
     __shared__ double Bs[2][BN][BK + PAD];
     const TA *__restrict__ A = (const TA *)g.A;
     const double *__restrict__ Bt = (const double *)g.Bt;
-    double *__restrict__ C = (double *)g.C;
+    double *C = (double *)g.C;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int ty, tx;
     tile_of(g.swizzle, &ty, &tx);
     const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
-    const int nk = (g.Kr + BK - 1) / BK;
+    // split K: slice blockIdx.z covers [kb, kend) and writes its own C plane
+    const int kb = g.kslices > 1 ? (int)blockIdx.z * g.kc : 0;
+    const int kend = g.kslices > 1 ? min(g.Kr, kb + g.kc) : g.Kr;
+    const int nk = (kend - kb + BK - 1) / BK;
+    C += (size_t)blockIdx.z * g.cslice;
 
     double2 ra[4], rb[4];
     auto gload = [&](int k0) {
@@ -293,8 +468,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
             int idx = tid + 256 * i;
             int row = idx >> 3, kk = (idx & 7) * 2;
             long long gr = bm + row;
-            int gk = k0 + kk;
-            if (gr < g.M && gk < g.Kr) {
+            int gk = kb + k0 + kk;
+            if (gr < g.M && gk < kend) {
                 if constexpr (sizeof(TA) == 4) {
                     float2 v = *(const float2 *)(A + gr * g.lda + gk);
                     ra[i] = make_double2(v.x, v.y);
@@ -305,7 +480,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
                 ra[i] = make_double2(0.0, 0.0);
             }
             long long gc = bn + row;
-            rb[i] = (gc < g.Nr && gk < g.Kr) ? *(const double2 *)(Bt + gc * g.ldb + gk) : make_double2(0.0, 0.0);
+            rb[i] = (gc < g.Nr && gk < kend) ? *(const double2 *)(Bt + gc * g.ldb + gk) : make_double2(0.0, 0.0);
         }
     };
     auto sstore = [&](int buf) {
@@ -407,6 +582,12 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
     }();
     const bool vec4 = (g.lda % 4) == 0 && (g.ldb % 4) == 0 && (g.Kr % 4) == 0 &&
                       ((uintptr_t)g.A % 16) == 0 && ((uintptr_t)g.Bt % 16) == 0;
+    if (tag == 1 && g.bf16x3) {
+        dim3 gb((g.Nr + bf3k::BN - 1) / bf3k::BN, (g.M + bf3k::BM - 1) / bf3k::BM);
+        k_gemm_bf3<1><<<gb, 512, 0, s>>>(g);
+        EMFA_HIP_CHECK(hipGetLastError());
+        return EMFA_OK;
+    }
     // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart
     if (vec4 && variant == 1) {
         if (tag == 1)
@@ -431,7 +612,7 @@ emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g0, int tag) {
     GemmArgs g = g0;
     g.swizzle = swizzle_default();
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
-    dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
+    dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM, std::max(1, g.kslices));
     if (tag == 1)
         k_gemm_f64<double, 1><<<grid, 256, 0, s>>>(g);
     else
@@ -444,7 +625,7 @@ emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g0) {
     GemmArgs g = g0;
     g.swizzle = swizzle_default();
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32/f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
-    dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM);
+    dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM, std::max(1, g.kslices));
     k_gemm_f64<float, 0><<<grid, 256, 0, s>>>(g);
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;

@@ -57,6 +57,7 @@ struct emfa_ctx {
     int cu_share = 0;  // cap on the cooperative panel's workgroups (0: num_cu)
     // LU parameters (0 / -1: built-in choice); set by emfa_lu_set_params or the autotuner
     int lu_nb = 0, panel_g = 0, panel_x = -1;
+    int lu_update_mode = 0;  // FP32 LU trailing update: 0 FP32 MFMA, 1 three-term bf16 split
     size_t bytes = 0;
     bool prof = false;
     std::map<std::string, emfa::ProfFamily> fam;

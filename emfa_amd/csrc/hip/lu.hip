@@ -923,6 +923,8 @@ static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols,
     g.beta = 1;
     ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K, s);
     const int tag = std::strcmp(fam, "lu_gemm") == 0 ? 1 : 0;
+    g.bf16x3 = ctx->lu_update_mode == 1 ? 1 : 0;
+    if (const char *e = getenv("EMFA_LU_BF16X3")) g.bf16x3 = atoi(e) != 0;
     if constexpr (sizeof(R) == 4)
         return gemm_launch_f32(s, g, tag);
     else
@@ -1292,10 +1294,8 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     // into S slices (multiples of 64 reals) with partial results summed in a
     // fixed order afterwards.
     const long long tiles = (long long)((N + 63) / 64) * ((2 * m + 63) / 64);
-    int S = 1;  // measured no faster on gfx950 for 32 right-hand sides; opt-in
-    if (const char *e = getenv("EMFA_RESID_SPLIT"))
-        S = e[0] == 'a' ? (int)std::min<long long>(16, std::max<long long>(1, (2LL * ctx->num_cu + tiles - 1) / tiles))
-                        : std::max(1, std::min(64, atoi(e)));
+    int S = (int)std::min<long long>(16, std::max<long long>(1, (2LL * ctx->num_cu + tiles - 1) / tiles));
+    if (const char *e = getenv("EMFA_RESID_SPLIT")) S = std::max(1, std::min(64, atoi(e)));
     long long kc = ((2LL * K + S - 1) / S + 63) / 64 * 64;
     S = (int)((2LL * K + kc - 1) / kc);
     const size_t p_b = S > 1 ? (size_t)S * N * 2 * m * sizeof(double) : 0;
@@ -1321,14 +1321,13 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     g.beta = accumulate ? 1 : 0;
     ProfScope ps(ctx, "refine_gemm", 8.0 * (double)N * m * K);
     if (S == 1) return A->prec == EMFA_F32 ? gemm_launch_f32_f64(ctx->stream, g) : gemm_launch_f64(ctx->stream, g);
-    const size_t es = A->prec == EMFA_F32 ? sizeof(float) : sizeof(double);
-    for (int sl = 0; sl < S; ++sl) {
-        const long long k0 = (long long)sl * kc;
+    {
+        // one launch, slices on blockIdx.z, each writing its own plane of P
         GemmArgs gs = g;
-        gs.Kr = (int)std::min<long long>(kc, 2LL * K - k0);
-        gs.A = (const char *)A->d + k0 * es;
-        gs.Bt = Bt + k0;
-        gs.C = P + (size_t)sl * N * 2 * m;
+        gs.kslices = S;
+        gs.kc = (int)kc;
+        gs.cslice = (long long)N * 2 * m;
+        gs.C = P;
         gs.ldc = 2LL * m;
         gs.alpha_neg = 0;
         gs.beta = 0;
@@ -1467,7 +1466,13 @@ emfa_status emfa_lu_set_params(emfa_ctx *ctx, int nb, int panel_workgroups, int 
     return EMFA_OK;
 }
 
-emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out) {
+emfa_status emfa_lu_set_update_mode(emfa_ctx *ctx, int mode) {
+    EMFA_REQUIRE(ctx && (mode == 0 || mode == 1), EMFA_E_INVALID, "emfa_lu_set_update_mode: bad argument");
+    ctx->lu_update_mode = mode;
+    return EMFA_OK;
+}
+
+emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int try_split_update, emfa_lu_tuning *out) {
     EMFA_REQUIRE(ctx && n > 0 && (prec == EMFA_F32 || prec == EMFA_F64), EMFA_E_INVALID,
                  "emfa_lu_autotune: bad argument");
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
@@ -1483,15 +1488,17 @@ emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out
             k_tune_matrix<double><<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>((cx<double> *)A->d, n);
         EMFA_HIP_CHECK(hipGetLastError());
     }
-    const int save_nb = ctx->lu_nb, save_g = ctx->panel_g, save_x = ctx->panel_x;
+    const int save_nb = ctx->lu_nb, save_g = ctx->panel_g, save_x = ctx->panel_x, save_m = ctx->lu_update_mode;
     struct Cand {
-        int nb, x, g;
+        int nb, x, g, mode;
     };
     std::vector<Cand> cands;
     for (int nb : {256, 512})
         if (nb <= std::max(256, n))
             for (int x : {1, 0})
-                for (int g : {0, 128}) cands.push_back({nb, x, g});
+                for (int g : {0, 128}) cands.push_back({nb, x, g, 0});
+    if (try_split_update && prec == EMFA_F32)
+        for (int g : {0, 128}) cands.push_back({512, 1, g, 1});
     hipEvent_t e0, e1;
     EMFA_HIP_CHECK(hipEventCreate(&e0));
     EMFA_HIP_CHECK(hipEventCreate(&e1));
@@ -1503,6 +1510,7 @@ emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out
         ctx->lu_nb = cands[i].nb;
         ctx->panel_g = cands[i].g;
         ctx->panel_x = cands[i].x;
+        ctx->lu_update_mode = cands[i].mode;
         if (i == 0) {  // warm-up (module load, pool)
             st = emfa_lu_factor(ctx, A, prec, &lu);
             emfa_lu_destroy(lu);
@@ -1518,10 +1526,11 @@ emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out
         if (st) break;
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (out && i < 8) {
+        if (out && i < 16) {
             out->cand_nb[i] = cands[i].nb;
             out->cand_variant[i] = cands[i].x;
             out->cand_workgroups[i] = cands[i].g;
+            out->cand_update_mode[i] = cands[i].mode;
             out->cand_ms[i] = ms;
         }
         ++tried;
@@ -1537,15 +1546,18 @@ emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out
         ctx->lu_nb = save_nb;
         ctx->panel_g = save_g;
         ctx->panel_x = save_x;
+        ctx->lu_update_mode = save_m;
         return st;
     }
     ctx->lu_nb = best.nb;
     ctx->panel_g = best.g;
     ctx->panel_x = best.x;
+    ctx->lu_update_mode = best.mode;
     if (out) {
         out->nb = best.nb;
         out->panel_variant = best.x;
         out->panel_workgroups = best.g;
+        out->update_mode = best.mode;
         out->best_ms = best_ms;
         out->candidates = tried;
     }

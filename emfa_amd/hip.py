@@ -26,6 +26,7 @@ EXPORTS = [
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_autotune",
+    "emfa_lu_set_update_mode",
 ]
 
 _vp = ctypes.c_void_p
@@ -48,9 +49,10 @@ class GmresReport(ctypes.Structure):
 
 class LuTuning(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int), ("panel_variant", ctypes.c_int), ("panel_workgroups", ctypes.c_int),
-                ("best_ms", ctypes.c_float), ("candidates", ctypes.c_int), ("cand_nb", ctypes.c_int * 8),
-                ("cand_variant", ctypes.c_int * 8), ("cand_workgroups", ctypes.c_int * 8),
-                ("cand_ms", ctypes.c_float * 8)]
+                ("update_mode", ctypes.c_int), ("best_ms", ctypes.c_float), ("candidates", ctypes.c_int),
+                ("cand_nb", ctypes.c_int * 16), ("cand_variant", ctypes.c_int * 16),
+                ("cand_workgroups", ctypes.c_int * 16), ("cand_update_mode", ctypes.c_int * 16),
+                ("cand_ms", ctypes.c_float * 16)]
 
 
 class MlfmaParams(ctypes.Structure):
@@ -188,16 +190,22 @@ class Context:
     def lu_params(self, nb=0, panel_workgroups=0, panel_variant=-1):
         _check(lib().emfa_lu_set_params(self.h, int(nb), int(panel_workgroups), int(panel_variant)))
 
-    def autotune_lu(self, n, prec):
+    def lu_update_mode(self, mode):
+        """0: FP32 MFMA trailing update; 1: three-term bf16 split on bf16 MFMA."""
+        _check(lib().emfa_lu_set_update_mode(self.h, int(mode)))
+
+    def autotune_lu(self, n, prec, try_split_update=False):
         """Time the LU candidates at size n, keep the fastest; returns the report."""
         t = LuTuning()
-        _check(lib().emfa_lu_autotune(self.h, int(n), int(prec), ctypes.byref(t)))
+        _check(lib().emfa_lu_autotune(self.h, int(n), int(prec), int(bool(try_split_update)), ctypes.byref(t)))
         c = t.candidates
         return {"nb": t.nb, "panel_variant": t.panel_variant, "panel_workgroups": t.panel_workgroups,
-                "best_ms": t.best_ms,
+                "update": "bf16x3" if t.update_mode == 1 else "fp32", "best_ms": t.best_ms,
                 "candidates": [{"nb": t.cand_nb[i], "panel_variant": t.cand_variant[i],
-                                "panel_workgroups": t.cand_workgroups[i], "ms": round(t.cand_ms[i], 2)}
-                               for i in range(min(c, 8))]}
+                                "panel_workgroups": t.cand_workgroups[i],
+                                "update": "bf16x3" if t.cand_update_mode[i] == 1 else "fp32",
+                                "ms": round(t.cand_ms[i], 2)}
+                               for i in range(min(c, 16))]}
 
     def set_cu_share(self, cus):
         _check(lib().emfa_ctx_set_cu_share(self.h, int(cus)))

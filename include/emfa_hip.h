@@ -144,17 +144,25 @@ int emfa_lu_info(const emfa_lu *lu);
  * 0: arrival counter, -1: built-in choice).  No counterpart in the reference
  * (LAPACK chooses its own blocking). */
 emfa_status emfa_lu_set_params(emfa_ctx *ctx, int nb, int panel_workgroups, int panel_variant);
+/* Trailing-update arithmetic of FP32 factorisations: 0 = FP32 MFMA
+ * (V_MFMA_F32_32X32X2_F32, default), 1 = FP32 operands split into two bf16
+ * terms on their way to LDS and multiplied on bf16 MFMA with FP32
+ * accumulation (hi*hi + hi*lo + lo*hi, about 16 significant bits per
+ * product; meant for FP32 + FP64-refinement solves).  No reference
+ * counterpart. */
+emfa_status emfa_lu_set_update_mode(emfa_ctx *ctx, int mode);
 typedef struct {
-    int nb, panel_variant, panel_workgroups; /* chosen */
-    float best_ms;                           /* its factorisation time */
-    int candidates;                          /* candidates timed */
-    int cand_nb[8], cand_variant[8], cand_workgroups[8];
-    float cand_ms[8];
+    int nb, panel_variant, panel_workgroups, update_mode; /* chosen */
+    float best_ms;                                        /* its factorisation time */
+    int candidates;                                       /* candidates timed */
+    int cand_nb[16], cand_variant[16], cand_workgroups[16], cand_update_mode[16];
+    float cand_ms[16];
 } emfa_lu_tuning;
 /* Autotuner: times the LU of a generated n x n matrix in precision prec for
- * each candidate (nb 256/512 x panel variant x panel workgroup cap all/128),
- * keeps the fastest in the context and reports all timings. */
-emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, emfa_lu_tuning *out);
+ * each candidate (nb 256/512 x panel variant x panel workgroup cap all/128;
+ * with try_split_update and FP32 also the bf16-split update), keeps the
+ * fastest in the context and reports all timings. */
+emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int try_split_update, emfa_lu_tuning *out);
 
 /* ---- mixed-precision refinement -------------------------------------- */
 typedef struct {

