@@ -30,11 +30,33 @@ struct GemmArgs {
     int alpha_neg;          // 1: C = beta C - A B ; 0: C = beta C + A B
     int beta;               // 0 or 1
     int swizzle = 1;        // XCD-aware tile order (set by the launchers)
-    int bf16x3 = 0;         // FP32 launcher, tag 1: three-term bf16 split on bf16 MFMA
     int kslices = 1;        // FP64 launchers: split K over blockIdx.z (C += z * cslice)
     int kc = 0;             // reals of K per slice (multiple of 64)
     long long cslice = 0;   // C elements between slice planes
 };
+
+// Three-term bf16 product from pre-split operand planes.  Each operand is
+// given as two bf16 planes (hi = bf16(x), lo = bf16(x - hi)), row-major with
+// ld elements per row; rows are padded with zeros to a multiple of 256
+// from the given pointer and K to a multiple of 16 (Kp), so the kernel
+// reads whole tiles with no bounds checks.  C (FP32, M x Nr, ldc) gets
+// beta C -/+ (Ah Bh^T + Ah Bl^T + Al Bh^T) on rows < M, columns < Nr.
+struct Bf3Planes {
+    int M, Nr, Kp;
+    const __bf16 *Ah, *Al;
+    long long lda;
+    const __bf16 *Bh, *Bl;
+    long long ldb;
+    float *C;
+    long long ldc;
+    int alpha_neg, beta;
+};
+
+// Split rows x cols FP32 (row stride ld, 8-byte aligned rows) into hi/lo bf16
+// planes of rows_p x cols_p (row stride ldp), zero outside rows x cols.
+emfa_status split_bf16_launch(hipStream_t s, const float *src, long long ld, int rows, int cols, __bf16 *hi,
+                              __bf16 *lo, long long ldp, int rows_p, int cols_p);
+emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g);
 
 emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=float,  TC=float
 emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=double, TC=double

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <type_traits>
 
 #include "gemm.h"
 
@@ -269,105 +270,98 @@ __global__ __launch_bounds__(256) void k_gemm_f32v(GemmArgs g) {
 // TA = float converts the matrix operand to double on its way into LDS.
 // ---------------------------------------------------------------------------
 // FP32 update emulated on bf16 MFMA (opt-in per context, LU trailing update
-// only).  Each FP32 operand x is split on its way to LDS into
-// hi = bf16(x), lo = bf16(x - hi); the product uses hi*hi + hi*lo + lo*hi
-// (three V_MFMA_F32_32X32X16_BF16 per block and 16-deep K step) with FP32
-// accumulation, which keeps about 16 significant bits of every product;
-// the FP64 refinement on the FP32 matrix removes the difference.
-// Workgroup tile 256 x 256, 8 waves in 4 x 2, wave tile 64 x 128 = 2 x 4
-// MFMA 32x32 blocks.  K step 32 (two MFMA steps), LDS double-buffered (hi
-// and lo planes of A and B, 128 KB), next tile prefetched into registers
-// during the MFMAs.
+// only).  Each FP32 operand x is split into hi = bf16(x), lo = bf16(x - hi);
+// the product uses hi*hi + hi*lo + lo*hi (three V_MFMA_F32_32X32X16_BF16 per
+// block and 16-deep K step) with FP32 accumulation, which keeps about 16
+// significant bits of every product; the FP64 refinement on the FP32 matrix
+// removes the difference.
 // MFMA lane layout: lane l holds row (A) / column (B) l % 32 and the eight
 // K values 8 * (l / 32) .. + 7; result element v of lane l is
 // C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-namespace bf3k {
-constexpr int BM = 256, BN = 256, BK = 32;  // LDS rows of BK bf16 = 64 B, 16-byte chunks XOR-swizzled
+// ---------------------------------------------------------------------------
+// Operand split for the plane kernel: one thread per 8 consecutive K of a row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ src, long long ld, int rows, int cols,
+                                                    __bf16 *__restrict__ hi, __bf16 *__restrict__ lo, long long ldp,
+                                                    int rows_p, int cols_p) {
+    const int chunks = cols_p >> 3;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)rows_p * chunks) return;
+    const int r = (int)(t / chunks), k0 = (int)(t % chunks) * 8;
+    bf16x8 h, l;
+#pragma unroll
+    for (int j = 0; j < 8; j += 2) {
+        float2 v = make_float2(0.f, 0.f);
+        if (r < rows) {
+            const float *q = src + (long long)r * ld + k0 + j;
+            if (k0 + j + 1 < cols)
+                v = *(const float2 *)q;
+            else if (k0 + j < cols)
+                v.x = q[0];
+        }
+        const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y;
+        h[j] = h0;
+        h[j + 1] = h1;
+        l[j] = (__bf16)(v.x - (float)h0);
+        l[j + 1] = (__bf16)(v.y - (float)h1);
+    }
+    *(bf16x8 *)(hi + (long long)r * ldp + k0) = h;
+    *(bf16x8 *)(lo + (long long)r * ldp + k0) = l;
 }
 
-// 16-byte chunk c (0..3) of LDS row r lives at chunk c ^ ((r >> 2) & 3): the 16
-// rows read together by a ds_read_b128 then hit 16 distinct 4-bank groups.
-__device__ __forceinline__ int bf3_off(int r, int k) {
-    return r * bf3k::BK + ((((k >> 3) ^ (r >> 2)) & 3) << 3) + (k & 7);
+// ---------------------------------------------------------------------------
+// Three-term bf16 GEMM from pre-split planes (Bf3Planes).  Workgroup tile
+// 256 x 256, 8 waves as 4 x 2 with wave tiles 64 x 128 (2 x 4 MFMA 32x32x16).
+// The operand tiles go straight from global memory into LDS with 16-byte
+// direct loads (global_load_lds_dwordx4: lane l of a wave writes LDS bytes
+// base + 16 l), four stages of K = 16 in flight: stage kt + 3 is issued while
+// stage kt is multiplied.  No registers stage the operands and no VALU work
+// sits in the loop.  An LDS plane row holds 16 bf16 (two 16-byte chunks);
+// chunk c of row r is stored at position c ^ ((r >> 2) & 1), so the eight
+// rows served per ds_read_b128 cycle hit eight distinct 4-bank groups.
+// Each wave waits (s_waitcnt vmcnt) for its own loads of the stage, then the
+// workgroup barrier publishes every wave's loads; the same barrier tells the
+// issuing waves that all reads of the buffer being refilled are done.
+// ---------------------------------------------------------------------------
+namespace bf3p {
+constexpr int BM = 256, BN = 256, BK = 16, NS = 4, PLANE = 256 * BK;
 }
 
-template <int TAG>
-__global__ __launch_bounds__(512) void k_gemm_bf3(GemmArgs g) {
-    using namespace bf3k;
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][BM * BK], Al[2][BM * BK], Bh[2][BN * BK], Bl[2][BN * BK];
-    const float *__restrict__ A = (const float *)g.A;
-    const float *__restrict__ Bt = (const float *)g.Bt;
-    float *__restrict__ C = (float *)g.C;
+__device__ __forceinline__ int bf3p_off(int r, int kg) { return r * bf3p::BK + ((kg ^ ((r >> 2) & 1)) << 3); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
+    using namespace bf3p;
+    __shared__ __attribute__((aligned(16))) __bf16 sm[NS][4][PLANE];  // stage, plane (Ah Al Bh Bl)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int ty, tx;
-    tile_of(g.swizzle, &ty, &tx);
+    tile_of(1, &ty, &tx);
     const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
-    const int nk = (g.Kr + BK - 1) / BK;
+    const int nk = g.Kp / BK;
 
-    // per k tile: 256 rows x 32 reals of A and of B = 2048 float4 each, 4 per thread
-    float4 ra[4], rb[4];
-    auto gload = [&](int k0) {
+    // this wave's direct loads: plane pl, rows half * 128 + 32 i + lane / 2
+    const int pl = wave >> 1, half = wave & 1;
+    const __bf16 *src = pl == 0 ? g.Ah : pl == 1 ? g.Al : pl == 2 ? g.Bh : g.Bl;
+    const long long ld = pl < 2 ? g.lda : g.ldb;
+    const long long row0 = pl < 2 ? bm : bn;
+    const __bf16 *gp[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int idx = tid + 512 * i;
-            const int row = idx >> 3, kq = (idx & 7) * 4;
-            const int gk = k0 + kq;
-            const long long gr = bm + row, gc = bn + row;
-            float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gr < g.M) {
-                const float *p = A + gr * g.lda + gk;
-                if (gk + 3 < g.Kr && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-                    va = *(const float4 *)p;
-                } else {
-                    va.x = gk < g.Kr ? p[0] : 0.f;
-                    va.y = gk + 1 < g.Kr ? p[1] : 0.f;
-                    va.z = gk + 2 < g.Kr ? p[2] : 0.f;
-                    va.w = gk + 3 < g.Kr ? p[3] : 0.f;
-                }
-            }
-            if (gc < g.Nr) {
-                const float *p = Bt + gc * g.ldb + gk;
-                if (gk + 3 < g.Kr && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-                    vb = *(const float4 *)p;
-                } else {
-                    vb.x = gk < g.Kr ? p[0] : 0.f;
-                    vb.y = gk + 1 < g.Kr ? p[1] : 0.f;
-                    vb.z = gk + 2 < g.Kr ? p[2] : 0.f;
-                    vb.w = gk + 3 < g.Kr ? p[3] : 0.f;
-                }
-            }
-            ra[i] = va;
-            rb[i] = vb;
-        }
-    };
-    auto split = [](float4 v, bf16x4 &h, bf16x4 &l) {
-        const float x[4] = {v.x, v.y, v.z, v.w};
+    for (int i = 0; i < 4; ++i) {
+        const int r = half * 128 + 32 * i + (lane >> 1);
+        const int c = (lane & 1) ^ ((r >> 2) & 1);
+        gp[i] = src + (row0 + r) * ld + 8 * c;
+    }
+    auto issue = [&](int kt) {
+        const int st = kt & (NS - 1);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const __bf16 hj = (__bf16)x[j];
-            h[j] = hj;
-            l[j] = (__bf16)(x[j] - (float)hj);
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int idx = tid + 512 * i;
-            const int row = idx >> 3, kq = (idx & 7) * 4;
-            const int o = bf3_off(row, kq);
-            bf16x4 h, l;
-            split(ra[i], h, l);
-            *(bf16x4 *)&Ah[buf][o] = h;
-            *(bf16x4 *)&Al[buf][o] = l;
-            split(rb[i], h, l);
-            *(bf16x4 *)&Bh[buf][o] = h;
-            *(bf16x4 *)&Bl[buf][o] = l;
-        }
+        for (int i = 0; i < 4; ++i)
+            __builtin_amdgcn_global_load_lds((const void *)(gp[i] + (long long)kt * BK),
+                                             (lds_void *)&sm[st][pl][(half * 128 + 32 * i) * BK], 16, 0, 0);
     };
 
     floatx16 acc[2][4];
@@ -378,63 +372,84 @@ __global__ __launch_bounds__(512) void k_gemm_bf3(GemmArgs g) {
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    const int r32 = lane & 31, kg = 8 * (lane >> 5);
+    issue(0);
+    if (nk > 1) issue(1);
+    if (nk > 2) issue(2);
+    const int r32 = lane & 31, kg = lane >> 5;
     for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * BK);
+        // loads of stages kt+1, kt+2 (4 per stage) may stay in flight
+        if (kt + 2 < nk)
+            asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+        else if (kt + 1 < nk)
+            asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (kt + 3 < nk) issue(kt + 3);
+        const int st = kt & (NS - 1);
+        bf16x8 ah[2], al[2], bh[4], bl[4];
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 16) {
-            bf16x8 ah[2], al[2], bh[4], bl[4];
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const int o = bf3_off(wm * 64 + mt * 32 + r32, ks + kg);
-                ah[mt] = *(const bf16x8 *)&Ah[buf][o];
-                al[mt] = *(const bf16x8 *)&Al[buf][o];
-            }
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const int o = bf3_off(wn * 128 + nt * 32 + r32, ks + kg);
-                bh[nt] = *(const bf16x8 *)&Bh[buf][o];
-                bl[nt] = *(const bf16x8 *)&Bl[buf][o];
-            }
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mt], bh[nt], acc[mt][nt], 0, 0, 0);
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bl[nt], acc[mt][nt], 0, 0, 0);
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+        for (int mt = 0; mt < 2; ++mt) {
+            const int o = bf3p_off(wm * 64 + mt * 32 + r32, kg);
+            ah[mt] = *(const bf16x8 *)&sm[st][0][o];
+            al[mt] = *(const bf16x8 *)&sm[st][1][o];
         }
-        if (kt + 1 < nk) sstore(buf ^ 1);
-        __syncthreads();
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int o = bf3p_off(wn * 128 + nt * 32 + r32, kg);
+            bh[nt] = *(const bf16x8 *)&sm[st][2][o];
+            bl[nt] = *(const bf16x8 *)&sm[st][3][o];
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bl[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bh[nt], acc[mt][nt], 0, 0, 0);
     }
 
+    // epilogue: all C loads of a 32-row slab are issued before its stores;
+    // tiles wholly inside C skip the per-element bounds checks
     const float sgn = g.alpha_neg ? -1.f : 1.f;
+    float *__restrict__ C = g.C + (bm + wm * 64 + 4 * (lane >> 5)) * g.ldc + bn + wn * 128 + r32;
+    const int mrem = (int)min((long long)BM, g.M - bm) - wm * 64 - 4 * (lane >> 5);
+    const int nrem = (int)min((long long)BN, (long long)g.Nr - bn) - wn * 128 - r32;
+    const bool full = bm + BM <= g.M && bn + BN <= g.Nr;
+    auto epilogue = [&](auto guarded) {
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < 2; ++mt) {
+            float cv[4][16];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
+            for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const long long i = bm + wm * 64 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const long long j = bn + wn * 128 + nt * 32 + r32;
-                if (i < g.M && j < g.Nr) {
-                    float *c = C + i * g.ldc + j;
-                    const float val = sgn * acc[mt][nt][v];
-                    *c = g.beta ? (*c + val) : val;
+                for (int v = 0; v < 16; ++v) {
+                    const int i = mt * 32 + 8 * (v >> 2) + (v & 3), j = nt * 32;
+                    const bool in = !guarded.value || (i < mrem && j < nrem);
+                    cv[nt][v] = (g.beta && in) ? C[i * g.ldc + j] : 0.f;
                 }
-            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = mt * 32 + 8 * (v >> 2) + (v & 3), j = nt * 32;
+                    if (!guarded.value || (i < mrem && j < nrem)) C[i * g.ldc + j] = cv[nt][v] + sgn * acc[mt][nt][v];
+                }
+        }
+    };
+    if (full)
+        epilogue(std::false_type{});
+    else
+        epilogue(std::true_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -582,12 +597,6 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
     }();
     const bool vec4 = (g.lda % 4) == 0 && (g.ldb % 4) == 0 && (g.Kr % 4) == 0 &&
                       ((uintptr_t)g.A % 16) == 0 && ((uintptr_t)g.Bt % 16) == 0;
-    if (tag == 1 && g.bf16x3) {
-        dim3 gb((g.Nr + bf3k::BN - 1) / bf3k::BN, (g.M + bf3k::BM - 1) / bf3k::BM);
-        k_gemm_bf3<1><<<gb, 512, 0, s>>>(g);
-        EMFA_HIP_CHECK(hipGetLastError());
-        return EMFA_OK;
-    }
     // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart
     if (vec4 && variant == 1) {
         if (tag == 1)
@@ -604,6 +613,31 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
     } else {
         k_gemm_f32<0><<<grid, 256, 0, s>>>(g);
     }
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+emfa_status split_bf16_launch(hipStream_t s, const float *src, long long ld, int rows, int cols, __bf16 *hi,
+                              __bf16 *lo, long long ldp, int rows_p, int cols_p) {
+    EMFA_REQUIRE(rows >= 0 && cols >= 0 && rows <= rows_p && cols <= cols_p && cols_p % 8 == 0 && ldp >= cols_p &&
+                     ldp % 8 == 0 && (ld % 2) == 0 && ((uintptr_t)src % 8) == 0 && ((uintptr_t)hi % 16) == 0 &&
+                     ((uintptr_t)lo % 16) == 0,
+                 EMFA_E_INVALID, "split_bf16: bad shape rows=%d/%d cols=%d/%d", rows, rows_p, cols, cols_p);
+    const long long work = (long long)rows_p * (cols_p / 8);
+    if (work == 0) return EMFA_OK;
+    k_split_bf16<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(src, ld, rows, cols, hi, lo, ldp, rows_p, cols_p);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g) {
+    auto al16 = [](const void *p) { return ((uintptr_t)p % 16) == 0; };
+    EMFA_REQUIRE(g.M > 0 && g.Nr > 0 && g.Kp > 0 && g.Kp % bf3p::BK == 0 && g.lda >= g.Kp && g.ldb >= g.Kp &&
+                     g.lda % 8 == 0 && g.ldb % 8 == 0 && al16(g.Ah) && al16(g.Al) && al16(g.Bh) && al16(g.Bl) &&
+                     g.M <= 65535LL * bf3p::BM && g.Nr <= 65535LL * bf3p::BN,
+                 EMFA_E_INVALID, "gemm bf16x3 planes: bad shape M=%d Nr=%d Kp=%d", g.M, g.Nr, g.Kp);
+    dim3 grid((g.Nr + bf3p::BN - 1) / bf3p::BN, (g.M + bf3p::BM - 1) / bf3p::BM);
+    k_gemm_bf3p<<<grid, 512, 0, s>>>(g);
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
 }

@@ -923,13 +923,71 @@ static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols,
     g.beta = 1;
     ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K, s);
     const int tag = std::strcmp(fam, "lu_gemm") == 0 ? 1 : 0;
-    g.bf16x3 = ctx->lu_update_mode == 1 ? 1 : 0;
-    if (const char *e = getenv("EMFA_LU_BF16X3")) g.bf16x3 = atoi(e) != 0;
     if constexpr (sizeof(R) == 4)
         return gemm_launch_f32(s, g, tag);
     else
         return gemm_launch_f64(s, g, tag);
 }
+
+// Operand planes of the three-term bf16 trailing update (FP32 factorisations):
+// the outer panel's L block (M x 2jb reals) and expanded U rows (2(N-next) x
+// 2jb reals), split once per outer step and read by both update halves.
+struct UpdatePlanes {
+    __bf16 *ah = nullptr, *al = nullptr, *bh = nullptr, *bl = nullptr;
+    int a_rows = 0, b_rows = 0, kp = 0;  // allocated rows per plane, K stride
+    static int rup(int a, int b) { return (a + b - 1) / b * b; }
+    static size_t bytes(int N, int nb) {
+        const size_t kp = (size_t)rup(2 * nb, 16);
+        return 2 * kp * sizeof(__bf16) * ((size_t)rup(N, 256) + (size_t)rup(2 * N, 256) + 256) + 1024;
+    }
+    void carve(void *p, int N, int nb) {
+        kp = rup(2 * nb, 16);
+        a_rows = rup(N, 256);
+        b_rows = rup(2 * N, 256) + 256;
+        char *q = (char *)p;
+        auto take = [&](size_t b) {
+            char *r = q;
+            q += (b + 255) & ~(size_t)255;
+            return (__bf16 *)r;
+        };
+        ah = take((size_t)a_rows * kp * sizeof(__bf16));
+        al = take((size_t)a_rows * kp * sizeof(__bf16));
+        bh = take((size_t)b_rows * kp * sizeof(__bf16));
+        bl = take((size_t)b_rows * kp * sizeof(__bf16));
+    }
+    // split L (M x K complex at Aop, lda) and the U rows (2*ncols x 2K reals, ldb)
+    emfa_status split(emfa_ctx *ctx, int M, int ncols, int K, const cx<float> *Aop, long long lda, const float *Bt,
+                      long long ldb) {
+        ProfScope ps(ctx, "lu_split", 0);
+        const int kr = 2 * K, k16 = rup(kr, 16);
+        emfa_status st = split_bf16_launch(ctx->stream, (const float *)Aop, 2 * lda, M, kr, ah, al, kp,
+                                           rup(M, 256), k16);
+        if (st) return st;
+        return split_bf16_launch(ctx->stream, Bt, ldb, 2 * ncols, kr, bh, bl, kp,
+                                 std::min(b_rows, rup(2 * ncols, 256) + 256), k16);
+    }
+    // C (M x ncols complex) -= L * U[:, c0 : c0 + ncols]
+    emfa_status update(emfa_ctx *ctx, int M, int c0, int ncols, int K, cx<float> *C, long long ldc,
+                       hipStream_t s) const {
+        if (M <= 0 || ncols <= 0 || K <= 0) return EMFA_OK;
+        ProfScope ps(ctx, "lu_gemm_split", 8.0 * (double)M * ncols * K, s);
+        Bf3Planes g;
+        g.M = M;
+        g.Nr = 2 * ncols;
+        g.Kp = rup(2 * K, 16);
+        g.Ah = ah;
+        g.Al = al;
+        g.lda = kp;
+        g.Bh = bh + (size_t)2 * c0 * kp;
+        g.Bl = bl + (size_t)2 * c0 * kp;
+        g.ldb = kp;
+        g.C = (float *)C;
+        g.ldc = 2 * ldc;
+        g.alpha_neg = 1;
+        g.beta = 1;
+        return gemm_launch_bf3_planes(s, g);
+    }
+};
 
 // Blocked triangular solve of rows [r0, r0+wb) of X (columns [c0,c1)) with the
 // diagonal block of F: 64-row diagonal solves (k_trsm64) + MFMA updates of
@@ -1047,8 +1105,15 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     // packet-exchange slots: packets, candidate rows, current row (zeroed below), values
     size_t x_pk_b = 2ull * kPanelMaxG * 8, x_rw_b = 2ull * kPanelMaxG * kPanelMaxW * 16,
            x_cw_b = 2ull * kPanelMaxW * 16, x_val_b = 2ull * kPanelMaxG * sizeof(R);
+    // FP32 factorisations with the bf16-split trailing update keep split operand planes
+    bool split_update = false;
+    if constexpr (sizeof(R) == 4) {
+        split_update = ctx->lu_update_mode == 1;
+        if (const char *e = getenv("EMFA_LU_BF16X3")) split_update = atoi(e) != 0;
+    }
+    size_t planes_b = split_update ? UpdatePlanes::bytes(N, nb) : 0;
     size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + 2 * bt_b + perm_b + x_pk_b + x_rw_b +
-                   x_cw_b + x_val_b + 8192;
+                   x_cw_b + x_val_b + planes_b + 8192;
     void *scr = nullptr;
     emfa_status st = ctx_scratch(ctx, total, &scr);
     if (st != EMFA_OK) return st;
@@ -1070,6 +1135,8 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     unsigned long long *x_pk = (unsigned long long *)take(x_pk_b + x_rw_b + x_cw_b);
     unsigned long long *x_rw = x_pk + x_pk_b / 8, *x_cw = x_rw + x_rw_b / 8;
     R *x_val = (R *)take(x_val_b);
+    UpdatePlanes planes;
+    if (split_update) planes.carve(take(planes_b), N, nb);
     {
         const int nw = (int)((x_pk_b + x_rw_b + x_cw_b) / 4);
         k_zero_u32<<<(nw + 255) / 256, 256, 0, ctx->stream>>>((unsigned *)x_pk, nw);
@@ -1169,23 +1236,27 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         const int M = N - next;
         const long long ldbt = 2LL * jb;
         const cx<R> *Aop = A + (size_t)next * lda + j0;
+        // C[:, c0 : c0 + ncols] of the trailing block -= L U[:, c0 : c0 + ncols]
+        auto trailing = [&](int c0, int ncols, hipStream_t s) -> emfa_status {
+            cx<R> *C = A + (size_t)next * lda + next + c0;
+            if constexpr (sizeof(R) == 4)
+                if (split_update) return planes.update(ctx, M, c0, ncols, jb, C, lda, s);
+            return gemm_update<R>(ctx, "lu_gemm", M, ncols, jb, Aop, lda, Bt_out + 2LL * c0 * ldbt, ldbt, C, lda, s);
+        };
+        if constexpr (sizeof(R) == 4)
+            if (split_update && (st = planes.split(ctx, M, M, jb, Aop, lda, Bt_out, ldbt))) return st;
         if (lookahead && N - next - nbn > 0) {
             // G2 (all columns but the next panel's) on the second stream, G1 and the next
             // panel's factorisation on the main stream meanwhile.
             EMFA_HIP_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
             EMFA_HIP_CHECK(hipStreamWaitEvent(ctx->aux, ctx->ev[0], 0));
-            st = gemm_update<R>(ctx, "lu_gemm", M, N - next - nbn, jb, Aop, lda, Bt_out + 2LL * nbn * ldbt, ldbt,
-                                A + (size_t)next * lda + next + nbn, lda, ctx->aux);
-            if (st) return st;
+            if ((st = trailing(nbn, N - next - nbn, ctx->aux))) return st;
             EMFA_HIP_CHECK(hipEventRecord(ctx->ev[1], ctx->aux));
-            st = gemm_update<R>(ctx, "lu_gemm", M, nbn, jb, Aop, lda, Bt_out, ldbt, A + (size_t)next * lda + next,
-                                lda);
-            if (st) return st;
+            if ((st = trailing(0, nbn, ctx->stream))) return st;
             if ((st = factor_outer(next, nbn))) return st;
             EMFA_HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->ev[1], 0));
         } else {
-            st = gemm_update<R>(ctx, "lu_gemm", M, M, jb, Aop, lda, Bt_out, ldbt, A + (size_t)next * lda + next, lda);
-            if (st) return st;
+            if ((st = trailing(0, M, ctx->stream))) return st;
             if ((st = factor_outer(next, nbn))) return st;
         }
         if ((st = outer_swap_trsm(next, nbn))) return st;

@@ -252,6 +252,36 @@ def test_bf16x3_update_lu_refines_to_fp64_accuracy(ctx):
     assert len(set(lu.pivots().tolist())) > 0
 
 
+@pytest.mark.parametrize("n,nb", [(1003, 256), (700, 512), (1600, 384)])
+def test_bf16x3_update_ragged_sizes(ctx, n, nb):
+    # ragged outer blocks (K not a multiple of 16, M not of 256) go through the
+    # zero-padded operand planes; the factor matches the FP32-MFMA one and
+    # refines to the FP64 solution
+    _say(f"bf16x3 ragged n={n} nb={nb}")
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    Z = hip.Mat.from_numpy(ctx, A, hip.F32)
+    ctx.lu_params(nb, 0, -1)
+    try:
+        lu0 = hip.LU(ctx, Z, hip.F32)
+        ctx.lu_update_mode(1)
+        lu1 = hip.LU(ctx, Z, hip.F32)
+    finally:
+        ctx.lu_update_mode(0)
+        ctx.lu_params(0, 0, -1)
+    assert np.array_equal(lu0.pivots(), lu1.pivots())  # diagonally dominant: no pivoting ties
+    f0, f1 = lu0.factor().numpy(), lu1.factor().numpy()
+    assert np.abs(f1 - f0).max() / np.abs(f0).max() < 1e-5
+    Bd = hip.Mat.from_numpy(ctx, B, hip.F64)
+    X = hip.Mat(ctx, n, 3, hip.F64)
+    rep, ok = hip.refine(ctx, Z, lu1, Bd, X, tol=1e-12, maxit=12)
+    assert ok and rep.relres_final < 1e-12
+    assert rel(X.numpy(), np.linalg.solve(Z.numpy(), B)) < 1e-10
+    for m in (lu0, lu1, Z, Bd, X):
+        m.close()
+
+
 def test_lu_singular_reports(ctx):
     _say("lu singular")
     n = 40
