@@ -32,6 +32,11 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK = {"f32_mfma": 157.3, "f64_mfma": 78.6, "bf16_mfma": 2500.0, "hbm": 8000.0}
+# HBM bytes per launch of the split trailing update on the 30720-unknown
+# workload (nb 512), from the FETCH_SIZE / WRITE_SIZE counter runs recorded in
+# profiles/lu_gemm_split_cfie_sphere_30720.md (fetch scaled by 2.0 after a 1 GiB
+# copy calibration), beside the algorithmic bytes of the same launches
+TRAFFIC_SPLIT_30720 = {"measured": 4.703e9, "algorithmic": 2.644e9}
 
 CONFIGS = {
     # name: (kind, subdiv, radius, freq, alpha, quad, nrhs-angles)
@@ -239,9 +244,12 @@ def main():
     if split:
         roof = {"bound": "mfma", "kernel": "lu_gemm_split (k_gemm_bf3p, V_MFMA_F32_32X32X16_BF16, 3 products)",
                 "achieved": gemm_tflops, "peak": PEAK["bf16_mfma"], "unit": "TFLOP/s",
-                "frac": gemm_tflops / PEAK["bf16_mfma"], "traffic": None,
+                "frac": gemm_tflops / PEAK["bf16_mfma"],
+                "traffic": TRAFFIC_SPLIT_30720["measured"] if n == 30720 else None,
+                "traffic_algorithmic": TRAFFIC_SPLIT_30720["algorithmic"] if n == 30720 else None,
                 "note": "achieved counts 8 M N K complex-FP32 flops once; the kernel issues three bf16 MFMA "
-                        "products per flop, so MFMA-pipe utilisation is 3 x frac",
+                        "products per flop, so MFMA-pipe utilisation is 3 x frac; traffic in bytes per launch "
+                        "from the counter runs of the same command (nb 512)",
                 "avg_launch_ms": avg_launch_ms, "launches_per_step": g["launches_per_step"]}
     else:
         roof = {"bound": "mfma", "kernel": "lu_gemm (k_gemm_f32, V_MFMA_F32_32X32X2_F32)",
