@@ -60,8 +60,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-stream summary of the default line")
     ap.add_argument("--no-tune", action="store_true", help="skip the LU autotuner (built-in nb / panel choice)")
-    ap.add_argument("--fp32-update-only", action="store_true",
-                    help="keep the FP32-MFMA trailing update (do not let the tuner pick the bf16-split update)")
+    ap.add_argument("--pivoted-fp32-only", action="store_true",
+                    help="keep the FP32 factor on FP32-MFMA updates with partial pivoting (the tuner does not "
+                         "consider the bf16-split update or panels without interchanges)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="copy a buffer of this many MiB once (a plain copy kernel of known size, to scale "
                          "hardware traffic counters)")
@@ -152,6 +153,7 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None):
         "served_solves_per_s": cap, "served_rhs_per_s": st["rhs_per_s"], "served_wall_s": st["wall_s"],
         "served_cache_hits": st["cache_hits"], "served_failed": st["failed"],
         "served_fp64_fallbacks": int(sum(r["fp64_fallback"] for r in sat)),
+        "served_pivot_fallbacks": int(st.get("pivot_fallbacks", 0)),
         "served_peak_gb": st["peak_bytes"] / 2**30,
         "plain_fp64_solves_per_s": len(reqs) / plain_wall, "plain_fp64_wall_s": plain_wall,
         "plain_fp64_p50_ms": float(np.percentile(pl, 50)), "plain_fp64_p99_ms": float(np.percentile(pl, 99)),
@@ -202,7 +204,7 @@ def main():
         Ca.close()
         Cb.close()
     # before the timed region; the FP32 path may pick the bf16-split update
-    tuning = None if a.no_tune else ctx.autotune_lu(n, hip.F32, try_split_update=not a.fp32_update_only)
+    tuning = None if a.no_tune else ctx.autotune_lu(n, hip.F32, for_refinement=not a.pivoted_fp32_only)
     mesh = hip.Mesh(ctx, xyz, tri, rwg)
     Z = hip.Mat(ctx, n, n, hip.F32)
     B = hip.Mat(ctx, n, nrhs, hip.F64)
@@ -231,7 +233,7 @@ def main():
     dt = (time.perf_counter() - t0) / a.steps
     fams = {}
     for f in ["fill_pairs", "fill_gather", "rhs", "lu_panel", "lu_laswp", "lu_trsm", "lu_gemm", "lu_split",
-              "lu_gemm_split", "solve_trsm", "solve_gemm", "refine_gemm", "rcs"]:
+              "lu_gemm_split", "lu_dinv", "solve", "solve_trsm", "solve_gemm", "refine_gemm", "rcs"]:
         ms, cnt, work = ctx.prof_get(f)
         fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
     ctx.prof(False)

@@ -56,7 +56,8 @@ struct emfa_ctx {
     int num_cu = 0;
     int cu_share = 0;  // cap on the cooperative panel's workgroups (0: num_cu)
     // LU parameters (0 / -1: built-in choice); set by emfa_lu_set_params or the autotuner
-    int lu_nb = 0, panel_g = 0, panel_x = -1;
+    int lu_nb = 0, panel_g = 0, panel_x = -1;  // panel_x 2: FP32 panels without interchanges
+    int force_pivoting = 0;                    // set while a grown pivot-free factor is redone
     int lu_update_mode = 0;  // FP32 LU trailing update: 0 FP32 MFMA, 1 three-term bf16 split
     size_t bytes = 0;
     bool prof = false;
@@ -66,7 +67,7 @@ struct emfa_ctx {
     // device scratch owned by the context (grown on demand)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
-    int *flags = nullptr;  // [0] spin-timeout flag, [1] singular info
+    int *flags = nullptr;  // [0] spin-timeout flag, [1] singular info, [2] pivot-free growth
 };
 
 struct emfa_mesh {
@@ -102,6 +103,10 @@ struct emfa_lu {
     int *ipiv = nullptr;    // device, 0-based absolute row of each pivot
     int *perm = nullptr;    // device: row i of P A is row perm[i] of A (the interchanges composed)
     int info = 0;
+    int interchanges = 1;   // 0: factored without row interchanges
+    int pivot_fallback = 0; // 1: a pivot-free attempt grew and was redone with interchanges
+    void *dinv = nullptr;   // device: inverses of the 256 x 256 diagonal blocks of L, then of U
+    size_t dinv_bytes = 0;
 };
 
 namespace emfa {
@@ -143,8 +148,10 @@ __host__ __device__ inline cx<R> cmul(cx<R> a, cx<R> b) {
 }
 
 // LU driver entry (lu.hip)
-emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info);
-emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, const int *perm, emfa_mat *B);
+emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info, int *interchanges);
+// bound on the multipliers of a factorisation without interchanges
+constexpr double kNoPivGrowth = 100.0;
+emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B);
 emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, emfa_mat *C, int negate,
                           int accumulate);
 emfa_status convert_copy(emfa_ctx *ctx, const emfa_mat *src, emfa_mat *dst);

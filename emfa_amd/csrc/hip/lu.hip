@@ -1037,6 +1037,281 @@ static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx
     return EMFA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Inverses of the 256 x 256 diagonal blocks of a factor, built once with the
+// factor, turn each diagonal solve of a triangular solve into a product:
+// a solve is then two launches per 256-row block (k_dinv_apply and the MFMA
+// update of the remaining rows).
+// k_tri_inv: one workgroup per block.  The block is handled as 4 x 4 tiles of
+// 64: each diagonal tile is inverted in LDS (one lane per column, unit
+// diagonal for L), the off-diagonal tiles of the inverse follow from
+//   L: X_ij = -D_i sum_{k=j}^{i-1} T_ik X_kj   (i > j, i ascending)
+//   U: X_ij = -D_i sum_{k=i+1}^{j} T_ik X_kj   (i < j, i descending)
+// with 64 x 64 tile products in LDS; the other triangle is zeroed.  Rows and
+// columns past the matrix are treated as identity.  Tiles of X are written
+// to the output and read back by the same workgroup (fenced).
+// ---------------------------------------------------------------------------
+constexpr int kDinvB = 256;
+
+template <typename R>
+struct TileAcc {  // 4 x 4 complex outputs per thread of a 64 x 64 tile
+    cx<R> v[4][4];
+};
+
+template <typename R>
+__device__ __forceinline__ void tile_mac(TileAcc<R> &acc, const cx<R> (*Ta)[65], const cx<R> (*Tb)[65], int ty,
+                                         int tx) {
+    for (int k = 0; k < 64; ++k) {
+        cx<R> a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = Ta[ty * 4 + i][k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Tb[k][tx + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc.v[i][j].re += a[i].re * b[j].re - a[i].im * b[j].im;
+                acc.v[i][j].im += a[i].re * b[j].im + a[i].im * b[j].re;
+            }
+    }
+}
+
+template <typename R, bool UPPER>
+__global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, long long ldf, int N,
+                                                 cx<R> *__restrict__ Dinv) {
+    __shared__ cx<R> Ta[64][65], Tb[64][65];
+    const int blk = blockIdx.x, r0 = blk * kDinvB, tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    cx<R> *X = Dinv + (size_t)blk * kDinvB * kDinvB;  // row-major 256 x 256
+    // tile (i, k) of the block's triangle; identity past the matrix
+    auto load_T = [&](cx<R>(*T)[65], int i, int k) {
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int rr = e >> 6, cc = e & 63;
+            const int gr = r0 + 64 * i + rr, gc = r0 + 64 * k + cc;
+            cx<R> v = {R(0), R(0)};
+            if (gr < N && gc < N)
+                v = F[(size_t)gr * ldf + gc];
+            else if (gr == gc)
+                v = {R(1), R(0)};
+            T[rr][cc] = v;
+        }
+    };
+    auto load_X = [&](cx<R>(*T)[65], int i, int j) {
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int rr = e >> 6, cc = e & 63;
+            T[rr][cc] = X[(size_t)(64 * i + rr) * kDinvB + 64 * j + cc];
+        }
+    };
+    auto store_X = [&](int i, int j, const TileAcc<R> &acc, R sign) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                X[(size_t)(64 * i + ty * 4 + a) * kDinvB + 64 * j + tx + 16 * b] = {sign * acc.v[a][b].re,
+                                                                                     sign * acc.v[a][b].im};
+    };
+    auto zero_acc = [](TileAcc<R> &acc) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc.v[a][b] = {R(0), R(0)};
+    };
+    auto publish = [&]() {
+        __threadfence_block();
+        __syncthreads();
+    };
+    for (int step = 0; step < 4; ++step) {
+        const int i = UPPER ? 3 - step : step;
+        // diagonal tile: Ta = T_ii, Tb = its inverse (lane t solves column t)
+        load_T(Ta, i, i);
+        for (int e = tid; e < 64 * 64; e += 256) Tb[e >> 6][e & 63] = {R(0), R(0)};
+        __syncthreads();
+        if (tid < 64) {
+            const int t = tid;
+            for (int q = 0; q < 64; ++q) {
+                const int r = UPPER ? 63 - q : q;
+                cx<R> acc = {r == t ? R(1) : R(0), R(0)};
+                const int kb = UPPER ? r + 1 : 0, ke = UPPER ? 64 : r;
+                for (int k = kb; k < ke; ++k) {
+                    const cx<R> t_rk = Ta[r][k], x = Tb[k][t];
+                    acc.re -= t_rk.re * x.re - t_rk.im * x.im;
+                    acc.im -= t_rk.re * x.im + t_rk.im * x.re;
+                }
+                if (UPPER) {
+                    const cx<R> d = Ta[r][r];
+                    const R den = d.re * d.re + d.im * d.im;
+                    acc = den > R(0) ? cmul(acc, cx<R>{d.re / den, -d.im / den}) : cx<R>{R(0), R(0)};
+                }
+                Tb[r][t] = acc;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < 64 * 64; e += 256) X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * i + (e & 63)] = Tb[e >> 6][e & 63];
+        // the other triangle of row tile i is zero
+        for (int j = 0; j < 4; ++j)
+            if (UPPER ? j < i : j > i)
+                for (int e = tid; e < 64 * 64; e += 256)
+                    X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * j + (e & 63)] = {R(0), R(0)};
+        publish();
+        // off-diagonal tiles of row tile i
+        const int jb = UPPER ? i + 1 : 0, je = UPPER ? 4 : i;
+        for (int j = jb; j < je; ++j) {
+            TileAcc<R> acc;
+            zero_acc(acc);
+            const int kb = UPPER ? i + 1 : j, ke = UPPER ? j + 1 : i;
+            for (int k = kb; k < ke; ++k) {
+                __syncthreads();
+                load_T(Ta, i, k);
+                load_X(Tb, k, j);
+                __syncthreads();
+                tile_mac(acc, Ta, Tb, ty, tx);
+            }
+            __syncthreads();
+            // X_ij = -D_i * acc
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) Tb[ty * 4 + a][tx + 16 * b] = acc.v[a][b];
+            load_X(Ta, i, i);
+            __syncthreads();
+            TileAcc<R> out;
+            zero_acc(out);
+            tile_mac(out, Ta, Tb, ty, tx);
+            store_X(i, j, out, R(-1));
+            publish();
+        }
+    }
+}
+
+// Y_b = Dinv_b W_b for one 256-row block (nb valid rows, m columns): 32 x 32
+// outputs per workgroup, K in chunks of 64 through LDS.  Writes Y (plain) and
+// the block's real-expanded transposed form Bt (2m x 2nb reals, the operand
+// layout of the MFMA update).
+template <typename R>
+__global__ __launch_bounds__(256) void k_dinv_apply(const cx<R> *__restrict__ Dinv, int nb,
+                                                    const cx<R> *__restrict__ W, long long ldw, int m,
+                                                    cx<R> *__restrict__ Y, long long ldy, R *__restrict__ Bt) {
+    __shared__ cx<R> Ds[32][65], Ws[64][33];
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;  // column tx, rows ty * 4 + a
+    const int rb = blockIdx.x * 32, cb = blockIdx.y * 32;
+    cx<R> acc[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc[a] = {R(0), R(0)};
+    for (int k0 = 0; k0 < nb; k0 += 64) {
+        for (int e = tid; e < 32 * 64; e += 256) {
+            const int rr = e >> 6, kk = e & 63;
+            Ds[rr][kk] = (rb + rr < nb && k0 + kk < nb) ? Dinv[(size_t)(rb + rr) * kDinvB + k0 + kk]
+                                                          : cx<R>{R(0), R(0)};
+        }
+        for (int e = tid; e < 64 * 32; e += 256) {
+            const int kk = e >> 5, cc = e & 31;
+            Ws[kk][cc] = (k0 + kk < nb && cb + cc < m) ? W[(size_t)(k0 + kk) * ldw + cb + cc] : cx<R>{R(0), R(0)};
+        }
+        __syncthreads();
+        for (int kk = 0; kk < 64; ++kk) {
+            const cx<R> w = Ws[kk][tx];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const cx<R> d = Ds[ty * 4 + a][kk];
+                acc[a].re += d.re * w.re - d.im * w.im;
+                acc[a].im += d.re * w.im + d.im * w.re;
+            }
+        }
+        __syncthreads();
+    }
+    const int j = cb + tx;
+    if (j >= m) return;
+    const long long ldbt = 2LL * nb;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int r = rb + ty * 4 + a;
+        if (r >= nb) continue;
+        const cx<R> x = acc[a];
+        Y[(size_t)r * ldy + j] = x;
+        if (!Bt) continue;
+        R *b0 = Bt + (2LL * j) * ldbt + 2 * r;
+        R *b1 = Bt + (2LL * j + 1) * ldbt + 2 * r;
+        b0[0] = x.re;
+        b0[1] = -x.im;
+        b1[0] = x.im;
+        b1[1] = x.re;
+    }
+}
+
+// Skinny products for few right-hand sides (m <= 64):
+//   C (M x m, ldc) = beta C + sign A (M x K, lda) X (K x m, ldx)
+// with A complex TA and X, C complex TC, accumulated in TC.  32 x 32 outputs
+// per workgroup (lane tx: column, rows ty * 4 + a), K in chunks of 64 through
+// LDS.  Every row block is its own workgroup, so an M x K operand with a
+// handful of columns spreads over the chip instead of sitting in a few MFMA
+// tiles with long K loops.  With P given, slice z = blockIdx.z covers K range
+// [z kc, (z + 1) kc) and writes its partial product to plane z of P (M x m
+// each), summed afterwards in slice order.
+template <typename TA, typename TC>
+__global__ __launch_bounds__(256) void k_skinny(const cx<TA> *__restrict__ A, long long lda,
+                                                const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C,
+                                                long long ldc, int M, int K, int m, int kc, TC sign, int beta,
+                                                cx<TC> *__restrict__ P) {
+    __shared__ cx<TA> As[32][65];
+    __shared__ cx<TC> Xs[64][33];
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+    const int rb = blockIdx.x * 32, cb = blockIdx.y * 32, z = blockIdx.z;
+    const int kb = z * kc, ke = min(K, kb + kc);
+    cx<TC> acc[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc[a] = {TC(0), TC(0)};
+    for (int k0 = kb; k0 < ke; k0 += 64) {
+        for (int e = tid; e < 32 * 64; e += 256) {
+            const int rr = e >> 6, kk = e & 63;
+            As[rr][kk] = (rb + rr < M && k0 + kk < ke) ? A[(size_t)(rb + rr) * lda + k0 + kk] : cx<TA>{TA(0), TA(0)};
+        }
+        for (int e = tid; e < 64 * 32; e += 256) {
+            const int kk = e >> 5, cc = e & 31;
+            Xs[kk][cc] = (k0 + kk < ke && cb + cc < m) ? X[(size_t)(k0 + kk) * ldx + cb + cc] : cx<TC>{TC(0), TC(0)};
+        }
+        __syncthreads();
+        for (int kk = 0; kk < 64; ++kk) {
+            const cx<TC> x = Xs[kk][tx];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const cx<TA> v = As[ty * 4 + a][kk];
+                const TC vr = (TC)v.re, vi = (TC)v.im;
+                acc[a].re += vr * x.re - vi * x.im;
+                acc[a].im += vr * x.im + vi * x.re;
+            }
+        }
+        __syncthreads();
+    }
+    const int j = cb + tx;
+    if (j >= m) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int r = rb + ty * 4 + a;
+        if (r >= M) continue;
+        if (P) {
+            P[(size_t)z * M * m + (size_t)r * m + j] = acc[a];
+        } else {
+            cx<TC> c = beta ? C[(size_t)r * ldc + j] : cx<TC>{TC(0), TC(0)};
+            c.re += sign * acc[a].re;
+            c.im += sign * acc[a].im;
+            C[(size_t)r * ldc + j] = c;
+        }
+    }
+}
+
+constexpr int kSkinnyMaxCols = 64;
+
+// W = P B: row i of W is row perm[i] of B
+template <typename R>
+__global__ __launch_bounds__(256) void k_gather_rows(const cx<R> *__restrict__ B, const int *__restrict__ perm, int N,
+                                                     int m, cx<R> *__restrict__ W) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)N * m) return;
+    const int i = (int)(e / m), c = (int)(e % m);
+    W[e] = B[(size_t)perm[i] * m + c];
+}
+
 // Workspace of the row interchanges: pair lists and a staging buffer of
 // 2*kPermChunk rows x max_cols.
 constexpr int kPermChunk = 256;
@@ -1055,6 +1330,141 @@ struct PermWs {
         stage = (cx<R> *)(q + 8192);
     }
 };
+
+// ---------------------------------------------------------------------------
+// Panel without row interchanges (FP32 factorisations that are refined in
+// FP64): the w x w diagonal block is factored in one workgroup, then every
+// row below solves x U11 = a on its own (L21 = A21 U11^-1), so a panel needs
+// no cross-workgroup exchange at all.  Both kernels raise flags[2] when a
+// pivot is zero or a multiplier exceeds sqrt(tau2) in modulus (partial
+// pivoting keeps them <= 1); the factorisation is then redone with
+// interchanges.
+// ---------------------------------------------------------------------------
+constexpr int kNoPivW = 64;  // widest pivot-free panel
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_diag_lu(cx<R> *__restrict__ A, long long lda, int js, int w,
+                                                 int *__restrict__ ipiv, int *flags, R tau2) {
+    __shared__ cx<R> a[kNoPivW][kNoPivW + 1];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < w * w; e += 256) {
+        const int i = e / w, j = e - i * w;
+        a[i][j] = A[(size_t)(js + i) * lda + js + j];
+    }
+    __syncthreads();
+    for (int c = 0; c < w; ++c) {
+        const cx<R> piv = a[c][c];
+        const R den = piv.re * piv.re + piv.im * piv.im;
+        const cx<R> inv = den > R(0) ? cx<R>{piv.re / den, -piv.im / den} : cx<R>{R(0), R(0)};
+        if (tid == 0 && !(den > R(0))) bad = 1;
+        for (int i = c + 1 + tid; i < w; i += 256) {
+            const cx<R> l = cmul(a[i][c], inv);
+            a[i][c] = l;
+            if (l.re * l.re + l.im * l.im > tau2) bad = 1;
+        }
+        __syncthreads();
+        const int m = w - c - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int i = c + 1 + e / m, j = c + 1 + e % m;
+            const cx<R> t = cmul(a[i][c], a[c][j]);
+            a[i][j].re -= t.re;
+            a[i][j].im -= t.im;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < w * w; e += 256) {
+        const int i = e / w, j = e - i * w;
+        A[(size_t)(js + i) * lda + js + j] = a[i][j];
+    }
+    if (tid < w) ipiv[js + tid] = js + tid;
+    if (tid == 0 && bad) flags[2] = 1;
+}
+
+// Rows r0..r1-1 of the panel: x U11 = a with U11 the upper triangle (and
+// diagonal) of the factored diagonal block.  One wave per 64 rows, one lane
+// per row; the rows sit in LDS transposed (x_k of lane t at k * 64 + t) so the
+// per-lane reads are bank-conflict free and the U11 reads are broadcasts.
+// Columns go in blocks of 8: the contributions of the solved columns to a
+// block are eight independent accumulations, then the 8 x 8 triangle of the
+// block is solved in registers.
+template <typename R>
+__global__ __launch_bounds__(64) void k_l21(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
+                                            int *flags, R tau2) {
+    __shared__ __align__(16) cx<R> U[kNoPivW * kNoPivW];  // U11[k][c] for k <= c < w, zero elsewhere
+    __shared__ cx<R> dinv[kNoPivW];
+    __shared__ R xre[kNoPivW * 64], xim[kNoPivW * 64];
+    const int lane = threadIdx.x;
+    const int base = r0 + blockIdx.x * 64, nrow = min(64, r1 - base);
+    for (int e = lane; e < kNoPivW * kNoPivW; e += 64) {
+        const int k = e / kNoPivW, c = e % kNoPivW;
+        U[e] = (k <= c && c < w) ? A[(size_t)(js + k) * lda + js + c] : cx<R>{R(0), R(0)};
+    }
+    {
+        cx<R> d = {R(0), R(0)};
+        if (lane < w) {
+            const cx<R> u = A[(size_t)(js + lane) * lda + js + lane];
+            const R den = u.re * u.re + u.im * u.im;
+            if (den > R(0)) d = {u.re / den, -u.im / den};
+        }
+        dinv[lane] = d;
+    }
+    for (int e = lane; e < nrow * w; e += 64) {
+        const int r = e / w, c = e - r * w;
+        const cx<R> v = A[(size_t)(base + r) * lda + js + c];
+        xre[c * 64 + r] = v.re;
+        xim[c * 64 + r] = v.im;
+    }
+    __syncthreads();
+    bool bad = false;
+    if (lane < nrow) {
+        for (int b0 = 0; b0 < w; b0 += 8) {
+            R ar[8], ai[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c = min(b0 + j, kNoPivW - 1);
+                ar[j] = b0 + j < w ? xre[c * 64 + lane] : R(0);
+                ai[j] = b0 + j < w ? xim[c * 64 + lane] : R(0);
+            }
+            for (int k = 0; k < b0; ++k) {
+                const R xr = xre[k * 64 + lane], xi = xim[k * 64 + lane];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const cx<R> u = U[k * kNoPivW + b0 + j];
+                    ar[j] -= xr * u.re - xi * u.im;
+                    ai[j] -= xr * u.im + xi * u.re;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+#pragma unroll
+                for (int i = 0; i < j; ++i) {
+                    const cx<R> u = U[(b0 + i) * kNoPivW + b0 + j];
+                    ar[j] -= ar[i] * u.re - ai[i] * u.im;
+                    ai[j] -= ar[i] * u.im + ai[i] * u.re;
+                }
+                const cx<R> d = dinv[b0 + j];
+                const R lr = ar[j] * d.re - ai[j] * d.im, li = ar[j] * d.im + ai[j] * d.re;
+                ar[j] = lr;
+                ai[j] = li;
+                if (b0 + j < w && lr * lr + li * li > tau2) bad = true;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (b0 + j < w) {
+                    xre[(b0 + j) * 64 + lane] = ar[j];
+                    xim[(b0 + j) * 64 + lane] = ai[j];
+                }
+        }
+    }
+    __syncthreads();
+    for (int e = lane; e < nrow * w; e += 64) {
+        const int r = e / w, c = e - r * w;
+        A[(size_t)(base + r) * lda + js + c] = cx<R>{xre[c * 64 + r], xim[c * 64 + r]};
+    }
+    if (bad) flags[2] = 1;
+}
 
 // Apply the interchanges k0..k1-1 to columns [c0,c1) and [c2,c3).
 template <typename R>
@@ -1077,7 +1487,7 @@ static emfa_status laswp_launch(emfa_ctx *ctx, cx<R> *A, long long lda, const in
 }
 
 template <typename R>
-static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out) {
+static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out, int *interchanges) {
     const int N = Am->rows;
     cx<R> *A = (cx<R> *)Am->d;
     const long long lda = N;
@@ -1157,11 +1567,39 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     EMFA_REQUIRE(G_max <= kPanelMaxG, EMFA_E_INVALID, "panel grid %d exceeds %d", G_max, kPanelMaxG);
     if (sizeof(R) == 4 && N >= 0xFFFFF) xchg = false;  // the FP32 packet holds a 20-bit row index
     if (const char *s = getenv("EMFA_LU_LOOKAHEAD")) lookahead = atoi(s) != 0;
+    // FP32 factorisations may run without interchanges (panel variant 2)
+    bool nopiv = false;
+    if constexpr (sizeof(R) == 4) {
+        nopiv = ctx->panel_x == 2 && !ctx->force_pivoting;
+        if (const char *s = getenv("EMFA_LU_NOPIV")) nopiv = atoi(s) != 0 && !ctx->force_pivoting;
+    }
+    if (interchanges) *interchanges = nopiv ? 0 : 1;
+    const R tau2 = R(kNoPivGrowth) * R(kNoPivGrowth);
 
     // Factor the outer panel [j0, j0+jb): inner panels of width w with the
     // in-panel row interchanges, triangular solves and updates (main stream).
     auto factor_outer = [&](int j0, int jb) -> emfa_status {
         int js = j0;
+        while (nopiv && js < j0 + jb) {
+            const int w = std::min(std::min(plan.w, kNoPivW), j0 + jb - js);
+            if constexpr (sizeof(R) == 4) {
+                ProfScope ps(ctx, "lu_panel", 0);
+                k_diag_lu<R><<<1, 256, 0, ctx->stream>>>(A, lda, js, w, ipiv, ctx->flags, tau2);
+                if (N - js - w > 0)
+                    k_l21<R><<<(N - js - w + 63) / 64, 64, 0, ctx->stream>>>(A, lda, js, w, js + w, N, ctx->flags,
+                                                                             tau2);
+                EMFA_HIP_CHECK(hipGetLastError());
+            }
+            if (js + w < j0 + jb) {
+                emfa_status s2 = trsm_blocked<R>(ctx, A, lda, A, lda, js, w, js + w, j0 + jb, 0, Bt_in, "lu_trsm");
+                if (s2) return s2;
+                s2 = gemm_update<R>(ctx, "lu_gemm", N - js - w, j0 + jb - js - w, w,
+                                    A + (size_t)(js + w) * lda + js, lda, Bt_in, 2LL * w,
+                                    A + (size_t)(js + w) * lda + js + w, lda);
+                if (s2) return s2;
+            }
+            js += w;
+        }
         while (js < j0 + jb) {
             const int M = N - js;
             int w = std::min(plan.w, j0 + jb - js);
@@ -1219,7 +1657,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     };
     // Apply the outer panel's interchanges outside it and solve for its U rows.
     auto outer_swap_trsm = [&](int j0, int jb) -> emfa_status {
-        emfa_status s2 = laswp_launch<R>(ctx, A, lda, ipiv, j0, j0 + jb, 0, j0, j0 + jb, N, pws);
+        emfa_status s2 = nopiv ? EMFA_OK : laswp_launch<R>(ctx, A, lda, ipiv, j0, j0 + jb, 0, j0, j0 + jb, N, pws);
         if (s2) return s2;
         if (j0 + jb < N) return trsm_blocked<R>(ctx, A, lda, A, lda, j0, jb, j0 + jb, N, 0, Bt_out, "lu_trsm");
         return EMFA_OK;
@@ -1261,7 +1699,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         }
         if ((st = outer_swap_trsm(next, nbn))) return st;
     }
-    int flags[2] = {0, 0};
+    int flags[3] = {0, 0, 0};
     EMFA_HIP_CHECK(hipMemcpyAsync(flags, ctx->flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
     EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (flags[0] != 0) {
@@ -1269,21 +1707,80 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         return EMFA_E_HIP;
     }
     *info_out = flags[1];
+    if (nopiv && flags[2] != 0) {
+        set_error("LU without interchanges: zero pivot or multiplier above %g", kNoPivGrowth);
+        return EMFA_E_NOCONV;
+    }
     return EMFA_OK;
 }
 
-emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info) {
-    if (A->prec == EMFA_F32) return lu_impl<float>(ctx, A, ipiv, info);
-    return lu_impl<double>(ctx, A, ipiv, info);
+emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info, int *interchanges) {
+    if (A->prec == EMFA_F32) return lu_impl<float>(ctx, A, ipiv, info, interchanges);
+    return lu_impl<double>(ctx, A, ipiv, info, interchanges);
 }
 
 // Solve with the factor (same precision as B).
 template <typename R>
-static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipiv, const int *perm, emfa_mat *Bm) {
+static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipiv, const int *perm, emfa_mat *Bm,
+                              const void *dinv, int interchanges) {
     const int N = LUm->rows, m = Bm->cols;
     const cx<R> *F = (const cx<R> *)LUm->d;
     cx<R> *B = (cx<R> *)Bm->d;
     const long long ldf = N, ldb = m;
+    if (dinv && !getenv("EMFA_SOLVE_BLOCKED")) {
+        // two launches per 256-row block with the diagonal-block inverses:
+        // forward W -> Y (W = P B, or B itself without interchanges), backward Y -> B
+        const int nblk = (N + kDinvB - 1) / kDinvB;
+        const cx<R> *DL = (const cx<R> *)dinv, *DU = DL + (size_t)nblk * kDinvB * kDinvB;
+        const size_t vec_b = (((size_t)N * m * sizeof(cx<R>)) + 255) & ~(size_t)255;
+        const size_t bt_b = (((size_t)2 * m * 2 * kDinvB * sizeof(R)) + 255) & ~(size_t)255;
+        void *scr = nullptr;
+        emfa_status st = ctx_scratch(ctx, 2 * vec_b + bt_b, &scr);
+        if (st) return st;
+        cx<R> *Y = (cx<R> *)scr;
+        cx<R> *W = interchanges ? (cx<R> *)((char *)scr + vec_b) : B;
+        R *Bt = (R *)((char *)scr + 2 * vec_b);
+        ProfScope ps(ctx, "solve", 8.0 * (double)N * N * m);
+        if (interchanges) {
+            const long long tot = (long long)N * m;
+            k_gather_rows<R><<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(B, perm, N, m, W);
+            EMFA_HIP_CHECK(hipGetLastError());
+        }
+        const dim3 g_apply(kDinvB / 32, (m + 31) / 32);
+        int skinny_max = kSkinnyMaxCols;
+        if (const char *e = getenv("EMFA_SKINNY_MAX")) skinny_max = std::min(kSkinnyMaxCols, atoi(e));
+        const bool skinny = m <= skinny_max;
+        // rows [c_r0, c_r0 + M) of C -= F[rows, cols r0 .. r0 + wb) X_b
+        auto update = [&](int M, const cx<R> *Fa, const cx<R> *Xb, cx<R> *C, int wb) -> emfa_status {
+            if (M <= 0) return EMFA_OK;
+            if (!skinny) return gemm_update<R>(ctx, "solve_gemm", M, m, wb, Fa, ldf, Bt, 2LL * wb, C, m);
+            ProfScope ps2(ctx, "solve_gemm", 8.0 * (double)M * m * wb);
+            k_skinny<R, R><<<dim3((M + 31) / 32, (m + 31) / 32, 1), 256, 0, ctx->stream>>>(
+                Fa, ldf, Xb, m, C, m, M, wb, m, wb, R(-1), 1, nullptr);
+            EMFA_HIP_CHECK(hipGetLastError());
+            return EMFA_OK;
+        };
+        for (int b = 0; b < nblk; ++b) {
+            const int r0 = b * kDinvB, wb = std::min(kDinvB, N - r0);
+            k_dinv_apply<R><<<g_apply, 256, 0, ctx->stream>>>(DL + (size_t)b * kDinvB * kDinvB, wb,
+                                                              W + (size_t)r0 * m, m, m, Y + (size_t)r0 * m, m,
+                                                              skinny ? nullptr : Bt);
+            EMFA_HIP_CHECK(hipGetLastError());
+            st = update(N - r0 - wb, F + (size_t)(r0 + wb) * ldf + r0, Y + (size_t)r0 * m, W + (size_t)(r0 + wb) * m,
+                        wb);
+            if (st) return st;
+        }
+        for (int b = nblk - 1; b >= 0; --b) {
+            const int r0 = b * kDinvB, wb = std::min(kDinvB, N - r0);
+            k_dinv_apply<R><<<g_apply, 256, 0, ctx->stream>>>(DU + (size_t)b * kDinvB * kDinvB, wb,
+                                                              Y + (size_t)r0 * m, m, m, B + (size_t)r0 * m, m,
+                                                              skinny ? nullptr : Bt);
+            EMFA_HIP_CHECK(hipGetLastError());
+            st = update(r0, F + r0, B + (size_t)r0 * m, Y, wb);
+            if (st) return st;
+        }
+        return EMFA_OK;
+    }
     // few right-hand sides: one look-back launch per direction and column chunk
     // (opt-in: the coherent tile loads make it slower than the blocked path)
     int lb_max = 0;
@@ -1339,9 +1836,37 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
     return EMFA_OK;
 }
 
-emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_mat *LU, const int *ipiv, const int *perm, emfa_mat *B) {
-    if (LU->prec == EMFA_F32) return solve_impl<float>(ctx, LU, ipiv, perm, B);
-    return solve_impl<double>(ctx, LU, ipiv, perm, B);
+emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B) {
+    if (lu->a->prec == EMFA_F32)
+        return solve_impl<float>(ctx, lu->a, lu->ipiv, lu->perm, B, lu->dinv, lu->interchanges);
+    return solve_impl<double>(ctx, lu->a, lu->ipiv, lu->perm, B, lu->dinv, lu->interchanges);
+}
+
+// Inverses of the diagonal blocks of L (unit) and U, both [nblk][256][256]
+static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
+    const int N = lu->a->rows, nblk = (N + kDinvB - 1) / kDinvB;
+    const size_t e = lu->a->prec == EMFA_F32 ? sizeof(cx<float>) : sizeof(cx<double>);
+    lu->dinv_bytes = 2 * (size_t)nblk * kDinvB * kDinvB * e;
+    if (ctx_alloc(ctx, &lu->dinv, lu->dinv_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        lu->dinv = nullptr;
+        set_error("emfa_lu_factor: diagonal-block inverse allocation failed");
+        return EMFA_E_OOM;
+    }
+    ProfScope ps(ctx, "lu_dinv", 0);
+    if (lu->a->prec == EMFA_F32) {
+        const cx<float> *F = (const cx<float> *)lu->a->d;
+        cx<float> *D = (cx<float> *)lu->dinv;
+        k_tri_inv<float, false><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D);
+        k_tri_inv<float, true><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D + (size_t)nblk * kDinvB * kDinvB);
+    } else {
+        const cx<double> *F = (const cx<double> *)lu->a->d;
+        cx<double> *D = (cx<double> *)lu->dinv;
+        k_tri_inv<double, false><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D);
+        k_tri_inv<double, true><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D + (size_t)nblk * kDinvB * kDinvB);
+    }
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
 }
 
 // C = (acc ? C : 0) + sign * sum_s P_s, slices summed in index order
@@ -1360,6 +1885,42 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     const int N = A->rows, K = A->cols, m = X->cols;
     EMFA_REQUIRE(X->rows == K && C->rows == N && C->cols == m && X->prec == EMFA_F64 && C->prec == EMFA_F64,
                  EMFA_E_INVALID, "gemm_residual: shape/precision mismatch");
+    int skinny_max = kSkinnyMaxCols;
+    if (const char *e = getenv("EMFA_SKINNY_MAX")) skinny_max = std::min(kSkinnyMaxCols, atoi(e));
+    if (m <= skinny_max) {
+        // row blocks x K slices (summed in slice order) on the VALU
+        const long long wgs = (long long)((N + 31) / 32) * ((m + 31) / 32);
+        int S = (int)std::min<long long>(32, std::max<long long>(1, (4LL * ctx->num_cu + wgs - 1) / wgs));
+        if (const char *e = getenv("EMFA_RESID_SPLIT")) S = std::max(1, std::min(64, atoi(e)));
+        const int kc = (int)(((long long)K + S - 1) / S + 63) / 64 * 64;
+        S = (K + kc - 1) / kc;
+        const size_t p_b = S > 1 ? (size_t)S * N * m * sizeof(cx<double>) : 0;
+        void *scr = nullptr;
+        if (p_b) {
+            emfa_status st = ctx_scratch(ctx, p_b + 256, &scr);
+            if (st) return st;
+        }
+        cx<double> *P = (cx<double> *)scr;
+        ProfScope ps(ctx, "refine_gemm", 8.0 * (double)N * m * K);
+        const dim3 grid((N + 31) / 32, (m + 31) / 32, S);
+        const double sign = negate ? -1.0 : 1.0;
+        if (A->prec == EMFA_F32)
+            k_skinny<float, double><<<grid, 256, 0, ctx->stream>>>(
+                (const cx<float> *)A->d, K, (const cx<double> *)X->d, m, (cx<double> *)C->d, m, N, K, m, kc, sign,
+                accumulate, P);
+        else
+            k_skinny<double, double><<<grid, 256, 0, ctx->stream>>>(
+                (const cx<double> *)A->d, K, (const cx<double> *)X->d, m, (cx<double> *)C->d, m, N, K, m, kc, sign,
+                accumulate, P);
+        EMFA_HIP_CHECK(hipGetLastError());
+        if (S > 1) {
+            const long long tot2 = (long long)N * 2 * m;
+            k_sum_slices<<<(unsigned)((tot2 + 255) / 256), 256, 0, ctx->stream>>>((const double *)P, S, tot2,
+                                                                                  (double *)C->d, sign, accumulate);
+            EMFA_HIP_CHECK(hipGetLastError());
+        }
+        return EMFA_OK;
+    }
     size_t bt_b = (((size_t)2 * m * 2 * K * sizeof(double)) + 255) & ~(size_t)255;
     // Few right-hand sides leave most CUs idle: split the inner dimension
     // into S slices (multiples of 64 reals) with partial results summed in a
@@ -1440,7 +2001,15 @@ emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu *
         return EMFA_E_OOM;
     }
     st = convert_copy(ctx, Z, lu->a);
-    if (!st) st = lu_factor_inplace(ctx, lu->a, lu->ipiv, &lu->info);
+    if (!st) st = lu_factor_inplace(ctx, lu->a, lu->ipiv, &lu->info, &lu->interchanges);
+    if (st == EMFA_E_NOCONV && lu->interchanges == 0) {
+        // the factorisation without interchanges grew: redo it with partial pivoting
+        ctx->force_pivoting = 1;
+        st = convert_copy(ctx, Z, lu->a);
+        if (!st) st = lu_factor_inplace(ctx, lu->a, lu->ipiv, &lu->info, &lu->interchanges);
+        ctx->force_pivoting = 0;
+        lu->pivot_fallback = 1;
+    }
     if (!st) {
         // compose the interchanges into one row permutation (used by the solves)
         const int n = Z->rows;
@@ -1467,6 +2036,7 @@ emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu *
             }
         }
     }
+    if (!st && !getenv("EMFA_SOLVE_BLOCKED")) st = build_dinv(ctx, lu);
     if (st) {
         emfa_lu_destroy(lu);
         return st;
@@ -1485,6 +2055,7 @@ emfa_status emfa_lu_destroy(emfa_lu *lu) {
     (void)hipStreamSynchronize(c->stream);
     ctx_free(c, lu->ipiv, sizeof(int) * lu->a->rows);
     if (lu->perm) ctx_free(c, lu->perm, sizeof(int) * lu->a->rows);
+    if (lu->dinv) ctx_free(c, lu->dinv, lu->dinv_bytes);
     emfa_mat_destroy(lu->a);
     delete lu;
     return EMFA_OK;
@@ -1497,13 +2068,13 @@ emfa_status emfa_lu_solve(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B) {
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     emfa_status st = EMFA_OK;
     if (B->prec == lu->a->prec) {
-        st = lu_solve_inplace(ctx, lu->a, lu->ipiv, lu->perm, B);
+        st = lu_solve_inplace(ctx, lu, B);
     } else {
         emfa_mat *T = nullptr;
         st = emfa_mat_create(ctx, B->rows, B->cols, lu->a->prec, &T);
         if (st) return st;
         st = convert_copy(ctx, B, T);
-        if (!st) st = lu_solve_inplace(ctx, lu->a, lu->ipiv, lu->perm, T);
+        if (!st) st = lu_solve_inplace(ctx, lu, T);
         if (!st) st = convert_copy(ctx, T, B);
         (void)hipStreamSynchronize(ctx->stream);
         emfa_mat_destroy(T);
@@ -1529,11 +2100,18 @@ emfa_mat *emfa_lu_matrix(emfa_lu *lu) { return lu ? lu->a : nullptr; }
 int emfa_lu_info(const emfa_lu *lu) { return lu ? lu->info : -1; }
 
 emfa_status emfa_lu_set_params(emfa_ctx *ctx, int nb, int panel_workgroups, int panel_variant) {
-    EMFA_REQUIRE(ctx && nb >= 0 && panel_workgroups >= 0 && panel_variant >= -1 && panel_variant <= 1,
+    EMFA_REQUIRE(ctx && nb >= 0 && panel_workgroups >= 0 && panel_variant >= -1 && panel_variant <= 2,
                  EMFA_E_INVALID, "emfa_lu_set_params: bad argument");
     ctx->lu_nb = nb;
     ctx->panel_g = panel_workgroups;
     ctx->panel_x = panel_variant;
+    return EMFA_OK;
+}
+
+emfa_status emfa_lu_get_pivoting(const emfa_lu *lu, int *interchanges, int *fallback) {
+    EMFA_REQUIRE(lu, EMFA_E_INVALID, "emfa_lu_get_pivoting: null factor");
+    if (interchanges) *interchanges = lu->interchanges;
+    if (fallback) *fallback = lu->pivot_fallback;
     return EMFA_OK;
 }
 
@@ -1543,7 +2121,7 @@ emfa_status emfa_lu_set_update_mode(emfa_ctx *ctx, int mode) {
     return EMFA_OK;
 }
 
-emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int try_split_update, emfa_lu_tuning *out) {
+emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int for_refinement, emfa_lu_tuning *out) {
     EMFA_REQUIRE(ctx && n > 0 && (prec == EMFA_F32 || prec == EMFA_F64), EMFA_E_INVALID,
                  "emfa_lu_autotune: bad argument");
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
@@ -1568,8 +2146,11 @@ emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int try_split_updat
         if (nb <= std::max(256, n))
             for (int x : {1, 0})
                 for (int g : {0, 128}) cands.push_back({nb, x, g, 0});
-    if (try_split_update && prec == EMFA_F32)
+    if (for_refinement && prec == EMFA_F32) {
         for (int g : {0, 128}) cands.push_back({512, 1, g, 1});
+        for (int nb : {256, 512})
+            if (nb <= std::max(256, n)) cands.push_back({nb, 2, 0, 1});
+    }
     hipEvent_t e0, e1;
     EMFA_HIP_CHECK(hipEventCreate(&e0));
     EMFA_HIP_CHECK(hipEventCreate(&e1));

@@ -79,7 +79,7 @@ extern "C" emfa_status emfa_refine(emfa_ctx *ctx, const emfa_mat *Z, const emfa_
     if (!st) st = colnorms(ctx, Bd, dn, bn);
     // initial solve in the factor's precision
     if (!st) st = convert_copy(ctx, Bd, D);
-    if (!st) st = lu_solve_inplace(ctx, lu->a, lu->ipiv, lu->perm, D);
+    if (!st) st = lu_solve_inplace(ctx, lu, D);
     if (!st) st = convert_copy(ctx, D, X);
     double r0 = 0, r = 0;
     int it = 0;
@@ -89,7 +89,7 @@ extern "C" emfa_status emfa_refine(emfa_ctx *ctx, const emfa_mat *Z, const emfa_
     r = r0;
     while (!st && it < maxit && r > tol) {
         st = convert_copy(ctx, R, D);
-        if (!st) st = lu_solve_inplace(ctx, lu->a, lu->ipiv, lu->perm, D);
+        if (!st) st = lu_solve_inplace(ctx, lu, D);
         if (!st) st = axpy_f64(ctx, X, D);
         if (!st) st = convert_copy(ctx, Bd, R);
         if (!st) st = gemm_residual(ctx, Z, X, R, 1, 1);

@@ -195,6 +195,7 @@ int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out) {
     out->solves_per_s = t.solves_per_s;
     out->rhs_per_s = t.rhs_per_s;
     out->peak_bytes = t.peak_bytes;
+    out->pivot_fallbacks = t.pivot_fallbacks;
     return 0;
 }
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <sstream>
 #include <stdexcept>
 
@@ -74,7 +75,22 @@ size_t Server::footprint(int n, int nrhs, int precision, bool factor_cached) {
     const size_t e = precision == EMFA_F32 ? 8 : 16;
     size_t vecs = (size_t)n * nrhs * 16 * 4 + (size_t)n * nrhs * e;  // B, X, R, Bd, D
     if (factor_cached) return vecs;
-    return 2 * (size_t)n * n * e + vecs + (size_t)n * 64 * 16;  // Z + factor + LU workspace share
+    // LU workspace at the default outer block of 256: two expanded U-row
+    // buffers (2n x 512 reals each) and, for FP32, the bf16 operand planes of
+    // the split update (hi and lo, 512 K, L rows + U rows + padding)
+    const size_t nbr = 512, rows = (size_t)n + 255;
+    size_t ws = 2 * (2 * (size_t)n) * nbr * (e / 2) + (1u << 20);
+    if (precision == EMFA_F32) ws += 2 * nbr * 2 * (rows + 2 * rows + 256);
+    // the factor keeps the inverses of its 256 x 256 diagonal blocks (L and U)
+    const size_t dinv = 2 * (((size_t)n + 255) / 256) * 256 * 256 * e;
+    // solve workspace: two n x nrhs vectors and an expanded block
+    ws += 2 * (size_t)n * nrhs * e + 2 * (size_t)nrhs * 512 * e;
+    return 2 * (size_t)n * n * e + dinv + vecs + ws;  // Z + factor + workspace
+}
+
+static bool env_flag(const char *name, bool dflt) {
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) != 0 : dflt;
 }
 
 Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
@@ -89,6 +105,11 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
         }
         // concurrent factorisations split the compute units between them
         if (workers > 1) emfa_ctx_set_cu_share(c, std::max(1, emfa_ctx_cu_count(c) / workers));
+        // FP32 factors are only ever used under FP64 refinement: factor them
+        // without interchanges (redone with pivoting when the multipliers
+        // grow) and with the bf16-split trailing update; FP64 factors pivot
+        if (env_flag("EMFA_SERVE_NOPIV", true)) emfa_lu_set_params(c, 0, 0, 2);
+        if (env_flag("EMFA_SERVE_SPLIT", true)) emfa_lu_set_update_mode(c, 1);
         ctxs_.push_back(c);
     }
     for (int w = 0; w < workers; ++w) threads_.emplace_back(&Server::worker, this, w);
@@ -314,7 +335,16 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
     emfa_fill_params fp{r.formulation, r.quad, k, r.alpha, 0.0, 0.0, 0};
     check(emfa_fill(ctx, nf->mesh, &fp, nf->Z), "fill");
     check(emfa_lu_factor(ctx, nf->Z, prec, &nf->lu), "LU");
-    nf->bytes = 2 * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16);
+    {
+        int inter = 1, fallback = 0;
+        check(emfa_lu_get_pivoting(nf->lu, &inter, &fallback), "LU");
+        if (fallback) {
+            std::lock_guard<std::mutex> l(mu_);
+            ++pivot_fallbacks_;
+        }
+    }
+    nf->bytes = 2 * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16) +
+                2 * (((size_t)n + 255) / 256) * 256 * 256 * (prec == EMFA_F32 ? 8 : 16);  // Z, factor, inverses
     // other workers use the factor on their own streams: finish it first
     check(emfa_ctx_sync(ctx), "sync");
     {
@@ -418,6 +448,7 @@ Stats Server::stats() {
     s.failed = failed_;
     s.cache_hits = hits_;
     s.peak_bytes = peak_;
+    s.pivot_fallbacks = pivot_fallbacks_;
     if (!lat_.empty()) {
         std::vector<double> v = lat_;
         std::sort(v.begin(), v.end());
@@ -444,7 +475,7 @@ Stats Server::stats() {
 void Server::reset_stats() {
     std::lock_guard<std::mutex> l(mu_);
     lat_.clear();
-    failed_ = hits_ = rhs_done_ = 0;
+    failed_ = hits_ = rhs_done_ = pivot_fallbacks_ = 0;
     have_t0_ = false;
     peak_ = reserved_ + cache_bytes_;
 }

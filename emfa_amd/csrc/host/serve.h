@@ -66,6 +66,7 @@ struct Stats {
     double wall_s = 0;  // first arrival to last completion
     double solves_per_s = 0, rhs_per_s = 0;
     size_t peak_bytes = 0;
+    int64_t pivot_fallbacks = 0;  // FP32 factors redone with interchanges
 };
 
 class Server {
@@ -124,7 +125,7 @@ class Server {
     std::set<std::string> building_;  // factors being computed by some worker
     // stats
     std::vector<double> lat_;
-    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0;
+    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0, pivot_fallbacks_ = 0;
     bool have_t0_ = false;
     std::chrono::steady_clock::time_point t0_, t1_;
 };

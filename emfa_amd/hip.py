@@ -25,7 +25,7 @@ EXPORTS = [
     "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
-    "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_autotune",
+    "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
     "emfa_lu_set_update_mode",
 ]
 
@@ -194,10 +194,12 @@ class Context:
         """0: FP32 MFMA trailing update; 1: three-term bf16 split on bf16 MFMA."""
         _check(lib().emfa_lu_set_update_mode(self.h, int(mode)))
 
-    def autotune_lu(self, n, prec, try_split_update=False):
-        """Time the LU candidates at size n, keep the fastest; returns the report."""
+    def autotune_lu(self, n, prec, for_refinement=False):
+        """Time the LU candidates at size n, keep the fastest; returns the report.
+        for_refinement admits the FP32 variants meant for FP64-refined solves
+        (bf16-split update, panels without interchanges)."""
         t = LuTuning()
-        _check(lib().emfa_lu_autotune(self.h, int(n), int(prec), int(bool(try_split_update)), ctypes.byref(t)))
+        _check(lib().emfa_lu_autotune(self.h, int(n), int(prec), int(bool(for_refinement)), ctypes.byref(t)))
         c = t.candidates
         return {"nb": t.nb, "panel_variant": t.panel_variant, "panel_workgroups": t.panel_workgroups,
                 "update": "bf16x3" if t.update_mode == 1 else "fp32", "best_ms": t.best_ms,
@@ -308,6 +310,13 @@ class LU:
         p = np.zeros(self.n, dtype=np.int32)
         _check(lib().emfa_lu_pivots(self.h, _pi(p)))
         return p
+
+    def pivoting(self):
+        """(interchanges, fallback): 1/0 partial pivoting or none; 1 when a
+        factorisation without interchanges grew and was redone with them."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().emfa_lu_get_pivoting(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def factor(self):
         return Mat(self.ctx, self.n, self.n, self.prec, handle=lib().emfa_lu_matrix(self.h), owned=False)

@@ -42,7 +42,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong), ("cache_hits", ctypes.c_longlong),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("solves_per_s", ctypes.c_double),
-                ("rhs_per_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t)]
+                ("rhs_per_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
+                ("pivot_fallbacks", ctypes.c_longlong)]
 
 
 _CORE = None

@@ -72,6 +72,7 @@ typedef struct {
     double wall_s;                          /* first arrival to last completion */
     double solves_per_s, rhs_per_s;
     size_t peak_bytes;                      /* admitted footprint + resident factors */
+    long long pivot_fallbacks;              /* FP32 factors redone with row interchanges */
 } emfa_serve_stats;
 
 /* workers >= 1 contexts on `device`; hbm_budget in bytes; keep_current != 0

@@ -141,16 +141,23 @@ int emfa_lu_info(const emfa_lu *lu);
 
 /* LU parameters of a context: outer block nb (0: 256), panel workgroup cap
  * (0: all CUs or the context's CU share), panel variant (1: packet exchange,
- * 0: arrival counter, -1: built-in choice).  No counterpart in the reference
- * (LAPACK chooses its own blocking). */
+ * 0: arrival counter, -1: built-in choice, 2: FP32 factorisations without
+ * row interchanges -- meant for FP32 + FP64-refinement solves; a zero pivot
+ * or a multiplier above 100 in modulus makes emfa_lu_factor redo the
+ * factorisation with partial pivoting; FP64 factorisations always pivot).
+ * No counterpart in the reference (LAPACK chooses its own blocking). */
 emfa_status emfa_lu_set_params(emfa_ctx *ctx, int nb, int panel_workgroups, int panel_variant);
 /* Trailing-update arithmetic of FP32 factorisations: 0 = FP32 MFMA
- * (V_MFMA_F32_32X32X2_F32, default), 1 = FP32 operands split into two bf16
- * terms on their way to LDS and multiplied on bf16 MFMA with FP32
- * accumulation (hi*hi + hi*lo + lo*hi, about 16 significant bits per
- * product; meant for FP32 + FP64-refinement solves).  No reference
- * counterpart. */
+ * (V_MFMA_F32_32X32X2_F32, default), 1 = both operands split once per outer
+ * step into bf16 planes hi = bf16(x), lo = bf16(x - hi) and multiplied on
+ * bf16 MFMA with FP32 accumulation (hi*hi + hi*lo + lo*hi, about 16
+ * significant bits per product; meant for FP32 + FP64-refinement solves).
+ * No reference counterpart. */
 emfa_status emfa_lu_set_update_mode(emfa_ctx *ctx, int mode);
+/* How a factor was computed: interchanges 1 (partial pivoting) or 0 (none);
+ * fallback 1 when a factorisation without interchanges grew and was redone
+ * with them. */
+emfa_status emfa_lu_get_pivoting(const emfa_lu *lu, int *interchanges, int *fallback);
 typedef struct {
     int nb, panel_variant, panel_workgroups, update_mode; /* chosen */
     float best_ms;                                        /* its factorisation time */
@@ -160,9 +167,10 @@ typedef struct {
 } emfa_lu_tuning;
 /* Autotuner: times the LU of a generated n x n matrix in precision prec for
  * each candidate (nb 256/512 x panel variant x panel workgroup cap all/128;
- * with try_split_update and FP32 also the bf16-split update), keeps the
- * fastest in the context and reports all timings. */
-emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int try_split_update, emfa_lu_tuning *out);
+ * with for_refinement and FP32 also the bf16-split update and the panel
+ * without interchanges), keeps the fastest in the context and reports all
+ * timings. */
+emfa_status emfa_lu_autotune(emfa_ctx *ctx, int n, int prec, int for_refinement, emfa_lu_tuning *out);
 
 /* ---- mixed-precision refinement -------------------------------------- */
 typedef struct {

@@ -208,9 +208,10 @@ def test_autotuner_and_every_lu_parameter_set_keep_getrf_pivots(ctx):
     n = 600
     rep = ctx.autotune_lu(n, hip.F64)
     assert rep["candidates"] and rep["nb"] in (256, 384, 512) and rep["best_ms"] > 0
-    rep32 = ctx.autotune_lu(n, hip.F32, try_split_update=True)
-    assert len(rep32["candidates"]) == 10 and rep32["update"] in ("fp32", "bf16x3")
+    rep32 = ctx.autotune_lu(n, hip.F32, for_refinement=True)
+    assert len(rep32["candidates"]) == 12 and rep32["update"] in ("fp32", "bf16x3")
     ctx.lu_update_mode(0)
+    ctx.lu_params(0, 0, -1)
     rng = np.random.default_rng(77)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     LUo, pivo, _ = O.lu(A)
@@ -267,19 +268,96 @@ def test_bf16x3_update_ragged_sizes(ctx, n, nb):
         lu0 = hip.LU(ctx, Z, hip.F32)
         ctx.lu_update_mode(1)
         lu1 = hip.LU(ctx, Z, hip.F32)
+        ctx.lu_params(nb, 0, 2)  # and without interchanges
+        lu2 = hip.LU(ctx, Z, hip.F32)
     finally:
         ctx.lu_update_mode(0)
         ctx.lu_params(0, 0, -1)
-    assert np.array_equal(lu0.pivots(), lu1.pivots())  # diagonally dominant: no pivoting ties
-    f0, f1 = lu0.factor().numpy(), lu1.factor().numpy()
-    assert np.abs(f1 - f0).max() / np.abs(f0).max() < 1e-5
+    # diagonally dominant: partial pivoting keeps the diagonal, so all three agree
+    assert np.array_equal(lu0.pivots(), np.arange(n)) and np.array_equal(lu1.pivots(), np.arange(n))
+    assert lu2.pivoting() == (0, 0) and np.array_equal(lu2.pivots(), np.arange(n))
+    f0 = lu0.factor().numpy()
+    for lu in (lu1, lu2):
+        assert np.abs(lu.factor().numpy() - f0).max() / np.abs(f0).max() < 1e-5
     Bd = hip.Mat.from_numpy(ctx, B, hip.F64)
     X = hip.Mat(ctx, n, 3, hip.F64)
-    rep, ok = hip.refine(ctx, Z, lu1, Bd, X, tol=1e-12, maxit=12)
-    assert ok and rep.relres_final < 1e-12
-    assert rel(X.numpy(), np.linalg.solve(Z.numpy(), B)) < 1e-10
-    for m in (lu0, lu1, Z, Bd, X):
+    for lu in (lu1, lu2):
+        rep, ok = hip.refine(ctx, Z, lu, Bd, X, tol=1e-12, maxit=12)
+        assert ok and rep.relres_final < 1e-12
+        assert rel(X.numpy(), np.linalg.solve(Z.numpy(), B)) < 1e-10
+    for m in (lu0, lu1, lu2, Z, Bd, X):
         m.close()
+
+
+def test_factor_without_interchanges_on_cfie_and_its_growth_fallback(ctx):
+    _say("no-interchange LU")
+    xyz, tri, rwg = sphere(3, 1.0)
+    k = hip.wavenumber(6e8)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, hip.F32)
+    hip.fill(ctx, M, Z, hip.CFIE, k, 3, alpha=0.5)
+    inc = hip.inc_table([(t, 0, 0) for t in range(0, 181, 60)])
+    B = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    hip.rhs_fill(ctx, M, B, hip.CFIE, k, 0.5, inc)
+    X = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    ctx.lu_params(256, 0, 2)
+    try:
+        lu = hip.LU(ctx, Z, hip.F32)
+        # the CFIE matrix factors without interchanges and refines to FP64 accuracy
+        assert lu.pivoting() == (0, 0)
+        rep, ok = hip.refine(ctx, Z, lu, B, X, tol=1e-12, maxit=12)
+        assert ok and rep.relres_final < 1e-12 and rep.relres_first < 1e-3
+        assert rel(X.numpy(), np.linalg.solve(Z.numpy(), B.numpy())) < 1e-10
+        lu.close()
+        # a tiny leading pivot makes the multipliers of column 0 ~1e4: redone with pivoting
+        n = 700
+        rng = np.random.default_rng(3)
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+        A[0, 0] = 1e-4
+        Zs = hip.Mat.from_numpy(ctx, A, hip.F32)
+        lus = hip.LU(ctx, Zs, hip.F32)
+        assert lus.pivoting() == (1, 1)
+        ctx.lu_params(256, 0, 1)
+        lup = hip.LU(ctx, Zs, hip.F32)
+        assert np.array_equal(lus.pivots(), lup.pivots()) and lus.pivots()[0] != 0
+        assert np.array_equal(lus.factor().numpy(), lup.factor().numpy())
+        # FP64 factorisations keep their interchanges whatever the variant
+        ctx.lu_params(256, 0, 2)
+        lud = hip.LU(ctx, hip.Mat.from_numpy(ctx, A, hip.F64), hip.F64)
+        assert lud.pivoting() == (1, 0)
+        LUo, pivo, _ = O.lu(A)
+        assert np.array_equal(lud.pivots(), pivo)
+        for m in (lus, lup, lud, Zs):
+            m.close()
+    finally:
+        ctx.lu_params(0, 0, -1)
+    for m in (Z, B, X):
+        m.close()
+    M.close()
+
+
+@pytest.mark.parametrize("prec", [hip.F32, hip.F64])
+@pytest.mark.parametrize("n,m", [(700, 1), (1003, 7), (513, 70), (256, 3)])
+def test_inverse_block_solve_matches_blocked_substitution(ctx, monkeypatch, prec, n, m):
+    # the solve through the diagonal-block inverses against the blocked
+    # substitution (no inverses) on the same factor data, pivoted and not
+    _say(f"dinv solve n={n} m={m} prec={prec}")
+    rng = np.random.default_rng(n + m)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    Z = hip.Mat.from_numpy(ctx, A, prec)
+    lu = hip.LU(ctx, Z, prec)
+    d1 = hip.Mat.from_numpy(ctx, B, prec)
+    lu.solve(d1)
+    monkeypatch.setenv("EMFA_SOLVE_BLOCKED", "1")
+    d0 = hip.Mat.from_numpy(ctx, B, prec)
+    lu.solve(d0)
+    # unshifted Gaussian matrices (kappa ~ n) pivot; FP32 solutions carry kappa * 6e-8
+    tol = 2e-3 if prec == hip.F32 else 1e-10
+    assert rel(d1.numpy(), d0.numpy()) < tol
+    assert rel(d1.numpy(), np.linalg.solve(Z.numpy(), B)) < tol
+    for x in (lu, Z, d0, d1):
+        x.close()
 
 
 def test_lu_singular_reports(ctx):
