@@ -1342,126 +1342,170 @@ struct PermWs {
 // ---------------------------------------------------------------------------
 constexpr int kNoPivW = 64;  // widest pivot-free panel
 
+// Diagonal block (w <= 64): lane i of every wave holds row i, wave v holds
+// columns 4 v .. 4 v + 3 in registers (16 waves).  Per column c the wave owning it
+// forms the multipliers l_i = a_ic / u_cc (pivot read with v_readlane) and
+// publishes them through LDS; after one barrier every wave updates its own
+// columns with the pivot row taken from lane c of its own registers.  Then
+// U11^-1 by Gauss-Jordan elimination on the same layout (the owner of column
+// c publishes it, one barrier, row c of the inverse scaled and read back per
+// wave with v_readlane), written to uinv (64 x 64, row-major).  Rows and
+// columns past w act as identity.  The LDS buffers alternate by step parity.
+// reciprocal for the pivot-free FP32 factor (refined in FP64 afterwards):
+// the hardware estimate (1 ulp) instead of a correctly rounded division
 template <typename R>
-__global__ __launch_bounds__(256) void k_diag_lu(cx<R> *__restrict__ A, long long lda, int js, int w,
-                                                 int *__restrict__ ipiv, int *flags, R tau2) {
-    __shared__ cx<R> a[kNoPivW][kNoPivW + 1];
-    __shared__ int bad;
-    const int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int e = tid; e < w * w; e += 256) {
-        const int i = e / w, j = e - i * w;
-        a[i][j] = A[(size_t)(js + i) * lda + js + j];
-    }
-    __syncthreads();
-    for (int c = 0; c < w; ++c) {
-        const cx<R> piv = a[c][c];
-        const R den = piv.re * piv.re + piv.im * piv.im;
-        const cx<R> inv = den > R(0) ? cx<R>{piv.re / den, -piv.im / den} : cx<R>{R(0), R(0)};
-        if (tid == 0 && !(den > R(0))) bad = 1;
-        for (int i = c + 1 + tid; i < w; i += 256) {
-            const cx<R> l = cmul(a[i][c], inv);
-            a[i][c] = l;
-            if (l.re * l.re + l.im * l.im > tau2) bad = 1;
-        }
-        __syncthreads();
-        const int m = w - c - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            const int i = c + 1 + e / m, j = c + 1 + e % m;
-            const cx<R> t = cmul(a[i][c], a[c][j]);
-            a[i][j].re -= t.re;
-            a[i][j].im -= t.im;
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < w * w; e += 256) {
-        const int i = e / w, j = e - i * w;
-        A[(size_t)(js + i) * lda + js + j] = a[i][j];
-    }
-    if (tid < w) ipiv[js + tid] = js + tid;
-    if (tid == 0 && bad) flags[2] = 1;
+__device__ __forceinline__ R fast_recip(R x) {
+    if constexpr (sizeof(R) == 4)
+        return __builtin_amdgcn_rcpf(x);
+    else
+        return R(1) / x;
 }
 
-// Rows r0..r1-1 of the panel: x U11 = a with U11 the upper triangle (and
-// diagonal) of the factored diagonal block.  One wave per 64 rows, one lane
-// per row; the rows sit in LDS transposed (x_k of lane t at k * 64 + t) so the
-// per-lane reads are bank-conflict free and the U11 reads are broadcasts.
-// Columns go in blocks of 8: the contributions of the solved columns to a
-// block are eight independent accumulations, then the 8 x 8 triangle of the
-// block is solved in registers.
 template <typename R>
-__global__ __launch_bounds__(64) void k_l21(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
-                                            int *flags, R tau2) {
-    __shared__ __align__(16) cx<R> U[kNoPivW * kNoPivW];  // U11[k][c] for k <= c < w, zero elsewhere
-    __shared__ cx<R> dinv[kNoPivW];
-    __shared__ R xre[kNoPivW * 64], xim[kNoPivW * 64];
-    const int lane = threadIdx.x;
-    const int base = r0 + blockIdx.x * 64, nrow = min(64, r1 - base);
-    for (int e = lane; e < kNoPivW * kNoPivW; e += 64) {
-        const int k = e / kNoPivW, c = e % kNoPivW;
-        U[e] = (k <= c && c < w) ? A[(size_t)(js + k) * lda + js + c] : cx<R>{R(0), R(0)};
+__device__ __forceinline__ R lane_read(R v, int l) {
+    if constexpr (sizeof(R) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
     }
-    {
-        cx<R> d = {R(0), R(0)};
-        if (lane < w) {
-            const cx<R> u = A[(size_t)(js + lane) * lda + js + lane];
-            const R den = u.re * u.re + u.im * u.im;
-            if (den > R(0)) d = {u.re / den, -u.im / den};
+}
+
+constexpr int kDiagCpw = 4, kDiagWaves = kNoPivW / kDiagCpw;  // columns per wave, waves
+
+template <typename R>
+__global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__ A, long long lda, int js, int w,
+                                                 int *__restrict__ ipiv, int *flags, R tau2, cx<R> *__restrict__ uinv) {
+    __shared__ cx<R> xbuf[2][kNoPivW];
+    __shared__ int bad;
+    // wave index as a scalar: column tests below are uniform per wave
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cx<R> a[kDiagCpw];
+#pragma unroll
+    for (int q = 0; q < kDiagCpw; ++q) {
+        const int j = kDiagCpw * wv + q;
+        a[q] = (lane < w && j < w) ? A[(size_t)(js + lane) * lda + js + j] : cx<R>{lane == j ? R(1) : R(0), R(0)};
+    }
+    if (threadIdx.x == 0) bad = 0;
+    bool mybad = false;
+    __syncthreads();
+    for (int c = 0; c < w; ++c) {
+        const int p = c & 1, cq = c % kDiagCpw;
+        if (wv == c / kDiagCpw) {
+            cx<R> v = a[0];
+#pragma unroll
+            for (int q = 1; q < kDiagCpw; ++q)
+                if (q == cq) v = a[q];
+            const R pr = lane_read(v.re, c), pi = lane_read(v.im, c);
+            const R den = pr * pr + pi * pi;
+            const R rd = den > R(0) ? fast_recip(den) : R(0);
+            if (!(den > R(0))) mybad = true;
+            const bool below = lane > c;
+            cx<R> l = cmul(v, cx<R>{pr * rd, -pi * rd});
+            if (!below) l = {R(0), R(0)};
+            if (l.re * l.re + l.im * l.im > tau2) mybad = true;
+#pragma unroll
+            for (int q = 0; q < kDiagCpw; ++q)
+                if (q == cq && below) a[q] = l;
+            xbuf[p][lane] = l;
         }
-        dinv[lane] = d;
+        __syncthreads();
+        // branch-free update: zero multiplier on rows <= c, zero pivot-row
+        // entry (uniform) on columns <= c
+        const cx<R> l = xbuf[p][lane];
+#pragma unroll
+        for (int q = 0; q < kDiagCpw; ++q) {
+            const R colact = kDiagCpw * wv + q > c ? R(1) : R(0);  // uniform: no branch around the reads
+            const R ur = colact * lane_read(a[q].re, c), ui = colact * lane_read(a[q].im, c);
+            a[q].re -= l.re * ur - l.im * ui;
+            a[q].im -= l.re * ui + l.im * ur;
+        }
     }
-    for (int e = lane; e < nrow * w; e += 64) {
-        const int r = e / w, c = e - r * w;
-        const cx<R> v = A[(size_t)(base + r) * lda + js + c];
-        xre[c * 64 + r] = v.re;
-        xim[c * 64 + r] = v.im;
+#pragma unroll
+    for (int q = 0; q < kDiagCpw; ++q) {
+        const int j = kDiagCpw * wv + q;
+        if (lane < w && j < w) A[(size_t)(js + lane) * lda + js + j] = a[q];
+    }
+    // X = U^-1 (rows by lane, columns by wave as above)
+    cx<R> x[kDiagCpw];
+#pragma unroll
+    for (int q = 0; q < kDiagCpw; ++q) x[q] = {lane == kDiagCpw * wv + q ? R(1) : R(0), R(0)};
+    __syncthreads();
+    for (int c = w - 1; c >= 0; --c) {
+        const int p = c & 1, cq = c % kDiagCpw;
+        if (wv == c / kDiagCpw) {  // column c of U (lanes <= c are used)
+            cx<R> v = a[0];
+#pragma unroll
+            for (int q = 1; q < kDiagCpw; ++q)
+                if (q == cq) v = a[q];
+            xbuf[p][lane] = v;
+        }
+        __syncthreads();
+        const cx<R> d = xbuf[p][c];
+        const cx<R> u = lane < c ? xbuf[p][lane] : cx<R>{R(0), R(0)};  // zero on rows >= c
+        const R den = d.re * d.re + d.im * d.im;
+        const R rd = den > R(0) ? fast_recip(den) : R(0);
+        const cx<R> inv = {d.re * rd, -d.im * rd};
+        // row c of X scaled (read back from lane c), rows < c updated, branch-free
+#pragma unroll
+        for (int q = 0; q < kDiagCpw; ++q) {
+            const R xr0 = lane_read(x[q].re, c), xi0 = lane_read(x[q].im, c);
+            const R xr = xr0 * inv.re - xi0 * inv.im, xi = xr0 * inv.im + xi0 * inv.re;
+            if (lane == c) x[q] = {xr, xi};
+            x[q].re -= u.re * xr - u.im * xi;
+            x[q].im -= u.re * xi + u.im * xr;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kDiagCpw; ++q) uinv[lane * kNoPivW + kDiagCpw * wv + q] = x[q];
+    if (threadIdx.x < w) ipiv[js + threadIdx.x] = js + threadIdx.x;
+    if (mybad) bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) flags[2] = 1;
+}
+
+// Rows r0..r1-1 of the panel: L21 = A21 U11^-1 with the inverse from
+// k_diag_lu.  32 rows x w columns per workgroup (lane tx: column, rows
+// ty * 8 + a), in place: a workgroup reads all of its rows before writing.
+template <typename R>
+__global__ __launch_bounds__(256) void k_l21(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
+                                            const cx<R> *__restrict__ uinv, int *flags, R tau2) {
+    __shared__ cx<R> Us[kNoPivW][kNoPivW + 1], As[32][kNoPivW + 1];
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    const int base = r0 + blockIdx.x * 32, nrow = min(32, r1 - base);
+    for (int e = tid; e < kNoPivW * kNoPivW; e += 256) {
+        const int k = e >> 6, c = e & 63;
+        Us[k][c] = (k < w && c < w) ? uinv[k * kNoPivW + c] : cx<R>{R(0), R(0)};
+    }
+    for (int e = tid; e < 32 * kNoPivW; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        As[r][c] = (r < nrow && c < w) ? A[(size_t)(base + r) * lda + js + c] : cx<R>{R(0), R(0)};
     }
     __syncthreads();
+    cx<R> acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = {R(0), R(0)};
+    for (int k = 0; k < w; ++k) {
+        const cx<R> u = Us[k][tx];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const cx<R> v = As[ty * 8 + q][k];
+            acc[q].re += v.re * u.re - v.im * u.im;
+            acc[q].im += v.re * u.im + v.im * u.re;
+        }
+    }
     bool bad = false;
-    if (lane < nrow) {
-        for (int b0 = 0; b0 < w; b0 += 8) {
-            R ar[8], ai[8];
+    if (tx < w) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int c = min(b0 + j, kNoPivW - 1);
-                ar[j] = b0 + j < w ? xre[c * 64 + lane] : R(0);
-                ai[j] = b0 + j < w ? xim[c * 64 + lane] : R(0);
+        for (int q = 0; q < 8; ++q) {
+            const int r = ty * 8 + q;
+            if (r < nrow) {
+                A[(size_t)(base + r) * lda + js + tx] = acc[q];
+                if (acc[q].re * acc[q].re + acc[q].im * acc[q].im > tau2) bad = true;
             }
-            for (int k = 0; k < b0; ++k) {
-                const R xr = xre[k * 64 + lane], xi = xim[k * 64 + lane];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const cx<R> u = U[k * kNoPivW + b0 + j];
-                    ar[j] -= xr * u.re - xi * u.im;
-                    ai[j] -= xr * u.im + xi * u.re;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-#pragma unroll
-                for (int i = 0; i < j; ++i) {
-                    const cx<R> u = U[(b0 + i) * kNoPivW + b0 + j];
-                    ar[j] -= ar[i] * u.re - ai[i] * u.im;
-                    ai[j] -= ar[i] * u.im + ai[i] * u.re;
-                }
-                const cx<R> d = dinv[b0 + j];
-                const R lr = ar[j] * d.re - ai[j] * d.im, li = ar[j] * d.im + ai[j] * d.re;
-                ar[j] = lr;
-                ai[j] = li;
-                if (b0 + j < w && lr * lr + li * li > tau2) bad = true;
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (b0 + j < w) {
-                    xre[(b0 + j) * 64 + lane] = ar[j];
-                    xim[(b0 + j) * 64 + lane] = ai[j];
-                }
         }
-    }
-    __syncthreads();
-    for (int e = lane; e < nrow * w; e += 64) {
-        const int r = e / w, c = e - r * w;
-        A[(size_t)(base + r) * lda + js + c] = cx<R>{xre[c * 64 + r], xim[c * 64 + r]};
     }
     if (bad) flags[2] = 1;
 }
@@ -1522,8 +1566,9 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         if (const char *e = getenv("EMFA_LU_BF16X3")) split_update = atoi(e) != 0;
     }
     size_t planes_b = split_update ? UpdatePlanes::bytes(N, nb) : 0;
+    const size_t uinv_b = (size_t)kNoPivW * kNoPivW * sizeof(cx<R>);
     size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + 2 * bt_b + perm_b + x_pk_b + x_rw_b +
-                   x_cw_b + x_val_b + planes_b + 8192;
+                   x_cw_b + x_val_b + planes_b + uinv_b + 8192;
     void *scr = nullptr;
     emfa_status st = ctx_scratch(ctx, total, &scr);
     if (st != EMFA_OK) return st;
@@ -1547,6 +1592,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     R *x_val = (R *)take(x_val_b);
     UpdatePlanes planes;
     if (split_update) planes.carve(take(planes_b), N, nb);
+    cx<R> *uinv = (cx<R> *)take(uinv_b);  // U11^-1 of the current pivot-free panel
     {
         const int nw = (int)((x_pk_b + x_rw_b + x_cw_b) / 4);
         k_zero_u32<<<(nw + 255) / 256, 256, 0, ctx->stream>>>((unsigned *)x_pk, nw);
@@ -1584,10 +1630,10 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             const int w = std::min(std::min(plan.w, kNoPivW), j0 + jb - js);
             if constexpr (sizeof(R) == 4) {
                 ProfScope ps(ctx, "lu_panel", 0);
-                k_diag_lu<R><<<1, 256, 0, ctx->stream>>>(A, lda, js, w, ipiv, ctx->flags, tau2);
+                k_diag_lu<R><<<1, kDiagWaves * 64, 0, ctx->stream>>>(A, lda, js, w, ipiv, ctx->flags, tau2, uinv);
                 if (N - js - w > 0)
-                    k_l21<R><<<(N - js - w + 63) / 64, 64, 0, ctx->stream>>>(A, lda, js, w, js + w, N, ctx->flags,
-                                                                             tau2);
+                    k_l21<R><<<(N - js - w + 31) / 32, 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv,
+                                                                               ctx->flags, tau2);
                 EMFA_HIP_CHECK(hipGetLastError());
             }
             if (js + w < j0 + jb) {
