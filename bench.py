@@ -282,6 +282,16 @@ def main():
         Z64.close()
         B64.close()
 
+    # the serving leg runs before the CPU baseline: the baseline's BLAS and
+    # OpenMP threads keep spinning for a while and would take the host cores
+    # that the server's worker threads launch from
+    for m_ in (B, X, mesh):
+        m_.close()
+    mixed = None
+    if not a.no_mixed:
+        from emfa_amd import workload
+        mixed = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers, int(a.budget_gb * 2**30))
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -315,10 +325,8 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
-    if not a.no_mixed:
-        from emfa_amd import workload
-        out["mixed_stream"] = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers,
-                                         int(a.budget_gb * 2**30))
+    if mixed is not None:
+        out["mixed_stream"] = mixed
     print(json.dumps(out))
 
 
