@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "gemm.h"
@@ -1239,68 +1240,117 @@ __global__ __launch_bounds__(256) void k_dinv_apply(const cx<R> *__restrict__ Di
     }
 }
 
-// Skinny products for few right-hand sides (m <= 64):
-//   C (M x m, ldc) = beta C + sign A (M x K, lda) X (K x m, ldx)
-// with A complex TA and X, C complex TC, accumulated in TC.  32 x 32 outputs
-// per workgroup (lane tx: column, rows ty * 4 + a), K in chunks of 64 through
-// LDS.  Every row block is its own workgroup, so an M x K operand with a
-// handful of columns spreads over the chip instead of sitting in a few MFMA
-// tiles with long K loops.  With P given, slice z = blockIdx.z covers K range
-// [z kc, (z + 1) kc) and writes its partial product to plane z of P (M x m
-// each), summed afterwards in slice order.
-template <typename TA, typename TC>
-__global__ __launch_bounds__(256) void k_skinny(const cx<TA> *__restrict__ A, long long lda,
-                                                const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C,
-                                                long long ldc, int M, int K, int m, int kc, TC sign, int beta,
-                                                cx<TC> *__restrict__ P) {
-    __shared__ cx<TA> As[32][65];
-    __shared__ cx<TC> Xs[64][33];
-    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
-    const int rb = blockIdx.x * 32, cb = blockIdx.y * 32, z = blockIdx.z;
+constexpr int kSkinnyMaxCols = 64;
+
+// Small products with at most 64 columns, the form every few-RHS solve step
+// and the refinement residual take:
+//   C (M x m) = beta C + sign A (M x K) X (K x m)      (P == null)
+//   plane z of P = A X over K slice [z kc, (z + 1) kc)  (P given)
+// The column tile MC (8, 16, 32, 64) follows m, so no lane works on padding
+// columns: lane tx owns column tx, rows ty + h RG (h < RPT) of the
+// workgroup's RPT RG rows (RG = 256 / MC).  Two rows per lane: more rows per
+// workgroup cut the re-reads of X but cost more in parallelism (measured).
+// K goes through LDS in chunks of 64 with the next chunk's operands loaded
+// into registers while the current chunk is multiplied.
+template <int MC>
+constexpr int smallmm_rpt() {
+    return 2;
+}
+
+template <typename TA, typename TC, int MC>
+__global__ __launch_bounds__(256) void k_smallmm(const cx<TA> *__restrict__ A, long long lda,
+                                                 const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C,
+                                                 long long ldc, int M, int K, int m, int kc, TC sign, int beta,
+                                                 cx<TC> *__restrict__ P) {
+    constexpr int RPT = smallmm_rpt<MC>(), RG = 256 / MC, RB = RPT * RG, KC = 64;
+    constexpr int AL = RB * KC / 256, XL = KC * MC / 256;  // chunk elements per thread
+    __shared__ cx<TA> As[RB][KC + 1];
+    __shared__ cx<TC> Xs[KC][MC];
+    const int tid = threadIdx.x, tx = tid % MC, ty = tid / MC;
+    const int rb = blockIdx.x * RB, z = blockIdx.y;
     const int kb = z * kc, ke = min(K, kb + kc);
-    cx<TC> acc[4];
+    cx<TA> ra[AL];
+    cx<TC> rx[XL];
+    auto gload = [&](int k0) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) acc[a] = {TC(0), TC(0)};
-    for (int k0 = kb; k0 < ke; k0 += 64) {
-        for (int e = tid; e < 32 * 64; e += 256) {
-            const int rr = e >> 6, kk = e & 63;
-            As[rr][kk] = (rb + rr < M && k0 + kk < ke) ? A[(size_t)(rb + rr) * lda + k0 + kk] : cx<TA>{TA(0), TA(0)};
+        for (int i = 0; i < AL; ++i) {
+            const int e = tid + 256 * i, r = e / KC, k = e % KC;
+            ra[i] = (rb + r < M && k0 + k < ke) ? A[(size_t)(rb + r) * lda + k0 + k] : cx<TA>{TA(0), TA(0)};
         }
-        for (int e = tid; e < 64 * 32; e += 256) {
-            const int kk = e >> 5, cc = e & 31;
-            Xs[kk][cc] = (k0 + kk < ke && cb + cc < m) ? X[(size_t)(k0 + kk) * ldx + cb + cc] : cx<TC>{TC(0), TC(0)};
+#pragma unroll
+        for (int i = 0; i < XL; ++i) {
+            const int e = tid + 256 * i, k = e / MC, c = e % MC;
+            rx[i] = (k0 + k < ke && c < m) ? X[(size_t)(k0 + k) * ldx + c] : cx<TC>{TC(0), TC(0)};
+        }
+    };
+    cx<TC> acc[RPT];
+#pragma unroll
+    for (int h = 0; h < RPT; ++h) acc[h] = {TC(0), TC(0)};
+    if (kb < ke) gload(kb);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < AL; ++i) {
+            const int e = tid + 256 * i;
+            As[e / KC][e % KC] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < XL; ++i) {
+            const int e = tid + 256 * i;
+            Xs[e / MC][e % MC] = rx[i];
         }
         __syncthreads();
-        for (int kk = 0; kk < 64; ++kk) {
-            const cx<TC> x = Xs[kk][tx];
+        if (k0 + KC < ke) gload(k0 + KC);
+#pragma unroll 8
+        for (int k = 0; k < KC; ++k) {
+            const cx<TC> x = Xs[k][tx];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const cx<TA> v = As[ty * 4 + a][kk];
-                const TC vr = (TC)v.re, vi = (TC)v.im;
-                acc[a].re += vr * x.re - vi * x.im;
-                acc[a].im += vr * x.im + vi * x.re;
+            for (int h = 0; h < RPT; ++h) {
+                const cx<TA> a = As[ty + h * RG][k];
+                acc[h].re += (TC)a.re * x.re - (TC)a.im * x.im;
+                acc[h].im += (TC)a.re * x.im + (TC)a.im * x.re;
             }
         }
-        __syncthreads();
     }
-    const int j = cb + tx;
-    if (j >= m) return;
+    if (tx >= m) return;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int r = rb + ty * 4 + a;
+    for (int h = 0; h < RPT; ++h) {
+        const int r = rb + ty + h * RG;
         if (r >= M) continue;
+        const cx<TC> v = acc[h];
         if (P) {
-            P[(size_t)z * M * m + (size_t)r * m + j] = acc[a];
+            P[(size_t)z * M * m + (size_t)r * m + tx] = v;
         } else {
-            cx<TC> c = beta ? C[(size_t)r * ldc + j] : cx<TC>{TC(0), TC(0)};
-            c.re += sign * acc[a].re;
-            c.im += sign * acc[a].im;
-            C[(size_t)r * ldc + j] = c;
+            cx<TC> c = beta ? C[(size_t)r * ldc + tx] : cx<TC>{TC(0), TC(0)};
+            c.re += sign * v.re;
+            c.im += sign * v.im;
+            C[(size_t)r * ldc + tx] = c;
         }
     }
 }
 
-constexpr int kSkinnyMaxCols = 64;
+// launch k_smallmm with the column tile for m (<= 64); slices on gridDim.y
+template <typename TA, typename TC>
+static emfa_status smallmm(hipStream_t s, const cx<TA> *A, long long lda, const cx<TC> *X, long long ldx, cx<TC> *C,
+                           long long ldc, int M, int K, int m, int slices, int kc, TC sign, int beta, cx<TC> *P) {
+    if (M <= 0 || m <= 0) return EMFA_OK;
+    EMFA_REQUIRE(m <= kSkinnyMaxCols && slices >= 1 && kc > 0, EMFA_E_INVALID, "smallmm: bad shape m=%d", m);
+    auto go = [&](auto mc) {
+        constexpr int MC = decltype(mc)::value, RB = smallmm_rpt<MC>() * (256 / MC);
+        const dim3 grid((M + RB - 1) / RB, slices);
+        k_smallmm<TA, TC, MC><<<grid, 256, 0, s>>>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P);
+    };
+    if (m <= 8)
+        go(std::integral_constant<int, 8>{});
+    else if (m <= 16)
+        go(std::integral_constant<int, 16>{});
+    else if (m <= 32)
+        go(std::integral_constant<int, 32>{});
+    else
+        go(std::integral_constant<int, 64>{});
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
 
 // W = P B: row i of W is row perm[i] of B
 template <typename R>
@@ -1801,27 +1851,27 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
             if (M <= 0) return EMFA_OK;
             if (!skinny) return gemm_update<R>(ctx, "solve_gemm", M, m, wb, Fa, ldf, Bt, 2LL * wb, C, m);
             ProfScope ps2(ctx, "solve_gemm", 8.0 * (double)M * m * wb);
-            k_skinny<R, R><<<dim3((M + 31) / 32, (m + 31) / 32, 1), 256, 0, ctx->stream>>>(
-                Fa, ldf, Xb, m, C, m, M, wb, m, wb, R(-1), 1, nullptr);
+            return smallmm<R, R>(ctx->stream, Fa, ldf, Xb, m, C, m, M, wb, m, 1, wb, R(-1), 1, nullptr);
+        };
+        // Y_b = D_b^-1 W_b
+        auto apply = [&](const cx<R> *D, int wb, const cx<R> *Wb, cx<R> *Yb) -> emfa_status {
+            if (skinny) return smallmm<R, R>(ctx->stream, D, kDinvB, Wb, m, Yb, m, wb, wb, m, 1, wb, R(1), 0, nullptr);
+            k_dinv_apply<R><<<g_apply, 256, 0, ctx->stream>>>(D, wb, Wb, m, m, Yb, m, Bt);
             EMFA_HIP_CHECK(hipGetLastError());
             return EMFA_OK;
         };
         for (int b = 0; b < nblk; ++b) {
             const int r0 = b * kDinvB, wb = std::min(kDinvB, N - r0);
-            k_dinv_apply<R><<<g_apply, 256, 0, ctx->stream>>>(DL + (size_t)b * kDinvB * kDinvB, wb,
-                                                              W + (size_t)r0 * m, m, m, Y + (size_t)r0 * m, m,
-                                                              skinny ? nullptr : Bt);
-            EMFA_HIP_CHECK(hipGetLastError());
+            st = apply(DL + (size_t)b * kDinvB * kDinvB, wb, W + (size_t)r0 * m, Y + (size_t)r0 * m);
+            if (st) return st;
             st = update(N - r0 - wb, F + (size_t)(r0 + wb) * ldf + r0, Y + (size_t)r0 * m, W + (size_t)(r0 + wb) * m,
                         wb);
             if (st) return st;
         }
         for (int b = nblk - 1; b >= 0; --b) {
             const int r0 = b * kDinvB, wb = std::min(kDinvB, N - r0);
-            k_dinv_apply<R><<<g_apply, 256, 0, ctx->stream>>>(DU + (size_t)b * kDinvB * kDinvB, wb,
-                                                              Y + (size_t)r0 * m, m, m, B + (size_t)r0 * m, m,
-                                                              skinny ? nullptr : Bt);
-            EMFA_HIP_CHECK(hipGetLastError());
+            st = apply(DU + (size_t)b * kDinvB * kDinvB, wb, Y + (size_t)r0 * m, B + (size_t)r0 * m);
+            if (st) return st;
             st = update(r0, F + r0, B + (size_t)r0 * m, Y, wb);
             if (st) return st;
         }
@@ -1935,7 +1985,9 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     if (const char *e = getenv("EMFA_SKINNY_MAX")) skinny_max = std::min(kSkinnyMaxCols, atoi(e));
     if (m <= skinny_max) {
         // row blocks x K slices (summed in slice order) on the VALU
-        const long long wgs = (long long)((N + 31) / 32) * ((m + 31) / 32);
+        const int mc = m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
+        const int rbk = 2 * (256 / mc);  // rows per workgroup of k_smallmm
+        const long long wgs = (N + rbk - 1) / rbk;
         int S = (int)std::min<long long>(32, std::max<long long>(1, (4LL * ctx->num_cu + wgs - 1) / wgs));
         if (const char *e = getenv("EMFA_RESID_SPLIT")) S = std::max(1, std::min(64, atoi(e)));
         const int kc = (int)(((long long)K + S - 1) / S + 63) / 64 * 64;
@@ -1948,17 +2000,14 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
         }
         cx<double> *P = (cx<double> *)scr;
         ProfScope ps(ctx, "refine_gemm", 8.0 * (double)N * m * K);
-        const dim3 grid((N + 31) / 32, (m + 31) / 32, S);
         const double sign = negate ? -1.0 : 1.0;
-        if (A->prec == EMFA_F32)
-            k_skinny<float, double><<<grid, 256, 0, ctx->stream>>>(
-                (const cx<float> *)A->d, K, (const cx<double> *)X->d, m, (cx<double> *)C->d, m, N, K, m, kc, sign,
-                accumulate, P);
-        else
-            k_skinny<double, double><<<grid, 256, 0, ctx->stream>>>(
-                (const cx<double> *)A->d, K, (const cx<double> *)X->d, m, (cx<double> *)C->d, m, N, K, m, kc, sign,
-                accumulate, P);
-        EMFA_HIP_CHECK(hipGetLastError());
+        emfa_status s2 =
+            A->prec == EMFA_F32
+                ? smallmm<float, double>(ctx->stream, (const cx<float> *)A->d, K, (const cx<double> *)X->d, m,
+                                         (cx<double> *)C->d, m, N, K, m, S, kc, sign, accumulate, P)
+                : smallmm<double, double>(ctx->stream, (const cx<double> *)A->d, K, (const cx<double> *)X->d, m,
+                                          (cx<double> *)C->d, m, N, K, m, S, kc, sign, accumulate, P);
+        if (s2) return s2;
         if (S > 1) {
             const long long tot2 = (long long)N * 2 * m;
             k_sum_slices<<<(unsigned)((tot2 + 255) / 256), 256, 0, ctx->stream>>>((const double *)P, S, tot2,
