@@ -153,6 +153,7 @@ int emfa_serve_wait(emfa_server *s, long long id, emfa_serve_result *res) {
     res->queue_ms = r.queue_ms;
     res->service_ms = r.service_ms;
     res->latency_ms = r.latency_ms;
+    res->batch_size = r.batch_size;
     if (r.status != 0) g_core_err = r.error;
     std::lock_guard<std::mutex> l(s->mu);
     s->taken[r.id] = std::move(r);
@@ -196,6 +197,7 @@ int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out) {
     out->rhs_per_s = t.rhs_per_s;
     out->peak_bytes = t.peak_bytes;
     out->pivot_fallbacks = t.pivot_fallbacks;
+    out->batched = t.batched;
     return 0;
 }
 

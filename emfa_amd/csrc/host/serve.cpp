@@ -110,6 +110,7 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
         // grow) and with the bf16-split trailing update; FP64 factors pivot
         if (env_flag("EMFA_SERVE_NOPIV", true)) emfa_lu_set_params(c, 0, 0, 2);
         if (const char *e = std::getenv("EMFA_SERVE_REFINE_TOL")) refine_tol_ = std::max(1e-14, std::atof(e));
+        if (const char *e = std::getenv("EMFA_SERVE_BATCH_RHS")) batch_max_rhs_ = std::max(0, std::atoi(e));
         if (env_flag("EMFA_SERVE_SPLIT", true)) emfa_lu_set_update_mode(c, 1);
         ctxs_.push_back(c);
     }
@@ -224,56 +225,87 @@ std::shared_ptr<Server::Factor> Server::cache_get(const std::string &key) {
     return it->second;
 }
 
-void Server::worker(int w) {
-    emfa_ctx *ctx = ctxs_[w];
-    for (;;) {
-        Job job;
-        {
-            std::unique_lock<std::mutex> l(mu_);
-            cv_work_.wait(l, [&] { return stop_ || !queue_.empty(); });
-            if (stop_) return;
-            job = std::move(queue_.front());
-            queue_.pop_front();
-            ++busy_;
-        }
-        Result res;
-        res.id = job.id;
-        auto start = clock_t_::now();
-        try {
-            run(ctx, job, res);
-        } catch (const StatusError &e) {
-            res.status = e.code;
-            res.error = e.what();
-        } catch (const std::exception &e) {
-            res.status = EMFA_E_INVALID;
-            res.error = e.what();
-        }
-        auto end = clock_t_::now();
-        res.queue_ms = ms_between(job.arrival, start);
-        res.service_ms = ms_between(start, end);
-        res.latency_ms = ms_between(job.arrival, end);
-        {
-            std::lock_guard<std::mutex> l(mu_);
-            --busy_;
-            t1_ = end;
-            if (res.status == EMFA_OK) {
-                lat_.push_back(res.latency_ms);
-                rhs_done_ += res.nrhs;
-                if (res.cache_hit) ++hits_;
-            } else {
-                ++failed_;
-            }
-            done_[res.id] = std::move(res);
-        }
-        cv_done_.notify_all();
-    }
-}
-
 static std::string factor_key(const Request &r, int prec) {
     std::ostringstream ks;
     ks.precision(17);
     ks << r.mesh.key() << '|' << r.freq << '|' << r.formulation << '|' << r.alpha << '|' << prec << '|' << r.quad;
     return ks.str();
+}
+
+// Continuous batching: a worker that takes a request also takes the queued
+// requests that share its factor (mesh, frequency, formulation, alpha,
+// precision, quadrature), up to batch_max_rhs_ right-hand sides in total, and
+// solves them with one excitation fill, one refinement and one far-field pass
+// on stacked right-hand sides.  Requests that arrive while a batch runs join
+// the next one.
+void Server::worker(int w) {
+    emfa_ctx *ctx = ctxs_[w];
+    for (;;) {
+        std::vector<Job> jobs;
+        {
+            std::unique_lock<std::mutex> l(mu_);
+            cv_work_.wait(l, [&] { return stop_ || !queue_.empty(); });
+            if (stop_) return;
+            jobs.push_back(std::move(queue_.front()));
+            queue_.pop_front();
+            int rhs = (int)(jobs[0].req.angles.size() / 3);
+            if (batch_max_rhs_ > 0 && rhs < batch_max_rhs_) {
+                const std::string key = factor_key(jobs[0].req, jobs[0].req.precision);
+                for (auto it = queue_.begin(); it != queue_.end();) {
+                    const int m = (int)(it->req.angles.size() / 3);
+                    if (m > 0 && rhs + m <= batch_max_rhs_ && factor_key(it->req, it->req.precision) == key) {
+                        rhs += m;
+                        jobs.push_back(std::move(*it));
+                        it = queue_.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            }
+            busy_ += (int)jobs.size();
+        }
+        std::vector<Result> res(jobs.size());
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            res[i].id = jobs[i].id;
+            res[i].batch_size = (int)jobs.size();
+        }
+        auto start = clock_t_::now();
+        try {
+            run(ctx, jobs, res);
+        } catch (const StatusError &e) {
+            for (auto &x : res) {
+                x.status = e.code;
+                x.error = e.what();
+            }
+        } catch (const std::exception &e) {
+            for (auto &x : res) {
+                x.status = EMFA_E_INVALID;
+                x.error = e.what();
+            }
+        }
+        auto end = clock_t_::now();
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            busy_ -= (int)jobs.size();
+            t1_ = end;
+            batched_ += (int64_t)jobs.size() - 1;
+            for (size_t i = 0; i < jobs.size(); ++i) {
+                Result &x = res[i];
+                x.queue_ms = ms_between(jobs[i].arrival, start);
+                x.service_ms = ms_between(start, end);
+                x.latency_ms = ms_between(jobs[i].arrival, end);
+                if (x.status == EMFA_OK) {
+                    lat_.push_back(x.latency_ms);
+                    rhs_done_ += x.nrhs;
+                    if (x.cache_hit) ++hits_;
+                } else {
+                    ++failed_;
+                }
+                done_[x.id] = std::move(x);
+            }
+        }
+        cv_done_.notify_all();
+    }
 }
 
 // The factor of (request, prec) from the cache, or computed and published.
@@ -363,10 +395,15 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
     return nf;
 }
 
-void Server::run(emfa_ctx *ctx, const Job &job, Result &res) {
-    const Request &r = job.req;
-    const int nrhs = (int)(r.angles.size() / 3);
-    if (nrhs <= 0) throw StatusError(EMFA_E_INVALID, "request without right-hand sides");
+void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result> &res) {
+    const Request &r = jobs[0].req;  // the batch shares mesh, frequency, formulation, precision
+    std::vector<int> first(jobs.size() + 1, 0);  // column range of each job in the stacked RHS
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const int m = (int)(jobs[i].req.angles.size() / 3);
+        if (m <= 0) throw StatusError(EMFA_E_INVALID, "request without right-hand sides");
+        first[i + 1] = first[i] + m;
+    }
+    const int nrhs = first.back();
     const double k = 2.0 * 3.14159265358979323846 * r.freq / 299792458.0;
 
     struct Guard {
@@ -379,30 +416,31 @@ void Server::run(emfa_ctx *ctx, const Job &job, Result &res) {
     } guard{this};
     bool hit = false;
     std::shared_ptr<Factor> f = acquire(ctx, r, r.precision, nrhs, &hit, &guard.b, 0);
-    res.cache_hit = hit;
     const int n = f->n;
-    res.unknowns = n;
-    res.nrhs = nrhs;
 
     // excitation (EMSolver/EM.cpp:29-41 per incidence), monostatic directions -k
     std::vector<double> inc(9 * (size_t)nrhs), dirs(3 * (size_t)nrhs);
     std::vector<int> cols(nrhs);
     const double d2r = 3.14159265358979323846 / 180.0;
-    for (int j = 0; j < nrhs; ++j) {
-        double th = r.angles[3 * j] * d2r, ph = r.angles[3 * j + 1] * d2r, po = r.angles[3 * j + 2] * d2r;
-        double kv[3] = {-std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
-        double ev[3] = {-std::cos(po) * std::cos(th) * std::cos(ph) - std::sin(po) * std::sin(ph),
-                        -std::cos(po) * std::cos(th) * std::sin(ph) + std::sin(po) * std::cos(ph),
-                        std::cos(po) * std::sin(th)};
-        double hv[3] = {kv[1] * ev[2] - kv[2] * ev[1], kv[2] * ev[0] - kv[0] * ev[2], kv[0] * ev[1] - kv[1] * ev[0]};
-        for (int c = 0; c < 3; ++c) {
-            inc[9 * j + c] = kv[c];
-            inc[9 * j + 3 + c] = ev[c];
-            inc[9 * j + 6 + c] = hv[c];
-            dirs[3 * j + c] = -kv[c];
+    for (size_t i = 0; i < jobs.size(); ++i)
+        for (int jj = 0; jj < first[i + 1] - first[i]; ++jj) {
+            const std::vector<double> &ang = jobs[i].req.angles;
+            const int j = first[i] + jj;
+            double th = ang[3 * jj] * d2r, ph = ang[3 * jj + 1] * d2r, po = ang[3 * jj + 2] * d2r;
+            double kv[3] = {-std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
+            double ev[3] = {-std::cos(po) * std::cos(th) * std::cos(ph) - std::sin(po) * std::sin(ph),
+                            -std::cos(po) * std::cos(th) * std::sin(ph) + std::sin(po) * std::cos(ph),
+                            std::cos(po) * std::sin(th)};
+            double hv[3] = {kv[1] * ev[2] - kv[2] * ev[1], kv[2] * ev[0] - kv[0] * ev[2],
+                            kv[0] * ev[1] - kv[1] * ev[0]};
+            for (int c = 0; c < 3; ++c) {
+                inc[9 * j + c] = kv[c];
+                inc[9 * j + 3 + c] = ev[c];
+                inc[9 * j + 6 + c] = hv[c];
+                dirs[3 * j + c] = -kv[c];
+            }
+            cols[j] = j;
         }
-        cols[j] = j;
-    }
     emfa_mat *B = nullptr, *X = nullptr;
     struct MatGuard {
         emfa_mat **a, **b;
@@ -414,32 +452,54 @@ void Server::run(emfa_ctx *ctx, const Job &job, Result &res) {
     check(emfa_mat_create(ctx, n, nrhs, EMFA_F64, &B), "B");
     check(emfa_rhs_fill(ctx, f->mesh, r.formulation, k, r.alpha, nrhs, inc.data(), B), "rhs");
     check(emfa_mat_create(ctx, n, nrhs, EMFA_F64, &X), "X");
+    int iters = 0;
+    double relres = 0;
+    bool fallback = false;
     if (r.precision == EMFA_F64) {
         check(emfa_mat_copy(X, B), "copy");
         check(emfa_lu_solve(ctx, f->lu, X), "solve");
     } else {
         emfa_refine_report rep{};
         emfa_status st = emfa_refine(ctx, f->Z, f->lu, B, X, refine_tol_, refine_maxit_, &rep);
-        res.refine_iterations = rep.iterations;
-        res.relres = rep.relres_final;
+        iters = rep.iterations;
+        relres = rep.relres_final;
         if (st == EMFA_E_NOCONV) {
             // the FP32 factor is too inaccurate for this matrix: solve with the FP64 factor
             bool hit64 = false;
             std::shared_ptr<Factor> f64 = acquire(ctx, r, EMFA_F64, nrhs, &hit64, &guard.b, f->bytes);
             check(emfa_mat_copy(X, B), "copy");
             check(emfa_lu_solve(ctx, f64->lu, X), "solve");
-            res.fp64_fallback = true;
+            fallback = true;
         } else {
             check(st, "refine");
         }
     }
-    res.rcs_db.resize(nrhs);
-    check(emfa_rcs(ctx, f->mesh, k, X, nrhs, dirs.data(), cols.data(), res.rcs_db.data()), "rcs");
+    std::vector<double> rcs(nrhs);
+    check(emfa_rcs(ctx, f->mesh, k, X, nrhs, dirs.data(), cols.data(), rcs.data()), "rcs");
+    std::vector<double> cur;
     if (keep_current_) {
-        res.current.resize(2 * (size_t)n * nrhs);
-        check(emfa_mat_download(X, res.current.data()), "download");
+        cur.resize(2 * (size_t)n * nrhs);
+        check(emfa_mat_download(X, cur.data()), "download");
     }
     check(emfa_ctx_sync(ctx), "sync");
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        Result &x = res[i];
+        const int c0 = first[i], m = first[i + 1] - first[i];
+        // batch mates did not pay for the factor: they count as reuse
+        x.cache_hit = hit || i > 0;
+        x.unknowns = n;
+        x.nrhs = m;
+        x.refine_iterations = iters;
+        x.relres = relres;
+        x.fp64_fallback = fallback;
+        x.rcs_db.assign(rcs.begin() + c0, rcs.begin() + c0 + m);
+        if (keep_current_) {
+            x.current.resize(2 * (size_t)n * m);
+            for (int row = 0; row < n; ++row)
+                std::copy(cur.begin() + 2 * ((size_t)row * nrhs + c0), cur.begin() + 2 * ((size_t)row * nrhs + c0 + m),
+                          x.current.begin() + 2 * (size_t)row * m);
+        }
+    }
 }
 
 Stats Server::stats() {
@@ -450,6 +510,7 @@ Stats Server::stats() {
     s.cache_hits = hits_;
     s.peak_bytes = peak_;
     s.pivot_fallbacks = pivot_fallbacks_;
+    s.batched = batched_;
     if (!lat_.empty()) {
         std::vector<double> v = lat_;
         std::sort(v.begin(), v.end());
@@ -476,7 +537,7 @@ Stats Server::stats() {
 void Server::reset_stats() {
     std::lock_guard<std::mutex> l(mu_);
     lat_.clear();
-    failed_ = hits_ = rhs_done_ = pivot_fallbacks_ = 0;
+    failed_ = hits_ = rhs_done_ = pivot_fallbacks_ = batched_ = 0;
     have_t0_ = false;
     peak_ = reserved_ + cache_bytes_;
 }

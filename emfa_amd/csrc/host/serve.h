@@ -56,6 +56,7 @@ struct Result {
     double relres = 0;
     bool fp64_fallback = false;  // refinement did not converge; solved with the FP64 factor
     double queue_ms = 0, service_ms = 0, latency_ms = 0;
+    int batch_size = 1;           // requests solved together with this one (same factor, one refinement)
     std::vector<double> rcs_db;   // monostatic RCS per right-hand side
     std::vector<double> current;  // optional: solution (interleaved complex, n x nrhs)
 };
@@ -67,6 +68,7 @@ struct Stats {
     double solves_per_s = 0, rhs_per_s = 0;
     size_t peak_bytes = 0;
     int64_t pivot_fallbacks = 0;  // FP32 factors redone with interchanges
+    int64_t batched = 0;          // requests that joined another request's batch
 };
 
 class Server {
@@ -95,7 +97,8 @@ class Server {
         std::chrono::steady_clock::time_point arrival;
     };
     void worker(int w);
-    void run(emfa_ctx *ctx, const Job &job, Result &res);
+    // solve the jobs of one batch (same factor key) with one refinement
+    void run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result> &res);
     std::shared_ptr<Factor> cache_get(const std::string &key);
     std::shared_ptr<Factor> acquire(emfa_ctx *ctx, const Request &r, int prec, int nrhs, bool *hit, size_t *held,
                                     size_t pinned);
@@ -129,7 +132,8 @@ class Server {
     std::set<std::string> building_;  // factors being computed by some worker
     // stats
     std::vector<double> lat_;
-    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0, pivot_fallbacks_ = 0;
+    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0, pivot_fallbacks_ = 0, batched_ = 0;
+    int batch_max_rhs_ = 256;  // right-hand sides one batch may stack (EMFA_SERVE_BATCH_RHS, 0: no batching)
     bool have_t0_ = false;
     std::chrono::steady_clock::time_point t0_, t1_;
 };

@@ -35,7 +35,7 @@ class Result(ctypes.Structure):
     _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("unknowns", ctypes.c_int),
                 ("nrhs", ctypes.c_int), ("cache_hit", ctypes.c_int), ("refine_iterations", ctypes.c_int),
                 ("relres", ctypes.c_double), ("fp64_fallback", ctypes.c_int), ("queue_ms", ctypes.c_double),
-                ("service_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+                ("service_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("batch_size", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("solves_per_s", ctypes.c_double),
                 ("rhs_per_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
-                ("pivot_fallbacks", ctypes.c_longlong)]
+                ("pivot_fallbacks", ctypes.c_longlong), ("batched", ctypes.c_longlong)]
 
 
 _CORE = None

@@ -64,6 +64,7 @@ typedef struct {
     double relres;      /* final relative residual of the refinement (FP32 requests) */
     int fp64_fallback;  /* refinement did not converge: solved with the FP64 factor */
     double queue_ms, service_ms, latency_ms;
+    int batch_size;     /* requests solved together (same factor, one stacked refinement) */
 } emfa_serve_result;
 
 typedef struct {
@@ -73,6 +74,7 @@ typedef struct {
     double solves_per_s, rhs_per_s;
     size_t peak_bytes;                      /* admitted footprint + resident factors */
     long long pivot_fallbacks;              /* FP32 factors redone with row interchanges */
+    long long batched;                      /* requests that joined another request's batch */
 } emfa_serve_stats;
 
 /* workers >= 1 contexts on `device`; hbm_budget in bytes; keep_current != 0

@@ -131,3 +131,28 @@ def test_mixed_stream_sample_served_matches_plain_fp64(ctx):
     assert st["completed"] == len(reqs) and st["failed"] == 0
     for p, s in zip(plain, served):
         assert np.abs(p - s["rcs_db"]).max() < 0.1
+
+
+def test_queued_requests_of_one_factor_are_batched_and_match_alone():
+    # continuous batching: requests queued behind a busy worker that share a
+    # factor are solved together (one stacked refinement); each answer stays
+    # within the refinement tolerance of the same request served alone
+    base = _req(serve.ICOSPHERE, 3, (0.7, 0.6, 0.5), hip.EFIE, 3.5e8, 5)
+    mates = [dict(base, angles=np.stack([th, 0 * th, 0 * th], 1))
+             for th in (np.linspace(0, 170, 5), np.linspace(10, 90, 9), np.array([33.0, 147.0]))]
+    blocker = _req(serve.PLATE, 40, (1.0, 1.0, 1.0), hip.EFIE, 4.0e8, 3, 80.0)
+    with serve.Server(workers=1, hbm_budget=16 * GIB, keep_current=True) as srv:
+        alone = [srv.wait(_submit(srv, r, hip.F32)) for r in mates]
+    with serve.Server(workers=1, hbm_budget=16 * GIB, keep_current=True) as srv:
+        b = _submit(srv, blocker, hip.F32)
+        ids = [_submit(srv, r, hip.F32) for r in mates]
+        assert srv.wait(b)["status"] == 0
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert all(a["batch_size"] == 1 for a in alone)
+    assert all(g["status"] == 0 and g["batch_size"] == 3 for g in got), [g["batch_size"] for g in got]
+    assert st["batched"] == 2
+    for a, g in zip(alone, got):
+        assert g["nrhs"] == a["nrhs"] and g["relres"] <= 1e-8
+        assert rel(g["current"], a["current"]) < 1e-6
+        assert np.abs(g["rcs_db"] - a["rcs_db"]).max() < 1e-4
