@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 # The solve server runs two streams per worker context; HIP maps streams onto
 # GPU_MAX_HW_QUEUES hardware queues (4 unless set), and streams that share a
 # queue execute in order.  Read once when HIP initialises, so set it here.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 
 PEAK = {"f32_mfma": 157.3, "f64_mfma": 78.6, "bf16_mfma": 2500.0, "hbm": 8000.0}
 # HBM bytes per launch of the split trailing update on the 30720-unknown
