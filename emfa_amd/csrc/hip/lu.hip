@@ -1082,6 +1082,9 @@ template <typename R, bool UPPER>
 __global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, long long ldf, int N,
                                                  cx<R> *__restrict__ Dinv) {
     __shared__ cx<R> Ta[64][65], Tb[64][65];
+    // staging for the doubling steps of the diagonal tiles (FP32: in LDS; the
+    // FP64 build keeps the third tile in global scratch next to the output)
+    __shared__ cx<float> Tc_lds[sizeof(R) == 4 ? 64 : 1][65];
     const int blk = blockIdx.x, r0 = blk * kDinvB, tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     cx<R> *X = Dinv + (size_t)blk * kDinvB * kDinvB;  // row-major 256 x 256
@@ -1124,30 +1127,99 @@ __global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, lo
     };
     for (int step = 0; step < 4; ++step) {
         const int i = UPPER ? 3 - step : step;
-        // diagonal tile: Ta = T_ii, Tb = its inverse (lane t solves column t)
         load_T(Ta, i, i);
-        for (int e = tid; e < 64 * 64; e += 256) Tb[e >> 6][e & 63] = {R(0), R(0)};
-        __syncthreads();
-        if (tid < 64) {
-            const int t = tid;
-            for (int q = 0; q < 64; ++q) {
-                const int r = UPPER ? 63 - q : q;
-                cx<R> acc = {r == t ? R(1) : R(0), R(0)};
-                const int kb = UPPER ? r + 1 : 0, ke = UPPER ? 64 : r;
-                for (int k = kb; k < ke; ++k) {
-                    const cx<R> t_rk = Ta[r][k], x = Tb[k][t];
-                    acc.re -= t_rk.re * x.re - t_rk.im * x.im;
-                    acc.im -= t_rk.re * x.im + t_rk.im * x.re;
+        if constexpr (sizeof(R) == 4) {
+            // diagonal tile: Ta = T_ii, Tb = its inverse by recursive doubling:
+            // inverses of the 1 x 1 diagonal blocks, then for b = 1, 2, .., 32 each
+            // pair of b-blocks is joined (L: X21 = -X22 L21 X11; U: X12 = -X11 U12
+            // X22), the product's middle factor staged in Tc
+            for (int e = tid; e < 64 * 64; e += 256) {
+                const int rr = e >> 6, cc = e & 63;
+                cx<R> v = {R(0), R(0)};
+                if (rr == cc) {
+                    if (UPPER) {
+                        const cx<R> d = Ta[rr][rr];
+                        const R den = d.re * d.re + d.im * d.im;
+                        v = den > R(0) ? cx<R>{d.re / den, -d.im / den} : cx<R>{R(0), R(0)};
+                    } else {
+                        v = {R(1), R(0)};
+                    }
                 }
-                if (UPPER) {
-                    const cx<R> d = Ta[r][r];
-                    const R den = d.re * d.re + d.im * d.im;
-                    acc = den > R(0) ? cmul(acc, cx<R>{d.re / den, -d.im / den}) : cx<R>{R(0), R(0)};
-                }
-                Tb[r][t] = acc;
+                Tb[rr][cc] = v;
             }
+            __syncthreads();
+            for (int b = 1; b < 64; b *= 2) {
+                // pairs p: block start s = 2 b p; off-diagonal block rows/cols
+                // L: Tc = L21 X11 (rows s+b.., cols s..), then X21 = -X22 Tc
+                // U: Tc = U12 X22 (rows s.., cols s+b..), then X12 = -X11 Tc
+                const int per = b * b, tot = (64 / (2 * b)) * per;
+                for (int e = tid; e < tot; e += 256) {
+                    const int pr = e / per, q = e % per, rr = q / b, cc = q % b, s0 = 2 * b * pr;
+                    cx<R> acc = {R(0), R(0)};
+                    if (!UPPER) {
+                        for (int k = 0; k < b; ++k) {
+                            const cx<R> a = Ta[s0 + b + rr][s0 + k], x = Tb[s0 + k][s0 + cc];
+                            acc.re += a.re * x.re - a.im * x.im;
+                            acc.im += a.re * x.im + a.im * x.re;
+                        }
+                        Tc_lds[s0 + b + rr][s0 + cc] = acc;
+                    } else {
+                        for (int k = 0; k < b; ++k) {
+                            const cx<R> a = Ta[s0 + rr][s0 + b + k], x = Tb[s0 + b + k][s0 + b + cc];
+                            acc.re += a.re * x.re - a.im * x.im;
+                            acc.im += a.re * x.im + a.im * x.re;
+                        }
+                        Tc_lds[s0 + rr][s0 + b + cc] = acc;
+                    }
+                }
+                __syncthreads();
+                for (int e = tid; e < tot; e += 256) {
+                    const int pr = e / per, q = e % per, rr = q / b, cc = q % b, s0 = 2 * b * pr;
+                    cx<R> acc = {R(0), R(0)};
+                    if (!UPPER) {
+                        for (int k = 0; k < b; ++k) {
+                            const cx<R> a = Tb[s0 + b + rr][s0 + b + k], x = Tc_lds[s0 + b + k][s0 + cc];
+                            acc.re -= a.re * x.re - a.im * x.im;
+                            acc.im -= a.re * x.im + a.im * x.re;
+                        }
+                        Tb[s0 + b + rr][s0 + cc] = acc;
+                    } else {
+                        for (int k = 0; k < b; ++k) {
+                            const cx<R> a = Tb[s0 + rr][s0 + k], x = Tc_lds[s0 + k][s0 + b + cc];
+                            acc.re -= a.re * x.re - a.im * x.im;
+                            acc.im -= a.re * x.im + a.im * x.re;
+                        }
+                        Tb[s0 + rr][s0 + b + cc] = acc;
+                    }
+                }
+                __syncthreads();
+            }
+
+        } else {
+            // FP64: lane t solves column t of the tile's inverse (LDS holds two tiles only)
+            for (int e = tid; e < 64 * 64; e += 256) Tb[e >> 6][e & 63] = {R(0), R(0)};
+            __syncthreads();
+            if (tid < 64) {
+                const int t = tid;
+                for (int q = 0; q < 64; ++q) {
+                    const int r = UPPER ? 63 - q : q;
+                    cx<R> acc = {r == t ? R(1) : R(0), R(0)};
+                    const int kb = UPPER ? r + 1 : 0, ke = UPPER ? 64 : r;
+                    for (int k = kb; k < ke; ++k) {
+                        const cx<R> t_rk = Ta[r][k], x = Tb[k][t];
+                        acc.re -= t_rk.re * x.re - t_rk.im * x.im;
+                        acc.im -= t_rk.re * x.im + t_rk.im * x.re;
+                    }
+                    if (UPPER) {
+                        const cx<R> d = Ta[r][r];
+                        const R den = d.re * d.re + d.im * d.im;
+                        acc = den > R(0) ? cmul(acc, cx<R>{d.re / den, -d.im / den}) : cx<R>{R(0), R(0)};
+                    }
+                    Tb[r][t] = acc;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (int e = tid; e < 64 * 64; e += 256) X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * i + (e & 63)] = Tb[e >> 6][e & 63];
         // the other triangle of row tile i is zero
         for (int j = 0; j < 4; ++j)
