@@ -50,6 +50,8 @@ struct Bf3Planes {
     float *C;
     long long ldc;
     int alpha_neg, beta;
+    int max_wgs = 0;     // > 0: at most this many workgroups (rounded down to a multiple of 8), looping over tiles
+    int persistent = 0;  // set by the launcher
 };
 
 // Split rows x cols FP32 (row stride ld, 8-byte aligned rows) into hi/lo bf16

@@ -16,6 +16,22 @@ namespace emfa {
 // depends on the dispatch order.
 constexpr int kGroupRows = 8;
 
+// The same order for linear tile index L of a TX x TY tile grid (persistent
+// workgroups: workgroup i takes tiles i, i + G, ..., all on XCD i % 8 when G is
+// a multiple of 8).
+__device__ __forceinline__ void tile_of_linear(int L, int TX, int TY, int *ty, int *tx) {
+    const int nT = TX * TY;
+    const int xcd = L & 7, slot = L >> 3;
+    const int q = nT >> 3, r = nT & 7;
+    const int t = xcd * q + min(xcd, r) + slot;
+    const int group = t / (kGroupRows * TX);
+    const int first = group * kGroupRows;
+    const int rows = min(kGroupRows, TY - first);
+    const int local = t - group * kGroupRows * TX;
+    *ty = first + local % rows;
+    *tx = local / rows;
+}
+
 __device__ __forceinline__ void tile_of(int swizzle, int *ty, int *tx) {
     const int TX = gridDim.x, TY = gridDim.y;
     if (!swizzle) {
@@ -339,8 +355,14 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
     __shared__ __attribute__((aligned(16))) __bf16 sm[NS][4][PLANE];  // stage, plane (Ah Al Bh Bl)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
+    const int TX = (g.Nr + BN - 1) / BN, TY = (g.M + BM - 1) / BM;
+    // one tile per workgroup, or (persistent grid) tiles blockIdx.x, + gridDim.x, ...
+    const int t0 = g.persistent ? (int)blockIdx.x : (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    const int tstep = g.persistent ? (int)gridDim.x : TX * TY;
+    for (int tl = t0; tl < TX * TY; tl += tstep) {
+    if (tl != t0) __syncthreads();  // the previous tile's LDS reads are done
     int ty, tx;
-    tile_of(1, &ty, &tx);
+    tile_of_linear(tl, TX, TY, &ty, &tx);
     const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
     const int nk = g.Kp / BK;
 
@@ -450,6 +472,7 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
         epilogue(std::false_type{});
     else
         epilogue(std::true_type{});
+    }  // tiles
 }
 
 // ---------------------------------------------------------------------------
@@ -636,8 +659,15 @@ emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g) {
                      g.lda % 8 == 0 && g.ldb % 8 == 0 && al16(g.Ah) && al16(g.Al) && al16(g.Bh) && al16(g.Bl) &&
                      g.M <= 65535LL * bf3p::BM && g.Nr <= 65535LL * bf3p::BN,
                  EMFA_E_INVALID, "gemm bf16x3 planes: bad shape M=%d Nr=%d Kp=%d", g.M, g.Nr, g.Kp);
-    dim3 grid((g.Nr + bf3p::BN - 1) / bf3p::BN, (g.M + bf3p::BM - 1) / bf3p::BM);
-    k_gemm_bf3p<<<grid, 512, 0, s>>>(g);
+    const int TX = (g.Nr + bf3p::BN - 1) / bf3p::BN, TY = (g.M + bf3p::BM - 1) / bf3p::BM;
+    Bf3Planes gg = g;
+    if (g.max_wgs >= 8 && TX * TY > g.max_wgs) {
+        gg.persistent = 1;
+        k_gemm_bf3p<<<(g.max_wgs / 8) * 8, 512, 0, s>>>(gg);
+    } else {
+        gg.persistent = 0;
+        k_gemm_bf3p<<<dim3(TX, TY), 512, 0, s>>>(gg);
+    }
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
 }

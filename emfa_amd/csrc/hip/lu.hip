@@ -986,6 +986,12 @@ struct UpdatePlanes {
         g.ldc = 2 * ldc;
         g.alpha_neg = 1;
         g.beta = 1;
+        // contexts sharing the device keep each update to a multiple of their CU share
+        if (ctx->cu_share > 0) {
+            int cap = 1;
+            if (const char *e = getenv("EMFA_GEMM_SHARE_CAP")) cap = atoi(e);
+            if (cap > 0) g.max_wgs = cap * ctx->cu_share;
+        }
         return gemm_launch_bf3_planes(s, g);
     }
 };
