@@ -7,6 +7,7 @@ with the monostatic RCS per incidence and, optionally, the surface current.
 import ctypes
 import os
 import threading
+import warnings
 
 import numpy as np
 
@@ -92,6 +93,18 @@ def footprint(n, nrhs, precision, factor_cached=False):
 
 class Server:
     def __init__(self, device=0, workers=2, hbm_budget=64 << 30, keep_current=False):
+        # Each worker drives two streams; HIP places streams on
+        # GPU_MAX_HW_QUEUES hardware queues (4 unless set before HIP
+        # initialises) and streams sharing a queue run in order.
+        want = min(32, 2 * workers)
+        try:
+            have = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        except ValueError:
+            have = 4
+        if have < want:
+            warnings.warn(f"Server with {workers} workers on {have} HIP hardware queues: the workers' streams "
+                          f"serialise on them; set GPU_MAX_HW_QUEUES={want} before HIP initialises",
+                          RuntimeWarning, stacklevel=2)
         self.h = ctypes.c_void_p()
         rc = core().emfa_serve_create(device, workers, int(hbm_budget), int(keep_current), ctypes.byref(self.h))
         if rc != 0:
