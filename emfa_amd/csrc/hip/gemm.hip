@@ -52,6 +52,35 @@ __device__ __forceinline__ void tile_of(int swizzle, int *ty, int *tx) {
     *tx = local / rows;
 }
 
+// C (+)= sgn * acc for one wave's 64 x 64 tile of 2 x 2 MFMA 32x32 results.
+// The loads of C of each 32-row half are issued before its stores: a store
+// through C may alias a later load, so a load-add-store per element makes each
+// one wait for the previous store's round trip (64 in a row per lane).  Halves
+// rather than the whole tile keep the kernel within 256 VGPRs.
+__device__ __forceinline__ void epilogue_f32(const GemmArgs &g, float *C, const floatx16 (&acc)[2][2],
+                                             long long r0, long long c0, int lane) {
+    const float sgn = g.alpha_neg ? -1.f : 1.f;
+    const long long j0 = c0 + (lane & 31);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        float old[2][16];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const long long i = r0 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), j = j0 + nt * 32;
+                old[nt][v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.f;
+            }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const long long i = r0 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), j = j0 + nt * 32;
+                if (i < g.M && j < g.Nr) C[i * g.ldc + j] = old[nt][v] + sgn * acc[mt][nt][v];
+            }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP32: workgroup tile 128 x 128 (reals), 4 waves in 2 x 2, wave tile 64 x 64 =
 // 2 x 2 MFMA 32x32 tiles.  K step 32: MFMA t (0..15) takes k = 16*(lane/32) + t,
@@ -153,21 +182,102 @@ __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
         __syncthreads();
     }
 
+    epilogue_f32(g, C, acc, bm + wm * 64, bn + wn * 64, lane);
+}
+
+// ---------------------------------------------------------------------------
+// FP32, small tiles: workgroup tile 64 x 64 (reals), 4 waves in 2 x 2, one
+// MFMA 32x32 tile per wave, K step 32 with the same K mapping as k_gemm_f32
+// (identical sums).  For launches whose 128 x 128 grid would leave most CUs
+// idle: four times the workgroups and a quarter of each wave's MFMA work
+// (a 128 x 128 tile issues 64 MFMAs per K step per wave: ~7 us at K = 128).
+// ---------------------------------------------------------------------------
+namespace f32s {
+constexpr int BM = 64, BN = 64, BK = 32, PAD = 4;
+}
+
+template <int TAG>
+__global__ __launch_bounds__(256) void k_gemm_f32s(GemmArgs g) {
+    using namespace f32s;
+    __shared__ float As[2][BM][BK + PAD];
+    __shared__ float Bs[2][BN][BK + PAD];
+    const float *__restrict__ A = (const float *)g.A;
+    const float *__restrict__ Bt = (const float *)g.Bt;
+    float *__restrict__ C = (float *)g.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    int ty, tx;
+    tile_of(g.swizzle, &ty, &tx);
+    const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
+    const int nk = (g.Kr + BK - 1) / BK;
+
+    float2 ra[4], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 256 * i;
+            const int row = idx >> 4, kk = (idx & 15) * 2;
+            const long long gr = bm + row, gc = bn + row;
+            const int gk = k0 + kk;
+            ra[i] = (gr < g.M && gk < g.Kr) ? *(const float2 *)(A + gr * g.lda + gk) : make_float2(0.f, 0.f);
+            rb[i] = (gc < g.Nr && gk < g.Kr) ? *(const float2 *)(Bt + gc * g.ldb + gk) : make_float2(0.f, 0.f);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 256 * i;
+            const int row = idx >> 4, kk = (idx & 15) * 2;
+            *(float2 *)&As[buf][row][kk] = ra[i];
+            *(float2 *)&Bs[buf][row][kk] = rb[i];
+        }
+    };
+
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int r32 = lane & 31, kg = 16 * (lane >> 5);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+        float av[16], bv[16];
+        const float4 *sa = (const float4 *)&As[buf][wm * 32 + r32][kg];
+        const float4 *sb = (const float4 *)&Bs[buf][wn * 32 + r32][kg];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 x = sa[q], y = sb[q];
+            av[4 * q] = x.x;
+            av[4 * q + 1] = x.y;
+            av[4 * q + 2] = x.z;
+            av[4 * q + 3] = x.w;
+            bv[4 * q] = y.x;
+            bv[4 * q + 1] = y.y;
+            bv[4 * q + 2] = y.z;
+            bv[4 * q + 3] = y.w;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], bv[t], acc, 0, 0, 0);
+        if (kt + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    // all loads of C before the first store (see epilogue_f32)
     const float sgn = g.alpha_neg ? -1.f : 1.f;
+    const long long j = bn + wn * 32 + r32;
+    float old[16];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+    for (int v = 0; v < 16; ++v) {
+        const long long i = bm + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        old[v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.f;
+    }
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                long long i = bm + wm * 64 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                long long j = bn + wn * 64 + nt * 32 + r32;
-                if (i < g.M && j < g.Nr) {
-                    float *c = C + i * g.ldc + j;
-                    float val = sgn * acc[mt][nt][v];
-                    *c = g.beta ? (*c + val) : val;
-                }
-            }
+    for (int v = 0; v < 16; ++v) {
+        const long long i = bm + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (i < g.M && j < g.Nr) C[i * g.ldc + j] = old[v] + sgn * acc[v];
+    }
 }
 
 // Variant with 16-byte global loads and a K step of BKV (32 or 64): same
@@ -263,21 +373,7 @@ __global__ __launch_bounds__(256) void k_gemm_f32v(GemmArgs g) {
         if (kt + 1 < nk) sstore(buf ^ 1);
         __syncthreads();
     }
-    const float sgn = g.alpha_neg ? -1.f : 1.f;
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                long long i = bm + wm * 64 + mt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                long long j = bn + wn * 64 + nt * 32 + r32;
-                if (i < g.M && j < g.Nr) {
-                    float *c = C + i * g.ldc + j;
-                    float val = sgn * acc[mt][nt][v];
-                    *c = g.beta ? (*c + val) : val;
-                }
-            }
+    epilogue_f32(g, C, acc, bm + wm * 64, bn + wn * 64, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -578,7 +674,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
         __syncthreads();
     }
 
+    // All loads of C before the first store (a store through C may alias a
+    // later load, which would otherwise serialise 64 round trips per lane).
     const double sgn = g.alpha_neg ? -1.0 : 1.0;
+    double old[4][4][4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
@@ -586,13 +685,19 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 // V_MFMA_F64_16X16X4_F64 result: lane l, element v holds row l/16 + 4v, column l%16
-                long long i = bm + wm * 64 + mt * 16 + (lane >> 4) + 4 * v;
-                long long j = bn + wn * 64 + nt * 16 + r16;
-                if (i < g.M && j < g.Nr) {
-                    double *c = C + i * g.ldc + j;
-                    double val = sgn * acc[mt][nt][v];
-                    *c = g.beta ? (*c + val) : val;
-                }
+                const long long i = bm + wm * 64 + mt * 16 + (lane >> 4) + 4 * v;
+                const long long j = bn + wn * 64 + nt * 16 + r16;
+                old[mt][nt][v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.0;
+            }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long i = bm + wm * 64 + mt * 16 + (lane >> 4) + 4 * v;
+                const long long j = bn + wn * 64 + nt * 16 + r16;
+                if (i < g.M && j < g.Nr) C[i * g.ldc + j] = old[mt][nt][v] + sgn * acc[mt][nt][v];
             }
 }
 
@@ -620,8 +725,21 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
     }();
     const bool vec4 = (g.lda % 4) == 0 && (g.ldb % 4) == 0 && (g.Kr % 4) == 0 &&
                       ((uintptr_t)g.A % 16) == 0 && ((uintptr_t)g.Bt % 16) == 0;
+    // launches whose 128 x 128 grid fills less than half of the CUs take the
+    // 64 x 64 tiles (EMFA_GEMM_SMALL=0 disables)
+    static const long long small_below = [] {
+        const char *e = getenv("EMFA_GEMM_SMALL");
+        return e && atoi(e) == 0 ? 0LL : 128LL;
+    }();
+    const long long tiles = (long long)grid.x * grid.y;
     // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart
-    if (vec4 && variant == 1) {
+    if (tiles < small_below && variant == 0) {
+        dim3 gs((g.Nr + f32s::BN - 1) / f32s::BN, (g.M + f32s::BM - 1) / f32s::BM);
+        if (tag == 1)
+            k_gemm_f32s<1><<<gs, 256, 0, s>>>(g);
+        else
+            k_gemm_f32s<0><<<gs, 256, 0, s>>>(g);
+    } else if (vec4 && variant == 1) {
         if (tag == 1)
             k_gemm_f32v<1, 32><<<grid, 256, 0, s>>>(g);
         else

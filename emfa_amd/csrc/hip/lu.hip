@@ -1084,13 +1084,81 @@ __device__ __forceinline__ void tile_mac(TileAcc<R> &acc, const cx<R> (*Ta)[65],
     }
 }
 
+// Inverse of a triangular 64 x 64 FP32 tile held in LDS, by recursive
+// doubling: inverses of the 1 x 1 diagonal blocks, then for b = 1, 2, .., 32
+// each pair of b-blocks is joined (L: X21 = -X22 L21 X11; U: X12 = -X11 U12
+// X22), the product's middle factor staged in C.  T is read in its triangle
+// only (unit diagonal taken for L); X receives the inverse, zero in the other
+// triangle.  All `nthr` threads of the workgroup call it (it synchronises).
+template <bool UPPER>
+__device__ void tile_inv64(const cx<float> (*T)[65], cx<float> (*X)[65], cx<float> (*C)[65], int tid, int nthr) {
+    using R = float;
+    for (int e = tid; e < 64 * 64; e += nthr) {
+        const int rr = e >> 6, cc = e & 63;
+        cx<R> v = {R(0), R(0)};
+        if (rr == cc) {
+            if (UPPER) {
+                const cx<R> d = T[rr][rr];
+                const R den = d.re * d.re + d.im * d.im;
+                v = den > R(0) ? cx<R>{d.re / den, -d.im / den} : cx<R>{R(0), R(0)};
+            } else {
+                v = {R(1), R(0)};
+            }
+        }
+        X[rr][cc] = v;
+    }
+    __syncthreads();
+    for (int b = 1; b < 64; b *= 2) {
+        // pairs p: block start s = 2 b p; off-diagonal block rows/cols
+        // L: C = L21 X11 (rows s+b.., cols s..), then X21 = -X22 C
+        // U: C = U12 X22 (rows s.., cols s+b..), then X12 = -X11 C
+        const int per = b * b, tot = (64 / (2 * b)) * per;
+        for (int e = tid; e < tot; e += nthr) {
+            const int pr = e / per, q = e % per, rr = q / b, cc = q % b, s0 = 2 * b * pr;
+            cx<R> acc = {R(0), R(0)};
+            if (!UPPER) {
+                for (int k = 0; k < b; ++k) {
+                    const cx<R> a = T[s0 + b + rr][s0 + k], x = X[s0 + k][s0 + cc];
+                    acc.re += a.re * x.re - a.im * x.im;
+                    acc.im += a.re * x.im + a.im * x.re;
+                }
+                C[s0 + b + rr][s0 + cc] = acc;
+            } else {
+                for (int k = 0; k < b; ++k) {
+                    const cx<R> a = T[s0 + rr][s0 + b + k], x = X[s0 + b + k][s0 + b + cc];
+                    acc.re += a.re * x.re - a.im * x.im;
+                    acc.im += a.re * x.im + a.im * x.re;
+                }
+                C[s0 + rr][s0 + b + cc] = acc;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < tot; e += nthr) {
+            const int pr = e / per, q = e % per, rr = q / b, cc = q % b, s0 = 2 * b * pr;
+            cx<R> acc = {R(0), R(0)};
+            if (!UPPER) {
+                for (int k = 0; k < b; ++k) {
+                    const cx<R> a = X[s0 + b + rr][s0 + b + k], x = C[s0 + b + k][s0 + cc];
+                    acc.re -= a.re * x.re - a.im * x.im;
+                    acc.im -= a.re * x.im + a.im * x.re;
+                }
+                X[s0 + b + rr][s0 + cc] = acc;
+            } else {
+                for (int k = 0; k < b; ++k) {
+                    const cx<R> a = X[s0 + rr][s0 + k], x = C[s0 + k][s0 + b + cc];
+                    acc.re -= a.re * x.re - a.im * x.im;
+                    acc.im -= a.re * x.im + a.im * x.re;
+                }
+                X[s0 + rr][s0 + b + cc] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <typename R, bool UPPER>
-__global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, long long ldf, int N,
-                                                 cx<R> *__restrict__ Dinv) {
-    __shared__ cx<R> Ta[64][65], Tb[64][65];
-    // staging for the doubling steps of the diagonal tiles (FP32: in LDS; the
-    // FP64 build keeps the third tile in global scratch next to the output)
-    __shared__ cx<float> Tc_lds[sizeof(R) == 4 ? 64 : 1][65];
+__device__ void tri_inv_block(const cx<R> *__restrict__ F, long long ldf, int N, cx<R> *__restrict__ Dinv,
+                              cx<R> (*Ta)[65], cx<R> (*Tb)[65], cx<float> (*Tc_lds)[65]) {
     const int blk = blockIdx.x, r0 = blk * kDinvB, tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     cx<R> *X = Dinv + (size_t)blk * kDinvB * kDinvB;  // row-major 256 x 256
@@ -1135,72 +1203,7 @@ __global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, lo
         const int i = UPPER ? 3 - step : step;
         load_T(Ta, i, i);
         if constexpr (sizeof(R) == 4) {
-            // diagonal tile: Ta = T_ii, Tb = its inverse by recursive doubling:
-            // inverses of the 1 x 1 diagonal blocks, then for b = 1, 2, .., 32 each
-            // pair of b-blocks is joined (L: X21 = -X22 L21 X11; U: X12 = -X11 U12
-            // X22), the product's middle factor staged in Tc
-            for (int e = tid; e < 64 * 64; e += 256) {
-                const int rr = e >> 6, cc = e & 63;
-                cx<R> v = {R(0), R(0)};
-                if (rr == cc) {
-                    if (UPPER) {
-                        const cx<R> d = Ta[rr][rr];
-                        const R den = d.re * d.re + d.im * d.im;
-                        v = den > R(0) ? cx<R>{d.re / den, -d.im / den} : cx<R>{R(0), R(0)};
-                    } else {
-                        v = {R(1), R(0)};
-                    }
-                }
-                Tb[rr][cc] = v;
-            }
-            __syncthreads();
-            for (int b = 1; b < 64; b *= 2) {
-                // pairs p: block start s = 2 b p; off-diagonal block rows/cols
-                // L: Tc = L21 X11 (rows s+b.., cols s..), then X21 = -X22 Tc
-                // U: Tc = U12 X22 (rows s.., cols s+b..), then X12 = -X11 Tc
-                const int per = b * b, tot = (64 / (2 * b)) * per;
-                for (int e = tid; e < tot; e += 256) {
-                    const int pr = e / per, q = e % per, rr = q / b, cc = q % b, s0 = 2 * b * pr;
-                    cx<R> acc = {R(0), R(0)};
-                    if (!UPPER) {
-                        for (int k = 0; k < b; ++k) {
-                            const cx<R> a = Ta[s0 + b + rr][s0 + k], x = Tb[s0 + k][s0 + cc];
-                            acc.re += a.re * x.re - a.im * x.im;
-                            acc.im += a.re * x.im + a.im * x.re;
-                        }
-                        Tc_lds[s0 + b + rr][s0 + cc] = acc;
-                    } else {
-                        for (int k = 0; k < b; ++k) {
-                            const cx<R> a = Ta[s0 + rr][s0 + b + k], x = Tb[s0 + b + k][s0 + b + cc];
-                            acc.re += a.re * x.re - a.im * x.im;
-                            acc.im += a.re * x.im + a.im * x.re;
-                        }
-                        Tc_lds[s0 + rr][s0 + b + cc] = acc;
-                    }
-                }
-                __syncthreads();
-                for (int e = tid; e < tot; e += 256) {
-                    const int pr = e / per, q = e % per, rr = q / b, cc = q % b, s0 = 2 * b * pr;
-                    cx<R> acc = {R(0), R(0)};
-                    if (!UPPER) {
-                        for (int k = 0; k < b; ++k) {
-                            const cx<R> a = Tb[s0 + b + rr][s0 + b + k], x = Tc_lds[s0 + b + k][s0 + cc];
-                            acc.re -= a.re * x.re - a.im * x.im;
-                            acc.im -= a.re * x.im + a.im * x.re;
-                        }
-                        Tb[s0 + b + rr][s0 + cc] = acc;
-                    } else {
-                        for (int k = 0; k < b; ++k) {
-                            const cx<R> a = Tb[s0 + rr][s0 + k], x = Tc_lds[s0 + k][s0 + b + cc];
-                            acc.re -= a.re * x.re - a.im * x.im;
-                            acc.im -= a.re * x.im + a.im * x.re;
-                        }
-                        Tb[s0 + rr][s0 + b + cc] = acc;
-                    }
-                }
-                __syncthreads();
-            }
-
+            tile_inv64<UPPER>(Ta, Tb, Tc_lds, tid, 256);
         } else {
             // FP64: lane t solves column t of the tile's inverse (LDS holds two tiles only)
             for (int e = tid; e < 64 * 64; e += 256) Tb[e >> 6][e & 63] = {R(0), R(0)};
@@ -1261,6 +1264,20 @@ __global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, lo
             publish();
         }
     }
+}
+
+// Grid (blocks, 2): blockIdx.y 0 builds the unit-lower inverses into Dinv,
+// 1 the upper ones into Dinv + blocks * 256 * 256, in one launch.
+template <typename R>
+__global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, long long ldf, int N,
+                                                 cx<R> *__restrict__ Dinv) {
+    __shared__ cx<R> Ta[64][65], Tb[64][65];
+    // staging for the doubling steps of the diagonal tiles (FP32 only)
+    __shared__ cx<float> Tc_lds[sizeof(R) == 4 ? 64 : 1][65];
+    if (blockIdx.y == 0)
+        tri_inv_block<R, false>(F, ldf, N, Dinv, Ta, Tb, Tc_lds);
+    else
+        tri_inv_block<R, true>(F, ldf, N, Dinv + (size_t)gridDim.x * kDinvB * kDinvB, Ta, Tb, Tc_lds);
 }
 
 // Y_b = Dinv_b W_b for one 256-row block (nb valid rows, m columns): 32 x 32
@@ -1476,11 +1493,9 @@ constexpr int kNoPivW = 64;  // widest pivot-free panel
 // publishes the four multiplier columns through LDS; after one barrier every
 // wave to the right forms the four pivot-row entries of its columns (a 4 x 4
 // unit-lower solve on wave-uniform values) and applies the rank-4 update.
-// Then U11^-1 by block Gauss-Jordan on the same layout (the owner publishes
-// its four U columns, every wave solves the 4 x 4 upper block for the
-// block's rows of its inverse columns and updates the rows above), written to
-// uinv (64 x 64, row-major).  16 + 16 barriers; rows and columns past w act
-// as identity; the LDS buffers alternate by block parity.
+// Then U11^-1 from a copy of U in LDS by recursive doubling on all 16 waves
+// (tile_inv64), written to uinv (64 x 64, row-major).  Rows and columns past
+// w act as identity; the LU's LDS buffers alternate by block parity.
 // reciprocal for the pivot-free FP32 factor (refined in FP64 afterwards):
 // the hardware estimate (1 ulp) instead of a correctly rounded division
 template <typename R>
@@ -1515,7 +1530,9 @@ template <typename R>
 __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__ A, long long lda, int js, int w,
                                                             int *__restrict__ ipiv, int *flags, R tau2,
                                                             cx<R> *__restrict__ uinv) {
+    static_assert(sizeof(R) == 4, "the pivot-free diagonal block is an FP32 kernel");
     __shared__ cx<R> buf[2][kDiagCpw][kNoPivW];
+    __shared__ cx<float> Tu[kNoPivW][kNoPivW + 1], Tx[kNoPivW][kNoPivW + 1], Tc[kNoPivW][kNoPivW + 1];
     __shared__ int bad;
     // wave index as a scalar: block tests below are uniform per wave
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1579,49 +1596,13 @@ __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__
         const int j = kDiagCpw * wv + q;
         if (lane < w && j < w) A[(size_t)(js + lane) * lda + js + j] = a[q];
     }
-    // X = U^-1 by 4-column blocks, last block first
-    cx<R> x[kDiagCpw];
+    // U11^-1 by recursive doubling on all waves, from a copy of U in LDS
+    // (the strictly lower part of the copy is never read)
 #pragma unroll
-    for (int q = 0; q < kDiagCpw; ++q) x[q] = {lane == kDiagCpw * wv + q ? R(1) : R(0), R(0)};
+    for (int q = 0; q < kDiagCpw; ++q) Tu[lane][kDiagCpw * wv + q] = a[q];
     __syncthreads();
-    for (int b = nbk - 1; b >= 0; --b) {
-        const int p = b & 1, c0 = kDiagCpw * b;
-        if (wv == b) {
-#pragma unroll
-            for (int cc = 0; cc < kDiagCpw; ++cc) buf[p][cc][lane] = a[cc];
-        }
-        __syncthreads();
-        cx<R> dinv[kDiagCpw];  // reciprocals of the block's diagonal (uniform)
-#pragma unroll
-        for (int k = 0; k < kDiagCpw; ++k) {
-            const cx<R> d = buf[p][k][c0 + k];
-            const R den = d.re * d.re + d.im * d.im;
-            const R rd = den > R(0) ? fast_recip(den) : R(0);
-            dinv[k] = {d.re * rd, -d.im * rd};
-        }
-        const bool above = lane < c0;
-        cx<R> ucol[kDiagCpw];  // U(lane, c0 + m) for rows above the block
-#pragma unroll
-        for (int m = 0; m < kDiagCpw; ++m) ucol[m] = above ? buf[p][m][lane] : cx<R>{R(0), R(0)};
-#pragma unroll
-        for (int q = 0; q < kDiagCpw; ++q) {
-            cx<R> y[kDiagCpw];
-#pragma unroll
-            for (int k = kDiagCpw - 1; k >= 0; --k) {
-                cx<R> t = {lane_read(x[q].re, c0 + k), lane_read(x[q].im, c0 + k)};
-#pragma unroll
-                for (int m = k + 1; m < kDiagCpw; ++m) cfms(t, buf[p][m][c0 + k], y[m]);
-                y[k] = cmul(t, dinv[k]);
-            }
-#pragma unroll
-            for (int k = 0; k < kDiagCpw; ++k)
-                if (lane == c0 + k) x[q] = y[k];
-#pragma unroll
-            for (int m = 0; m < kDiagCpw; ++m) cfms(x[q], ucol[m], y[m]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < kDiagCpw; ++q) uinv[lane * kNoPivW + kDiagCpw * wv + q] = x[q];
+    tile_inv64<true>(Tu, Tx, Tc, threadIdx.x, kDiagWaves * 64);
+    for (int e = threadIdx.x; e < kNoPivW * kNoPivW; e += kDiagWaves * 64) uinv[e] = Tx[e >> 6][e & 63];
     if (threadIdx.x < w) ipiv[js + threadIdx.x] = js + threadIdx.x;
     if (mybad) bad = 1;
     __syncthreads();
@@ -2065,13 +2046,11 @@ static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
     if (lu->a->prec == EMFA_F32) {
         const cx<float> *F = (const cx<float> *)lu->a->d;
         cx<float> *D = (cx<float> *)lu->dinv;
-        k_tri_inv<float, false><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D);
-        k_tri_inv<float, true><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D + (size_t)nblk * kDinvB * kDinvB);
+        k_tri_inv<float><<<dim3(nblk, 2), 256, 0, ctx->stream>>>(F, N, N, D);
     } else {
         const cx<double> *F = (const cx<double> *)lu->a->d;
         cx<double> *D = (cx<double> *)lu->dinv;
-        k_tri_inv<double, false><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D);
-        k_tri_inv<double, true><<<nblk, 256, 0, ctx->stream>>>(F, N, N, D + (size_t)nblk * kDinvB * kDinvB);
+        k_tri_inv<double><<<dim3(nblk, 2), 256, 0, ctx->stream>>>(F, N, N, D);
     }
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
