@@ -556,12 +556,12 @@ __device__ __forceinline__ double bcast<double>(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// cols_per_wg = kTrsmColsPerPass * passes
+// cols_per_wg = kTrsmColsPerPass * passes; workgroup `bx` of the solve
 template <typename R>
-__global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
-                                                long long ldx, int r0, int wb, int c0, int c1, int upper,
-                                                R *__restrict__ Bt, long long ldbt, int kofs, int passes) {
-    __shared__ cx<R> Lt[64][65];
+__device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
+                                            long long ldx, int r0, int wb, int c0, int c1, int upper,
+                                            R *__restrict__ Bt, long long ldbt, int kofs, int passes, int bx,
+                                            cx<R> (*Lt)[65]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int e = tid; e < wb * wb; e += 256) {
         int r = e / wb, i = e % wb;
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, lon
     }
     __syncthreads();
     const int cols_per_wg = kTrsmColsPerPass * passes;
-    const int cb = c0 + blockIdx.x * cols_per_wg;
+    const int cb = c0 + bx * cols_per_wg;
     for (int ps = 0; ps < passes; ++ps) {
         int col[kTrsmCPW];
         bool ok[kTrsmCPW];
@@ -625,6 +625,14 @@ __global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, lon
             }
         }
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
+                                                long long ldx, int r0, int wb, int c0, int c1, int upper,
+                                                R *__restrict__ Bt, long long ldbt, int kofs, int passes) {
+    __shared__ cx<R> Lt[64][65];
+    trsm64_body<R>(F, ldf, X, ldx, r0, wb, c0, c1, upper, Bt, ldbt, kofs, passes, blockIdx.x, Lt);
 }
 
 // --------------------------------------------------------------------------
@@ -1049,9 +1057,11 @@ static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx
 // factor, turn each diagonal solve of a triangular solve into a product:
 // a solve is then two launches per 256-row block (k_dinv_apply and the MFMA
 // update of the remaining rows).
-// k_tri_inv: one workgroup per block.  The block is handled as 4 x 4 tiles of
-// 64: each diagonal tile is inverted in LDS (one lane per column, unit
-// diagonal for L), the off-diagonal tiles of the inverse follow from
+// k_tri_inv (FP64 factors; FP32 factors use k_dinv_diag / k_dinv_off below,
+// the same recurrences spread over one workgroup per tile): one workgroup
+// per block and triangle.  The block is handled as 4 x 4 tiles of 64: each
+// diagonal tile is inverted in LDS (one lane per column, unit diagonal for
+// L), the off-diagonal tiles of the inverse follow from
 //   L: X_ij = -D_i sum_{k=j}^{i-1} T_ik X_kj   (i > j, i ascending)
 //   U: X_ij = -D_i sum_{k=i+1}^{j} T_ik X_kj   (i < j, i descending)
 // with 64 x 64 tile products in LDS; the other triangle is zeroed.  Rows and
@@ -1158,7 +1168,7 @@ __device__ void tile_inv64(const cx<float> (*T)[65], cx<float> (*X)[65], cx<floa
 
 template <typename R, bool UPPER>
 __device__ void tri_inv_block(const cx<R> *__restrict__ F, long long ldf, int N, cx<R> *__restrict__ Dinv,
-                              cx<R> (*Ta)[65], cx<R> (*Tb)[65], cx<float> (*Tc_lds)[65]) {
+                              cx<R> (*Ta)[65], cx<R> (*Tb)[65]) {
     const int blk = blockIdx.x, r0 = blk * kDinvB, tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     cx<R> *X = Dinv + (size_t)blk * kDinvB * kDinvB;  // row-major 256 x 256
@@ -1202,33 +1212,29 @@ __device__ void tri_inv_block(const cx<R> *__restrict__ F, long long ldf, int N,
     for (int step = 0; step < 4; ++step) {
         const int i = UPPER ? 3 - step : step;
         load_T(Ta, i, i);
-        if constexpr (sizeof(R) == 4) {
-            tile_inv64<UPPER>(Ta, Tb, Tc_lds, tid, 256);
-        } else {
-            // FP64: lane t solves column t of the tile's inverse (LDS holds two tiles only)
-            for (int e = tid; e < 64 * 64; e += 256) Tb[e >> 6][e & 63] = {R(0), R(0)};
-            __syncthreads();
-            if (tid < 64) {
-                const int t = tid;
-                for (int q = 0; q < 64; ++q) {
-                    const int r = UPPER ? 63 - q : q;
-                    cx<R> acc = {r == t ? R(1) : R(0), R(0)};
-                    const int kb = UPPER ? r + 1 : 0, ke = UPPER ? 64 : r;
-                    for (int k = kb; k < ke; ++k) {
-                        const cx<R> t_rk = Ta[r][k], x = Tb[k][t];
-                        acc.re -= t_rk.re * x.re - t_rk.im * x.im;
-                        acc.im -= t_rk.re * x.im + t_rk.im * x.re;
-                    }
-                    if (UPPER) {
-                        const cx<R> d = Ta[r][r];
-                        const R den = d.re * d.re + d.im * d.im;
-                        acc = den > R(0) ? cmul(acc, cx<R>{d.re / den, -d.im / den}) : cx<R>{R(0), R(0)};
-                    }
-                    Tb[r][t] = acc;
+        // lane t solves column t of the tile's inverse (LDS holds two FP64 tiles only)
+        for (int e = tid; e < 64 * 64; e += 256) Tb[e >> 6][e & 63] = {R(0), R(0)};
+        __syncthreads();
+        if (tid < 64) {
+            const int t = tid;
+            for (int q = 0; q < 64; ++q) {
+                const int r = UPPER ? 63 - q : q;
+                cx<R> acc = {r == t ? R(1) : R(0), R(0)};
+                const int kb = UPPER ? r + 1 : 0, ke = UPPER ? 64 : r;
+                for (int k = kb; k < ke; ++k) {
+                    const cx<R> t_rk = Ta[r][k], x = Tb[k][t];
+                    acc.re -= t_rk.re * x.re - t_rk.im * x.im;
+                    acc.im -= t_rk.re * x.im + t_rk.im * x.re;
                 }
+                if (UPPER) {
+                    const cx<R> d = Ta[r][r];
+                    const R den = d.re * d.re + d.im * d.im;
+                    acc = den > R(0) ? cmul(acc, cx<R>{d.re / den, -d.im / den}) : cx<R>{R(0), R(0)};
+                }
+                Tb[r][t] = acc;
             }
-            __syncthreads();
         }
+        __syncthreads();
         for (int e = tid; e < 64 * 64; e += 256) X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * i + (e & 63)] = Tb[e >> 6][e & 63];
         // the other triangle of row tile i is zero
         for (int j = 0; j < 4; ++j)
@@ -1272,12 +1278,101 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_tri_inv(const cx<R> *__restrict__ F, long long ldf, int N,
                                                  cx<R> *__restrict__ Dinv) {
     __shared__ cx<R> Ta[64][65], Tb[64][65];
-    // staging for the doubling steps of the diagonal tiles (FP32 only)
-    __shared__ cx<float> Tc_lds[sizeof(R) == 4 ? 64 : 1][65];
     if (blockIdx.y == 0)
-        tri_inv_block<R, false>(F, ldf, N, Dinv, Ta, Tb, Tc_lds);
+        tri_inv_block<R, false>(F, ldf, N, Dinv, Ta, Tb);
     else
-        tri_inv_block<R, true>(F, ldf, N, Dinv + (size_t)gridDim.x * kDinvB * kDinvB, Ta, Tb, Tc_lds);
+        tri_inv_block<R, true>(F, ldf, N, Dinv + (size_t)gridDim.x * kDinvB * kDinvB, Ta, Tb);
+}
+
+// FP32 diagonal-block inverses spread over the chip.  k_dinv_diag inverts
+// every 64 x 64 diagonal tile (one workgroup per tile and triangle, grid
+// (blocks * 4, 2), blockIdx.y 0: unit lower, 1: upper) and zeroes the other
+// triangle of the tile's row; k_dinv_off(d), d = 1, 2, 3, forms the tiles on
+// the d-th sub-diagonal (L) or super-diagonal (U) of every 256-block, one
+// workgroup per tile (grid (blocks * (4 - d), 2)):
+//   L: X_{i,i-d} = -X_ii sum_{k=i-d}^{i-1} L_ik X_{k,i-d}
+//   U: X_{i,i+d} = -X_ii sum_{k=i+1}^{i+d} U_ik X_{k,i+d}
+// so each launch reads only tiles written by the earlier ones.  Layout as
+// k_tri_inv: L inverses at Dinv, U inverses at Dinv + blocks * 256 * 256,
+// row-major 256 x 256 per block; rows and columns past N act as identity.
+__device__ __forceinline__ void dinv_load_F(cx<float> (*T)[65], const cx<float> *__restrict__ F, long long ldf,
+                                            int N, int gr0, int gc0, int tid) {
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int rr = e >> 6, cc = e & 63, gr = gr0 + rr, gc = gc0 + cc;
+        cx<float> v = {0.f, 0.f};
+        if (gr < N && gc < N)
+            v = F[(size_t)gr * ldf + gc];
+        else if (gr == gc)
+            v = {1.f, 0.f};
+        T[rr][cc] = v;
+    }
+}
+
+__device__ __forceinline__ void dinv_load_X(cx<float> (*T)[65], const cx<float> *__restrict__ X, int i, int j,
+                                            int tid) {
+    for (int e = tid; e < 64 * 64; e += 256) T[e >> 6][e & 63] = X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * j + (e & 63)];
+}
+
+__global__ __launch_bounds__(256) void k_dinv_diag(const cx<float> *__restrict__ F, long long ldf, int N,
+                                                   cx<float> *__restrict__ Dinv, int nblk) {
+    __shared__ cx<float> Ta[64][65], Tb[64][65], Tc[64][65];
+    const int blk = blockIdx.x >> 2, i = blockIdx.x & 3, tid = threadIdx.x;
+    const bool upper = blockIdx.y != 0;
+    cx<float> *X = Dinv + (upper ? (size_t)nblk * kDinvB * kDinvB : 0) + (size_t)blk * kDinvB * kDinvB;
+    const int r0 = blk * kDinvB + 64 * i;
+    dinv_load_F(Ta, F, ldf, N, r0, r0, tid);
+    __syncthreads();
+    if (upper)
+        tile_inv64<true>(Ta, Tb, Tc, tid, 256);
+    else
+        tile_inv64<false>(Ta, Tb, Tc, tid, 256);
+    for (int e = tid; e < 64 * 64; e += 256) X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * i + (e & 63)] = Tb[e >> 6][e & 63];
+    for (int j = 0; j < 4; ++j)
+        if (upper ? j < i : j > i)
+            for (int e = tid; e < 64 * 64; e += 256)
+                X[(size_t)(64 * i + (e >> 6)) * kDinvB + 64 * j + (e & 63)] = {0.f, 0.f};
+}
+
+__global__ __launch_bounds__(256) void k_dinv_off(const cx<float> *__restrict__ F, long long ldf, int N,
+                                                  cx<float> *__restrict__ Dinv, int nblk, int d) {
+    __shared__ cx<float> Ta[64][65], Tb[64][65];
+    const int per = 4 - d, blk = blockIdx.x / per, t = blockIdx.x % per, tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    const bool upper = blockIdx.y != 0;
+    cx<float> *X = Dinv + (upper ? (size_t)nblk * kDinvB * kDinvB : 0) + (size_t)blk * kDinvB * kDinvB;
+    const int r0 = blk * kDinvB;
+    const int i = upper ? t : d + t, j = upper ? i + d : i - d;
+    const int kb = upper ? i + 1 : j, ke = upper ? j : i - 1;  // inclusive
+    TileAcc<float> acc;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc.v[a][b] = {0.f, 0.f};
+    for (int k = kb; k <= ke; ++k) {
+        if (k != kb) __syncthreads();
+        dinv_load_F(Ta, F, ldf, N, r0 + 64 * i, r0 + 64 * k, tid);
+        dinv_load_X(Tb, X, k, j, tid);
+        __syncthreads();
+        tile_mac(acc, Ta, Tb, ty, tx);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) Tb[ty * 4 + a][tx + 16 * b] = acc.v[a][b];
+    dinv_load_X(Ta, X, i, i, tid);
+    __syncthreads();
+    TileAcc<float> out;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) out.v[a][b] = {0.f, 0.f};
+    tile_mac(out, Ta, Tb, ty, tx);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            X[(size_t)(64 * i + ty * 4 + a) * kDinvB + 64 * j + tx + 16 * b] = {-out.v[a][b].re, -out.v[a][b].im};
 }
 
 // Y_b = Dinv_b W_b for one 256-row block (nb valid rows, m columns): 32 x 32
@@ -1613,11 +1708,11 @@ __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__
 // k_diag_lu.  32 rows x w columns per workgroup (lane tx: column, rows
 // ty * 8 + a), in place: a workgroup reads all of its rows before writing.
 template <typename R>
-__global__ __launch_bounds__(256) void k_l21(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
-                                            const cx<R> *__restrict__ uinv, int *flags, R tau2) {
-    __shared__ cx<R> Us[kNoPivW][kNoPivW + 1], As[32][kNoPivW + 1];
+__device__ __forceinline__ void l21_body(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
+                                         const cx<R> *__restrict__ uinv, int *flags, R tau2, int bx,
+                                         cx<R> (*Us)[kNoPivW + 1], cx<R> (*As)[kNoPivW + 1]) {
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    const int base = r0 + blockIdx.x * 32, nrow = min(32, r1 - base);
+    const int base = r0 + bx * 32, nrow = min(32, r1 - base);
     for (int e = tid; e < kNoPivW * kNoPivW; e += 256) {
         const int k = e >> 6, c = e & 63;
         Us[k][c] = (k < w && c < w) ? uinv[k * kNoPivW + c] : cx<R>{R(0), R(0)};
@@ -1651,6 +1746,31 @@ __global__ __launch_bounds__(256) void k_l21(cx<R> *__restrict__ A, long long ld
         }
     }
     if (bad) flags[2] = 1;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_l21(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
+                                            const cx<R> *__restrict__ uinv, int *flags, R tau2) {
+    __shared__ cx<R> Us[kNoPivW][kNoPivW + 1], As[32][kNoPivW + 1];
+    l21_body<R>(A, lda, js, w, r0, r1, uinv, flags, tau2, blockIdx.x, Us, As);
+}
+
+// The two products that follow a pivot-free diagonal block, in one launch:
+// workgroups [0, nl21) form L21 rows r0..r1-1 (as k_l21), the rest solve the
+// block's U rows for panel columns c0..c1-1 (as k_trsm64 with a unit-lower
+// 64-row block, writing Bt for the in-panel update).  The two touch disjoint
+// parts of A and only read the diagonal block, so they run side by side.
+template <typename R>
+__global__ __launch_bounds__(256) void k_l21_u12(cx<R> *__restrict__ A, long long lda, int js, int w, int r0,
+                                                int r1, const cx<R> *__restrict__ uinv, int *flags, R tau2,
+                                                int nl21, int c0, int c1, R *__restrict__ Bt, long long ldbt,
+                                                int passes) {
+    __shared__ cx<R> smem[(kNoPivW + 32) * (kNoPivW + 1)];
+    auto *T0 = (cx<R>(*)[kNoPivW + 1])smem;
+    if ((int)blockIdx.x < nl21)
+        l21_body<R>(A, lda, js, w, r0, r1, uinv, flags, tau2, blockIdx.x, T0, T0 + kNoPivW);
+    else
+        trsm64_body<R>(A, lda, A, lda, js, w, c0, c1, 0, Bt, ldbt, 0, passes, blockIdx.x - nl21, T0);
 }
 
 // Apply the interchanges k0..k1-1 to columns [c0,c1) and [c2,c3).
@@ -1771,20 +1891,27 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         int js = j0;
         while (nopiv && js < j0 + jb) {
             const int w = std::min(std::min(plan.w, kNoPivW), j0 + jb - js);
+            const bool right = js + w < j0 + jb;  // panel columns right of the block
             if constexpr (sizeof(R) == 4) {
                 ProfScope ps(ctx, "lu_panel", 0);
                 k_diag_lu<R><<<1, kDiagWaves * 64, 0, ctx->stream>>>(A, lda, js, w, ipiv, ctx->flags, tau2, uinv);
-                if (N - js - w > 0)
-                    k_l21<R><<<(N - js - w + 31) / 32, 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv,
-                                                                               ctx->flags, tau2);
+                const int nl21 = (N - js - w + 31) / 32;
+                if (right) {
+                    // L21 and the block's U rows (one 64-row step of trsm_blocked) in one launch
+                    const int c0 = js + w, c1 = j0 + jb, ncols = c1 - c0;
+                    const int passes = std::max(1, std::min(8, ncols / (kTrsmColsPerPass * 1024)));
+                    const int cpw = kTrsmColsPerPass * passes, nt = (ncols + cpw - 1) / cpw;
+                    k_l21_u12<R><<<nl21 + nt, 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv, ctx->flags,
+                                                                     tau2, nl21, c0, c1, Bt_in, 2LL * w, passes);
+                } else if (nl21 > 0) {
+                    k_l21<R><<<nl21, 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv, ctx->flags, tau2);
+                }
                 EMFA_HIP_CHECK(hipGetLastError());
             }
-            if (js + w < j0 + jb) {
-                emfa_status s2 = trsm_blocked<R>(ctx, A, lda, A, lda, js, w, js + w, j0 + jb, 0, Bt_in, "lu_trsm");
-                if (s2) return s2;
-                s2 = gemm_update<R>(ctx, "lu_gemm", N - js - w, j0 + jb - js - w, w,
-                                    A + (size_t)(js + w) * lda + js, lda, Bt_in, 2LL * w,
-                                    A + (size_t)(js + w) * lda + js + w, lda);
+            if (right) {
+                emfa_status s2 = gemm_update<R>(ctx, "lu_gemm", N - js - w, j0 + jb - js - w, w,
+                                                A + (size_t)(js + w) * lda + js, lda, Bt_in, 2LL * w,
+                                                A + (size_t)(js + w) * lda + js + w, lda);
                 if (s2) return s2;
             }
             js += w;
@@ -2046,7 +2173,8 @@ static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
     if (lu->a->prec == EMFA_F32) {
         const cx<float> *F = (const cx<float> *)lu->a->d;
         cx<float> *D = (cx<float> *)lu->dinv;
-        k_tri_inv<float><<<dim3(nblk, 2), 256, 0, ctx->stream>>>(F, N, N, D);
+        k_dinv_diag<<<dim3(nblk * 4, 2), 256, 0, ctx->stream>>>(F, N, N, D, nblk);
+        for (int d = 1; d < 4; ++d) k_dinv_off<<<dim3(nblk * (4 - d), 2), 256, 0, ctx->stream>>>(F, N, N, D, nblk, d);
     } else {
         const cx<double> *F = (const cx<double> *)lu->a->d;
         cx<double> *D = (cx<double> *)lu->dinv;
