@@ -33,6 +33,7 @@ struct GemmArgs {
     int kslices = 1;        // FP64 launchers: split K over blockIdx.z (C += z * cslice)
     int kc = 0;             // reals of K per slice (multiple of 64)
     long long cslice = 0;   // C elements between slice planes
+    Batch bt;               // FP32 launchers: A and C in the batch's matrices, Bt in its scratch slabs
 };
 
 // Three-term bf16 product from pre-split operand planes.  Each operand is
@@ -52,12 +53,15 @@ struct Bf3Planes {
     int alpha_neg, beta;
     int max_wgs = 0;     // > 0: at most this many workgroups (rounded down to a multiple of 8), looping over tiles
     int persistent = 0;  // set by the launcher
+    Batch bt;            // C in the batch's matrices, the planes in its scratch slabs
 };
 
 // Split rows x cols FP32 (row stride ld, 8-byte aligned rows) into hi/lo bf16
 // planes of rows_p x cols_p (row stride ldp), zero outside rows x cols.
+// With a batch, hi / lo are scratch and src is a matrix (src_mat) or scratch.
 emfa_status split_bf16_launch(hipStream_t s, const float *src, long long ld, int rows, int cols, __bf16 *hi,
-                              __bf16 *lo, long long ldp, int rows_p, int cols_p);
+                              __bf16 *lo, long long ldp, int rows_p, int cols_p, const Batch &bt = Batch(),
+                              bool src_mat = true);
 emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g);
 
 emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g, int tag = 0);  // TA=float,  TC=float

@@ -96,9 +96,9 @@ __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
     using namespace f32k;
     __shared__ float As[2][BM][BK + PAD];
     __shared__ float Bs[2][BN][BK + PAD];
-    const float *__restrict__ A = (const float *)g.A;
-    const float *__restrict__ Bt = (const float *)g.Bt;
-    float *__restrict__ C = (float *)g.C;
+    const float *__restrict__ A = batch_mat(g.bt, (const float *)g.A);
+    const float *__restrict__ Bt = batch_scr(g.bt, (const float *)g.Bt);
+    float *__restrict__ C = batch_mat(g.bt, (float *)g.C);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int ty, tx;
@@ -201,9 +201,9 @@ __global__ __launch_bounds__(256) void k_gemm_f32s(GemmArgs g) {
     using namespace f32s;
     __shared__ float As[2][BM][BK + PAD];
     __shared__ float Bs[2][BN][BK + PAD];
-    const float *__restrict__ A = (const float *)g.A;
-    const float *__restrict__ Bt = (const float *)g.Bt;
-    float *__restrict__ C = (float *)g.C;
+    const float *__restrict__ A = batch_mat(g.bt, (const float *)g.A);
+    const float *__restrict__ Bt = batch_scr(g.bt, (const float *)g.Bt);
+    float *__restrict__ C = batch_mat(g.bt, (float *)g.C);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int ty, tx;
@@ -288,9 +288,9 @@ __global__ __launch_bounds__(256) void k_gemm_f32v(GemmArgs g) {
     constexpr int LOADS = BM * KP / 256;                       // float4 loads per thread per operand
     __shared__ float As[2][BM][BKV + PAD];
     __shared__ float Bs[2][BN][BKV + PAD];
-    const float *__restrict__ A = (const float *)g.A;
-    const float *__restrict__ Bt = (const float *)g.Bt;
-    float *__restrict__ C = (float *)g.C;
+    const float *__restrict__ A = batch_mat(g.bt, (const float *)g.A);
+    const float *__restrict__ Bt = batch_scr(g.bt, (const float *)g.Bt);
+    float *__restrict__ C = batch_mat(g.bt, (float *)g.C);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     int ty, tx;
@@ -398,7 +398,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ src, long long ld, int rows, int cols,
                                                     __bf16 *__restrict__ hi, __bf16 *__restrict__ lo, long long ldp,
-                                                    int rows_p, int cols_p) {
+                                                    int rows_p, int cols_p, Batch bt, int src_mat) {
+    src = src_mat ? batch_mat(bt, src) : batch_scr(bt, src);
+    hi = batch_scr(bt, hi);
+    lo = batch_scr(bt, lo);
     const int chunks = cols_p >> 3;
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long long)rows_p * chunks) return;
@@ -451,20 +454,22 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
     __shared__ __attribute__((aligned(16))) __bf16 sm[NS][4][PLANE];  // stage, plane (Ah Al Bh Bl)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int TX = (g.Nr + BN - 1) / BN, TY = (g.M + BM - 1) / BM;
+    const int TX = (g.Nr + BN - 1) / BN, TY = (g.M + BM - 1) / BM, TT = TX * TY;
     // one tile per workgroup, or (persistent grid) tiles blockIdx.x, + gridDim.x, ...
-    const int t0 = g.persistent ? (int)blockIdx.x : (int)(blockIdx.y * gridDim.x + blockIdx.x);
-    const int tstep = g.persistent ? (int)gridDim.x : TX * TY;
-    for (int tl = t0; tl < TX * TY; tl += tstep) {
+    // over the tiles of every matrix of the batch (matrix tl / TT)
+    const int t0 = g.persistent ? (int)blockIdx.x : (int)((blockIdx.z * TY + blockIdx.y) * gridDim.x + blockIdx.x);
+    const int tstep = g.persistent ? (int)gridDim.x : TT * g.bt.count;
+    for (int tl = t0; tl < TT * g.bt.count; tl += tstep) {
     if (tl != t0) __syncthreads();  // the previous tile's LDS reads are done
+    const int zb = tl / TT;
     int ty, tx;
-    tile_of_linear(tl, TX, TY, &ty, &tx);
+    tile_of_linear(tl - zb * TT, TX, TY, &ty, &tx);
     const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
     const int nk = g.Kp / BK;
 
     // this wave's direct loads: plane pl, rows half * 128 + 32 i + lane / 2
     const int pl = wave >> 1, half = wave & 1;
-    const __bf16 *src = pl == 0 ? g.Ah : pl == 1 ? g.Al : pl == 2 ? g.Bh : g.Bl;
+    const __bf16 *src = batch_scr(g.bt, pl == 0 ? g.Ah : pl == 1 ? g.Al : pl == 2 ? g.Bh : g.Bl, zb);
     const long long ld = pl < 2 ? g.lda : g.ldb;
     const long long row0 = pl < 2 ? bm : bn;
     const __bf16 *gp[4];
@@ -539,7 +544,7 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
     // epilogue: all C loads of a 32-row slab are issued before its stores;
     // tiles wholly inside C skip the per-element bounds checks
     const float sgn = g.alpha_neg ? -1.f : 1.f;
-    float *__restrict__ C = g.C + (bm + wm * 64 + 4 * (lane >> 5)) * g.ldc + bn + wn * 128 + r32;
+    float *__restrict__ C = batch_mat(g.bt, g.C, zb) + (bm + wm * 64 + 4 * (lane >> 5)) * g.ldc + bn + wn * 128 + r32;
     const int mrem = (int)min((long long)BM, g.M - bm) - wm * 64 - 4 * (lane >> 5);
     const int nrem = (int)min((long long)BN, (long long)g.Nr - bn) - wn * 128 - r32;
     const bool full = bm + BM <= g.M && bn + BN <= g.Nr;
@@ -717,8 +722,9 @@ static int swizzle_default() {
 emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
     GemmArgs g = g0;
     g.swizzle = swizzle_default();
-    EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
-    dim3 grid((g.Nr + f32k::BN - 1) / f32k::BN, (g.M + f32k::BM - 1) / f32k::BM);
+    EMFA_REQUIRE(gemm_args_ok(g) && g.bt.count >= 1 && g.bt.count <= 65535, EMFA_E_INVALID,
+                 "gemm f32: bad shape M=%d Nr=%d Kr=%d batch %d", g.M, g.Nr, g.Kr, g.bt.count);
+    dim3 grid((g.Nr + f32k::BN - 1) / f32k::BN, (g.M + f32k::BM - 1) / f32k::BM, g.bt.count);
     static const int variant = [] {
         const char *e = getenv("EMFA_GEMM_VARIANT");
         return e ? atoi(e) : 0;
@@ -731,10 +737,10 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
         const char *e = getenv("EMFA_GEMM_SMALL");
         return e && atoi(e) == 0 ? 0LL : 128LL;
     }();
-    const long long tiles = (long long)grid.x * grid.y;
+    const long long tiles = (long long)grid.x * grid.y * grid.z;
     // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart
     if (tiles < small_below && variant == 0) {
-        dim3 gs((g.Nr + f32s::BN - 1) / f32s::BN, (g.M + f32s::BM - 1) / f32s::BM);
+        dim3 gs((g.Nr + f32s::BN - 1) / f32s::BN, (g.M + f32s::BM - 1) / f32s::BM, g.bt.count);
         if (tag == 1)
             k_gemm_f32s<1><<<gs, 256, 0, s>>>(g);
         else
@@ -759,14 +765,15 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
 }
 
 emfa_status split_bf16_launch(hipStream_t s, const float *src, long long ld, int rows, int cols, __bf16 *hi,
-                              __bf16 *lo, long long ldp, int rows_p, int cols_p) {
+                              __bf16 *lo, long long ldp, int rows_p, int cols_p, const Batch &bt, bool src_mat) {
     EMFA_REQUIRE(rows >= 0 && cols >= 0 && rows <= rows_p && cols <= cols_p && cols_p % 8 == 0 && ldp >= cols_p &&
                      ldp % 8 == 0 && (ld % 2) == 0 && ((uintptr_t)src % 8) == 0 && ((uintptr_t)hi % 16) == 0 &&
                      ((uintptr_t)lo % 16) == 0,
                  EMFA_E_INVALID, "split_bf16: bad shape rows=%d/%d cols=%d/%d", rows, rows_p, cols, cols_p);
     const long long work = (long long)rows_p * (cols_p / 8);
     if (work == 0) return EMFA_OK;
-    k_split_bf16<<<(unsigned)((work + 255) / 256), 256, 0, s>>>(src, ld, rows, cols, hi, lo, ldp, rows_p, cols_p);
+    k_split_bf16<<<dim3((unsigned)((work + 255) / 256), 1, bt.count), 256, 0, s>>>(src, ld, rows, cols, hi, lo, ldp,
+                                                                                  rows_p, cols_p, bt, src_mat ? 1 : 0);
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
 }
@@ -778,13 +785,14 @@ emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g) {
                      g.M <= 65535LL * bf3p::BM && g.Nr <= 65535LL * bf3p::BN,
                  EMFA_E_INVALID, "gemm bf16x3 planes: bad shape M=%d Nr=%d Kp=%d", g.M, g.Nr, g.Kp);
     const int TX = (g.Nr + bf3p::BN - 1) / bf3p::BN, TY = (g.M + bf3p::BM - 1) / bf3p::BM;
+    EMFA_REQUIRE(g.bt.count >= 1 && g.bt.count <= 65535, EMFA_E_INVALID, "gemm bf16x3 planes: batch %d", g.bt.count);
     Bf3Planes gg = g;
-    if (g.max_wgs >= 8 && TX * TY > g.max_wgs) {
+    if (g.max_wgs >= 8 && (long long)TX * TY * g.bt.count > g.max_wgs) {
         gg.persistent = 1;
         k_gemm_bf3p<<<(g.max_wgs / 8) * 8, 512, 0, s>>>(gg);
     } else {
         gg.persistent = 0;
-        k_gemm_bf3p<<<dim3(TX, TY), 512, 0, s>>>(gg);
+        k_gemm_bf3p<<<dim3(TX, TY, g.bt.count), 512, 0, s>>>(gg);
     }
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;
@@ -793,6 +801,7 @@ emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g) {
 emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g0, int tag) {
     GemmArgs g = g0;
     g.swizzle = swizzle_default();
+    EMFA_REQUIRE(g.bt.count == 1 && !g.bt.mat, EMFA_E_INVALID, "gemm f64: batches are FP32 only");
     EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM, std::max(1, g.kslices));
     if (tag == 1)
@@ -806,7 +815,8 @@ emfa_status gemm_launch_f64(hipStream_t s, const GemmArgs &g0, int tag) {
 emfa_status gemm_launch_f32_f64(hipStream_t s, const GemmArgs &g0) {
     GemmArgs g = g0;
     g.swizzle = swizzle_default();
-    EMFA_REQUIRE(gemm_args_ok(g), EMFA_E_INVALID, "gemm f32/f64: bad shape M=%d Nr=%d Kr=%d", g.M, g.Nr, g.Kr);
+    EMFA_REQUIRE(gemm_args_ok(g) && g.bt.count == 1 && !g.bt.mat, EMFA_E_INVALID,
+                 "gemm f32/f64: bad shape M=%d Nr=%d Kr=%d (or a batch)", g.M, g.Nr, g.Kr);
     dim3 grid((g.Nr + f64k::BN - 1) / f64k::BN, (g.M + f64k::BM - 1) / f64k::BM, std::max(1, g.kslices));
     k_gemm_f64<float, 0><<<grid, 256, 0, s>>>(g);
     EMFA_HIP_CHECK(hipGetLastError());

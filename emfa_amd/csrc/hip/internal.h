@@ -46,6 +46,40 @@ struct PendingEvent {
     double work;
 };
 
+// A batch of same-size matrices factored by one launch sequence: kernels take
+// matrix z on blockIdx.z.  Pointer arithmetic on the host starts from matrix
+// 0 (mat0); a kernel moves a matrix pointer onto matrix z's base (same offset)
+// and a scratch pointer by z slabs (every matrix owns one scratch slab of the
+// same layout).  The default value is no batch: both moves are identities.
+struct Batch {
+    const char *const *mat = nullptr;  // device array: base of matrix z
+    const char *mat0 = nullptr;        // base of matrix 0
+    long long scratch = 0;             // bytes between consecutive scratch slabs
+    int count = 1;
+};
+
+template <typename T>
+__device__ __forceinline__ T *batch_mat(const Batch &b, T *p, int z) {
+    if (!b.mat) return p;
+    return (T *)(b.mat[z] + ((const char *)p - b.mat0));
+}
+
+template <typename T>
+__device__ __forceinline__ T *batch_scr(const Batch &b, T *p, int z) {
+    return (T *)((char *)p + (long long)z * b.scratch);
+}
+
+// matrix / scratch pointers of the matrix on blockIdx.z
+template <typename T>
+__device__ __forceinline__ T *batch_mat(const Batch &b, T *p) {
+    return batch_mat(b, p, (int)blockIdx.z);
+}
+
+template <typename T>
+__device__ __forceinline__ T *batch_scr(const Batch &b, T *p) {
+    return batch_scr(b, p, (int)blockIdx.z);
+}
+
 }  // namespace emfa
 
 struct emfa_ctx {
@@ -68,6 +102,7 @@ struct emfa_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int *flags = nullptr;  // [0] spin-timeout flag, [1] singular info, [2] pivot-free growth
+    emfa::Batch batch;     // set while a batch of factorisations runs (the LU helpers launch over it)
 };
 
 struct emfa_mesh {

@@ -627,12 +627,15 @@ __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long lo
     }
 }
 
+// With a batch, F and X are in the batch's matrices and Bt in its scratch.
 template <typename R>
 __global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
                                                 long long ldx, int r0, int wb, int c0, int c1, int upper,
-                                                R *__restrict__ Bt, long long ldbt, int kofs, int passes) {
+                                                R *__restrict__ Bt, long long ldbt, int kofs, int passes,
+                                                Batch bt = Batch()) {
     __shared__ cx<R> Lt[64][65];
-    trsm64_body<R>(F, ldf, X, ldx, r0, wb, c0, c1, upper, Bt, ldbt, kofs, passes, blockIdx.x, Lt);
+    trsm64_body<R>(batch_mat(bt, F), ldf, batch_mat(bt, X), ldx, r0, wb, c0, c1, upper, Bt ? batch_scr(bt, Bt) : Bt,
+                   ldbt, kofs, passes, blockIdx.x, Lt);
 }
 
 // --------------------------------------------------------------------------
@@ -882,7 +885,8 @@ __global__ __launch_bounds__(256) void k_expand(const cx<R> *__restrict__ X, int
     b1[1] = v.re;
 }
 
-__global__ void k_zero_u32(unsigned *p, int n) {
+__global__ void k_zero_u32(unsigned *p, int n, Batch bt = Batch()) {
+    p = batch_scr(bt, p);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
@@ -930,6 +934,7 @@ static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols,
     g.ldc = 2 * ldc;
     g.alpha_neg = 1;
     g.beta = 1;
+    g.bt = ctx->batch;
     ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K, s);
     const int tag = std::strcmp(fam, "lu_gemm") == 0 ? 1 : 0;
     if constexpr (sizeof(R) == 4)
@@ -970,10 +975,10 @@ struct UpdatePlanes {
         ProfScope ps(ctx, "lu_split", 0);
         const int kr = 2 * K, k16 = rup(kr, 16);
         emfa_status st = split_bf16_launch(ctx->stream, (const float *)Aop, 2 * lda, M, kr, ah, al, kp,
-                                           rup(M, 256), k16);
+                                           rup(M, 256), k16, ctx->batch, true);
         if (st) return st;
         return split_bf16_launch(ctx->stream, Bt, ldb, 2 * ncols, kr, bh, bl, kp,
-                                 std::min(b_rows, rup(2 * ncols, 256) + 256), k16);
+                                 std::min(b_rows, rup(2 * ncols, 256) + 256), k16, ctx->batch, false);
     }
     // C (M x ncols complex) -= L * U[:, c0 : c0 + ncols]
     emfa_status update(emfa_ctx *ctx, int M, int c0, int ncols, int K, cx<float> *C, long long ldc,
@@ -994,6 +999,7 @@ struct UpdatePlanes {
         g.ldc = 2 * ldc;
         g.alpha_neg = 1;
         g.beta = 1;
+        g.bt = ctx->batch;
         // contexts sharing the device keep each update to a multiple of their CU share
         if (ctx->cu_share > 0) {
             int cap = 1;
@@ -1023,7 +1029,9 @@ static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx
             int b = std::min(64, wb - s);
             {
                 ProfScope ps(ctx, fam, 0);
-                k_trsm64<R><<<blocks, 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0, c1, 0, Bt, ldbt, s, passes);
+                k_trsm64<R><<<dim3(blocks, 1, ctx->batch.count), 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0,
+                                                                                       c1, 0, Bt, ldbt, s, passes,
+                                                                                       ctx->batch);
                 EMFA_HIP_CHECK(hipGetLastError());
             }
             if (s + b < wb) {
@@ -1039,7 +1047,9 @@ static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx
             int b = std::min(64, wb - s);
             {
                 ProfScope ps(ctx, fam, 0);
-                k_trsm64<R><<<blocks, 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0, c1, 1, Bt, ldbt, s, passes);
+                k_trsm64<R><<<dim3(blocks, 1, ctx->batch.count), 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0,
+                                                                                       c1, 1, Bt, ldbt, s, passes,
+                                                                                       ctx->batch);
                 EMFA_HIP_CHECK(hipGetLastError());
             }
             if (s > 0) {
@@ -1314,8 +1324,10 @@ __device__ __forceinline__ void dinv_load_X(cx<float> (*T)[65], const cx<float> 
 }
 
 __global__ __launch_bounds__(256) void k_dinv_diag(const cx<float> *__restrict__ F, long long ldf, int N,
-                                                   cx<float> *__restrict__ Dinv, int nblk) {
+                                                   cx<float> *__restrict__ Dinv, int nblk, Batch bf, Batch bd) {
     __shared__ cx<float> Ta[64][65], Tb[64][65], Tc[64][65];
+    F = batch_mat(bf, F);  // a batch: factor z and its inverses on blockIdx.z
+    Dinv = batch_mat(bd, Dinv);
     const int blk = blockIdx.x >> 2, i = blockIdx.x & 3, tid = threadIdx.x;
     const bool upper = blockIdx.y != 0;
     cx<float> *X = Dinv + (upper ? (size_t)nblk * kDinvB * kDinvB : 0) + (size_t)blk * kDinvB * kDinvB;
@@ -1334,8 +1346,10 @@ __global__ __launch_bounds__(256) void k_dinv_diag(const cx<float> *__restrict__
 }
 
 __global__ __launch_bounds__(256) void k_dinv_off(const cx<float> *__restrict__ F, long long ldf, int N,
-                                                  cx<float> *__restrict__ Dinv, int nblk, int d) {
+                                                  cx<float> *__restrict__ Dinv, int nblk, int d, Batch bf, Batch bd) {
     __shared__ cx<float> Ta[64][65], Tb[64][65];
+    F = batch_mat(bf, F);
+    Dinv = batch_mat(bd, Dinv);
     const int per = 4 - d, blk = blockIdx.x / per, t = blockIdx.x % per, tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     const bool upper = blockIdx.y != 0;
@@ -1624,7 +1638,12 @@ __device__ __forceinline__ void cfms(cx<R> &acc, const cx<R> &a, const cx<R> &b)
 template <typename R>
 __global__ __launch_bounds__(kDiagWaves * 64) void k_diag_lu(cx<R> *__restrict__ A, long long lda, int js, int w,
                                                             int *__restrict__ ipiv, int *flags, R tau2,
-                                                            cx<R> *__restrict__ uinv) {
+                                                            cx<R> *__restrict__ uinv, Batch bt) {
+    // the matrix, its (scratch) pivots, flags and U11^-1 for the batch's matrix z
+    A = batch_mat(bt, A);
+    ipiv = batch_scr(bt, ipiv);
+    flags = batch_scr(bt, flags);
+    uinv = batch_scr(bt, uinv);
     static_assert(sizeof(R) == 4, "the pivot-free diagonal block is an FP32 kernel");
     __shared__ cx<R> buf[2][kDiagCpw][kNoPivW];
     __shared__ cx<float> Tu[kNoPivW][kNoPivW + 1], Tx[kNoPivW][kNoPivW + 1], Tc[kNoPivW][kNoPivW + 1];
@@ -1750,9 +1769,10 @@ __device__ __forceinline__ void l21_body(cx<R> *__restrict__ A, long long lda, i
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_l21(cx<R> *__restrict__ A, long long lda, int js, int w, int r0, int r1,
-                                            const cx<R> *__restrict__ uinv, int *flags, R tau2) {
+                                            const cx<R> *__restrict__ uinv, int *flags, R tau2, Batch bt) {
     __shared__ cx<R> Us[kNoPivW][kNoPivW + 1], As[32][kNoPivW + 1];
-    l21_body<R>(A, lda, js, w, r0, r1, uinv, flags, tau2, blockIdx.x, Us, As);
+    l21_body<R>(batch_mat(bt, A), lda, js, w, r0, r1, batch_scr(bt, uinv), batch_scr(bt, flags), tau2, blockIdx.x,
+                Us, As);
 }
 
 // The two products that follow a pivot-free diagonal block, in one launch:
@@ -1764,8 +1784,12 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_l21_u12(cx<R> *__restrict__ A, long long lda, int js, int w, int r0,
                                                 int r1, const cx<R> *__restrict__ uinv, int *flags, R tau2,
                                                 int nl21, int c0, int c1, R *__restrict__ Bt, long long ldbt,
-                                                int passes) {
+                                                int passes, Batch bt) {
     __shared__ cx<R> smem[(kNoPivW + 32) * (kNoPivW + 1)];
+    A = batch_mat(bt, A);
+    uinv = batch_scr(bt, uinv);
+    flags = batch_scr(bt, flags);
+    Bt = batch_scr(bt, Bt);
     auto *T0 = (cx<R>(*)[kNoPivW + 1])smem;
     if ((int)blockIdx.x < nl21)
         l21_body<R>(A, lda, js, w, r0, r1, uinv, flags, tau2, blockIdx.x, T0, T0 + kNoPivW);
@@ -1794,8 +1818,14 @@ static emfa_status laswp_launch(emfa_ctx *ctx, cx<R> *A, long long lda, const in
 }
 
 template <typename R>
-static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out, int *interchanges) {
+// With ctx->batch set (count matrices of Am's size, matrix 0 = Am), the
+// pivot-free FP32 path with the split update factors all of them with one
+// launch sequence; growth[z] then reports matrix z's growth flag and the
+// call itself fails only on errors that concern the whole batch.
+static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out, int *interchanges,
+                           int *growth = nullptr) {
     const int N = Am->rows;
+    const bool batched = ctx->batch.mat != nullptr;
     cx<R> *A = (cx<R> *)Am->d;
     const long long lda = N;
     LuPlan<R> plan;
@@ -1832,9 +1862,16 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     const size_t uinv_b = (size_t)kNoPivW * kNoPivW * sizeof(cx<R>);
     size_t total = slot_rows_b + slot_val_b + slot_idx_b + rowc_b + bar_b + 2 * bt_b + perm_b + x_pk_b + x_rw_b +
                    x_cw_b + x_val_b + planes_b + uinv_b + 8192;
+    // a batch: one slab of this layout per matrix (plus its flags and pivots)
+    const size_t slab = batched ? (total + 256 + (size_t)N * sizeof(int) + 4096 + 4095) & ~(size_t)4095 : total;
+    if (batched) {
+        EMFA_REQUIRE(sizeof(R) == 4 && split_update && ctx->batch.count >= 1, EMFA_E_INVALID,
+                     "LU batch: FP32 with the split update only");
+    }
     void *scr = nullptr;
-    emfa_status st = ctx_scratch(ctx, total, &scr);
+    emfa_status st = ctx_scratch(ctx, slab * (batched ? ctx->batch.count : 1), &scr);
     if (st != EMFA_OK) return st;
+    if (batched) ctx->batch.scratch = (long long)slab;
     char *p = (char *)scr;
     auto take = [&](size_t b) {
         char *q = p;
@@ -1856,13 +1893,16 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     UpdatePlanes planes;
     if (split_update) planes.carve(take(planes_b), N, nb);
     cx<R> *uinv = (cx<R> *)take(uinv_b);  // U11^-1 of the current pivot-free panel
+    // growth flags and pivots: the context's for one matrix, each slab's for a batch
+    int *flagsp = batched ? (int *)take(256) : ctx->flags;
+    int *ipivp = batched ? (int *)take((size_t)N * sizeof(int)) : ipiv;
     {
         const int nw = (int)((x_pk_b + x_rw_b + x_cw_b) / 4);
         k_zero_u32<<<(nw + 255) / 256, 256, 0, ctx->stream>>>((unsigned *)x_pk, nw);
     }
     k_zero_u32<<<1, 64, 0, ctx->stream>>>(bar, 64);
     EMFA_HIP_CHECK(hipGetLastError());
-    k_zero_u32<<<1, 64, 0, ctx->stream>>>((unsigned *)ctx->flags, 16);
+    k_zero_u32<<<dim3(1, 1, ctx->batch.count), 64, 0, ctx->stream>>>((unsigned *)flagsp, 16, ctx->batch);
     EMFA_HIP_CHECK(hipGetLastError());
     unsigned bar_base = 0;
     // Cooperative launches of concurrent contexts are serialised by the
@@ -1883,6 +1923,7 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         if (const char *s = getenv("EMFA_LU_NOPIV")) nopiv = atoi(s) != 0 && !ctx->force_pivoting;
     }
     if (interchanges) *interchanges = nopiv ? 0 : 1;
+    EMFA_REQUIRE(!batched || nopiv, EMFA_E_INVALID, "LU batch: pivot-free panels only");
     const R tau2 = R(kNoPivGrowth) * R(kNoPivGrowth);
 
     // Factor the outer panel [j0, j0+jb): inner panels of width w with the
@@ -1894,17 +1935,21 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             const bool right = js + w < j0 + jb;  // panel columns right of the block
             if constexpr (sizeof(R) == 4) {
                 ProfScope ps(ctx, "lu_panel", 0);
-                k_diag_lu<R><<<1, kDiagWaves * 64, 0, ctx->stream>>>(A, lda, js, w, ipiv, ctx->flags, tau2, uinv);
+                const dim3 g1(1, 1, ctx->batch.count);
+                k_diag_lu<R><<<g1, kDiagWaves * 64, 0, ctx->stream>>>(A, lda, js, w, ipivp, flagsp, tau2, uinv,
+                                                                      ctx->batch);
                 const int nl21 = (N - js - w + 31) / 32;
                 if (right) {
                     // L21 and the block's U rows (one 64-row step of trsm_blocked) in one launch
                     const int c0 = js + w, c1 = j0 + jb, ncols = c1 - c0;
                     const int passes = std::max(1, std::min(8, ncols / (kTrsmColsPerPass * 1024)));
                     const int cpw = kTrsmColsPerPass * passes, nt = (ncols + cpw - 1) / cpw;
-                    k_l21_u12<R><<<nl21 + nt, 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv, ctx->flags,
-                                                                     tau2, nl21, c0, c1, Bt_in, 2LL * w, passes);
+                    k_l21_u12<R><<<dim3(nl21 + nt, 1, ctx->batch.count), 256, 0, ctx->stream>>>(
+                        A, lda, js, w, js + w, N, uinv, flagsp, tau2, nl21, c0, c1, Bt_in, 2LL * w, passes,
+                        ctx->batch);
                 } else if (nl21 > 0) {
-                    k_l21<R><<<nl21, 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv, ctx->flags, tau2);
+                    k_l21<R><<<dim3(nl21, 1, ctx->batch.count), 256, 0, ctx->stream>>>(A, lda, js, w, js + w, N, uinv,
+                                                                                        flagsp, tau2, ctx->batch);
                 }
                 EMFA_HIP_CHECK(hipGetLastError());
             }
@@ -2014,6 +2059,18 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             if ((st = factor_outer(next, nbn))) return st;
         }
         if ((st = outer_swap_trsm(next, nbn))) return st;
+    }
+    if (batched) {
+        // per-matrix growth flags from each slab
+        std::vector<int> hf(3 * (size_t)ctx->batch.count, 0);
+        for (int z = 0; z < ctx->batch.count; ++z)
+            EMFA_HIP_CHECK(hipMemcpyAsync(&hf[3 * (size_t)z], (const char *)flagsp + (size_t)z * slab, 3 * sizeof(int),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (int z = 0; z < ctx->batch.count; ++z)
+            if (growth) growth[z] = hf[3 * (size_t)z + 2] != 0;
+        *info_out = 0;
+        return EMFA_OK;
     }
     int flags[3] = {0, 0, 0};
     EMFA_HIP_CHECK(hipMemcpyAsync(flags, ctx->flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
@@ -2159,6 +2216,13 @@ emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B) {
 }
 
 // Inverses of the diagonal blocks of L (unit) and U, both [nblk][256][256]
+// FP32 diagonal-block inverses of one factor, or (bf, bd) of a batch of factors
+static void dinv_launch_f32(hipStream_t s, const cx<float> *F, int N, cx<float> *D, int nblk, const Batch &bf,
+                            const Batch &bd) {
+    k_dinv_diag<<<dim3(nblk * 4, 2, bf.count), 256, 0, s>>>(F, N, N, D, nblk, bf, bd);
+    for (int d = 1; d < 4; ++d) k_dinv_off<<<dim3(nblk * (4 - d), 2, bf.count), 256, 0, s>>>(F, N, N, D, nblk, d, bf, bd);
+}
+
 static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
     const int N = lu->a->rows, nblk = (N + kDinvB - 1) / kDinvB;
     const size_t e = lu->a->prec == EMFA_F32 ? sizeof(cx<float>) : sizeof(cx<double>);
@@ -2173,8 +2237,7 @@ static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
     if (lu->a->prec == EMFA_F32) {
         const cx<float> *F = (const cx<float> *)lu->a->d;
         cx<float> *D = (cx<float> *)lu->dinv;
-        k_dinv_diag<<<dim3(nblk * 4, 2), 256, 0, ctx->stream>>>(F, N, N, D, nblk);
-        for (int d = 1; d < 4; ++d) k_dinv_off<<<dim3(nblk * (4 - d), 2), 256, 0, ctx->stream>>>(F, N, N, D, nblk, d);
+        dinv_launch_f32(ctx->stream, F, N, D, nblk, Batch(), Batch());
     } else {
         const cx<double> *F = (const cx<double> *)lu->a->d;
         cx<double> *D = (cx<double> *)lu->dinv;
@@ -2293,6 +2356,33 @@ using namespace emfa;
 
 extern "C" {
 
+// Compose the interchanges of lu->ipiv into one row permutation (used by the
+// solves): row i of P A is row perm[i] of A.
+static emfa_status compose_perm(emfa_ctx *ctx, emfa_lu *lu) {
+    const int n = lu->a->rows;
+    std::vector<int> piv(n), perm(n);
+    if (hipMemcpy(piv.data(), lu->ipiv, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        (!lu->perm && ctx_alloc(ctx, (void **)&lu->perm, sizeof(int) * n) != hipSuccess)) {
+        (void)hipGetLastError();
+        set_error("emfa_lu_factor: permutation setup failed");
+        return EMFA_E_HIP;
+    }
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const int p = piv[i];
+        if (p < 0 || p >= n) {
+            set_error("emfa_lu_factor: pivot %d out of range at column %d", p, i);
+            return EMFA_E_HIP;
+        }
+        std::swap(perm[i], perm[p]);
+    }
+    if (hipMemcpy(lu->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("emfa_lu_factor: permutation upload failed");
+        return EMFA_E_HIP;
+    }
+    return EMFA_OK;
+}
+
 emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu **out) {
     EMFA_REQUIRE(ctx && Z && out, EMFA_E_INVALID, "emfa_lu_factor: null argument");
     EMFA_REQUIRE(Z->rows == Z->cols, EMFA_E_INVALID, "emfa_lu_factor: matrix is %dx%d, not square", Z->rows,
@@ -2324,32 +2414,7 @@ emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu *
         ctx->force_pivoting = 0;
         lu->pivot_fallback = 1;
     }
-    if (!st) {
-        // compose the interchanges into one row permutation (used by the solves)
-        const int n = Z->rows;
-        std::vector<int> piv(n), perm(n);
-        if (hipMemcpy(piv.data(), lu->ipiv, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess ||
-            ctx_alloc(ctx, (void **)&lu->perm, sizeof(int) * n) != hipSuccess) {
-            (void)hipGetLastError();
-            set_error("emfa_lu_factor: permutation setup failed");
-            st = EMFA_E_HIP;
-        } else {
-            for (int i = 0; i < n; ++i) perm[i] = i;
-            for (int i = 0; i < n; ++i) {
-                const int p = piv[i];
-                if (p < 0 || p >= n) {
-                    set_error("emfa_lu_factor: pivot %d out of range at column %d", p, i);
-                    st = EMFA_E_HIP;
-                    break;
-                }
-                std::swap(perm[i], perm[p]);
-            }
-            if (!st && hipMemcpy(lu->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
-                set_error("emfa_lu_factor: permutation upload failed");
-                st = EMFA_E_HIP;
-            }
-        }
-    }
+    if (!st) st = compose_perm(ctx, lu);
     if (!st && !getenv("EMFA_SOLVE_BLOCKED")) st = build_dinv(ctx, lu);
     if (st) {
         emfa_lu_destroy(lu);
@@ -2361,6 +2426,148 @@ emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu *
         return EMFA_E_SINGULAR;
     }
     return EMFA_OK;
+}
+
+emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int count, int prec, emfa_lu **out) {
+    EMFA_REQUIRE(ctx && Z && out && count >= 1, EMFA_E_INVALID, "emfa_lu_factor_batch: bad arguments");
+    for (int z = 0; z < count; ++z) out[z] = nullptr;
+    for (int z = 0; z < count; ++z)
+        EMFA_REQUIRE(Z[z] && Z[z]->rows == Z[z]->cols && Z[z]->rows == Z[0]->rows, EMFA_E_INVALID,
+                     "emfa_lu_factor_batch: matrix %d is not square of the batch's size", z);
+    EMFA_REQUIRE(prec == EMFA_F32 || prec == EMFA_F64, EMFA_E_INVALID, "emfa_lu_factor_batch: bad precision");
+    const int n = Z[0]->rows;
+    // one launch sequence for the batch: FP32, pivot-free panels, split update, inverse-block solves
+    const bool batchable = count > 1 && prec == EMFA_F32 && ctx->panel_x == 2 && ctx->lu_update_mode == 1 &&
+                           !ctx->force_pivoting && !getenv("EMFA_SOLVE_BLOCKED") && !getenv("EMFA_LU_NOPIV") &&
+                           !getenv("EMFA_LU_BF16X3") && !getenv("EMFA_LU_NOBATCH");
+    if (!batchable) {
+        emfa_status worst = EMFA_OK;
+        for (int z = 0; z < count; ++z) {
+            emfa_status st = emfa_lu_factor(ctx, Z[z], prec, &out[z]);
+            if (st == EMFA_E_SINGULAR) {
+                worst = st;
+            } else if (st) {
+                for (int y = 0; y < z; ++y) {
+                    emfa_lu_destroy(out[y]);
+                    out[y] = nullptr;
+                }
+                return st;
+            }
+        }
+        return worst;
+    }
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    std::vector<emfa_lu *> lus(count, nullptr);
+    void *dmat = nullptr, *ddinv = nullptr;
+    auto fail = [&](emfa_status st) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->batch = Batch();
+        if (dmat) ctx_free(ctx, dmat, sizeof(void *) * count);
+        if (ddinv) ctx_free(ctx, ddinv, sizeof(void *) * count);
+        for (auto l : lus)
+            if (l) emfa_lu_destroy(l);
+        return st;
+    };
+    for (int z = 0; z < count; ++z) {
+        emfa_lu *lu = new emfa_lu();
+        lu->ctx = ctx;
+        emfa_status st = emfa_mat_create(ctx, n, n, EMFA_F32, &lu->a);
+        if (st) {
+            delete lu;
+            return fail(st);
+        }
+        lus[z] = lu;
+        if (ctx_alloc(ctx, (void **)&lu->ipiv, sizeof(int) * n) != hipSuccess ||
+            ctx_alloc(ctx, (void **)&lu->perm, sizeof(int) * n) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("emfa_lu_factor_batch: pivot allocation failed");
+            return fail(EMFA_E_OOM);
+        }
+        if ((st = convert_copy(ctx, Z[z], lu->a))) return fail(st);
+    }
+    // device array of the factors' bases; host copies stay alive until the stream syncs below
+    std::vector<const char *> hmat(count), hdinv(count);
+    for (int z = 0; z < count; ++z) hmat[z] = (const char *)lus[z]->a->d;
+    if (ctx_alloc(ctx, &dmat, sizeof(void *) * count) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("emfa_lu_factor_batch: allocation failed");
+        return fail(EMFA_E_OOM);
+    }
+    EMFA_HIP_CHECK(hipMemcpyAsync(dmat, hmat.data(), sizeof(void *) * count, hipMemcpyHostToDevice, ctx->stream));
+    ctx->batch = Batch();
+    ctx->batch.mat = (const char *const *)dmat;
+    ctx->batch.mat0 = hmat[0];
+    ctx->batch.count = count;
+    std::vector<int> growth(count, 0);
+    int info = 0, inter = 0;
+    emfa_status st = lu_impl<float>(ctx, lus[0]->a, lus[0]->ipiv, &info, &inter, growth.data());
+    ctx->batch = Batch();
+    if (st) return fail(st);
+    // no interchanges: identity pivots and permutation
+    std::vector<int> iota(n);
+    for (int i = 0; i < n; ++i) iota[i] = i;
+    for (int z = 0; z < count; ++z) {
+        lus[z]->interchanges = 0;
+        lus[z]->info = 0;
+        EMFA_HIP_CHECK(hipMemcpyAsync(lus[z]->ipiv, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+        EMFA_HIP_CHECK(hipMemcpyAsync(lus[z]->perm, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    // a factor that grew is redone alone with partial pivoting
+    for (int z = 0; z < count; ++z) {
+        if (!growth[z]) continue;
+        emfa_lu *lu = lus[z];
+        ctx->force_pivoting = 1;
+        st = convert_copy(ctx, Z[z], lu->a);
+        if (!st) st = lu_factor_inplace(ctx, lu->a, lu->ipiv, &lu->info, &lu->interchanges);
+        ctx->force_pivoting = 0;
+        lu->pivot_fallback = 1;
+        if (!st) st = compose_perm(ctx, lu);
+        if (st) return fail(st);
+    }
+    // inverses of the diagonal blocks of every factor, one launch sequence
+    {
+        const int nblk = (n + kDinvB - 1) / kDinvB;
+        for (int z = 0; z < count; ++z) {
+            emfa_lu *lu = lus[z];
+            lu->dinv_bytes = 2 * (size_t)nblk * kDinvB * kDinvB * sizeof(cx<float>);
+            if (ctx_alloc(ctx, &lu->dinv, lu->dinv_bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                lu->dinv = nullptr;
+                set_error("emfa_lu_factor_batch: diagonal-block inverse allocation failed");
+                return fail(EMFA_E_OOM);
+            }
+            hdinv[z] = (const char *)lu->dinv;
+        }
+        if (ctx_alloc(ctx, &ddinv, sizeof(void *) * count) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("emfa_lu_factor_batch: allocation failed");
+            return fail(EMFA_E_OOM);
+        }
+        EMFA_HIP_CHECK(hipMemcpyAsync(ddinv, hdinv.data(), sizeof(void *) * count, hipMemcpyHostToDevice, ctx->stream));
+        Batch bf, bd;
+        bf.mat = (const char *const *)dmat;
+        bf.mat0 = hmat[0];
+        bf.count = count;
+        bd.mat = (const char *const *)ddinv;
+        bd.mat0 = hdinv[0];
+        bd.count = count;
+        ProfScope ps(ctx, "lu_dinv", 0);
+        dinv_launch_f32(ctx->stream, (const cx<float> *)lus[0]->a->d, n, (cx<float> *)lus[0]->dinv, nblk, bf, bd);
+        EMFA_HIP_CHECK(hipGetLastError());
+    }
+    EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, dmat, sizeof(void *) * count);
+    ctx_free(ctx, ddinv, sizeof(void *) * count);
+    emfa_status worst = EMFA_OK;
+    for (int z = 0; z < count; ++z) {
+        out[z] = lus[z];
+        if (lus[z]->info != 0) {
+            set_error("emfa_lu_factor_batch: exactly zero pivot in matrix %d at column %d", z, lus[z]->info);
+            worst = EMFA_E_SINGULAR;
+        }
+    }
+    return worst;
 }
 
 emfa_status emfa_lu_destroy(emfa_lu *lu) {

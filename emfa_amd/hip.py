@@ -22,7 +22,7 @@ EXPORTS = [
     "emfa_ctx_bytes_in_use", "emfa_ctx_cu_count", "emfa_ctx_set_cu_share", "emfa_geom_icosphere", "emfa_geom_plate", "emfa_rwg_build", "emfa_mesh_upload",
     "emfa_mesh_destroy", "emfa_mesh_unknowns", "emfa_mat_create", "emfa_mat_destroy", "emfa_mat_upload",
     "emfa_mat_download", "emfa_mat_copy", "emfa_mat_rows", "emfa_mat_cols", "emfa_mat_prec", "emfa_mat_device_ptr",
-    "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
+    "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_factor_batch", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
@@ -293,15 +293,19 @@ class Mat:
 
 
 class LU:
-    def __init__(self, ctx, Z, prec):
+    def __init__(self, ctx, Z, prec, handle=None):
         self.ctx = ctx
+        self.n = Z.rows
+        self.prec = prec
+        if handle is not None:  # a factor made by lu_factor_batch
+            self.h = handle
+            self.singular = lib().emfa_lu_info(handle) != 0
+            return
         self.h = _vp()
         code = lib().emfa_lu_factor(ctx.h, Z.h, prec, ctypes.byref(self.h))
         self.singular = code == EMFA_E_SINGULAR
         if code not in (EMFA_OK, EMFA_E_SINGULAR):
             _check(code)
-        self.n = Z.rows
-        self.prec = prec
 
     def info(self):
         return lib().emfa_lu_info(self.h)
@@ -334,6 +338,18 @@ class LU:
             self.close()
         except Exception:
             pass
+
+
+def lu_factor_batch(ctx, mats, prec):
+    """Factor same-size matrices with one launch sequence where the context
+    allows it (emfa_lu_factor_batch); returns one LU per matrix."""
+    mats = list(mats)
+    arr = (_vp * len(mats))(*[m.h for m in mats])
+    outs = (_vp * len(mats))()
+    code = lib().emfa_lu_factor_batch(ctx.h, arr, len(mats), prec, outs)
+    if code not in (EMFA_OK, EMFA_E_SINGULAR):
+        _check(code)
+    return [LU(ctx, m, prec, handle=_vp(h)) for m, h in zip(mats, outs)]
 
 
 # ------------------------------------------------------------------ operations

@@ -131,6 +131,14 @@ emfa_status emfa_rhs_fill(emfa_ctx *ctx, const emfa_mesh *mesh, int kind, double
  * (EMSolver/MFIE.cpp:64).  Returns EMFA_E_SINGULAR for an exactly zero
  * pivot (the factor is still produced, like LAPACK's info > 0). */
 emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu **out);
+/* Factor `count` matrices of one size: out[z] as emfa_lu_factor(Z[z]) would
+ * give it.  In FP32 with panel variant 2 and update mode 1 the batch shares
+ * one launch sequence (every kernel takes the batch on blockIdx.z; a factor
+ * that grows is redone alone with interchanges); otherwise the matrices are
+ * factored one after another.  EMFA_E_SINGULAR if any factor has an exactly
+ * zero pivot (all factors are still produced).  No reference counterpart:
+ * the reference factors one matrix per solve (EMSolver/MFIE.cpp:64). */
+emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int count, int prec, emfa_lu **out);
 emfa_status emfa_lu_destroy(emfa_lu *lu);
 /* In place: B (n x nrhs, any precision) <- A^{-1} B using the factor. */
 emfa_status emfa_lu_solve(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B);

@@ -443,3 +443,55 @@ def test_end_to_end_efie_sphere(ctx):
     assert ok
     r32 = hip.rcs(ctx, M, k, X2, dirs)
     assert np.abs(r32 - ro).max() <= 0.1
+
+
+def test_batched_factor_matches_single_factors_and_redoes_a_grown_member(ctx):
+    _say("batched LU")
+    ctx.lu_params(256, 0, 2)
+    ctx.lu_update_mode(1)
+    rng = np.random.default_rng(11)
+    try:
+        for sub in (2, 3):  # 480 and 1920 unknowns: a partial last block, several outer steps
+            xyz, tri, rwg = sphere(sub, 1.0)
+            M = hip.Mesh(ctx, xyz, tri, rwg)
+            Zs = []
+            for f in (2e8, 3e8, 4.5e8):
+                Z = hip.Mat(ctx, M.n, M.n, hip.F32)
+                hip.fill(ctx, M, Z, hip.CFIE, hip.wavenumber(f), 3, alpha=0.5)
+                Zs.append(Z)
+            single = [hip.LU(ctx, Z, hip.F32) for Z in Zs]
+            batch = hip.lu_factor_batch(ctx, Zs, hip.F32)
+            B0 = (rng.standard_normal((M.n, 5)) + 1j * rng.standard_normal((M.n, 5))).astype(np.complex64)
+            for s1, b1 in zip(single, batch):
+                assert b1.pivoting() == (0, 0) and s1.pivoting() == (0, 0)
+                Fs, Fb = s1.factor().numpy(), b1.factor().numpy()
+                _say(f"n={M.n} factor identical: {np.array_equal(Fs, Fb)}, rel {rel(Fb, Fs):.2e}")
+                assert rel(Fb, Fs) < 1e-6
+                Xs, Xb = hip.Mat.from_numpy(ctx, B0, hip.F32), hip.Mat.from_numpy(ctx, B0, hip.F32)
+                s1.solve(Xs)
+                b1.solve(Xb)
+                assert rel(Xb.numpy(), Xs.numpy()) < 1e-5
+                for m in (s1, b1, Xs, Xb):
+                    m.close()
+            for Z in Zs:
+                Z.close()
+            M.close()
+        # a member whose multipliers grow is redone alone with interchanges
+        n = 700
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+        G = A.copy()
+        G[0, 0] = 1e-4
+        Zo, Zg, Zo2 = (hip.Mat.from_numpy(ctx, X, hip.F32) for X in (A, G, A.T.copy()))
+        lb = hip.lu_factor_batch(ctx, [Zo, Zg, Zo2], hip.F32)
+        assert [l.pivoting() for l in lb] == [(0, 0), (1, 1), (0, 0)]
+        lg = hip.LU(ctx, Zg, hip.F32)  # alone: the same fallback
+        assert lg.pivoting() == (1, 1)
+        assert np.array_equal(lb[1].pivots(), lg.pivots())
+        assert np.array_equal(lb[1].factor().numpy(), lg.factor().numpy())
+        lo = hip.LU(ctx, Zo, hip.F32)
+        assert rel(lb[0].factor().numpy(), lo.factor().numpy()) < 1e-6
+        for m in lb + [lg, lo, Zo, Zg, Zo2]:
+            m.close()
+    finally:
+        ctx.lu_params(0, 0, -1)
+        ctx.lu_update_mode(0)
