@@ -1004,7 +1004,8 @@ struct UpdatePlanes {
         if (ctx->cu_share > 0) {
             int cap = 1;
             if (const char *e = getenv("EMFA_GEMM_SHARE_CAP")) cap = atoi(e);
-            if (cap > 0) g.max_wgs = cap * ctx->cu_share;
+            // a batch of factorisations does the work of that many workers
+            if (cap > 0) g.max_wgs = std::min(ctx->num_cu, cap * ctx->cu_share * std::max(1, ctx->batch.count));
         }
         return gemm_launch_bf3_planes(s, g);
     }

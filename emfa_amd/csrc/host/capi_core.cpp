@@ -198,6 +198,8 @@ int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out) {
     out->peak_bytes = t.peak_bytes;
     out->pivot_fallbacks = t.pivot_fallbacks;
     out->batched = t.batched;
+    out->factor_batches = t.factor_batches;
+    out->batch_factors = t.batch_factors;
     return 0;
 }
 

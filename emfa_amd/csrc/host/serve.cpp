@@ -71,6 +71,13 @@ static void build_mesh(const MeshSpec &m, std::vector<double> &xyz, std::vector<
     if (st) throw StatusError(st, emfa_last_error());
 }
 
+// RWG count of a mesh spec without building it (closed icosphere: 1.5 T =
+// 30 4^s edges; plate of c x c cells: 3 c^2 - 2 c interior edges)
+static int spec_unknowns(const MeshSpec &m) {
+    if (m.kind == PLATE) return 3 * m.subdiv * m.subdiv - 2 * m.subdiv;
+    return 30 * (1 << (2 * m.subdiv));
+}
+
 size_t Server::footprint(int n, int nrhs, int precision, bool factor_cached) {
     const size_t e = precision == EMFA_F32 ? 8 : 16;
     size_t vecs = (size_t)n * nrhs * 16 * 4 + (size_t)n * nrhs * e;  // B, X, R, Bd, D
@@ -111,6 +118,7 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
         if (env_flag("EMFA_SERVE_NOPIV", true)) emfa_lu_set_params(c, 0, 0, 2);
         if (const char *e = std::getenv("EMFA_SERVE_REFINE_TOL")) refine_tol_ = std::max(1e-14, std::atof(e));
         if (const char *e = std::getenv("EMFA_SERVE_BATCH_RHS")) batch_max_rhs_ = std::max(0, std::atoi(e));
+        if (const char *e = std::getenv("EMFA_SERVE_FACTOR_BATCH")) factor_batch_max_ = std::max(0, std::atoi(e));
         if (env_flag("EMFA_SERVE_SPLIT", true)) emfa_lu_set_update_mode(c, 1);
         ctxs_.push_back(c);
     }
@@ -241,16 +249,22 @@ static std::string factor_key(const Request &r, int prec) {
 void Server::worker(int w) {
     emfa_ctx *ctx = ctxs_[w];
     for (;;) {
-        std::vector<Job> jobs;
+        // groups[0]: the first queued request and the queued requests that
+        // share its factor; groups[1..]: requests for other new FP32 factors
+        // of the same size, built together with groups[0]'s (factor batch)
+        std::vector<std::vector<Job>> groups(1);
+        groups.reserve((size_t)std::max(1, factor_batch_max_) + 1);  // references into groups[0] stay valid
+        bool factor_batch = false;
         {
             std::unique_lock<std::mutex> l(mu_);
             cv_work_.wait(l, [&] { return stop_ || !queue_.empty(); });
             if (stop_) return;
+            std::vector<Job> &jobs = groups[0];
             jobs.push_back(std::move(queue_.front()));
             queue_.pop_front();
             int rhs = (int)(jobs[0].req.angles.size() / 3);
+            const std::string key = factor_key(jobs[0].req, jobs[0].req.precision);
             if (batch_max_rhs_ > 0 && rhs < batch_max_rhs_) {
-                const std::string key = factor_key(jobs[0].req, jobs[0].req.precision);
                 for (auto it = queue_.begin(); it != queue_.end();) {
                     const int m = (int)(it->req.angles.size() / 3);
                     if (m > 0 && rhs + m <= batch_max_rhs_ && factor_key(it->req, it->req.precision) == key) {
@@ -262,50 +276,179 @@ void Server::worker(int w) {
                     }
                 }
             }
-            busy_ += (int)jobs.size();
-        }
-        std::vector<Result> res(jobs.size());
-        for (size_t i = 0; i < jobs.size(); ++i) {
-            res[i].id = jobs[i].id;
-            res[i].batch_size = (int)jobs.size();
-        }
-        auto start = clock_t_::now();
-        try {
-            run(ctx, jobs, res);
-        } catch (const StatusError &e) {
-            for (auto &x : res) {
-                x.status = e.code;
-                x.error = e.what();
+            const Request &r0 = jobs[0].req;
+            if (factor_batch_max_ > 1 && r0.precision == EMFA_F32 && !cache_.count(key) && !building_.count(key)) {
+                const int n0 = spec_unknowns(r0.mesh);
+                std::map<std::string, size_t> group_of{{key, 0}};
+                std::vector<int> group_rhs{rhs};
+                std::deque<Job> rest;
+                for (auto &q : queue_) {
+                    const int m = (int)(q.req.angles.size() / 3);
+                    bool taken = false;
+                    if (q.req.precision == EMFA_F32 && m > 0 && spec_unknowns(q.req.mesh) == n0) {
+                        const std::string k = factor_key(q.req, EMFA_F32);
+                        auto g = group_of.find(k);
+                        if (g != group_of.end() && g->second > 0 && group_rhs[g->second] + m <= batch_max_rhs_) {
+                            group_rhs[g->second] += m;
+                            groups[g->second].push_back(std::move(q));
+                            taken = true;
+                        } else if (g == group_of.end() && (int)groups.size() < factor_batch_max_ && !cache_.count(k) &&
+                                   !building_.count(k)) {
+                            group_of[k] = groups.size();
+                            group_rhs.push_back(m);
+                            groups.emplace_back();
+                            groups.back().push_back(std::move(q));
+                            taken = true;
+                        }
+                    }
+                    if (!taken) rest.push_back(std::move(q));
+                }
+                queue_.swap(rest);
+                if (groups.size() > 1) {
+                    factor_batch = true;
+                    for (auto &g : group_of) building_.insert(g.first);
+                }
             }
-        } catch (const std::exception &e) {
-            for (auto &x : res) {
-                x.status = EMFA_E_INVALID;
-                x.error = e.what();
-            }
+            for (auto &g : groups) busy_ += (int)g.size();
         }
-        auto end = clock_t_::now();
+        const auto start = clock_t_::now();
+        if (factor_batch) prebuild(ctx, groups);
+        for (auto &jobs : groups) {
+            std::vector<Result> res(jobs.size());
+            for (size_t i = 0; i < jobs.size(); ++i) {
+                res[i].id = jobs[i].id;
+                res[i].batch_size = (int)jobs.size();
+            }
+            try {
+                run(ctx, jobs, res, factor_batch);
+            } catch (const StatusError &e) {
+                for (auto &x : res) {
+                    x.status = e.code;
+                    x.error = e.what();
+                }
+            } catch (const std::exception &e) {
+                for (auto &x : res) {
+                    x.status = EMFA_E_INVALID;
+                    x.error = e.what();
+                }
+            }
+            auto end = clock_t_::now();
+            {
+                std::lock_guard<std::mutex> l(mu_);
+                busy_ -= (int)jobs.size();
+                t1_ = end;
+                batched_ += (int64_t)jobs.size() - 1;
+                for (size_t i = 0; i < jobs.size(); ++i) {
+                    Result &x = res[i];
+                    x.queue_ms = ms_between(jobs[i].arrival, start);
+                    x.service_ms = ms_between(start, end);
+                    x.latency_ms = ms_between(jobs[i].arrival, end);
+                    if (x.status == EMFA_OK) {
+                        lat_.push_back(x.latency_ms);
+                        rhs_done_ += x.nrhs;
+                        if (x.cache_hit) ++hits_;
+                    } else {
+                        ++failed_;
+                    }
+                    done_[x.id] = std::move(x);
+                }
+            }
+            cv_done_.notify_all();
+        }
+    }
+}
+
+// Factor batch: the groups' meshes are built and filled one by one, then all
+// matrices are factored with one emfa_lu_factor_batch call and published
+// together.  The reservation covers every factor and its LU workspace while
+// they are built; the factors' bytes then move to the cache.  On any failure
+// the keys are released and run() builds each factor on its own.
+void Server::prebuild(emfa_ctx *ctx, const std::vector<std::vector<Job>> &groups) {
+    const size_t count = groups.size();
+    std::vector<std::string> keys(count);
+    for (size_t g = 0; g < count; ++g) keys[g] = factor_key(groups[g][0].req, EMFA_F32);
+    size_t held = 0;
+    bool published = false;
+    try {
+        std::vector<std::vector<double>> xyz(count);
+        std::vector<std::vector<int>> tri(count), rwg(count);
+        std::vector<int> n(count, 0);
+        for (size_t g = 0; g < count; ++g) {
+            build_mesh(groups[g][0].req.mesh, xyz[g], tri[g], rwg[g], n[g]);
+            if (n[g] != n[0]) throw StatusError(EMFA_E_INVALID, "factor batch: sizes differ");
+        }
+        const size_t need = count * footprint(n[0], 0, EMFA_F32, false);
+        {
+            mu_.lock();
+            bool ok = reserve(need, 0);
+            mu_.unlock();
+            if (!ok) throw StatusError(EMFA_E_OOM, "factor batch exceeds the HBM budget");
+        }
+        held = need;
+        std::vector<std::shared_ptr<Factor>> fs(count);
+        std::vector<const emfa_mat *> Zs(count);
+        for (size_t g = 0; g < count; ++g) {
+            fs[g] = make_factor(ctx, groups[g][0].req, EMFA_F32, xyz[g], tri[g], rwg[g], n[g]);
+            Zs[g] = fs[g]->Z;
+        }
+        std::vector<emfa_lu *> lus(count, nullptr);
+        check(emfa_lu_factor_batch(ctx, Zs.data(), (int)count, EMFA_F32, lus.data()), "LU batch");
+        int fallbacks = 0;
+        for (size_t g = 0; g < count; ++g) {
+            fs[g]->lu = lus[g];
+            int inter = 1, fallback = 0;
+            check(emfa_lu_get_pivoting(lus[g], &inter, &fallback), "LU");
+            fallbacks += fallback;
+        }
+        // other workers use the factors on their own streams: finish them first
+        check(emfa_ctx_sync(ctx), "sync");
         {
             std::lock_guard<std::mutex> l(mu_);
-            busy_ -= (int)jobs.size();
-            t1_ = end;
-            batched_ += (int64_t)jobs.size() - 1;
-            for (size_t i = 0; i < jobs.size(); ++i) {
-                Result &x = res[i];
-                x.queue_ms = ms_between(jobs[i].arrival, start);
-                x.service_ms = ms_between(start, end);
-                x.latency_ms = ms_between(jobs[i].arrival, end);
-                if (x.status == EMFA_OK) {
-                    lat_.push_back(x.latency_ms);
-                    rhs_done_ += x.nrhs;
-                    if (x.cache_hit) ++hits_;
-                } else {
-                    ++failed_;
-                }
-                done_[x.id] = std::move(x);
+            for (size_t g = 0; g < count; ++g) {
+                cache_[keys[g]] = fs[g];
+                lru_.push_back(keys[g]);
+                cache_bytes_ += fs[g]->bytes;
+                reserved_ -= fs[g]->bytes;
+                held -= fs[g]->bytes;
+                building_.erase(keys[g]);
             }
+            release(held);
+            held = 0;
+            pivot_fallbacks_ += fallbacks;
+            ++factor_batches_;
+            batch_factors_ += (int64_t)count;
+            published = true;
         }
-        cv_done_.notify_all();
+    } catch (const std::exception &) {
+        // fall through: run() builds what is missing one factor at a time
     }
+    if (!published) {
+        std::lock_guard<std::mutex> l(mu_);
+        release(held);
+        for (auto &k : keys) building_.erase(k);
+    }
+    cv_build_.notify_all();
+}
+
+// Mesh on the device and the filled matrix of a new factor (no LU yet); its
+// bytes count Z, the factor and the diagonal-block inverses.
+std::shared_ptr<Server::Factor> Server::make_factor(emfa_ctx *ctx, const Request &r, int prec,
+                                                    const std::vector<double> &xyz, const std::vector<int> &tri,
+                                                    const std::vector<int> &rwg, int n) {
+    const double k = 2.0 * 3.14159265358979323846 * r.freq / 299792458.0;
+    auto nf = std::make_shared<Factor>();
+    nf->n = n;
+    nf->prec = prec;
+    nf->k = k;
+    check(emfa_mesh_upload(ctx, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3), rwg.data(), n,
+                           &nf->mesh),
+          "mesh");
+    check(emfa_mat_create(ctx, n, n, prec, &nf->Z), "Z");
+    emfa_fill_params fp{r.formulation, r.quad, k, r.alpha, 0.0, 0.0, 0};
+    check(emfa_fill(ctx, nf->mesh, &fp, nf->Z), "fill");
+    nf->bytes = 2 * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16) +
+                2 * (((size_t)n + 255) / 256) * 256 * 256 * (prec == EMFA_F32 ? 8 : 16);
+    return nf;
 }
 
 // The factor of (request, prec) from the cache, or computed and published.
@@ -356,17 +499,7 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
     *held += need;
     if (f) return f;
 
-    const double k = 2.0 * 3.14159265358979323846 * r.freq / 299792458.0;
-    auto nf = std::make_shared<Factor>();
-    nf->n = n;
-    nf->prec = prec;
-    nf->k = k;
-    check(emfa_mesh_upload(ctx, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3), rwg.data(), n,
-                           &nf->mesh),
-          "mesh");
-    check(emfa_mat_create(ctx, n, n, prec, &nf->Z), "Z");
-    emfa_fill_params fp{r.formulation, r.quad, k, r.alpha, 0.0, 0.0, 0};
-    check(emfa_fill(ctx, nf->mesh, &fp, nf->Z), "fill");
+    auto nf = make_factor(ctx, r, prec, xyz, tri, rwg, n);
     check(emfa_lu_factor(ctx, nf->Z, prec, &nf->lu), "LU");
     {
         int inter = 1, fallback = 0;
@@ -376,8 +509,6 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
             ++pivot_fallbacks_;
         }
     }
-    nf->bytes = 2 * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16) +
-                2 * (((size_t)n + 255) / 256) * 256 * 256 * (prec == EMFA_F32 ? 8 : 16);  // Z, factor, inverses
     // other workers use the factor on their own streams: finish it first
     check(emfa_ctx_sync(ctx), "sync");
     {
@@ -395,7 +526,7 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
     return nf;
 }
 
-void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result> &res) {
+void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result> &res, bool fresh) {
     const Request &r = jobs[0].req;  // the batch shares mesh, frequency, formulation, precision
     std::vector<int> first(jobs.size() + 1, 0);  // column range of each job in the stacked RHS
     for (size_t i = 0; i < jobs.size(); ++i) {
@@ -486,7 +617,7 @@ void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result
         Result &x = res[i];
         const int c0 = first[i], m = first[i + 1] - first[i];
         // batch mates did not pay for the factor: they count as reuse
-        x.cache_hit = hit || i > 0;
+        x.cache_hit = (hit && !fresh) || i > 0;
         x.unknowns = n;
         x.nrhs = m;
         x.refine_iterations = iters;
@@ -511,6 +642,8 @@ Stats Server::stats() {
     s.peak_bytes = peak_;
     s.pivot_fallbacks = pivot_fallbacks_;
     s.batched = batched_;
+    s.factor_batches = factor_batches_;
+    s.batch_factors = batch_factors_;
     if (!lat_.empty()) {
         std::vector<double> v = lat_;
         std::sort(v.begin(), v.end());
@@ -537,7 +670,7 @@ Stats Server::stats() {
 void Server::reset_stats() {
     std::lock_guard<std::mutex> l(mu_);
     lat_.clear();
-    failed_ = hits_ = rhs_done_ = pivot_fallbacks_ = batched_ = 0;
+    failed_ = hits_ = rhs_done_ = pivot_fallbacks_ = batched_ = factor_batches_ = batch_factors_ = 0;
     have_t0_ = false;
     peak_ = reserved_ + cache_bytes_;
 }

@@ -69,6 +69,8 @@ struct Stats {
     size_t peak_bytes = 0;
     int64_t pivot_fallbacks = 0;  // FP32 factors redone with interchanges
     int64_t batched = 0;          // requests that joined another request's batch
+    int64_t factor_batches = 0;   // factorisations that built several new factors together
+    int64_t batch_factors = 0;    // factors built that way
 };
 
 class Server {
@@ -97,8 +99,15 @@ class Server {
         std::chrono::steady_clock::time_point arrival;
     };
     void worker(int w);
-    // solve the jobs of one batch (same factor key) with one refinement
-    void run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result> &res);
+    // solve the jobs of one batch (same factor key) with one refinement;
+    // fresh: the factor was just built for them (not a cache hit)
+    void run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result> &res, bool fresh = false);
+    // factor the groups' matrices (distinct keys, one size, FP32) with one
+    // batched factorisation and publish them; on failure the groups fall
+    // back to building their factors one by one
+    void prebuild(emfa_ctx *ctx, const std::vector<std::vector<Job>> &groups);
+    std::shared_ptr<Factor> make_factor(emfa_ctx *ctx, const Request &r, int prec, const std::vector<double> &xyz,
+                                        const std::vector<int> &tri, const std::vector<int> &rwg, int n);
     std::shared_ptr<Factor> cache_get(const std::string &key);
     std::shared_ptr<Factor> acquire(emfa_ctx *ctx, const Request &r, int prec, int nrhs, bool *hit, size_t *held,
                                     size_t pinned);
@@ -132,8 +141,11 @@ class Server {
     std::set<std::string> building_;  // factors being computed by some worker
     // stats
     std::vector<double> lat_;
-    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0, pivot_fallbacks_ = 0, batched_ = 0;
+    int64_t failed_ = 0, hits_ = 0, rhs_done_ = 0, pivot_fallbacks_ = 0, batched_ = 0, batch_factors_ = 0;
     int batch_max_rhs_ = 256;  // right-hand sides one batch may stack (EMFA_SERVE_BATCH_RHS, 0: no batching)
+    // distinct new FP32 factors of one size built together (EMFA_SERVE_FACTOR_BATCH, <= 1: one at a time)
+    int factor_batch_max_ = 8;
+    int64_t factor_batches_ = 0;  // batched factorisations run (stats)
     bool have_t0_ = false;
     std::chrono::steady_clock::time_point t0_, t1_;
 };

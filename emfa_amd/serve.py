@@ -44,7 +44,8 @@ class Stats(ctypes.Structure):
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("solves_per_s", ctypes.c_double),
                 ("rhs_per_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
-                ("pivot_fallbacks", ctypes.c_longlong), ("batched", ctypes.c_longlong)]
+                ("pivot_fallbacks", ctypes.c_longlong), ("batched", ctypes.c_longlong),
+                ("factor_batches", ctypes.c_longlong), ("batch_factors", ctypes.c_longlong)]
 
 
 _CORE = None

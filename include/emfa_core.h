@@ -75,6 +75,8 @@ typedef struct {
     size_t peak_bytes;                      /* admitted footprint + resident factors */
     long long pivot_fallbacks;              /* FP32 factors redone with row interchanges */
     long long batched;                      /* requests that joined another request's batch */
+    long long factor_batches;               /* factorisations that built several new FP32 factors together */
+    long long batch_factors;                /* factors built that way */
 } emfa_serve_stats;
 
 /* workers >= 1 contexts on `device`; hbm_budget in bytes; keep_current != 0

@@ -156,3 +156,28 @@ def test_queued_requests_of_one_factor_are_batched_and_match_alone():
         assert g["nrhs"] == a["nrhs"] and g["relres"] <= 1e-8
         assert rel(g["current"], a["current"]) < 1e-6
         assert np.abs(g["rcs_db"] - a["rcs_db"]).max() < 1e-4
+
+
+def test_new_factors_of_one_size_are_factored_together_and_match_alone(monkeypatch):
+    # factor batching: requests for different new factors of one size queued
+    # behind a busy worker are factored with one batched factorisation; each
+    # answer equals the same request served with factor batching off
+    reqs = [_req(serve.ICOSPHERE, 2, (0.8, 0.7, 0.6), form, f, 4)
+            for form, f in ((hip.EFIE, 2.0e8), (hip.CFIE, 2.6e8), (hip.EFIE, 3.1e8), (hip.CFIE, 3.7e8))]
+    blocker = _req(serve.PLATE, 40, (1.0, 1.0, 1.0), hip.EFIE, 4.0e8, 3, 80.0)
+    with serve.Server(workers=1, hbm_budget=16 * GIB, keep_current=True) as srv:
+        b = _submit(srv, blocker, hip.F32)
+        ids = [_submit(srv, r, hip.F32) for r in reqs]
+        assert srv.wait(b)["status"] == 0
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["factor_batches"] == 1 and st["batch_factors"] == 4, st
+    assert all(g["status"] == 0 and not g["cache_hit"] and g["relres"] <= 1e-8 for g in got)
+    monkeypatch.setenv("EMFA_SERVE_FACTOR_BATCH", "0")
+    with serve.Server(workers=1, hbm_budget=16 * GIB, keep_current=True) as srv:
+        alone = [srv.wait(_submit(srv, r, hip.F32)) for r in reqs]
+        assert srv.stats()["factor_batches"] == 0
+    for a, g in zip(alone, got):
+        assert g["refine_iterations"] == a["refine_iterations"]
+        assert rel(g["current"], a["current"]) < 1e-12
+        assert np.abs(g["rcs_db"] - a["rcs_db"]).max() < 1e-9
