@@ -164,6 +164,8 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None):
         "served_fp64_fallbacks": int(sum(r["fp64_fallback"] for r in sat)),
         "served_pivot_fallbacks": int(st.get("pivot_fallbacks", 0)),
         "served_batched_requests": int(st.get("batched", 0)),
+        "served_factor_batches": int(st.get("factor_batches", 0)),
+        "served_factors_in_batches": int(st.get("batch_factors", 0)),
         "served_peak_gb": st["peak_bytes"] / 2**30,
         "plain_fp64_solves_per_s": len(reqs) / plain_wall, "plain_fp64_wall_s": plain_wall,
         "plain_fp64_p50_ms": float(np.percentile(pl, 50)), "plain_fp64_p99_ms": float(np.percentile(pl, 99)),

@@ -1454,7 +1454,9 @@ constexpr int kSkinnyMaxCols = 64;
 // The column tile MC (8, 16, 32, 64) follows m, so no lane works on padding
 // columns: lane tx owns column tx, rows ty + h RG (h < RPT) of the
 // workgroup's RPT RG rows (RG = 256 / MC).  Two rows per lane: more rows per
-// workgroup cut the re-reads of X but cost more in parallelism (measured).
+// workgroup cut the re-reads of X but cost more in parallelism (measured:
+// eight rows per lane on tall 32 / 64-column products made a 7680-row solve
+// 2.5 -> 4.2 ms and the served stream 223 -> 187 solves/s).
 // K goes through LDS in chunks of 64 with the next chunk's operands loaded
 // into registers while the current chunk is multiplied.
 template <int MC>
