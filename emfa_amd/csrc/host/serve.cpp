@@ -140,7 +140,7 @@ Server::~Server() {
 
 int64_t Server::submit(const Request &r) {
     std::lock_guard<std::mutex> l(mu_);
-    Job j{next_id_++, r, clock_t_::now()};
+    Job j{next_id_++, r, clock_t_::now(), false};
     if (!have_t0_) {
         t0_ = j.arrival;
         have_t0_ = true;
@@ -312,7 +312,21 @@ void Server::worker(int w) {
             for (auto &g : groups) busy_ += (int)g.size();
         }
         const auto start = clock_t_::now();
-        if (factor_batch) prebuild(ctx, groups);
+        if (factor_batch) {
+            prebuild(ctx, groups);
+            // the other groups go back to the front of the queue: their
+            // factors are resident now, so idle workers answer them meanwhile
+            std::lock_guard<std::mutex> l(mu_);
+            for (size_t g = groups.size() - 1; g >= 1; --g) {
+                busy_ -= (int)groups[g].size();
+                for (size_t i = groups[g].size(); i-- > 0;) {
+                    groups[g][i].fresh = i == 0;
+                    queue_.push_front(std::move(groups[g][i]));
+                }
+            }
+            groups.resize(1);
+            cv_work_.notify_all();
+        }
         for (auto &jobs : groups) {
             std::vector<Result> res(jobs.size());
             for (size_t i = 0; i < jobs.size(); ++i) {
@@ -320,7 +334,7 @@ void Server::worker(int w) {
                 res[i].batch_size = (int)jobs.size();
             }
             try {
-                run(ctx, jobs, res, factor_batch);
+                run(ctx, jobs, res, factor_batch || jobs[0].fresh);
             } catch (const StatusError &e) {
                 for (auto &x : res) {
                     x.status = e.code;

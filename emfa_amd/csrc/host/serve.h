@@ -97,6 +97,7 @@ class Server {
         int64_t id;
         Request req;
         std::chrono::steady_clock::time_point arrival;
+        bool fresh = false;  // its factor was built by a factor batch for it (not a cache hit)
     };
     void worker(int w);
     // solve the jobs of one batch (same factor key) with one refinement;
