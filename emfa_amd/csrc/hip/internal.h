@@ -142,6 +142,7 @@ struct emfa_lu {
     int pivot_fallback = 0; // 1: a pivot-free attempt grew and was redone with interchanges
     void *dinv = nullptr;   // device: inverses of the 256 x 256 diagonal blocks of L, then of U
     size_t dinv_bytes = 0;
+    void *gh = nullptr;     // device (FP32 factors): L21 DL^-1 / U12 DU^-1 per 256-block column, N x N
 };
 
 namespace emfa {

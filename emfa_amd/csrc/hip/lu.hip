@@ -1390,6 +1390,51 @@ __global__ __launch_bounds__(256) void k_dinv_off(const cx<float> *__restrict__ 
             X[(size_t)(64 * i + ty * 4 + a) * kDinvB + 64 * j + tx + 16 * b] = {-out.v[a][b].re, -out.v[a][b].im};
 }
 
+// Inverse-scaled off-diagonal blocks of an FP32 factor, for one-launch solve
+// steps: G = L[rows below block b, block b] DL_b^-1 and H = U[rows above
+// block b, block b] DU_b^-1, each stored where its L or U block sits in GH
+// (N x N, row-major; the diagonal blocks of GH are not used).  Grid (row
+// tiles of 64, 8 per block: 4 column tiles x {L, U}, batch): a workgroup
+// forms one 64 x 64 tile from four tile products; tiles outside the
+// triangle return at once.
+__global__ __launch_bounds__(256) void k_dinv_scale(const cx<float> *__restrict__ F, int N,
+                                                    const cx<float> *__restrict__ Dinv, int nblk,
+                                                    cx<float> *__restrict__ GH, Batch bf, Batch bd, Batch bg) {
+    __shared__ cx<float> Ta[64][65], Tb[64][65];
+    const int r0 = 64 * blockIdx.x, b = blockIdx.y >> 3, upper = (blockIdx.y >> 2) & 1, ct = blockIdx.y & 3;
+    const int c0 = kDinvB * b + 64 * ct, tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    if (r0 >= N || c0 >= N) return;
+    if (upper ? r0 + 64 > kDinvB * b : r0 < kDinvB * (b + 1)) return;
+    F = batch_mat(bf, F);
+    Dinv = batch_mat(bd, Dinv);
+    GH = batch_mat(bg, GH);
+    const cx<float> *D = Dinv + (upper ? (size_t)nblk * kDinvB * kDinvB : 0) + (size_t)b * kDinvB * kDinvB;
+    TileAcc<float> acc;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc.v[a][q] = {0.f, 0.f};
+    for (int kc = 0; kc < 4; ++kc) {
+        const int k0 = kDinvB * b + 64 * kc;
+        if (k0 >= N) break;
+        if (kc) __syncthreads();
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int rr = e >> 6, cc = e & 63;
+            Ta[rr][cc] = (r0 + rr < N && k0 + cc < N) ? F[(size_t)(r0 + rr) * N + k0 + cc] : cx<float>{0.f, 0.f};
+            Tb[rr][cc] = D[(size_t)(64 * kc + rr) * kDinvB + 64 * ct + cc];
+        }
+        __syncthreads();
+        tile_mac(acc, Ta, Tb, ty, tx);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = r0 + ty * 4 + a, c = c0 + tx + 16 * q;
+            if (r < N && c < N) GH[(size_t)r * N + c] = acc.v[a][q];
+        }
+}
+
 // Y_b = Dinv_b W_b for one 256-row block (nb valid rows, m columns): 32 x 32
 // outputs per workgroup, K in chunks of 64 through LDS.  Writes Y (plain) and
 // the block's real-expanded transposed form Bt (2m x 2nb reals, the operand
@@ -1464,17 +1509,21 @@ constexpr int smallmm_rpt() {
     return 2;
 }
 
+template <int MC>
+constexpr int smallmm_rb() {
+    return smallmm_rpt<MC>() * (256 / MC);
+}
+
+// row block bx, K slice z of C (+)= sign A X (see k_smallmm); LDS from the caller
 template <typename TA, typename TC, int MC>
-__global__ __launch_bounds__(256) void k_smallmm(const cx<TA> *__restrict__ A, long long lda,
-                                                 const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C,
-                                                 long long ldc, int M, int K, int m, int kc, TC sign, int beta,
-                                                 cx<TC> *__restrict__ P) {
+__device__ __forceinline__ void smallmm_body(const cx<TA> *__restrict__ A, long long lda,
+                                             const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C, long long ldc,
+                                             int M, int K, int m, int kc, TC sign, int beta, cx<TC> *__restrict__ P,
+                                             int bx, int z, cx<TA> (*As)[65], cx<TC> (*Xs)[MC]) {
     constexpr int RPT = smallmm_rpt<MC>(), RG = 256 / MC, RB = RPT * RG, KC = 64;
     constexpr int AL = RB * KC / 256, XL = KC * MC / 256;  // chunk elements per thread
-    __shared__ cx<TA> As[RB][KC + 1];
-    __shared__ cx<TC> Xs[KC][MC];
     const int tid = threadIdx.x, tx = tid % MC, ty = tid / MC;
-    const int rb = blockIdx.x * RB, z = blockIdx.y;
+    const int rb = bx * RB;
     const int kb = z * kc, ke = min(K, kb + kc);
     cx<TA> ra[AL];
     cx<TC> rx[XL];
@@ -1536,6 +1585,34 @@ __global__ __launch_bounds__(256) void k_smallmm(const cx<TA> *__restrict__ A, l
     }
 }
 
+template <typename TA, typename TC, int MC>
+__global__ __launch_bounds__(256) void k_smallmm(const cx<TA> *__restrict__ A, long long lda,
+                                                 const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C,
+                                                 long long ldc, int M, int K, int m, int kc, TC sign, int beta,
+                                                 cx<TC> *__restrict__ P) {
+    __shared__ cx<TA> As[smallmm_rb<MC>()][65];
+    __shared__ cx<TC> Xs[64][MC];
+    smallmm_body<TA, TC, MC>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P, blockIdx.x, blockIdx.y, As, Xs);
+}
+
+// One step of a triangular solve through the diagonal-block inverses, in one
+// launch: workgroups [0, n0) form Y_b = D_b W_b on the block's rows (C0 =
+// D W, K = wb), the rest update the other rows with the same W_b through the
+// inverse-scaled off-diagonal block (C1 -= G W, G = L21 D_L^-1 or U12 D_U^-1
+// stored with the factor).  Both only read W_b and write disjoint rows.
+template <typename R, int MC>
+__global__ __launch_bounds__(256) void k_solve_step(const cx<R> *__restrict__ D, const cx<R> *__restrict__ G,
+                                                    long long ldg, const cx<R> *__restrict__ Wb, int m,
+                                                    cx<R> *__restrict__ Yb, cx<R> *__restrict__ C1, int wb, int M1,
+                                                    int n0) {
+    __shared__ cx<R> As[smallmm_rb<MC>()][65];
+    __shared__ cx<R> Xs[64][MC];
+    if ((int)blockIdx.x < n0)
+        smallmm_body<R, R, MC>(D, kDinvB, Wb, m, Yb, m, wb, wb, m, wb, R(1), 0, nullptr, blockIdx.x, 0, As, Xs);
+    else
+        smallmm_body<R, R, MC>(G, ldg, Wb, m, C1, m, M1, wb, m, wb, R(-1), 1, nullptr, blockIdx.x - n0, 0, As, Xs);
+}
+
 // launch k_smallmm with the column tile for m (<= 64); slices on gridDim.y
 template <typename TA, typename TC>
 static emfa_status smallmm(hipStream_t s, const cx<TA> *A, long long lda, const cx<TC> *X, long long ldx, cx<TC> *C,
@@ -1546,6 +1623,29 @@ static emfa_status smallmm(hipStream_t s, const cx<TA> *A, long long lda, const 
         constexpr int MC = decltype(mc)::value, RB = smallmm_rpt<MC>() * (256 / MC);
         const dim3 grid((M + RB - 1) / RB, slices);
         k_smallmm<TA, TC, MC><<<grid, 256, 0, s>>>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P);
+    };
+    if (m <= 8)
+        go(std::integral_constant<int, 8>{});
+    else if (m <= 16)
+        go(std::integral_constant<int, 16>{});
+    else if (m <= 32)
+        go(std::integral_constant<int, 32>{});
+    else
+        go(std::integral_constant<int, 64>{});
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+// one solve step (k_solve_step) with the column tile for m (<= 64)
+template <typename R>
+static emfa_status solve_step(hipStream_t s, const cx<R> *D, const cx<R> *G, long long ldg, const cx<R> *Wb, int m,
+                              cx<R> *Yb, cx<R> *C1, int wb, int M1) {
+    EMFA_REQUIRE(m >= 1 && m <= kSkinnyMaxCols && wb >= 1 && wb <= kDinvB && M1 >= 0, EMFA_E_INVALID,
+                 "solve step: bad shape m=%d wb=%d", m, wb);
+    auto go = [&](auto mc) {
+        constexpr int MC = decltype(mc)::value, RB = smallmm_rb<MC>();
+        const int n0 = (wb + RB - 1) / RB, n1 = (M1 + RB - 1) / RB;
+        k_solve_step<R, MC><<<n0 + n1, 256, 0, s>>>(D, G, ldg, Wb, m, Yb, C1, wb, M1, n0);
     };
     if (m <= 8)
         go(std::integral_constant<int, 8>{});
@@ -2098,7 +2198,7 @@ emfa_status lu_factor_inplace(emfa_ctx *ctx, emfa_mat *A, int *ipiv, int *info, 
 // Solve with the factor (same precision as B).
 template <typename R>
 static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipiv, const int *perm, emfa_mat *Bm,
-                              const void *dinv, int interchanges) {
+                              const void *dinv, int interchanges, const void *gh = nullptr) {
     const int N = LUm->rows, m = Bm->cols;
     const cx<R> *F = (const cx<R> *)LUm->d;
     cx<R> *B = (cx<R> *)Bm->d;
@@ -2140,8 +2240,18 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
             EMFA_HIP_CHECK(hipGetLastError());
             return EMFA_OK;
         };
+        // with the inverse-scaled blocks GH a step is one launch: the apply and
+        // the other rows' update both read the step's input block
+        const cx<R> *GH = skinny ? (const cx<R> *)gh : nullptr;
         for (int b = 0; b < nblk; ++b) {
             const int r0 = b * kDinvB, wb = std::min(kDinvB, N - r0);
+            if (GH) {
+                st = solve_step<R>(ctx->stream, DL + (size_t)b * kDinvB * kDinvB, GH + (size_t)(r0 + wb) * ldf + r0,
+                                   ldf, W + (size_t)r0 * m, m, Y + (size_t)r0 * m, W + (size_t)(r0 + wb) * m, wb,
+                                   N - r0 - wb);
+                if (st) return st;
+                continue;
+            }
             st = apply(DL + (size_t)b * kDinvB * kDinvB, wb, W + (size_t)r0 * m, Y + (size_t)r0 * m);
             if (st) return st;
             st = update(N - r0 - wb, F + (size_t)(r0 + wb) * ldf + r0, Y + (size_t)r0 * m, W + (size_t)(r0 + wb) * m,
@@ -2150,6 +2260,12 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
         }
         for (int b = nblk - 1; b >= 0; --b) {
             const int r0 = b * kDinvB, wb = std::min(kDinvB, N - r0);
+            if (GH) {
+                st = solve_step<R>(ctx->stream, DU + (size_t)b * kDinvB * kDinvB, GH + r0, ldf, Y + (size_t)r0 * m, m,
+                                   B + (size_t)r0 * m, Y, wb, r0);
+                if (st) return st;
+                continue;
+            }
             st = apply(DU + (size_t)b * kDinvB * kDinvB, wb, Y + (size_t)r0 * m, B + (size_t)r0 * m);
             if (st) return st;
             st = update(r0, F + r0, B + (size_t)r0 * m, Y, wb);
@@ -2214,7 +2330,7 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
 
 emfa_status lu_solve_inplace(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B) {
     if (lu->a->prec == EMFA_F32)
-        return solve_impl<float>(ctx, lu->a, lu->ipiv, lu->perm, B, lu->dinv, lu->interchanges);
+        return solve_impl<float>(ctx, lu->a, lu->ipiv, lu->perm, B, lu->dinv, lu->interchanges, lu->gh);
     return solve_impl<double>(ctx, lu->a, lu->ipiv, lu->perm, B, lu->dinv, lu->interchanges);
 }
 
@@ -2224,6 +2340,22 @@ static void dinv_launch_f32(hipStream_t s, const cx<float> *F, int N, cx<float> 
                             const Batch &bd) {
     k_dinv_diag<<<dim3(nblk * 4, 2, bf.count), 256, 0, s>>>(F, N, N, D, nblk, bf, bd);
     for (int d = 1; d < 4; ++d) k_dinv_off<<<dim3(nblk * (4 - d), 2, bf.count), 256, 0, s>>>(F, N, N, D, nblk, d, bf, bd);
+}
+
+// the inverse-scaled off-diagonal blocks (GH) after the inverses
+static void gh_launch_f32(hipStream_t s, const cx<float> *F, int N, const cx<float> *D, int nblk, cx<float> *GH,
+                          const Batch &bf, const Batch &bd, const Batch &bg) {
+    k_dinv_scale<<<dim3((N + 63) / 64, nblk * 8, bf.count), 256, 0, s>>>(F, N, D, nblk, GH, bf, bd, bg);
+}
+
+// With EMFA_SOLVE_GH=1, FP32 factors also keep L21 DL^-1 / U12 DU^-1 for
+// one-launch solve steps: a solve at N = 7680 takes 1.3 ms instead of 2.5,
+// but building them costs ~2.5 ms per factor there and n^2 more memory, and
+// in a saturated solve server (factors reused ~1.6 times, eight workers) it
+// cost 13 % of the throughput, so it is off by default.
+static bool gh_wanted() {
+    const char *e = getenv("EMFA_SOLVE_GH");
+    return e && atoi(e) != 0;
 }
 
 static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
@@ -2241,6 +2373,15 @@ static emfa_status build_dinv(emfa_ctx *ctx, emfa_lu *lu) {
         const cx<float> *F = (const cx<float> *)lu->a->d;
         cx<float> *D = (cx<float> *)lu->dinv;
         dinv_launch_f32(ctx->stream, F, N, D, nblk, Batch(), Batch());
+        if (gh_wanted()) {
+            if (ctx_alloc(ctx, &lu->gh, (size_t)N * N * sizeof(cx<float>)) != hipSuccess) {
+                (void)hipGetLastError();
+                lu->gh = nullptr;
+                set_error("emfa_lu_factor: scaled off-diagonal block allocation failed");
+                return EMFA_E_OOM;
+            }
+            gh_launch_f32(ctx->stream, F, N, D, nblk, (cx<float> *)lu->gh, Batch(), Batch(), Batch());
+        }
     } else {
         const cx<double> *F = (const cx<double> *)lu->a->d;
         cx<double> *D = (cx<double> *)lu->dinv;
@@ -2461,12 +2602,13 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
     }
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     std::vector<emfa_lu *> lus(count, nullptr);
-    void *dmat = nullptr, *ddinv = nullptr;
+    void *dmat = nullptr, *ddinv = nullptr, *dgh = nullptr;
     auto fail = [&](emfa_status st) {
         (void)hipStreamSynchronize(ctx->stream);
         ctx->batch = Batch();
         if (dmat) ctx_free(ctx, dmat, sizeof(void *) * count);
         if (ddinv) ctx_free(ctx, ddinv, sizeof(void *) * count);
+        if (dgh) ctx_free(ctx, dgh, sizeof(void *) * count);
         for (auto l : lus)
             if (l) emfa_lu_destroy(l);
         return st;
@@ -2489,7 +2631,7 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
         if ((st = convert_copy(ctx, Z[z], lu->a))) return fail(st);
     }
     // device array of the factors' bases; host copies stay alive until the stream syncs below
-    std::vector<const char *> hmat(count), hdinv(count);
+    std::vector<const char *> hmat(count), hdinv(count), hgh(count);
     for (int z = 0; z < count; ++z) hmat[z] = (const char *)lus[z]->a->d;
     if (ctx_alloc(ctx, &dmat, sizeof(void *) * count) != hipSuccess) {
         (void)hipGetLastError();
@@ -2558,10 +2700,35 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
         ProfScope ps(ctx, "lu_dinv", 0);
         dinv_launch_f32(ctx->stream, (const cx<float> *)lus[0]->a->d, n, (cx<float> *)lus[0]->dinv, nblk, bf, bd);
         EMFA_HIP_CHECK(hipGetLastError());
+        if (gh_wanted()) {
+            for (int z = 0; z < count; ++z) {
+                if (ctx_alloc(ctx, &lus[z]->gh, (size_t)n * n * sizeof(cx<float>)) != hipSuccess) {
+                    (void)hipGetLastError();
+                    lus[z]->gh = nullptr;
+                    set_error("emfa_lu_factor_batch: scaled off-diagonal block allocation failed");
+                    return fail(EMFA_E_OOM);
+                }
+                hgh[z] = (const char *)lus[z]->gh;
+            }
+            if (ctx_alloc(ctx, &dgh, sizeof(void *) * count) != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("emfa_lu_factor_batch: allocation failed");
+                return fail(EMFA_E_OOM);
+            }
+            EMFA_HIP_CHECK(hipMemcpyAsync(dgh, hgh.data(), sizeof(void *) * count, hipMemcpyHostToDevice, ctx->stream));
+            Batch bg;
+            bg.mat = (const char *const *)dgh;
+            bg.mat0 = hgh[0];
+            bg.count = count;
+            gh_launch_f32(ctx->stream, (const cx<float> *)lus[0]->a->d, n, (const cx<float> *)lus[0]->dinv, nblk,
+                          (cx<float> *)lus[0]->gh, bf, bd, bg);
+            EMFA_HIP_CHECK(hipGetLastError());
+        }
     }
     EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx_free(ctx, dmat, sizeof(void *) * count);
     ctx_free(ctx, ddinv, sizeof(void *) * count);
+    if (dgh) ctx_free(ctx, dgh, sizeof(void *) * count);
     emfa_status worst = EMFA_OK;
     for (int z = 0; z < count; ++z) {
         out[z] = lus[z];
@@ -2580,6 +2747,7 @@ emfa_status emfa_lu_destroy(emfa_lu *lu) {
     ctx_free(c, lu->ipiv, sizeof(int) * lu->a->rows);
     if (lu->perm) ctx_free(c, lu->perm, sizeof(int) * lu->a->rows);
     if (lu->dinv) ctx_free(c, lu->dinv, lu->dinv_bytes);
+    if (lu->gh) ctx_free(c, lu->gh, (size_t)lu->a->rows * lu->a->rows * sizeof(cx<float>));
     emfa_mat_destroy(lu->a);
     delete lu;
     return EMFA_OK;

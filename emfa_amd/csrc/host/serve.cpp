@@ -71,6 +71,11 @@ static void build_mesh(const MeshSpec &m, std::vector<double> &xyz, std::vector<
     if (st) throw StatusError(st, emfa_last_error());
 }
 
+static bool env_flag(const char *name, bool dflt) {
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) != 0 : dflt;
+}
+
 // RWG count of a mesh spec without building it (closed icosphere: 1.5 T =
 // 30 4^s edges; plate of c x c cells: 3 c^2 - 2 c interior edges)
 static int spec_unknowns(const MeshSpec &m) {
@@ -89,15 +94,12 @@ size_t Server::footprint(int n, int nrhs, int precision, bool factor_cached) {
     size_t ws = 2 * (2 * (size_t)n) * nbr * (e / 2) + (1u << 20);
     if (precision == EMFA_F32) ws += 2 * nbr * 2 * (rows + 2 * rows + 256);
     // the factor keeps the inverses of its 256 x 256 diagonal blocks (L and U)
-    const size_t dinv = 2 * (((size_t)n + 255) / 256) * 256 * 256 * e;
+    // and, in FP32, its off-diagonal blocks scaled by them (n x n)
+    size_t dinv = 2 * (((size_t)n + 255) / 256) * 256 * 256 * e;
+    if (precision == EMFA_F32 && env_flag("EMFA_SOLVE_GH", false)) dinv += (size_t)n * n * e;
     // solve workspace: two n x nrhs vectors and an expanded block
     ws += 2 * (size_t)n * nrhs * e + 2 * (size_t)nrhs * 512 * e;
     return 2 * (size_t)n * n * e + dinv + vecs + ws;  // Z + factor + workspace
-}
-
-static bool env_flag(const char *name, bool dflt) {
-    const char *v = std::getenv(name);
-    return v ? std::atoi(v) != 0 : dflt;
 }
 
 Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
@@ -460,7 +462,9 @@ std::shared_ptr<Server::Factor> Server::make_factor(emfa_ctx *ctx, const Request
     check(emfa_mat_create(ctx, n, n, prec, &nf->Z), "Z");
     emfa_fill_params fp{r.formulation, r.quad, k, r.alpha, 0.0, 0.0, 0};
     check(emfa_fill(ctx, nf->mesh, &fp, nf->Z), "fill");
-    nf->bytes = 2 * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16) +
+    // Z, the factor, the diagonal-block inverses and (FP32, EMFA_SOLVE_GH=1) the scaled off-diagonal blocks
+    const bool gh = prec == EMFA_F32 && env_flag("EMFA_SOLVE_GH", false);
+    nf->bytes = (gh ? 3 : 2) * (size_t)n * n * (prec == EMFA_F32 ? 8 : 16) +
                 2 * (((size_t)n + 255) / 256) * 256 * 256 * (prec == EMFA_F32 ? 8 : 16);
     return nf;
 }

@@ -498,3 +498,38 @@ def test_batched_factor_matches_single_factors_and_redoes_a_grown_member(ctx):
     finally:
         ctx.lu_params(0, 0, -1)
         ctx.lu_update_mode(0)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_one_launch_solve_steps_match_two_launch_steps(ctx, monkeypatch, variant):
+    # EMFA_SOLVE_GH=1: FP32 factors keep L21 DL^-1 / U12 DU^-1 and solve with
+    # one launch per 256-row step; answers match the two-launch steps
+    _say(f"scaled-block solve variant={variant}")
+    rng = np.random.default_rng(17 + variant)
+    n = 1300
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * np.sqrt(n) * np.eye(n)
+    ctx.lu_params(256, 0, variant)
+    ctx.lu_update_mode(1)
+    try:
+        Z = hip.Mat.from_numpy(ctx, A, hip.F32)
+        lu0 = hip.LU(ctx, Z, hip.F32)
+        monkeypatch.setenv("EMFA_SOLVE_GH", "1")
+        lu1 = hip.LU(ctx, Z, hip.F32)
+        Zs = [hip.Mat.from_numpy(ctx, X, hip.F32) for X in (A, A.T.copy())]
+        lub = hip.lu_factor_batch(ctx, Zs, hip.F32)
+        for m in (1, 13, 40, 64):
+            B = (rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))).astype(np.complex64)
+            x0, x1, xb = (hip.Mat.from_numpy(ctx, B, hip.F32) for _ in range(3))
+            lu0.solve(x0)
+            lu1.solve(x1)
+            lub[0].solve(xb)
+            ref = x0.numpy()
+            assert rel(x1.numpy(), ref) < 1e-5 and rel(xb.numpy(), ref) < 1e-5
+            assert rel(ref, np.linalg.solve(A.astype(np.complex64).astype(np.complex128), B)) < 1e-4
+            for x in (x0, x1, xb):
+                x.close()
+        for m in [lu0, lu1, Z] + lub + Zs:
+            m.close()
+    finally:
+        ctx.lu_params(0, 0, -1)
+        ctx.lu_update_mode(0)
