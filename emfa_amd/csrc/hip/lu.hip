@@ -556,8 +556,10 @@ __device__ __forceinline__ double bcast<double>(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// cols_per_wg = kTrsmColsPerPass * passes; workgroup `bx` of the solve
-template <typename R>
+// cols_per_wg = 4 CPW passes; workgroup `bx` of the solve.  CPW columns per
+// wave: more interleave more independent chains per wave and need fewer
+// workgroups (less CU time in all), at a longer time per workgroup.
+template <typename R, int CPW = kTrsmCPW>
 __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
                                             long long ldx, int r0, int wb, int c0, int c1, int upper,
                                             R *__restrict__ Bt, long long ldbt, int kofs, int passes, int bx,
@@ -568,15 +570,15 @@ __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long lo
         Lt[i][r] = F[(size_t)(r0 + r) * ldf + r0 + i];
     }
     __syncthreads();
-    const int cols_per_wg = kTrsmColsPerPass * passes;
+    const int cols_per_wg = (4 * CPW) * passes;
     const int cb = c0 + bx * cols_per_wg;
     for (int ps = 0; ps < passes; ++ps) {
-        int col[kTrsmCPW];
-        bool ok[kTrsmCPW];
-        cx<R> x[kTrsmCPW];
+        int col[CPW];
+        bool ok[CPW];
+        cx<R> x[CPW];
 #pragma unroll
-        for (int q = 0; q < kTrsmCPW; ++q) {
-            col[q] = cb + ps * kTrsmColsPerPass + wave * kTrsmCPW + q;
+        for (int q = 0; q < CPW; ++q) {
+            col[q] = cb + ps * (4 * CPW) + wave * CPW + q;
             ok[q] = col[q] < c1;
             x[q] = (ok[q] && lane < wb) ? X[(size_t)(r0 + lane) * ldx + col[q]] : cx<R>{R(0), R(0)};
         }
@@ -585,7 +587,7 @@ __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long lo
                 cx<R> l = Lt[i][lane];
                 const bool act = lane > i && lane < wb;
 #pragma unroll
-                for (int q = 0; q < kTrsmCPW; ++q) {
+                for (int q = 0; q < CPW; ++q) {
                     R xr = bcast(x[q].re, i), xi = bcast(x[q].im, i);
                     if (act) {
                         x[q].re -= l.re * xr - l.im * xi;
@@ -600,7 +602,7 @@ __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long lo
                 cx<R> inv = den > R(0) ? cx<R>{d.re / den, -d.im / den} : cx<R>{R(0), R(0)};
                 cx<R> u = Lt[i][lane];
 #pragma unroll
-                for (int q = 0; q < kTrsmCPW; ++q) {
+                for (int q = 0; q < CPW; ++q) {
                     if (lane == i) x[q] = cmul(x[q], inv);
                     R xr = bcast(x[q].re, i), xi = bcast(x[q].im, i);
                     if (lane < i) {
@@ -611,7 +613,7 @@ __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long lo
             }
         }
 #pragma unroll
-        for (int q = 0; q < kTrsmCPW; ++q) {
+        for (int q = 0; q < CPW; ++q) {
             if (!ok[q] || lane >= wb) continue;
             X[(size_t)(r0 + lane) * ldx + col[q]] = x[q];
             if (Bt) {
@@ -628,14 +630,14 @@ __device__ __forceinline__ void trsm64_body(const cx<R> *__restrict__ F, long lo
 }
 
 // With a batch, F and X are in the batch's matrices and Bt in its scratch.
-template <typename R>
+template <typename R, int CPW>
 __global__ __launch_bounds__(256) void k_trsm64(const cx<R> *__restrict__ F, long long ldf, cx<R> *__restrict__ X,
                                                 long long ldx, int r0, int wb, int c0, int c1, int upper,
                                                 R *__restrict__ Bt, long long ldbt, int kofs, int passes,
                                                 Batch bt = Batch()) {
     __shared__ cx<R> Lt[64][65];
-    trsm64_body<R>(batch_mat(bt, F), ldf, batch_mat(bt, X), ldx, r0, wb, c0, c1, upper, Bt ? batch_scr(bt, Bt) : Bt,
-                   ldbt, kofs, passes, blockIdx.x, Lt);
+    trsm64_body<R, CPW>(batch_mat(bt, F), ldf, batch_mat(bt, X), ldx, r0, wb, c0, c1, upper,
+                        Bt ? batch_scr(bt, Bt) : Bt, ldbt, kofs, passes, blockIdx.x, Lt);
 }
 
 // --------------------------------------------------------------------------
@@ -1021,18 +1023,31 @@ static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx
     if (c1 <= c0 || wb <= 0) return EMFA_OK;
     const long long ldbt = 2LL * wb;
     const int ncols = c1 - c0;
+    // columns per wave: 8 for the LU's U rows in contexts that share the
+    // device (a solve server: fewer workgroups, less CU time while the others
+    // wait for it), else 2 (shortest time per workgroup); EMFA_TRSM_CPW sets it
+    int wide = ctx->cu_share > 0 && std::strcmp(fam, "lu_trsm") == 0 ? 8 : 2;
+    if (const char *e = getenv("EMFA_TRSM_CPW")) wide = atoi(e) == 8 ? 8 : 2;
+    const int cpp = 4 * wide;  // columns per pass of a workgroup
     // enough workgroups to fill the chip; more columns per workgroup when wide
-    int passes = std::max(1, std::min(8, ncols / (kTrsmColsPerPass * 1024)));
-    const int cpw = kTrsmColsPerPass * passes;
+    int passes = std::max(1, std::min(8, ncols / (cpp * 1024)));
+    const int cpw = cpp * passes;
     const int blocks = (ncols + cpw - 1) / cpw;
+    auto trsm = [&](int r, int b, int up, int kofs) {
+        const dim3 g(blocks, 1, ctx->batch.count);
+        if (wide == 8)
+            k_trsm64<R, 8><<<g, 256, 0, ctx->stream>>>(F, ldf, X, ldx, r, b, c0, c1, up, Bt, ldbt, kofs, passes,
+                                                      ctx->batch);
+        else
+            k_trsm64<R, 2><<<g, 256, 0, ctx->stream>>>(F, ldf, X, ldx, r, b, c0, c1, up, Bt, ldbt, kofs, passes,
+                                                      ctx->batch);
+    };
     if (!upper) {
         for (int s = 0; s < wb; s += 64) {
             int b = std::min(64, wb - s);
             {
                 ProfScope ps(ctx, fam, 0);
-                k_trsm64<R><<<dim3(blocks, 1, ctx->batch.count), 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0,
-                                                                                       c1, 0, Bt, ldbt, s, passes,
-                                                                                       ctx->batch);
+                trsm(r0 + s, b, 0, s);
                 EMFA_HIP_CHECK(hipGetLastError());
             }
             if (s + b < wb) {
@@ -1048,9 +1063,7 @@ static emfa_status trsm_blocked(emfa_ctx *ctx, const cx<R> *F, long long ldf, cx
             int b = std::min(64, wb - s);
             {
                 ProfScope ps(ctx, fam, 0);
-                k_trsm64<R><<<dim3(blocks, 1, ctx->batch.count), 256, 0, ctx->stream>>>(F, ldf, X, ldx, r0 + s, b, c0,
-                                                                                       c1, 1, Bt, ldbt, s, passes,
-                                                                                       ctx->batch);
+                trsm(r0 + s, b, 1, s);
                 EMFA_HIP_CHECK(hipGetLastError());
             }
             if (s > 0) {
