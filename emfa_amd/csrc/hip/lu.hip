@@ -1522,18 +1522,18 @@ constexpr int smallmm_rpt() {
     return 2;
 }
 
-template <int MC>
+template <int MC, int RPT = smallmm_rpt<MC>()>
 constexpr int smallmm_rb() {
-    return smallmm_rpt<MC>() * (256 / MC);
+    return RPT * (256 / MC);
 }
 
 // row block bx, K slice z of C (+)= sign A X (see k_smallmm); LDS from the caller
-template <typename TA, typename TC, int MC>
+template <typename TA, typename TC, int MC, int RPT = smallmm_rpt<MC>()>
 __device__ __forceinline__ void smallmm_body(const cx<TA> *__restrict__ A, long long lda,
                                              const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C, long long ldc,
                                              int M, int K, int m, int kc, TC sign, int beta, cx<TC> *__restrict__ P,
                                              int bx, int z, cx<TA> (*As)[65], cx<TC> (*Xs)[MC]) {
-    constexpr int RPT = smallmm_rpt<MC>(), RG = 256 / MC, RB = RPT * RG, KC = 64;
+    constexpr int RG = 256 / MC, RB = RPT * RG, KC = 64;
     constexpr int AL = RB * KC / 256, XL = KC * MC / 256;  // chunk elements per thread
     const int tid = threadIdx.x, tx = tid % MC, ty = tid / MC;
     const int rb = bx * RB;
@@ -1598,14 +1598,15 @@ __device__ __forceinline__ void smallmm_body(const cx<TA> *__restrict__ A, long 
     }
 }
 
-template <typename TA, typename TC, int MC>
+template <typename TA, typename TC, int MC, int RPT>
 __global__ __launch_bounds__(256) void k_smallmm(const cx<TA> *__restrict__ A, long long lda,
                                                  const cx<TC> *__restrict__ X, long long ldx, cx<TC> *C,
                                                  long long ldc, int M, int K, int m, int kc, TC sign, int beta,
                                                  cx<TC> *__restrict__ P) {
-    __shared__ cx<TA> As[smallmm_rb<MC>()][65];
+    __shared__ cx<TA> As[smallmm_rb<MC, RPT>()][65];
     __shared__ cx<TC> Xs[64][MC];
-    smallmm_body<TA, TC, MC>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P, blockIdx.x, blockIdx.y, As, Xs);
+    smallmm_body<TA, TC, MC, RPT>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P, blockIdx.x, blockIdx.y, As,
+                                  Xs);
 }
 
 // One step of a triangular solve through the diagonal-block inverses, in one
@@ -1626,16 +1627,29 @@ __global__ __launch_bounds__(256) void k_solve_step(const cx<R> *__restrict__ D,
         smallmm_body<R, R, MC>(G, ldg, Wb, m, C1, m, M1, wb, m, wb, R(-1), 1, nullptr, blockIdx.x - n0, 0, As, Xs);
 }
 
-// launch k_smallmm with the column tile for m (<= 64); slices on gridDim.y
+// launch k_smallmm with the column tile for m (<= 64); slices on gridDim.y.
+// rpt: rows per lane, 2 (short products: the most workgroups) or 8 (long-K
+// products such as the refinement residual: every workgroup reads its slice
+// of X, so fewer, taller workgroups read it fewer times; 32 / 64-column tiles)
 template <typename TA, typename TC>
 static emfa_status smallmm(hipStream_t s, const cx<TA> *A, long long lda, const cx<TC> *X, long long ldx, cx<TC> *C,
-                           long long ldc, int M, int K, int m, int slices, int kc, TC sign, int beta, cx<TC> *P) {
+                           long long ldc, int M, int K, int m, int slices, int kc, TC sign, int beta, cx<TC> *P,
+                           int rpt = 2) {
     if (M <= 0 || m <= 0) return EMFA_OK;
     EMFA_REQUIRE(m <= kSkinnyMaxCols && slices >= 1 && kc > 0, EMFA_E_INVALID, "smallmm: bad shape m=%d", m);
     auto go = [&](auto mc) {
-        constexpr int MC = decltype(mc)::value, RB = smallmm_rpt<MC>() * (256 / MC);
+        constexpr int MC = decltype(mc)::value;
+        if constexpr (MC >= 32) {
+            if (rpt == 8) {
+                constexpr int RB = smallmm_rb<MC, 8>();
+                const dim3 grid((M + RB - 1) / RB, slices);
+                k_smallmm<TA, TC, MC, 8><<<grid, 256, 0, s>>>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P);
+                return;
+            }
+        }
+        constexpr int RB = smallmm_rb<MC, 2>();
         const dim3 grid((M + RB - 1) / RB, slices);
-        k_smallmm<TA, TC, MC><<<grid, 256, 0, s>>>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P);
+        k_smallmm<TA, TC, MC, 2><<<grid, 256, 0, s>>>(A, lda, X, ldx, C, ldc, M, K, m, kc, sign, beta, P);
     };
     if (m <= 8)
         go(std::integral_constant<int, 8>{});
@@ -2425,7 +2439,10 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     if (m <= skinny_max) {
         // row blocks x K slices (summed in slice order) on the VALU
         const int mc = m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
-        const int rbk = 2 * (256 / mc);  // rows per workgroup of k_smallmm
+        // tall residuals with wide column tiles: 8 rows per lane (X re-read 4x less)
+        int rpt = N >= 2048 && mc >= 32 ? 8 : 2;
+        if (const char *e = getenv("EMFA_RESID_RPT")) rpt = atoi(e) == 8 && mc >= 32 ? 8 : 2;
+        const int rbk = rpt * (256 / mc);  // rows per workgroup of k_smallmm
         const long long wgs = (N + rbk - 1) / rbk;
         int S = (int)std::min<long long>(32, std::max<long long>(1, (4LL * ctx->num_cu + wgs - 1) / wgs));
         if (const char *e = getenv("EMFA_RESID_SPLIT")) S = std::max(1, std::min(64, atoi(e)));
@@ -2443,9 +2460,9 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
         emfa_status s2 =
             A->prec == EMFA_F32
                 ? smallmm<float, double>(ctx->stream, (const cx<float> *)A->d, K, (const cx<double> *)X->d, m,
-                                         (cx<double> *)C->d, m, N, K, m, S, kc, sign, accumulate, P)
+                                         (cx<double> *)C->d, m, N, K, m, S, kc, sign, accumulate, P, rpt)
                 : smallmm<double, double>(ctx->stream, (const cx<double> *)A->d, K, (const cx<double> *)X->d, m,
-                                          (cx<double> *)C->d, m, N, K, m, S, kc, sign, accumulate, P);
+                                          (cx<double> *)C->d, m, N, K, m, S, kc, sign, accumulate, P, rpt);
         if (s2) return s2;
         if (S > 1) {
             const long long tot2 = (long long)N * 2 * m;
