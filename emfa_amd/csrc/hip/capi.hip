@@ -25,8 +25,12 @@ void set_error(const char *fmt, ...) {
 // handed out again.  A freed block carries an event recorded on the freeing
 // context's stream; a context that takes it first makes both its streams
 // wait on that event, so reuse across contexts stays in stream order.  The
-// cache is trimmed when it exceeds EMFA_POOL_CACHE_MB (default 32768), when
-// hipMalloc fails, and when the device's last context is destroyed.
+// cache is trimmed when it exceeds EMFA_POOL_CACHE_MB (default 32768) and
+// when hipMalloc fails.  It outlives the contexts: the next context in the
+// process (a new solve server) takes its blocks instead of allocating again.
+// Handing tens of GB back to the driver at each server's end and allocating
+// them again for the next one stalled every worker's first allocation of the
+// new server for 0.6-2.5 s now and then.
 struct PoolBlock {
     void *p;
     size_t bytes;
@@ -39,7 +43,6 @@ struct DevicePool {
     std::map<void *, size_t> live;  // handed-out block -> its size
     std::vector<hipEvent_t> events;
     size_t cached = 0;
-    int contexts = 0;
 };
 
 static DevicePool &pool(int device) {
@@ -133,21 +136,6 @@ void ctx_free(emfa_ctx *ctx, void *p, size_t bytes) {
     P.cached += rb;
     ctx->bytes -= rb;
     if (P.cached > pool_limit()) pool_trim_locked(P, pool_limit() / 2);
-}
-
-static void pool_attach(int device) {
-    DevicePool &P = pool(device);
-    std::lock_guard<std::mutex> l(P.mu);
-    ++P.contexts;
-}
-
-static void pool_detach(int device) {
-    DevicePool &P = pool(device);
-    std::lock_guard<std::mutex> l(P.mu);
-    if (--P.contexts > 0) return;
-    pool_trim_locked(P, 0);
-    for (auto e : P.events) (void)hipEventDestroy(e);
-    P.events.clear();
 }
 
 emfa_status ctx_scratch(emfa_ctx *ctx, size_t bytes, void **p) {
@@ -252,12 +240,10 @@ emfa_status emfa_ctx_create(int device, emfa_ctx **out) {
         return EMFA_E_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    pool_attach(device);
     void *f = nullptr;
     if (ctx_alloc(c, &f, 64) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         (void)hipStreamDestroy(c->aux);
-        pool_detach(device);
         delete c;
         set_error("flag allocation failed");
         return EMFA_E_OOM;
@@ -281,7 +267,6 @@ emfa_status emfa_ctx_destroy(emfa_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->aux);
     (void)hipStreamDestroy(ctx->stream);
-    pool_detach(ctx->device);
     delete ctx;
     return EMFA_OK;
 }
