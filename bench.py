@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--mixed-requests", type=int, default=32)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=96.0)
+    ap.add_argument("--served-passes", type=int, default=5,
+                    help="saturated server passes over the request stream; the median is reported")
     return ap.parse_args()
 
 
@@ -126,12 +128,13 @@ def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
     }
 
 
-def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None):
+def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5):
     """Answer `reqs` on the plain FP64 path (sequential, nothing cached) and
     through the solve server (FP32+refine, `workers` contexts, resident
-    factors): once with every request queued at t=0 (capacity), once on
-    Poisson arrivals at `latency_rate` x that capacity (latency), each on a
-    fresh server.  Returns a summary dict."""
+    factors): `passes` times with every request queued at t=0 (capacity: the
+    median pass; one pass varies by about +-3 %), then once on Poisson
+    arrivals at `latency_rate` x that capacity (latency), each pass on a
+    fresh server.  Every pass's RCS is checked.  Returns a summary dict."""
     from emfa_amd import hip, serve, workload
     precision = hip.F32 if precision is None else precision
     # Kernel code objects load on first use: both paths are warmed on the
@@ -146,22 +149,28 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None):
     ctx.close()
     with serve.Server(workers=workers, hbm_budget=budget) as srv:
         workload.run_served(srv, reqs, precision)
-    with serve.Server(workers=workers, hbm_budget=budget) as srv:
-        sat, st = workload.run_served(srv, reqs, precision)
+    runs = []
+    for _ in range(max(1, passes)):
+        with serve.Server(workers=workers, hbm_budget=budget) as srv:
+            runs.append(workload.run_served(srv, reqs, precision))
+    per_pass = [st_["solves_per_s"] for _, st_ in runs]
+    st = runs[sorted(range(len(runs)), key=lambda i: per_pass[i])[len(runs) // 2]][1]
     cap = st["solves_per_s"]
     with serve.Server(workers=workers, hbm_budget=budget) as srv:
         lat, st2 = workload.run_served(srv, reqs, precision, rate=latency_rate * cap)
-    err = max(float(np.abs(p - r["rcs_db"]).max()) for p, r in zip(plain, sat))
+    err = max(float(np.abs(p - r["rcs_db"]).max()) for out, _ in runs for p, r in zip(plain, out))
     err2 = max(float(np.abs(p - r["rcs_db"]).max()) for p, r in zip(plain, lat))
+    fb = max(int(sum(r["fp64_fallback"] for r in out)) for out, _ in runs)
     ns = [workload.unknowns(r) for r in reqs]
     pl = np.sort(plain_lat)
     return {
         "requests": len(reqs), "unknowns_min": int(min(ns)), "unknowns_max": int(max(ns)),
         "rhs_total": int(sum(len(r["angles"]) for r in reqs)),
         "workers": workers, "hbm_budget_gb": budget / 2**30,
-        "served_solves_per_s": cap, "served_rhs_per_s": st["rhs_per_s"], "served_wall_s": st["wall_s"],
-        "served_cache_hits": st["cache_hits"], "served_failed": st["failed"],
-        "served_fp64_fallbacks": int(sum(r["fp64_fallback"] for r in sat)),
+        "served_solves_per_s": cap, "served_passes_solves_per_s": per_pass,
+        "served_rhs_per_s": st["rhs_per_s"], "served_wall_s": st["wall_s"],
+        "served_cache_hits": st["cache_hits"], "served_failed": max(st_["failed"] for _, st_ in runs),
+        "served_fp64_fallbacks": fb,
         "served_pivot_fallbacks": int(st.get("pivot_fallbacks", 0)),
         "served_batched_requests": int(st.get("batched", 0)),
         "served_factor_batches": int(st.get("factor_batches", 0)),
@@ -304,7 +313,8 @@ def main():
     mixed = None
     if not a.no_mixed:
         from emfa_amd import workload
-        mixed = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers, int(a.budget_gb * 2**30))
+        mixed = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers, int(a.budget_gb * 2**30),
+                           passes=a.served_passes)
 
     cpu = None
     if not a.no_cpu:
@@ -438,7 +448,7 @@ def main_stream(a):
         reqs = workload.mixed_stream(a.steps)
         name = ("mixed stream: spheres/ellipsoids (s=3,4) and 1x1 m plates, 500-15000 RWG, EFIE/CFIE, 1-64 RHS, "
                 "30% repeat meshes")
-    leg = stream_leg(reqs, a.workers, int(a.budget_gb * 2**30))
+    leg = stream_leg(reqs, a.workers, int(a.budget_gb * 2**30), passes=a.served_passes)
     lr = leg["latency_run"]
     out = {
         "metric": "solves/sec (served FP32+refine) + p50/p99 latency",

@@ -1,4 +1,5 @@
 // C ABI: errors, contexts, meshes, matrices, profiling.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -25,8 +26,10 @@ void set_error(const char *fmt, ...) {
 // handed out again.  A freed block carries an event recorded on the freeing
 // context's stream; a context that takes it first makes both its streams
 // wait on that event, so reuse across contexts stays in stream order.  The
-// cache is trimmed when it exceeds EMFA_POOL_CACHE_MB (default 32768) and
-// when hipMalloc fails.  It outlives the contexts: the next context in the
+// cache is trimmed, longest-cached blocks first, to three quarters of its
+// limit when it exceeds it (EMFA_POOL_CACHE_MB, default half the device's
+// memory, so a solve server's whole working set fits), and emptied when
+// hipMalloc fails.  It outlives the contexts: the next context in the
 // process (a new solve server) takes its blocks instead of allocating again.
 // Handing tens of GB back to the driver at each server's end and allocating
 // them again for the next one stalled every worker's first allocation of the
@@ -35,6 +38,7 @@ struct PoolBlock {
     void *p;
     size_t bytes;
     hipEvent_t ev;
+    unsigned long long seq;  // order of the frees
 };
 
 struct DevicePool {
@@ -43,6 +47,8 @@ struct DevicePool {
     std::map<void *, size_t> live;  // handed-out block -> its size
     std::vector<hipEvent_t> events;
     size_t cached = 0;
+    unsigned long long seq = 0;
+    size_t limit = 0;  // set with the first allocation on the device
 };
 
 static DevicePool &pool(int device) {
@@ -50,12 +56,18 @@ static DevicePool &pool(int device) {
     return pools[device & 63];
 }
 
-static size_t pool_limit() {
-    static const size_t lim = [] {
-        const char *e = getenv("EMFA_POOL_CACHE_MB");
-        return (size_t)(e ? atoll(e) : 32768) << 20;
-    }();
-    return lim;
+static size_t pool_limit(int device) {
+    if (const char *e = getenv("EMFA_POOL_CACHE_MB")) return (size_t)std::max(0LL, atoll(e)) << 20;
+    size_t freeb = 0, total = 0;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(device);
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        total = (size_t)64 << 30;
+    }
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return total / 2;
 }
 
 static size_t pool_round(size_t bytes) {
@@ -63,10 +75,14 @@ static size_t pool_round(size_t bytes) {
     return (bytes + g - 1) / g * g;
 }
 
-// frees cached blocks (largest first) until at most `keep` bytes stay cached; call with mu held
+// frees cached blocks, longest cached first (blocks of sizes no longer asked
+// for go before the ones in use again), until at most `keep` bytes stay
+// cached; call with mu held
 static void pool_trim_locked(DevicePool &P, size_t keep) {
     while (P.cached > keep && !P.free_blocks.empty()) {
-        auto it = std::prev(P.free_blocks.end());
+        auto it = P.free_blocks.begin();
+        for (auto j = std::next(it); j != P.free_blocks.end(); ++j)
+            if (j->second.seq < it->second.seq) it = j;
         (void)hipEventSynchronize(it->second.ev);
         (void)hipFree(it->second.p);
         P.events.push_back(it->second.ev);
@@ -82,6 +98,7 @@ hipError_t ctx_alloc(emfa_ctx *ctx, void **p, size_t bytes) {
     DevicePool &P = pool(ctx->device);
     {
         std::lock_guard<std::mutex> l(P.mu);
+        if (!P.limit) P.limit = std::max<size_t>(1, pool_limit(ctx->device));
         auto it = P.free_blocks.lower_bound(rb);
         if (it != P.free_blocks.end() && it->first <= rb + rb / 4 + (2u << 20)) {
             PoolBlock b = it->second;
@@ -132,10 +149,10 @@ void ctx_free(emfa_ctx *ctx, void *p, size_t bytes) {
         return;
     }
     (void)hipEventRecord(ev, ctx->stream);
-    P.free_blocks.emplace(rb, PoolBlock{p, rb, ev});
+    P.free_blocks.emplace(rb, PoolBlock{p, rb, ev, P.seq++});
     P.cached += rb;
     ctx->bytes -= rb;
-    if (P.cached > pool_limit()) pool_trim_locked(P, pool_limit() / 2);
+    if (P.cached > P.limit) pool_trim_locked(P, P.limit / 4 * 3);
 }
 
 emfa_status ctx_scratch(emfa_ctx *ctx, size_t bytes, void **p) {
