@@ -1947,6 +1947,18 @@ static emfa_status laswp_launch(emfa_ctx *ctx, cx<R> *A, long long lda, const in
     return EMFA_OK;
 }
 
+// Cooperative launch of a panel kernel.  False, with *coop cleared so the
+// rest of the factorisation launches its panels normally, when the runtime
+// refuses it: it does while another process on the device holds the
+// cooperative-launch resources (a process that has launched cooperatively
+// keeps them), e.g. a solver run started beside a server process.
+static bool coop_launch(hipStream_t s, const void *fn, int G, void **args, size_t lds, bool *coop) {
+    if (hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, lds, s) == hipSuccess) return true;
+    (void)hipGetLastError();
+    *coop = false;
+    return false;
+}
+
 template <typename R>
 // With ctx->batch set (count matrices of Am's size, matrix 0 = Am), the
 // pivot-free FP32 path with the split update factors all of them with one
@@ -2104,31 +2116,27 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             {
                 ProfScope ps(ctx, "lu_panel", 0);
                 if (xchg) {
-                    if (coop) {
-                        void *args[] = {(void *)&A,    (void *)&lda,  (void *)&js,    (void *)&M,
-                                        (void *)&w,    (void *)&RB,   (void *)&ipiv,  (void *)&x_pk,
-                                        (void *)&x_rw, (void *)&x_cw, (void *)&x_val, (void *)&ctx->flags};
-                        EMFA_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_panel_x<R>, dim3(G), dim3(256),
-                                                                  args, lds_x, ctx->stream));
-                    } else {
+                    void *args[] = {(void *)&A,    (void *)&lda,  (void *)&js,    (void *)&M,
+                                    (void *)&w,    (void *)&RB,   (void *)&ipiv,  (void *)&x_pk,
+                                    (void *)&x_rw, (void *)&x_cw, (void *)&x_val, (void *)&ctx->flags};
+                    if (!(coop && coop_launch(ctx->stream, (const void *)k_panel_x<R>, G, args, lds_x, &coop))) {
                         k_panel_x<R><<<G, 256, lds_x, ctx->stream>>>(A, lda, js, M, w, RB, ipiv, x_pk, x_rw, x_cw,
                                                                      x_val, ctx->flags);
                         EMFA_HIP_CHECK(hipGetLastError());
                     }
-                } else if (coop) {
+                } else {
                     void *args[] = {(void *)&A,        (void *)&lda,      (void *)&js,   (void *)&M,
                                     (void *)&w,        (void *)&RB,       (void *)&ipiv, (void *)&slot_rows,
                                     (void *)&slot_val, (void *)&slot_idx, (void *)&rowc, (void *)&bar,
                                     (void *)&bar_base, (void *)&ctx->flags};
-                    EMFA_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_panel<R>, dim3(G), dim3(256), args,
-                                                              lds, ctx->stream));
-                } else {
                     // G workgroups of 256 threads with <= 160 KB LDS each always fit one per CU, and
                     // the concurrent update never waits on the panel, so all of them become
                     // resident; the barrier's bounded spin reports it if not.
-                    k_panel<R><<<G, 256, lds, ctx->stream>>>(A, lda, js, M, w, RB, ipiv, slot_rows, slot_val,
-                                                             slot_idx, rowc, bar, bar_base, ctx->flags);
-                    EMFA_HIP_CHECK(hipGetLastError());
+                    if (!(coop && coop_launch(ctx->stream, (const void *)k_panel<R>, G, args, lds, &coop))) {
+                        k_panel<R><<<G, 256, lds, ctx->stream>>>(A, lda, js, M, w, RB, ipiv, slot_rows, slot_val,
+                                                                 slot_idx, rowc, bar, bar_base, ctx->flags);
+                        EMFA_HIP_CHECK(hipGetLastError());
+                    }
                 }
             }
             bar_base += (unsigned)w * G;
@@ -2250,7 +2258,13 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
             EMFA_HIP_CHECK(hipGetLastError());
         }
         const dim3 g_apply(kDinvB / 32, (m + 31) / 32);
-        int skinny_max = kSkinnyMaxCols;
+        // k_smallmm up to 64 columns; above that the expanded-operand MFMA
+        // GEMM.  Contexts that share the device take the GEMM from 32
+        // columns: k_smallmm is VALU-bound there and the MFMA product needs
+        // about half the CU time (served 64-request stream: 193 -> 201
+        // solves/s, median of 14 passes), while a lone solve finishes sooner
+        // on k_smallmm's many small workgroups.
+        int skinny_max = ctx->cu_share > 0 ? 31 : kSkinnyMaxCols;
         if (const char *e = getenv("EMFA_SKINNY_MAX")) skinny_max = std::min(kSkinnyMaxCols, atoi(e));
         const bool skinny = m <= skinny_max;
         // rows [c_r0, c_r0 + M) of C -= F[rows, cols r0 .. r0 + wb) X_b

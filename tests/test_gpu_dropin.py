@@ -170,3 +170,34 @@ def test_mlfma_solver_files_near_dense_cfie(tmp_path):
     assert np.abs(val - ref)[lit].max() <= 1.0
     p, q = 10 ** (val / 10), 10 ** (ref / 10)
     assert np.linalg.norm(p - q) / np.linalg.norm(q) < 5e-2
+
+
+def test_launcher_beside_a_process_that_launched_cooperatively(tmp_path):
+    # This process factors a matrix on a plain context first (cooperative
+    # panel launches) and keeps the context open; the launcher process that
+    # follows may then have its cooperative launches refused and must fall
+    # back to normal panel launches, with the same results.
+    from emfa_amd import hip
+    rng = np.random.default_rng(11)
+    n = 1000
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    ctx = hip.Context(0)
+    held = []
+    try:
+        Z = hip.Mat.from_numpy(ctx, A, hip.F64)
+        held.append(Z)
+        held.append(hip.LU(ctx, Z, hip.F64))
+        ctx.sync()
+        xyz, tri = _run(tmp_path, "EFIE", "BistaticRCS", extra="    precision: FP64\n", tto=60, tdelta=10)
+    finally:
+        for x in reversed(held):
+            x.close()
+        ctx.close()
+    rwg, _ = O.build_rwg(tri)
+    k = O.wavenumber(3e8)
+    Z = O.fill_efie(xyz, tri, rwg, k, 3)
+    X = np.linalg.solve(Z, O.rhs(xyz, tri, rwg, 0, k, 1.0, O.inc_table([(0, 0, 0)])))
+    th, val = _read_rcs(tmp_path / "sphere_EFIE" / "bistatic_rcs_phi_0.dat")
+    assert np.array_equal(th, np.arange(0, 61, 10.0))
+    ref = O.rcs(xyz, tri, rwg, k, X, _dirs(th, 0))
+    assert np.abs(val - ref).max() <= 0.05 + 1e-4

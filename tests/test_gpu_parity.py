@@ -363,6 +363,37 @@ def test_inverse_block_solve_matches_blocked_substitution(ctx, monkeypatch, prec
         x.close()
 
 
+@pytest.mark.parametrize("m", [40, 64, 20])
+def test_shared_context_solve_matches_skinny_products(monkeypatch, m):
+    # a context with a CU share (as the solve server's workers) solves 32-64
+    # columns with the expanded-operand MFMA GEMM instead of k_smallmm; 20
+    # columns stay on k_smallmm.  Same factor, same answer to FP32 accuracy.
+    _say(f"shared-context solve m={m}")
+    n = 2600
+    rng = np.random.default_rng(77 + m)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    c = hip.Context(0)
+    mats = []
+    try:
+        c.set_cu_share(32)
+        Z = hip.Mat.from_numpy(c, A, hip.F32)
+        lu = hip.LU(c, Z, hip.F32)
+        d1 = hip.Mat.from_numpy(c, B, hip.F32)
+        mats += [Z, lu, d1]
+        lu.solve(d1)
+        monkeypatch.setenv("EMFA_SKINNY_MAX", "64")
+        d0 = hip.Mat.from_numpy(c, B, hip.F32)
+        mats.append(d0)
+        lu.solve(d0)
+        assert rel(d1.numpy(), d0.numpy()) < 1e-4
+        assert rel(d1.numpy(), np.linalg.solve(A, B)) < 2e-3
+    finally:
+        for x in mats:
+            x.close()
+        c.close()
+
+
 def test_lu_singular_reports(ctx):
     _say("lu singular")
     n = 40
