@@ -582,7 +582,7 @@ constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
 }
 
 template <typename TA, int TAG>
-__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(GemmArgs g) {
     using namespace f64k;
     __shared__ double As[2][BM][BK + PAD];
     __shared__ double Bs[2][BN][BK + PAD];
@@ -680,12 +680,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
         __syncthreads();
     }
 
-    // All loads of C before the first store (a store through C may alias a
-    // later load, which would otherwise serialise 64 round trips per lane).
+    // Per 16-row slab, all loads of C before the first store (a store
+    // through C may alias a later load, which would otherwise serialise a
+    // round trip per element); slab by slab keeps the registers for two
+    // waves per SIMD.
     const double sgn = g.alpha_neg ? -1.0 : 1.0;
-    double old[4][4][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int mt = 0; mt < 4; ++mt) {
+        double old[4][4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
@@ -693,18 +695,17 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs g) {
                 // V_MFMA_F64_16X16X4_F64 result: lane l, element v holds row l/16 + 4v, column l%16
                 const long long i = bm + wm * 64 + mt * 16 + (lane >> 4) + 4 * v;
                 const long long j = bn + wn * 64 + nt * 16 + r16;
-                old[mt][nt][v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.0;
+                old[nt][v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.0;
             }
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const long long i = bm + wm * 64 + mt * 16 + (lane >> 4) + 4 * v;
                 const long long j = bn + wn * 64 + nt * 16 + r16;
-                if (i < g.M && j < g.Nr) C[i * g.ldc + j] = old[mt][nt][v] + sgn * acc[mt][nt][v];
+                if (i < g.M && j < g.Nr) C[i * g.ldc + j] = old[nt][v] + sgn * acc[mt][nt][v];
             }
+    }
 }
 
 static bool gemm_args_ok(const GemmArgs &g) {
