@@ -91,8 +91,10 @@ namespace f32k {
 constexpr int BM = 128, BN = 128, BK = 32, PAD = 4;
 }
 
+// Two waves per SIMD (two 72 KB workgroups per CU): the bound keeps it at 180
+// VGPRs without spills; unbounded it took 216 + 64 and ran one.
 template <int TAG>
-__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs g) {
+__global__ __launch_bounds__(256, 2) void k_gemm_f32(GemmArgs g) {
     using namespace f32k;
     __shared__ float As[2][BM][BK + PAD];
     __shared__ float Bs[2][BN][BK + PAD];
