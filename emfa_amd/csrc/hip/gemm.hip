@@ -238,6 +238,16 @@ __global__ __launch_bounds__(256) void k_gemm_f32s(GemmArgs g) {
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    // C is read before the K loop: with the short K of the panel and solve
+    // updates (a few steps of 32) its load latency would otherwise follow the
+    // last MFMA.  Only this workgroup writes its tile of C during the launch.
+    const long long j = bn + wn * 32 + (lane & 31);
+    float old[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const long long i = bm + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        old[v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.f;
+    }
     gload(0);
     sstore(0);
     __syncthreads();
@@ -266,15 +276,7 @@ __global__ __launch_bounds__(256) void k_gemm_f32s(GemmArgs g) {
         __syncthreads();
     }
 
-    // all loads of C before the first store (see epilogue_f32)
     const float sgn = g.alpha_neg ? -1.f : 1.f;
-    const long long j = bn + wn * 32 + r32;
-    float old[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const long long i = bm + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        old[v] = (g.beta && i < g.M && j < g.Nr) ? C[i * g.ldc + j] : 0.f;
-    }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const long long i = bm + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
