@@ -737,11 +737,15 @@ emfa_status gemm_launch_f32(hipStream_t s, const GemmArgs &g0, int tag) {
     }();
     const bool vec4 = (g.lda % 4) == 0 && (g.ldb % 4) == 0 && (g.Kr % 4) == 0 &&
                       ((uintptr_t)g.A % 16) == 0 && ((uintptr_t)g.Bt % 16) == 0;
-    // launches whose 128 x 128 grid fills less than half of the CUs take the
-    // 64 x 64 tiles (EMFA_GEMM_SMALL=0 disables)
+    // launches of fewer than 2048 tiles of 128 x 128 take the 64 x 64 tiles:
+    // the in-panel, U-row and solve updates (short K or few columns), where
+    // four times the workgroups, four waves per SIMD and C read before the
+    // K loop beat the larger tile (C2: in-panel updates 62.5 -> 55.4 ms,
+    // solve updates 46.6 -> 40.0 ms); the long-K outer updates keep 128 x
+    // 128.  EMFA_GEMM_SMALL sets the tile count (0 disables).
     static const long long small_below = [] {
         const char *e = getenv("EMFA_GEMM_SMALL");
-        return e && atoi(e) == 0 ? 0LL : 128LL;
+        return e ? std::max(0LL, atoll(e)) : 2048LL;
     }();
     const long long tiles = (long long)grid.x * grid.y * grid.z;
     // tag 1: the LU trailing update, a distinct symbol so profilers can tell it apart
