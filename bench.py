@@ -256,7 +256,7 @@ def main():
     dt = (time.perf_counter() - t0) / a.steps
     fams = {}
     for f in ["fill_pairs", "fill_gather", "rhs", "lu_panel", "lu_laswp", "lu_trsm", "lu_gemm", "lu_split",
-              "lu_gemm_split", "lu_dinv", "solve", "solve_trsm", "solve_gemm", "refine_gemm", "rcs"]:
+              "lu_gemm_split", "lu_dinv", "solve", "solve_trsm", "solve_apply", "solve_gemm", "refine_gemm", "rcs"]:
         ms, cnt, work = ctx.prof_get(f)
         fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
     ctx.prof(False)

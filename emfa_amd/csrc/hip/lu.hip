@@ -917,11 +917,12 @@ struct LuPlan {
 };
 
 // C (complex, M x ncols, ldc) -= Aop (M x K, lda) * B, with B given as its
-// real-expanded transposed form Bt (ldb reals per row).
+// real-expanded transposed form Bt (ldb reals per row); with negate = 0 and
+// accumulate = 0, C = Aop * B.
 template <typename R>
 static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols, int K, const cx<R> *Aop,
                                long long lda, const R *Bt, long long ldb, cx<R> *C, long long ldc,
-                               hipStream_t stream = nullptr) {
+                               hipStream_t stream = nullptr, int negate = 1, int accumulate = 1) {
     hipStream_t s = stream ? stream : ctx->stream;
     if (M <= 0 || ncols <= 0 || K <= 0) return EMFA_OK;
     GemmArgs g;
@@ -934,8 +935,8 @@ static emfa_status gemm_update(emfa_ctx *ctx, const char *fam, int M, int ncols,
     g.ldb = ldb;
     g.C = C;
     g.ldc = 2 * ldc;
-    g.alpha_neg = 1;
-    g.beta = 1;
+    g.alpha_neg = negate;
+    g.beta = accumulate;
     g.bt = ctx->batch;
     ProfScope ps(ctx, fam, 8.0 * (double)M * ncols * K, s);
     const int tag = std::strcmp(fam, "lu_gemm") == 0 ? 1 : 0;
@@ -2246,11 +2247,12 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
         const size_t vec_b = (((size_t)N * m * sizeof(cx<R>)) + 255) & ~(size_t)255;
         const size_t bt_b = (((size_t)2 * m * 2 * kDinvB * sizeof(R)) + 255) & ~(size_t)255;
         void *scr = nullptr;
-        emfa_status st = ctx_scratch(ctx, 2 * vec_b + bt_b, &scr);
+        emfa_status st = ctx_scratch(ctx, 2 * vec_b + 2 * bt_b, &scr);
         if (st) return st;
         cx<R> *Y = (cx<R> *)scr;
         cx<R> *W = interchanges ? (cx<R> *)((char *)scr + vec_b) : B;
         R *Bt = (R *)((char *)scr + 2 * vec_b);
+        R *Bw = (R *)((char *)scr + 2 * vec_b + bt_b);  // expanded W_b (MFMA apply)
         ProfScope ps(ctx, "solve", 8.0 * (double)N * N * m);
         if (interchanges) {
             const long long tot = (long long)N * m;
@@ -2267,6 +2269,8 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
         int skinny_max = ctx->cu_share > 0 ? 31 : kSkinnyMaxCols;
         if (const char *e = getenv("EMFA_SKINNY_MAX")) skinny_max = std::min(kSkinnyMaxCols, atoi(e));
         const bool skinny = m <= skinny_max;
+        const char *apply_env = getenv("EMFA_APPLY_MFMA");  // 0: k_dinv_apply for every width
+        const bool apply_mfma = !(apply_env && atoi(apply_env) == 0);
         // rows [c_r0, c_r0 + M) of C -= F[rows, cols r0 .. r0 + wb) X_b
         auto update = [&](int M, const cx<R> *Fa, const cx<R> *Xb, cx<R> *C, int wb) -> emfa_status {
             if (M <= 0) return EMFA_OK;
@@ -2277,6 +2281,21 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
         // Y_b = D_b^-1 W_b
         auto apply = [&](const cx<R> *D, int wb, const cx<R> *Wb, cx<R> *Yb) -> emfa_status {
             if (skinny) return smallmm<R, R>(ctx->stream, D, kDinvB, Wb, m, Yb, m, wb, wb, m, 1, wb, R(1), 0, nullptr);
+            if (sizeof(R) == 4 && m > kSkinnyMaxCols && apply_mfma) {
+                // many FP32 right-hand sides: Y_b = D_b W_b on the MFMA GEMM
+                // (W_b expanded first), then Y_b expanded for the update.
+                // k_dinv_apply ran 48 VALU workgroups at ~2 TFLOP/s here
+                // (181 columns: 42 us per block)
+                const long long tot = (long long)wb * m;
+                k_expand<R><<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(Wb, wb, m, Bw);
+                EMFA_HIP_CHECK(hipGetLastError());
+                emfa_status s2 = gemm_update<R>(ctx, "solve_apply", wb, m, wb, D, kDinvB, Bw, 2LL * wb, Yb, m,
+                                                nullptr, 0, 0);
+                if (s2) return s2;
+                k_expand<R><<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(Yb, wb, m, Bt);
+                EMFA_HIP_CHECK(hipGetLastError());
+                return EMFA_OK;
+            }
             k_dinv_apply<R><<<g_apply, 256, 0, ctx->stream>>>(D, wb, Wb, m, m, Yb, m, Bt);
             EMFA_HIP_CHECK(hipGetLastError());
             return EMFA_OK;
