@@ -363,6 +363,28 @@ def test_inverse_block_solve_matches_blocked_substitution(ctx, monkeypatch, prec
         x.close()
 
 
+@pytest.mark.parametrize("n,m", [(1100, 65), (2600, 181)])
+def test_mfma_block_apply_matches_valu_apply(ctx, monkeypatch, n, m):
+    # more than 64 FP32 right-hand sides: the diagonal-block applies run as
+    # MFMA products of the expanded blocks; k_dinv_apply (EMFA_APPLY_MFMA=0)
+    # gives the same solution to FP32 accuracy on the same factor
+    _say(f"mfma block apply n={n} m={m}")
+    rng = np.random.default_rng(n + 7 * m)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    Z = hip.Mat.from_numpy(ctx, A, hip.F32)
+    lu = hip.LU(ctx, Z, hip.F32)
+    d1 = hip.Mat.from_numpy(ctx, B, hip.F32)
+    lu.solve(d1)
+    monkeypatch.setenv("EMFA_APPLY_MFMA", "0")
+    d0 = hip.Mat.from_numpy(ctx, B, hip.F32)
+    lu.solve(d0)
+    assert rel(d1.numpy(), d0.numpy()) < 1e-4
+    assert rel(d1.numpy(), np.linalg.solve(A, B)) < 2e-3
+    for x in (lu, Z, d0, d1):
+        x.close()
+
+
 @pytest.mark.parametrize("m", [40, 64, 20])
 def test_shared_context_solve_matches_skinny_products(monkeypatch, m):
     # a context with a CU share (as the solve server's workers) solves 32-64
