@@ -2271,6 +2271,11 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
         const bool skinny = m <= skinny_max;
         const char *apply_env = getenv("EMFA_APPLY_MFMA");  // 0: k_dinv_apply for every width
         const bool apply_mfma = !(apply_env && atoi(apply_env) == 0);
+        // fewest columns for the MFMA applies: above 64 only (from 32 columns in
+        // server contexts the served 64-request stream measured 187-192
+        // against 195-196 solves/s: three launches instead of one per block)
+        int apply_min = kSkinnyMaxCols + 1;
+        if (const char *e = getenv("EMFA_APPLY_MFMA_MIN")) apply_min = std::max(1, atoi(e));
         // rows [c_r0, c_r0 + M) of C -= F[rows, cols r0 .. r0 + wb) X_b
         auto update = [&](int M, const cx<R> *Fa, const cx<R> *Xb, cx<R> *C, int wb) -> emfa_status {
             if (M <= 0) return EMFA_OK;
@@ -2281,7 +2286,7 @@ static emfa_status solve_impl(emfa_ctx *ctx, const emfa_mat *LUm, const int *ipi
         // Y_b = D_b^-1 W_b
         auto apply = [&](const cx<R> *D, int wb, const cx<R> *Wb, cx<R> *Yb) -> emfa_status {
             if (skinny) return smallmm<R, R>(ctx->stream, D, kDinvB, Wb, m, Yb, m, wb, wb, m, 1, wb, R(1), 0, nullptr);
-            if (sizeof(R) == 4 && m > kSkinnyMaxCols && apply_mfma) {
+            if (sizeof(R) == 4 && m >= apply_min && apply_mfma) {
                 // many FP32 right-hand sides: Y_b = D_b W_b on the MFMA GEMM
                 // (W_b expanded first), then Y_b expanded for the update.
                 // k_dinv_apply ran 48 VALU workgroups at ~2 TFLOP/s here
