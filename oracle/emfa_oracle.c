@@ -29,6 +29,23 @@
  *     replaces the EFIE part by the EFIE-style analytic extraction.
  *   - the Sigma (surface impedance) self term is added without the RWG sign
  *     factors, exactly as EFIE.cpp:297-306 (and its pm/mp/mm copies) do.
+ *   - surface-current corner magnitudes carry over between triangles as
+ *     CFIE.cpp:740 lets them (open-boundary triangles).
+ *
+ * Pins (what checks each routine; tests/ and tests/golden/):
+ *   orc_quad_rule       exactness on monomials of degree 2 / 5, weights sum 1
+ *   orc_build_rwg       lists of the reference-compiled builder; brute force
+ *   orc_incidence       reference-compiled V of EFIE/CFIE/MFIE (angles != 0)
+ *   orc_fill_efie       reference-compiled Z (with and without Sigma, plate),
+ *                       Duffy quadrature of the singular potentials, Mie
+ *   orc_fill_cfie       reference-compiled Z of CFIE and of MFIE (alpha = 0)
+ *   orc_rhs             reference-compiled V of EFIE, CFIE and MFIE
+ *   orc_rcs             reference-compiled RCS cuts; Mie series
+ *   orc_lu, orc_lu_solve  LAPACK getrf / getrs (scipy) pivots and factors
+ *   orc_gmres, orc_bicgstab  reach the LU solution
+ *   orc_current         reference-compiled CFIE::calculateSurfaceCurrent on
+ *                       the reference's own I (closed meshes and a plate)
+ *   orc_potentials      adaptive Duffy quadrature, off-plane 2-D quadrature
  */
 #include <complex.h>
 #include <math.h>
@@ -815,6 +832,11 @@ void orc_current(const double *xyz, int nt, int n, const int *rwg, const double 
         if (cnt[tm] < 3) lst[3 * tm + cnt[tm]++] = u;
     }
     const cplx *X = (const cplx *)Xd;
+    /* The corner magnitudes live outside the triangle loop, as `mag[3]` does
+     * at CFIE.cpp:740: a triangle with fewer than three RWGs (an open
+     * boundary) keeps the previous triangle's values at its remaining
+     * corners.  The reference's current.sc carries those values. */
+    double mg[3] = {0, 0, 0};
     for (int t = 0; t < nt; ++t) {
         v3 vx[3] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
         double len[3] = {0, 0, 0};
@@ -837,7 +859,6 @@ void orc_current(const double *xyz, int nt, int n, const int *rwg, const double 
         v3 cen = scl(add(add(vx[0], vx[1]), vx[2]), 1.0 / 3.0);
         double dom = 2.0 * tri_area(vx[0], vx[1], vx[2]);
         cplx cc[3] = {0, 0, 0};
-        double mg[3] = {0, 0, 0};
         for (int i = 0; i < m; ++i) {
             const int *r = rwg + 6 * lst[3 * t + i];
             double sg = (t == r[4]) ? 1.0 : -1.0;

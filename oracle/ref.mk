@@ -16,7 +16,7 @@ CXXFLAGS := -std=c++14 -O2 -fpermissive -w -DARMA_DONT_USE_LAPACK -DARMA_DONT_US
 SRCS := EMComponent/Quadrature.cpp EMComponent/Mesh.cpp EMComponent/CommonEdge.cpp EMComponent/ConfigLoader.cpp \
   EMComponent/Triangle.cpp EMComponent/Tetrahedron.cpp EMComponent/Result.cpp EMComponent/ThreadPool.cpp \
   EMComponent/tools.cpp EMIterative/iml.cpp EMSolver/EM.cpp EMSolver/EFIE.cpp EMSolver/CFIE.cpp \
-  EMSolver/Miscellaneous.cpp
+  EMSolver/Miscellaneous.cpp EMSolver/MFIE.cpp
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.cpp=.o)))
 
 all: $(OUT)/emfa_ref

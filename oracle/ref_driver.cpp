@@ -6,10 +6,16 @@
 //   Mesh::loadMeshFile + CommonEdge::buildCommonEdge   (mesh as the reference reads it, RWG order)
 //   EFIE::fillZ / fillV  or  CFIE::fillZ / fillV        (complex<float> Z and V)
 //   a dense solve with Eigen's partial-pivot LU (the alternative left in EMSolver/EFIE.cpp:88)
-//   EFIE::getBiRCS / CFIE::getBiRCS on a theta cut at phi = 0
+//   MFIE::fillZ / fillV (EMSolver/MFIE.cpp:324-437) for kind mfie
+//   EFIE::getBiRCS / CFIE::getBiRCS / MFIE::getBiRCS on a theta cut at phi = 0
+//   CFIE::calculateSurfaceCurrent (EMSolver/CFIE.cpp:727-794) on the solved I
 // and writes everything to a binary file (layout in write_output).
 //
-// usage: emfa_ref efie|cfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin
+// MFIE keeps its kernels and members private; the driver reaches them by
+// compiling MFIE.h with `private` read as `protected` (the class layout is
+// unchanged, only access checking differs).
+//
+// usage: emfa_ref efie|cfie|mfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +26,9 @@
 #include "CommonEdge.h"
 #include "EFIE.h"
 #include "Mesh.h"
+#define private protected
+#include "MFIE.h"
+#undef private
 
 using component::VectorR3;
 
@@ -29,6 +38,26 @@ struct Out {
     std::vector<float> xyz;
     std::vector<int> tri, rwg;
     std::vector<float> Z, V, I, rcs, theta;
+    std::vector<float> cur;  // per triangle: v1, v2, v3 (xyz), magn1, magn2, magn3, magnc
+};
+
+// The reference's per-triangle current magnitudes, computed from a given
+// solution by the reference's own CFIE::calculateSurfaceCurrent.
+class CurrentProbe : public mom::CFIE {
+  public:
+    void run(std::shared_ptr<component::Mesh> m, std::shared_ptr<component::CommonEdge> ce, const Out &o_in,
+             Out &o) {
+        mesh_ptr_ = m;
+        ce_ptr_ = ce;
+        unknowns_ = ce->getCommonEdgeNum();
+        I.zeros(unknowns_);
+        for (int r = 0; r < unknowns_; ++r) I(r) = Complex(o_in.I[2 * r], o_in.I[2 * r + 1]);
+        std::vector<component::CurrentData> cd;
+        calculateSurfaceCurrent(&cd);
+        for (const auto &d : cd)
+            o.cur.insert(o.cur.end(), {d.v1.x, d.v1.y, d.v1.z, d.v2.x, d.v2.y, d.v2.z, d.v3.x, d.v3.y, d.v3.z,
+                                       d.magn1, d.magn2, d.magn3, d.magnc});
+    }
 };
 
 void incidence(float theta_deg, float phi_deg, float pol_deg, VectorR3 &k, VectorR3 &e, VectorR3 &h) {
@@ -89,6 +118,7 @@ class RefEFIE : public mom::EFIE {
             o.I.insert(o.I.end(), {I_eigen(r).real(), I_eigen(r).imag()});
         }
         rcs_cut(o);
+        CurrentProbe().run(mesh_ptr_, ce_ptr_, o, o);
     }
     std::shared_ptr<component::Mesh> mesh() { return mesh_ptr_; }
     std::shared_ptr<component::CommonEdge> edges() { return ce_ptr_; }
@@ -141,13 +171,59 @@ class RefCFIE : public mom::CFIE {
             o.theta.push_back(thetaDegree);
             o.rcs.push_back(getBiRCS(sk));
         }
+        CurrentProbe().run(mesh_ptr_, ce_ptr_, o, o);
+    }
+    std::shared_ptr<component::Mesh> mesh() { return mesh_ptr_; }
+    std::shared_ptr<component::CommonEdge> edges() { return ce_ptr_; }
+};
+
+class RefMFIE : public mom::MFIE {
+  public:
+    void run(const std::string &nas, float freq, float th, float ph, float po, Out &o) {
+        mesh_ptr_ = std::make_shared<component::Mesh>();
+        ce_ptr_ = std::make_shared<component::CommonEdge>();
+        mesh_ptr_->loadMeshFile(nas);
+        ce_ptr_->buildCommonEdge(mesh_ptr_);
+        unknowns_ = ce_ptr_->getCommonEdgeNum();
+        k_ = component::PI2 * freq / component::cc;
+        incidence(th, ph, po, inc_k_, inc_e_, inc_h_);
+        // MFIE::solve (EMSolver/MFIE.cpp:50-73) without its LAPACK call
+        Z.zeros(unknowns_, unknowns_);
+        I.zeros(unknowns_);
+        V.zeros(unknowns_);
+        fillZ();
+        fillV();
+        int n = unknowns_;
+        CMatrix Ze(n, n);
+        CVector Ve(n);
+        for (int r = 0; r < n; ++r) {
+            Ve(r) = V(r);
+            for (int c = 0; c < n; ++c) Ze(r, c) = Z(r, c);
+        }
+        CVector Ie = Ze.partialPivLu().solve(Ve);
+        for (int r = 0; r < n; ++r) I(r) = Ie(r);
+        collect_mesh(this, o);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) o.Z.insert(o.Z.end(), {Z(r, c).real(), Z(r, c).imag()});
+        for (int r = 0; r < n; ++r) {
+            o.V.insert(o.V.end(), {V(r).real(), V(r).imag()});
+            o.I.insert(o.I.end(), {I(r).real(), I(r).imag()});
+        }
+        for (value_t thetaDegree = 0; thetaDegree <= 360; thetaDegree += 5) {
+            value_t theta = thetaDegree * component::DegreesToRadians;
+            VectorR3 sk(sin(theta), 0, cos(theta));
+            o.theta.push_back(thetaDegree);
+            o.rcs.push_back(getBiRCS(sk));
+        }
+        CurrentProbe().run(mesh_ptr_, ce_ptr_, o, o);
     }
     std::shared_ptr<component::Mesh> mesh() { return mesh_ptr_; }
     std::shared_ptr<component::CommonEdge> edges() { return ce_ptr_; }
 };
 
 // layout: int32 nv, nt, n, nang; float xyz[nv*3]; int32 tri[nt*3]; int32 rwg[n*6];
-//         float Z[n*n*2]; float V[n*2]; float I[n*2]; float theta[nang]; float rcs[nang]
+//         float Z[n*n*2]; float V[n*2]; float I[n*2]; float theta[nang]; float rcs[nang];
+//         int32 ncur; float cur[ncur*13]
 void write_output(const char *path, const Out &o) {
     FILE *f = std::fopen(path, "wb");
     if (!f) std::exit(3);
@@ -161,6 +237,9 @@ void write_output(const char *path, const Out &o) {
     std::fwrite(o.I.data(), sizeof(float), o.I.size(), f);
     std::fwrite(o.theta.data(), sizeof(float), o.theta.size(), f);
     std::fwrite(o.rcs.data(), sizeof(float), o.rcs.size(), f);
+    int ncur = (int)o.cur.size() / 13;
+    std::fwrite(&ncur, sizeof(int), 1, f);
+    std::fwrite(o.cur.data(), sizeof(float), o.cur.size(), f);
     std::fclose(f);
 }
 
@@ -168,7 +247,7 @@ void write_output(const char *path, const Out &o) {
 
 int main(int argc, char **argv) {
     if (argc != 11) {
-        std::fprintf(stderr, "usage: %s efie|cfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin\n",
+        std::fprintf(stderr, "usage: %s efie|cfie|mfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin\n",
                      argv[0]);
         return 2;
     }
@@ -183,6 +262,9 @@ int main(int argc, char **argv) {
     } else if (kind == "cfie") {
         RefCFIE c;
         c.run(argv[2], freq, alpha, th, ph, po, o);
+    } else if (kind == "mfie") {
+        RefMFIE m;
+        m.run(argv[2], freq, th, ph, po, o);
     } else {
         return 2;
     }

@@ -3,7 +3,8 @@ reference's own EFIE/CFIE code (oracle/_ref/emfa_ref, built by
 `make -C oracle -f ref.mk` from the reference sources) on small synthetic
 meshes.  Each fixture holds the inputs (mesh as the reference read it, RWG
 list, frequency, incidence) and the reference's outputs (complex<float> Z
-rows, V, the LU solution I, the RCS cut).
+rows, V, the LU solution I, the RCS cut, and the per-triangle current
+magnitudes of CFIE::calculateSurfaceCurrent computed from that I).
 
   python tests/golden/make_ref_golden.py
 """
@@ -29,6 +30,8 @@ CASES = {
     "efie_plate4": ("efie", lambda: meshgen.plate(4, 1.0), 3e8, 1.0, 0j, (30, 45, 0), None),
     "cfie_sphere_s2": ("cfie", lambda: meshgen.icosphere(2, 1.0), 6e8, 0.5, 0j, (0, 0, 0), 64),
     "cfie_ellipsoid_s1": ("cfie", lambda: meshgen.ellipsoid(1, 1.0, 0.6, 0.4), 4e8, 0.3, 0j, (40, 20, 90), None),
+    "mfie_sphere_s1": ("mfie", lambda: meshgen.icosphere(1, 0.5), 3e8, 0.0, 0j, (0, 0, 0), None),
+    "mfie_ellipsoid_s1": ("mfie", lambda: meshgen.ellipsoid(1, 0.8, 0.5, 0.6), 4.5e8, 0.0, 0j, (60, 30, 45), None),
 }
 
 
@@ -43,8 +46,10 @@ def read_out(path):
         I = np.frombuffer(f.read(8 * n), dtype=np.float32).reshape(n, 2)
         th = np.frombuffer(f.read(4 * na), dtype=np.float32)
         rcs = np.frombuffer(f.read(4 * na), dtype=np.float32)
+        (nc,) = np.frombuffer(f.read(4), dtype=np.int32)
+        cur = np.frombuffer(f.read(52 * nc), dtype=np.float32).reshape(nc, 13)
     c = lambda a: a[..., 0].astype(np.float64) + 1j * a[..., 1].astype(np.float64)
-    return xyz, tri, rwg, c(Z), c(V), c(I), th, rcs
+    return xyz, tri, rwg, c(Z), c(V), c(I), th, rcs, cur
 
 
 def main():
@@ -59,12 +64,12 @@ def main():
             subprocess.run([EXE, kind, nas, repr(freq), repr(alpha), repr(sigma.real), repr(sigma.imag),
                             str(inc[0]), str(inc[1]), str(inc[2]), out], cwd=d, check=True,
                            stdout=subprocess.DEVNULL)
-            rx, rt, rr, Z, V, I, th, rcs = read_out(out)
+            rx, rt, rr, Z, V, I, th, rcs, cur = read_out(out)
         keep = Z if rows is None else Z[:rows]
         np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), kind=kind, freq=freq, alpha=alpha,
                             sigma=sigma, inc=np.array(inc, dtype=np.float64), xyz=rx, tri=rt, rwg=rr,
                             Z=keep.astype(np.complex64), V=V.astype(np.complex64), I=I.astype(np.complex64),
-                            theta=th, rcs=rcs)
+                            theta=th, rcs=rcs, cur_vertices=cur[:, :9], cur_magn=cur[:, 9:])
         print(name, "N =", rr.shape[0], "Z rows kept:", keep.shape[0])
 
 
