@@ -82,9 +82,104 @@ def monostatic_dirs(theta_deg):
     return np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
 
 
+def host_cpu():
+    """What the host offers: logical CPUs, the ones this process may use, model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0)), "model": model}
+
+
+def write_nas(path, xyz, tri):
+    """NASTRAN in the dialect the reference's Mesh::loadNasFile reads
+    (EMComponent/Mesh.cpp:261-309): 9 header lines, GRID* pairs, CTRIA3."""
+    with open(path, "w") as f:
+        for i in range(9):
+            f.write(f"$ header line {i}\n")
+        for i, p in enumerate(xyz):
+            f.write(f"GRID*   {i + 1:8d}  {p[0]: .9e}  {p[1]: .9e}\n")
+            f.write(f"*       {i + 1:8d}  {p[2]: .9e}\n")
+        for i, t in enumerate(tri):
+            f.write(f"CTRIA3 {i + 1} 1 {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+        f.write("ENDDATA\n")
+
+
+def cpu_baseline_reference(kind, radius, freq, alpha, n, nrhs, sample_subdiv=3, reps=3, lu_n=2048):
+    """The reference's own code, single-threaded as it runs, on this host.
+
+    Fill, excitation and far field: oracle/_ref/emfa_ref (the reference's
+    EFIE/CFIE sources compiled unmodified, complex<float>) times its own
+    fillZ, one fillV and getBiRCS on an icosphere of the same radius,
+    frequency and formulation with `sample_subdiv` subdivisions.  Every RWG
+    pair costs the fill the same whatever the mesh size, so fillZ scales by
+    (N / n)^2, fillV and the far field by N / n per right-hand side.
+    Solve: LAPACK cgesv (the routine behind the reference's arma::solve,
+    EMSolver/MFIE.cpp:64; here scipy's OpenBLAS limited to one thread) on a
+    lu_n x lu_n matrix with the workload's right-hand sides, scaled by N^3.
+    Each part is the median of `reps` runs.  Returns None when the
+    reference binary has not been built."""
+    import subprocess
+    import tempfile
+    import scipy.linalg.lapack as lp
+    from threadpoolctl import threadpool_limits
+    from emfa_amd import hip
+    exe = os.path.join(ROOT, "oracle", "_ref", "emfa_ref")
+    if not os.path.exists(exe):
+        return None
+    xyz, tri = hip.icosphere(sample_subdiv, radius)
+    runs = []
+    with tempfile.TemporaryDirectory() as d:
+        nas = os.path.join(d, "sample.nas")
+        write_nas(nas, xyz, tri)
+        for _ in range(reps):
+            r = subprocess.run([exe, "time", kind.lower(), nas, repr(float(freq)), repr(float(alpha))], cwd=d,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, check=True)
+            ns, tz, tv, trcs = r.stderr.split()[-4:]
+            runs.append((int(ns), float(tz), float(tv), float(trcs)))
+    ns = runs[0][0]
+    tz, tv, trcs = (float(np.median([x[i] for x in runs])) for i in (1, 2, 3))
+    fill_s = tz * (n / ns) ** 2
+    rhs_s = tv * (n / ns) * nrhs
+    rcs_s = trcs * (n / ns) * nrhs
+    lu_n = min(lu_n, n)
+    rng = np.random.default_rng(0)
+    A0 = (rng.standard_normal((lu_n, lu_n)) + 1j * rng.standard_normal((lu_n, lu_n))).astype(np.complex64)
+    B0 = (rng.standard_normal((lu_n, nrhs)) + 1j * rng.standard_normal((lu_n, nrhs))).astype(np.complex64)
+    tl = []
+    with threadpool_limits(limits=1):
+        for _ in range(reps):
+            A, B = A0.copy(), B0.copy()
+            t0 = time.perf_counter()
+            lp.cgesv(A, B, overwrite_a=1, overwrite_b=1)
+            tl.append(time.perf_counter() - t0)
+    lu_s = float(np.median(tl)) * (n / lu_n) ** 3
+    total = fill_s + rhs_s + lu_s + rcs_s
+    return {
+        "value": 1.0 / total,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": (f"reference {kind} code compiled from its sources (oracle/_ref/emfa_ref, complex<float>, one "
+                   f"thread): fillZ on icosphere s={sample_subdiv} ({ns} RWG) {tz:.2f}s -> {fill_s:.0f}s at "
+                   f"{n} RWG by (N/n)^2; fillV {tv * 1e3:.2f} ms and getBiRCS {trcs * 1e3:.2f} ms per direction "
+                   f"scaled by N/n x {nrhs}; LAPACK cgesv one thread {lu_n}x{lu_n} with {nrhs} RHS "
+                   f"{np.median(tl):.2f}s -> {lu_s:.0f}s by N^3; medians of {reps} runs; est. full solve "
+                   f"{total:.0f}s"),
+        "phases_s": {"fill": fill_s, "rhs": rhs_s, "lu_solve": lu_s, "rcs": rcs_s},
+        "host": host_cpu(),
+    }
+
+
 def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
-    """Oracle fill on a row sample + LAPACK cgesv (the routine behind the
-    reference's arma::solve) on a smaller matrix, scaled to one full solve."""
+    """Parallel port: the oracle's FP64 fill (the reference's RWG-pair loop,
+    OpenMP over test functions on every host core) on a row sample + LAPACK
+    cgesv on all cores on a smaller matrix, scaled to one full solve."""
     from oracle import oracle as O
     import scipy.linalg.lapack as lp
     n = rwg.shape[0]
@@ -121,6 +216,7 @@ def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
         "unit": "solves/s",
         "cores": int(threads),
         "kind": "port",
+        "host": host_cpu(),
         "sample": (f"oracle {kind} fill (FP64, RWG-pair loop as EFIE.cpp:603-696/CFIE.cpp:485-568) on rows 0..{rows} "
                    f"of {n} ({tf:.1f}s -> {fill_est:.0f}s for all rows) + LAPACK cgesv (arma::solve's routine) "
                    f"{ns}x{ns} with {n_rhs} RHS ({tl:.1f}s -> {lu_est:.0f}s scaled by N^3); est. full solve "
@@ -249,9 +345,14 @@ def main():
     ctx.sync()
     ctx.prof_reset()
     ctx.prof(True)
+    # each step ends in a device-to-host copy of its RCS values, so the host
+    # clock around one step is that step's latency
+    step_s = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
+        ts = time.perf_counter()
         r32 = step()
+        step_s.append(time.perf_counter() - ts)
     ctx.sync()
     dt = (time.perf_counter() - t0) / a.steps
     fams = {}
@@ -316,12 +417,18 @@ def main():
         mixed = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers, int(a.budget_gb * 2**30),
                            passes=a.served_passes)
 
-    cpu = None
+    cpu = cpu_par = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline(xyz, tri, rwg, kind_s, k, alpha, quad, nrhs, a.cpu_seconds)
+            cpu = cpu_baseline_reference(kind_s, radius, freq, alpha, n, nrhs)
         except Exception as e:  # the baseline is reported, never required
-            cpu = {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}"}
+            cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "reference", "sample": f"failed: {e}"}
+        try:
+            cpu_par = cpu_baseline(xyz, tri, rwg, kind_s, k, alpha, quad, nrhs, a.cpu_seconds)
+        except Exception as e:
+            cpu_par = {"value": None, "unit": "solves/s", "cores": None, "kind": "port", "sample": f"failed: {e}"}
+        if cpu is None:
+            cpu, cpu_par = cpu_par, None
 
     out = {
         "metric": "solves/sec (FP32+refine path) + p50/p99 latency",
@@ -340,13 +447,16 @@ def main():
         "config": {"workload": f"{kind_s} PEC sphere r={radius} m f={freq / 1e6:.0f} MHz icosphere s={subdiv}",
                    "unknowns": n, "triangles": int(tri.shape[0]), "rhs": nrhs, "quadrature": f"{quad}x{quad}",
                    "parallelism": "single GPU"},
-        "latency_ms": {"p50": dt * 1e3, "p99": dt * 1e3, "note": "one solve per step, sequential"},
+        "latency_ms": {"p50": float(np.percentile(step_s, 50)) * 1e3, "p99": float(np.percentile(step_s, 99)) * 1e3,
+                       "min": min(step_s) * 1e3, "max": max(step_s) * 1e3,
+                       "note": f"percentiles over the {a.steps} timed steps (one solve each, back to back)"},
         "roofline": roof,
         "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
         "lu_tuning": tuning,
         "refine": {"iterations": reports[-1][0], "relres_first": reports[-1][1], "relres_final": reports[-1][2],
                    "converged": reports[-1][3]},
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
     }
     out.update(extra)
     if mixed is not None:

@@ -2188,11 +2188,22 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
             // panel's factorisation on the main stream meanwhile.
             EMFA_HIP_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
             EMFA_HIP_CHECK(hipStreamWaitEvent(ctx->aux, ctx->ev[0], 0));
-            if ((st = trailing(nbn, N - next - nbn, ctx->aux))) return st;
-            EMFA_HIP_CHECK(hipEventRecord(ctx->ev[1], ctx->aux));
-            if ((st = trailing(0, nbn, ctx->stream))) return st;
-            if ((st = factor_outer(next, nbn))) return st;
-            EMFA_HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->ev[1], 0));
+            st = trailing(nbn, N - next - nbn, ctx->aux);
+            // The second stream joins the main one on every path, errors
+            // included: callers order the context's buffers (and their return
+            // to the device pool) through the main stream alone.
+            hipError_t je = hipEventRecord(ctx->ev[1], ctx->aux);
+            if (!st) st = trailing(0, nbn, ctx->stream);
+            if (!st) st = factor_outer(next, nbn);
+            if (je == hipSuccess) je = hipStreamWaitEvent(ctx->stream, ctx->ev[1], 0);
+            if (je != hipSuccess) {
+                (void)hipStreamSynchronize(ctx->aux);
+                if (!st) {
+                    set_error("lu: joining the look-ahead stream failed: %s", hipGetErrorString(je));
+                    st = EMFA_E_HIP;
+                }
+            }
+            if (st) return st;
         } else {
             if ((st = trailing(0, M, ctx->stream))) return st;
             if ((st = factor_outer(next, nbn))) return st;
@@ -2573,7 +2584,10 @@ extern "C" {
 static emfa_status compose_perm(emfa_ctx *ctx, emfa_lu *lu) {
     const int n = lu->a->rows;
     std::vector<int> piv(n), perm(n);
-    if (hipMemcpy(piv.data(), lu->ipiv, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+    // Copies run on the context's stream: blocks from the shared device pool
+    // are ordered against their previous owner only through that stream.
+    if (hipMemcpyAsync(piv.data(), lu->ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess ||
         (!lu->perm && ctx_alloc(ctx, (void **)&lu->perm, sizeof(int) * n) != hipSuccess)) {
         (void)hipGetLastError();
         set_error("emfa_lu_factor: permutation setup failed");
@@ -2588,7 +2602,8 @@ static emfa_status compose_perm(emfa_ctx *ctx, emfa_lu *lu) {
         }
         std::swap(perm[i], perm[p]);
     }
-    if (hipMemcpy(lu->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(lu->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
         set_error("emfa_lu_factor: permutation upload failed");
         return EMFA_E_HIP;
     }
@@ -2852,7 +2867,9 @@ emfa_status emfa_lu_solve(emfa_ctx *ctx, const emfa_lu *lu, emfa_mat *B) {
 
 emfa_status emfa_lu_pivots(const emfa_lu *lu, int *ipiv_host) {
     EMFA_REQUIRE(lu && ipiv_host, EMFA_E_INVALID, "emfa_lu_pivots: null argument");
-    EMFA_HIP_CHECK(hipMemcpy(ipiv_host, lu->ipiv, sizeof(int) * lu->a->rows, hipMemcpyDeviceToHost));
+    EMFA_HIP_CHECK(hipMemcpyAsync(ipiv_host, lu->ipiv, sizeof(int) * lu->a->rows, hipMemcpyDeviceToHost,
+                                  lu->ctx->stream));
+    EMFA_HIP_CHECK(hipStreamSynchronize(lu->ctx->stream));
     return EMFA_OK;
 }
 

@@ -378,6 +378,11 @@ void DenseMoM::calculateSurfaceCurrent(std::vector<component::CurrentData> *curr
         return std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
     };
     currents->resize(nt);
+    // Corner magnitudes persist across triangles as in the reference
+    // (CFIE.cpp:740 declares them outside the loop): at an open boundary a
+    // triangle with fewer than three RWGs reports the previous triangle's
+    // values at its other corners, and current.sc matches the reference's.
+    double mg[3] = {0, 0, 0};
     for (size_t t = 0; t < nt; ++t) {
         auto &unks = tris[t];
         component::Vec3 vx[3];
@@ -403,7 +408,6 @@ void DenseMoM::calculateSurfaceCurrent(std::vector<component::CurrentData> *curr
         double s = 0.5 * (la + lb + lc);
         double dom = 2 * std::sqrt(std::max(0.0, s * (s - la) * (s - lb) * (s - lc)));
         std::complex<double> cc[3] = {0, 0, 0};
-        double mg[3] = {0, 0, 0};
         size_t m = std::min<size_t>(unks.size(), 3);
         for (size_t i = 0; i < m; ++i) {
             const auto &r = ce_.getRWGRef(unks[i]);

@@ -86,10 +86,9 @@ def lib():
         L.emfa_ctx_bytes_in_use.restype = ctypes.c_size_t
         L.emfa_mat_device_ptr.restype = _vp
         L.emfa_lu_matrix.restype = _vp
-        for name in EXPORTS:
-            f = getattr(L, name)
-            if f.restype is ctypes.c_int and name not in ("emfa_version",):
-                pass
+        missing = [name for name in EXPORTS if not hasattr(L, name)]
+        if missing:
+            raise EmfaError(EMFA_E_NODEVICE, f"{LIB_PATH} lacks {', '.join(missing)}: rebuild it")
         _LIB = L
     return _LIB
 

@@ -42,7 +42,10 @@
  *   orc_rhs             reference-compiled V of EFIE, CFIE and MFIE
  *   orc_rcs             reference-compiled RCS cuts; Mie series
  *   orc_lu, orc_lu_solve  LAPACK getrf / getrs (scipy) pivots and factors
- *   orc_gmres, orc_bicgstab  reach the LU solution
+ *   orc_gmres           reaches the LU solution
+ *   orc_bicgstab, orc_bicgstab_mf  reference-compiled iml::BICGSTAB (both
+ *                       overloads) on the reference's own CFIE Z and V:
+ *                       step counts and solutions; reach the LU solution
  *   orc_current         reference-compiled CFIE::calculateSurfaceCurrent on
  *                       the reference's own I (closed meshes and a plate)
  *   orc_potentials      adaptive Duffy quadrature, off-plane 2-D quadrature
@@ -813,6 +816,67 @@ int orc_bicgstab(int n, const double *Ad, const double *bd, double *xd, double e
     }
     if (ok) *ok = (it != max_iter);
     free(r0); free(r1); free(v); free(p0); free(p1); free(s); free(t);
+    return it;
+}
+
+/* ------------------------------------------------------------------ */
+/* BiCGSTAB, matrix-free overload EMIterative/iml.cpp:88-154: x0 = 0,  */
+/* stop when ||r||^2 / ||b||^2 <= eps^2 or after max_iter steps; when  */
+/* |rho| < eps_f^2 ||r0||^2 (eps_f = float epsilon, the reference's    */
+/* NumTraits<complex<float>>::epsilon()) the shadow residual restarts  */
+/* at r, and the first restart resets the step count.  Returns the     */
+/* step count; *relres = ||r|| / ||b|| at exit.                        */
+/* ------------------------------------------------------------------ */
+int orc_bicgstab_mf(int n, const double *Ad, const double *bd, double *xd, double eps, int max_iter, double *relres)
+{
+    const cplx *A = (const cplx *)Ad;
+    const cplx *b = (const cplx *)bd;
+    cplx *x = (cplx *)xd;
+    cplx *r = malloc(sizeof(cplx) * n), *r0 = malloc(sizeof(cplx) * n), *v = calloc(n, sizeof(cplx));
+    cplx *p = calloc(n, sizeof(cplx)), *sv = malloc(sizeof(cplx) * n), *t = malloc(sizeof(cplx) * n);
+    for (int i = 0; i < n; ++i) { x[i] = 0; r[i] = b[i]; r0[i] = b[i]; }
+    double r0_sq = 0, rhs_sq = 0;
+    for (int i = 0; i < n; ++i) { r0_sq += creal(conj(r0[i]) * r0[i]); rhs_sq += creal(conj(b[i]) * b[i]); }
+    int it = 0;
+    if (rhs_sq == 0) {
+        if (relres) *relres = 0;
+        free(r); free(r0); free(v); free(p); free(sv); free(t);
+        return 0;
+    }
+    const double epsf = 1.1920928955078125e-07, eps2 = epsf * epsf, tol2 = eps * eps;
+    cplx rho = 1, alpha = 1, w = 1;
+    int restarts = 0;
+    for (;;) {
+        double rr = 0;
+        for (int i = 0; i < n; ++i) rr += creal(conj(r[i]) * r[i]);
+        if (!(rr / rhs_sq > tol2 && it < max_iter)) break;
+        cplx rho_old = rho;
+        rho = 0;
+        for (int i = 0; i < n; ++i) rho += conj(r0[i]) * r[i];
+        if (cabs(rho) < eps2 * r0_sq) {
+            memcpy(r0, r, sizeof(cplx) * n);
+            rho = r0_sq = rr;
+            if (restarts++ == 0) it = 0;
+        }
+        cplx beta = (rho / rho_old) * (alpha / w);
+        for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - w * v[i]);
+        matvec(n, A, p, v);
+        cplx d = 0;
+        for (int i = 0; i < n; ++i) d += conj(r0[i]) * v[i];
+        alpha = rho / d;
+        for (int i = 0; i < n; ++i) sv[i] = r[i] - alpha * v[i];
+        matvec(n, A, sv, t);
+        double tt = 0;
+        cplx ts = 0;
+        for (int i = 0; i < n; ++i) { tt += creal(conj(t[i]) * t[i]); ts += conj(t[i]) * sv[i]; }
+        w = tt > 0 ? ts / tt : 0;
+        for (int i = 0; i < n; ++i) { x[i] += alpha * p[i] + w * sv[i]; r[i] = sv[i] - w * t[i]; }
+        ++it;
+    }
+    double rr = 0;
+    for (int i = 0; i < n; ++i) rr += creal(conj(r[i]) * r[i]);
+    if (relres) *relres = sqrt(rr / rhs_sq);
+    free(r); free(r0); free(v); free(p); free(sv); free(t);
     return it;
 }
 

@@ -45,6 +45,8 @@ def lib():
         L.orc_gmres.argtypes = [ctypes.c_int, _d, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_int, _d]
         L.orc_bicgstab.restype = ctypes.c_int
         L.orc_bicgstab.argtypes = [ctypes.c_int, _d, _d, _d, ctypes.c_double, ctypes.c_int, _i]
+        L.orc_bicgstab_mf.restype = ctypes.c_int
+        L.orc_bicgstab_mf.argtypes = [ctypes.c_int, _d, _d, _d, ctypes.c_double, ctypes.c_int, _d]
         L.orc_current.argtypes = [_d, ctypes.c_int, ctypes.c_int, _i, _d, ctypes.c_int, ctypes.c_int, _d, _d]
         L.orc_quad_rule.restype = ctypes.c_int
         L.orc_quad_rule.argtypes = [ctypes.c_int, _d, _d]
@@ -219,6 +221,18 @@ def bicgstab(A, b, eps=1e-2, max_iter=500):
     it = lib().orc_bicgstab(A.shape[0], A.ctypes.data_as(_d), b.ctypes.data_as(_d), x.ctypes.data_as(_d),
                             float(eps), int(max_iter), ctypes.byref(ok))
     return x, it, bool(ok.value)
+
+
+def bicgstab_mf(A, b, eps=1e-2, max_iter=500):
+    """BiCGSTAB of EMIterative/iml.cpp:88-154 (the matrix-free overload, with
+    its shadow-residual restart).  Returns (x, steps, relres)."""
+    A = _c(A, np.complex128)
+    b = _c(b, np.complex128).ravel()
+    x = np.zeros_like(b)
+    rr = ctypes.c_double(0)
+    it = lib().orc_bicgstab_mf(A.shape[0], A.ctypes.data_as(_d), b.ctypes.data_as(_d), x.ctypes.data_as(_d),
+                               float(eps), int(max_iter), ctypes.byref(rr))
+    return x, it, rr.value
 
 
 def current(xyz, nt, rwg, X, col=0):

@@ -16,6 +16,11 @@
 // unchanged, only access checking differs).
 //
 // usage: emfa_ref efie|cfie|mfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin
+//        emfa_ref time cfie|efie mesh.nas freq alpha
+// The second form times the reference's own fillZ, one fillV and the far
+// field of 37 directions (getBiRCS) on the mesh, single-threaded as the
+// reference runs, and prints "n fillZ_s fillV_s rcs_s_per_direction".
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,6 +31,7 @@
 #include "CommonEdge.h"
 #include "EFIE.h"
 #include "Mesh.h"
+#include "iml.h"
 #define private protected
 #include "MFIE.h"
 #undef private
@@ -39,7 +45,30 @@ struct Out {
     std::vector<int> tri, rwg;
     std::vector<float> Z, V, I, rcs, theta;
     std::vector<float> cur;  // per triangle: v1, v2, v3 (xyz), magn1, magn2, magn3, magnc
+    int it_b = -1, it_mf = -1;
+    std::vector<float> xb, xmf;  // BiCGSTAB solutions (both overloads)
 };
+
+// The reference's BiCGSTAB overloads on (Z, V): the Armadillo one that
+// CFIE::solve calls (EMIterative/iml.cpp:3-39, CFIE.cpp:76: eps 0.01, 500
+// steps) and the matrix-free one (iml.cpp:88-154) with the product Z x.
+void run_bicgstab(const Qcx_mat &Z, const Qcx_vec &V, Out &o) {
+    const int n = (int)V.n_elem;
+    Qcx_vec xb;
+    iml::BICGSTAB(Z, V, xb, 0.01f, 500, o.it_b);
+    CMatrix Ze(n, n);
+    CVector Ve(n), xmf;
+    for (int r = 0; r < n; ++r) {
+        Ve(r) = V(r);
+        for (int c = 0; c < n; ++c) Ze(r, c) = Z(r, c);
+    }
+    MulVct mul = [&Ze](const CVector &v) -> CVector { return Ze * v; };
+    iml::BICGSTAB(xmf, mul, Ve, 500, 0.01f, o.it_mf);
+    for (int r = 0; r < n; ++r) {
+        o.xb.insert(o.xb.end(), {xb(r).real(), xb(r).imag()});
+        o.xmf.insert(o.xmf.end(), {xmf(r).real(), xmf(r).imag()});
+    }
+}
 
 // The reference's per-triangle current magnitudes, computed from a given
 // solution by the reference's own CFIE::calculateSurfaceCurrent.
@@ -92,6 +121,30 @@ void collect_mesh(S *s, Out &o) {
 
 class RefEFIE : public mom::EFIE {
   public:
+    void setup(const std::string &nas, float freq, float) {
+        mesh_ptr_ = std::make_shared<component::Mesh>();
+        ce_ptr_ = std::make_shared<component::CommonEdge>();
+        mesh_ptr_->loadMeshFile(nas);
+        ce_ptr_->buildCommonEdge(mesh_ptr_);
+        unknowns_ = ce_ptr_->getCommonEdgeNum();
+        k_ = component::PI2 * freq / component::cc;
+        threshold_edm = 3.0f * component::cc / freq;
+        Sigma = Complex(0, 0);
+        incidence(0, 0, 0, inc_k_, inc_e_, inc_h_);
+        I_eigen.setZero(unknowns_);
+    }
+    int unknowns() const { return unknowns_; }
+    void fill_z() {
+        Z.zeros(unknowns_, unknowns_);
+        Z_eigen.setZero(unknowns_, unknowns_);
+        fillZ();
+    }
+    void fill_v() {
+        V.zeros(unknowns_);
+        V_eigen.setZero(unknowns_);
+        fillV();
+    }
+    value_t bi_rcs(const VectorR3 &d) const { return getBiRCS(d); }
     void run(const std::string &nas, float freq, Complex sigma, float th, float ph, float po, Out &o) {
         mesh_ptr_ = std::make_shared<component::Mesh>();
         ce_ptr_ = std::make_shared<component::CommonEdge>();
@@ -135,8 +188,46 @@ class RefEFIE : public mom::EFIE {
     }
 };
 
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Per-phase wall time of the reference's own EFIE/CFIE code on one mesh.
+template <typename S>
+void time_phases(S &solver, double out[4]) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    solver.fill_z();
+    out[1] = seconds_since(t0);
+    t0 = clk::now();
+    solver.fill_v();
+    out[2] = seconds_since(t0);
+    t0 = clk::now();
+    volatile value_t sink = 0;
+    for (int d = 0; d < 37; ++d) {
+        value_t theta = value_t(5 * d) * component::DegreesToRadians;
+        sink = sink + solver.bi_rcs(VectorR3(sin(theta), 0, cos(theta)));
+    }
+    out[3] = seconds_since(t0) / 37;
+}
+
 class RefCFIE : public mom::CFIE {
   public:
+    void setup(const std::string &nas, float freq, float alpha) {
+        mesh_ptr_ = std::make_shared<component::Mesh>();
+        ce_ptr_ = std::make_shared<component::CommonEdge>();
+        mesh_ptr_->loadMeshFile(nas);
+        ce_ptr_->buildCommonEdge(mesh_ptr_);
+        unknowns_ = ce_ptr_->getCommonEdgeNum();
+        k_ = component::PI2 * freq / component::cc;
+        alpha_ = alpha;
+        incidence(0, 0, 0, inc_k_, inc_e_, inc_h_);
+        I.zeros(unknowns_);
+    }
+    int unknowns() const { return unknowns_; }
+    void fill_z() { fillZ(); }
+    void fill_v() { fillV(); }
+    value_t bi_rcs(const VectorR3 &d) const { return getBiRCS(d); }
     void run(const std::string &nas, float freq, float alpha, float th, float ph, float po, Out &o) {
         mesh_ptr_ = std::make_shared<component::Mesh>();
         ce_ptr_ = std::make_shared<component::CommonEdge>();
@@ -172,6 +263,7 @@ class RefCFIE : public mom::CFIE {
             o.rcs.push_back(getBiRCS(sk));
         }
         CurrentProbe().run(mesh_ptr_, ce_ptr_, o, o);
+        run_bicgstab(Z, V, o);
     }
     std::shared_ptr<component::Mesh> mesh() { return mesh_ptr_; }
     std::shared_ptr<component::CommonEdge> edges() { return ce_ptr_; }
@@ -223,7 +315,8 @@ class RefMFIE : public mom::MFIE {
 
 // layout: int32 nv, nt, n, nang; float xyz[nv*3]; int32 tri[nt*3]; int32 rwg[n*6];
 //         float Z[n*n*2]; float V[n*2]; float I[n*2]; float theta[nang]; float rcs[nang];
-//         int32 ncur; float cur[ncur*13]
+//         int32 ncur; float cur[ncur*13];
+//         int32 it_b; int32 it_mf; if it_b >= 0: float xb[n*2]; float xmf[n*2]
 void write_output(const char *path, const Out &o) {
     FILE *f = std::fopen(path, "wb");
     if (!f) std::exit(3);
@@ -240,12 +333,36 @@ void write_output(const char *path, const Out &o) {
     int ncur = (int)o.cur.size() / 13;
     std::fwrite(&ncur, sizeof(int), 1, f);
     std::fwrite(o.cur.data(), sizeof(float), o.cur.size(), f);
+    std::fwrite(&o.it_b, sizeof(int), 1, f);
+    std::fwrite(&o.it_mf, sizeof(int), 1, f);
+    std::fwrite(o.xb.data(), sizeof(float), o.xb.size(), f);
+    std::fwrite(o.xmf.data(), sizeof(float), o.xmf.size(), f);
     std::fclose(f);
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
+    if (argc == 6 && std::string(argv[1]) == "time") {
+        std::string kind = argv[2];
+        float freq = std::strtof(argv[4], nullptr), alpha = std::strtof(argv[5], nullptr);
+        double t[4] = {0, 0, 0, 0};
+        if (kind == "cfie") {
+            RefCFIE c;
+            c.setup(argv[3], freq, alpha);
+            t[0] = c.unknowns();
+            time_phases(c, t);
+        } else if (kind == "efie") {
+            RefEFIE e;
+            e.setup(argv[3], freq, alpha);
+            t[0] = e.unknowns();
+            time_phases(e, t);
+        } else {
+            return 2;
+        }
+        std::fprintf(stderr, "%d %.6f %.6f %.6e\n", (int)t[0], t[1], t[2], t[3]);
+        return 0;
+    }
     if (argc != 11) {
         std::fprintf(stderr, "usage: %s efie|cfie|mfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin\n",
                      argv[0]);

@@ -4,7 +4,8 @@ reference's own EFIE/CFIE code (oracle/_ref/emfa_ref, built by
 meshes.  Each fixture holds the inputs (mesh as the reference read it, RWG
 list, frequency, incidence) and the reference's outputs (complex<float> Z
 rows, V, the LU solution I, the RCS cut, and the per-triangle current
-magnitudes of CFIE::calculateSurfaceCurrent computed from that I).
+magnitudes of CFIE::calculateSurfaceCurrent computed from that I; for
+CFIE and MFIE also the reference's BiCGSTAB solutions of its own Z, V).
 
   python tests/golden/make_ref_golden.py
 """
@@ -48,8 +49,14 @@ def read_out(path):
         rcs = np.frombuffer(f.read(4 * na), dtype=np.float32)
         (nc,) = np.frombuffer(f.read(4), dtype=np.int32)
         cur = np.frombuffer(f.read(52 * nc), dtype=np.float32).reshape(nc, 13)
+        it_b, it_mf = np.frombuffer(f.read(8), dtype=np.int32)
+        xb = xmf = None
+        if it_b >= 0:
+            xb = np.frombuffer(f.read(8 * n), dtype=np.float32).reshape(n, 2)
+            xmf = np.frombuffer(f.read(8 * n), dtype=np.float32).reshape(n, 2)
     c = lambda a: a[..., 0].astype(np.float64) + 1j * a[..., 1].astype(np.float64)
-    return xyz, tri, rwg, c(Z), c(V), c(I), th, rcs, cur
+    bicg = None if xb is None else (int(it_b), c(xb), int(it_mf), c(xmf))
+    return xyz, tri, rwg, c(Z), c(V), c(I), th, rcs, cur, bicg
 
 
 def main():
@@ -64,12 +71,18 @@ def main():
             subprocess.run([EXE, kind, nas, repr(freq), repr(alpha), repr(sigma.real), repr(sigma.imag),
                             str(inc[0]), str(inc[1]), str(inc[2]), out], cwd=d, check=True,
                            stdout=subprocess.DEVNULL)
-            rx, rt, rr, Z, V, I, th, rcs, cur = read_out(out)
+            rx, rt, rr, Z, V, I, th, rcs, cur, bicg = read_out(out)
         keep = Z if rows is None else Z[:rows]
+        extra = {}
+        if bicg is not None:
+            # BiCGSTAB of the reference on its own Z and V (both overloads; the
+            # matrix-free one never writes its step count, EMIterative/iml.cpp:88-154)
+            extra = dict(bicgstab_steps=bicg[0], bicgstab_x=bicg[1].astype(np.complex64),
+                         bicgstab_mf_x=bicg[3].astype(np.complex64))
         np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), kind=kind, freq=freq, alpha=alpha,
                             sigma=sigma, inc=np.array(inc, dtype=np.float64), xyz=rx, tri=rt, rwg=rr,
                             Z=keep.astype(np.complex64), V=V.astype(np.complex64), I=I.astype(np.complex64),
-                            theta=th, rcs=rcs, cur_vertices=cur[:, :9], cur_magn=cur[:, 9:])
+                            theta=th, rcs=rcs, cur_vertices=cur[:, :9], cur_magn=cur[:, 9:], **extra)
         print(name, "N =", rr.shape[0], "Z rows kept:", keep.shape[0])
 
 

@@ -88,3 +88,30 @@ def test_nas_reader_reads_what_the_reference_reads(path, tmp_path):
     assert rc == 0
     assert np.array_equal(t2, g["tri"])
     assert np.array_equal(x2.astype(np.float32), g["xyz"]) and np.array_equal(x2, g["xyz"].astype(np.float64))
+
+
+BICG = [p for p in FIXTURES if "bicgstab_x" in np.load(p).files]
+
+
+@pytest.mark.parametrize("path", BICG, ids=_name)
+def test_oracle_bicgstab_matches_reference_golden(path):
+    """O.bicgstab / O.bicgstab_mf against the reference's own iml::BICGSTAB
+    overloads (EMIterative/iml.cpp:3-39, :88-154) on its CFIE system at the
+    settings of CFIE.cpp:76 (eps 0.01, 500 steps).  The oracle runs on its FP64
+    Z and V, the reference on its complex<float> ones: the solutions agree to
+    FP32 rounding and the Armadillo overload takes the same number of steps
+    (the matrix-free overload does not report its count)."""
+    g = np.load(path)
+    xyz = g["xyz"].astype(np.float64)
+    tri, rwg = g["tri"], g["rwg"]
+    k = O.wavenumber(float(g["freq"]))
+    alpha = float(g["alpha"])
+    inc = O.inc_table([tuple(g["inc"])])
+    Z = O.fill_cfie(xyz, tri, rwg, k, alpha, 3, 0)
+    b = O.rhs(xyz, tri, rwg, 1, k, alpha, inc)[:, 0]
+    x, it, ok = O.bicgstab(Z, b, 0.01, 500)
+    assert ok and it == int(g["bicgstab_steps"])
+    assert np.abs(x - g["bicgstab_x"]).max() < 2e-5 * np.abs(x).max()
+    x2, it2, rr = O.bicgstab_mf(Z, b, 0.01, 500)
+    assert rr <= 0.01
+    assert np.abs(x2 - g["bicgstab_mf_x"]).max() < 2e-5 * np.abs(x2).max()

@@ -168,6 +168,31 @@ def test_lu_f64_matches_getrf(ctx, n):
     assert rel(dB.numpy(), np.linalg.solve(A, B)) < 1e-10
 
 
+@pytest.mark.parametrize("xchg", [0, 1])
+def test_lu_f64_normal_launch_panels_match_getrf(ctx, monkeypatch, xchg):
+    """The panel launched without the cooperative API (what a context does
+    when the runtime refuses a cooperative launch, and what contexts sharing
+    the device always do), for both pivot-exchange designs: getrf pivots."""
+    _say(f"lu f64 normal-launch panel xchg={xchg}")
+    monkeypatch.setenv("EMFA_PANEL_COOP", "0")
+    monkeypatch.setenv("EMFA_PANEL_X", str(xchg))
+    n = 777
+    rng = np.random.default_rng(77 + xchg)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    dA = hip.Mat.from_numpy(ctx, A, hip.F64)
+    lu = hip.LU(ctx, dA, hip.F64)
+    LUo, pivo, info = O.lu(A)
+    assert info == 0 and lu.info() == 0
+    assert np.array_equal(lu.pivots(), pivo)
+    assert np.abs(lu.factor().numpy() - LUo).max() / np.abs(LUo).max() < 1e-10
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    dB = hip.Mat.from_numpy(ctx, B, hip.F64)
+    lu.solve(dB)
+    assert rel(dB.numpy(), np.linalg.solve(A, B)) < 1e-10
+    for m in (lu, dA, dB):
+        m.close()
+
+
 @pytest.mark.parametrize("n", [100, 517])
 def test_lu_f32(ctx, n):
     _say(f"lu f32 n={n}")
@@ -464,13 +489,18 @@ def test_gmres(ctx):
 
 
 # ---------------------------------------------------------------- end to end (C1 mesh)
-def test_end_to_end_efie_sphere(ctx):
-    """EFIE, PEC sphere r=0.5 m, 300 MHz, icosphere s=3 (1920 RWG), x-pol from +z."""
-    _say("end-to-end C1")
+@pytest.mark.parametrize("quad", [3, 7])
+def test_end_to_end_efie_sphere(ctx, quad):
+    """EFIE, PEC sphere r=0.5 m, 300 MHz, icosphere s=3 (1920 RWG), x-pol from +z,
+    bistatic RCS at 361 angles.  quad=7 is the configured workload (7-point
+    outer and inner rules); quad=3 is the reference's own fill rule.  Bounds:
+    FP64 path surface-current coefficients rel L2 <= 1e-5 and RCS <= 0.05 dB,
+    FP32 + FP64 refinement RCS <= 0.1 dB, both against the FP64 oracle."""
+    _say(f"end-to-end C1 quad={quad}")
     xyz, tri, rwg = sphere(3, 0.5)
     k = hip.wavenumber(3e8)
     inc = hip.inc_table([(0, 0, 0)])
-    Zo = O.fill_efie(xyz, tri, rwg, k, 3)
+    Zo = O.fill_efie(xyz, tri, rwg, k, quad)
     Bo = O.rhs(xyz, tri, rwg, 0, k, 1.0, inc)
     Xo = sl.solve(Zo, Bo)
     th = np.arange(0, 361, 1.0)
@@ -479,18 +509,22 @@ def test_end_to_end_efie_sphere(ctx):
     M = hip.Mesh(ctx, xyz, tri, rwg)
     # FP64 plain path
     Z = hip.Mat(ctx, M.n, M.n, hip.F64)
-    hip.fill(ctx, M, Z, hip.EFIE, k, 3)
+    hip.fill(ctx, M, Z, hip.EFIE, k, quad)
     B = hip.Mat(ctx, M.n, 1, hip.F64)
     hip.rhs_fill(ctx, M, B, hip.EFIE, k, 1.0, inc)
     lu = hip.LU(ctx, Z, hip.F64)
     lu.solve(B)
     X = B.numpy()
     assert rel(X, Xo) <= 1e-5
+    mag, _ = O.current(xyz, tri.shape[0], rwg, X)
+    mago, _ = O.current(xyz, tri.shape[0], rwg, Xo)
+    assert rel(mag, mago) <= 1e-5
     r64 = hip.rcs(ctx, M, k, B, dirs)
+    _say(f"  FP64 path: current rel {rel(X, Xo):.2e}, RCS max |diff| {np.abs(r64 - ro).max():.2e} dB")
     assert np.abs(r64 - ro).max() <= 0.05
     # FP32 fill + FP32 LU + FP64 refinement
     Z32 = hip.Mat(ctx, M.n, M.n, hip.F32)
-    hip.fill(ctx, M, Z32, hip.EFIE, k, 3)
+    hip.fill(ctx, M, Z32, hip.EFIE, k, quad)
     B2 = hip.Mat(ctx, M.n, 1, hip.F64)
     hip.rhs_fill(ctx, M, B2, hip.EFIE, k, 1.0, inc)
     lu32 = hip.LU(ctx, Z32, hip.F32)
@@ -498,6 +532,7 @@ def test_end_to_end_efie_sphere(ctx):
     rep, ok = hip.refine(ctx, Z32, lu32, B2, X2, tol=1e-10, maxit=10)
     assert ok
     r32 = hip.rcs(ctx, M, k, X2, dirs)
+    _say(f"  FP32+refine: RCS max |diff| {np.abs(r32 - ro).max():.2e} dB")
     assert np.abs(r32 - ro).max() <= 0.1
 
 
