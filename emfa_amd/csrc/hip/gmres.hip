@@ -269,20 +269,23 @@ emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(con
 
 using namespace emfa;
 
+emfa_status emfa::gemv(emfa_ctx *ctx, const emfa_mat *Z, const double2 *in, double2 *out) {
+    const int n = Z->rows;
+    hipStream_t s = ctx->stream;
+    ProfScope ps(ctx, "gmres_matvec", (double)n * n * elem_bytes(Z->prec));
+    if (Z->prec == EMFA_F32)
+        k_gemv<float><<<(n + 3) / 4, 256, 0, s>>>((const cx<float> *)Z->d, n, in, out);
+    else
+        k_gemv<double><<<(n + 3) / 4, 256, 0, s>>>((const cx<double> *)Z->d, n, in, out);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
 extern "C" emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, int m, double tol,
                                   int max_restart, emfa_gmres_report *rep) {
     EMFA_REQUIRE(ctx && Z && B && X, EMFA_E_INVALID, "emfa_gmres: null argument");
     const int n = Z->rows;
     EMFA_REQUIRE(Z->cols == n, EMFA_E_INVALID, "emfa_gmres: Z must be square");
-    hipStream_t s = ctx->stream;
-    auto matvec = [&](const double2 *in, double2 *out) -> emfa_status {
-        ProfScope ps(ctx, "gmres_matvec", (double)n * n * elem_bytes(Z->prec));
-        if (Z->prec == EMFA_F32)
-            k_gemv<float><<<(n + 3) / 4, 256, 0, s>>>((const cx<float> *)Z->d, n, in, out);
-        else
-            k_gemv<double><<<(n + 3) / 4, 256, 0, s>>>((const cx<double> *)Z->d, n, in, out);
-        EMFA_HIP_CHECK(hipGetLastError());
-        return EMFA_OK;
-    };
+    auto matvec = [&](const double2 *in, double2 *out) -> emfa_status { return gemv(ctx, Z, in, out); };
     return gmres_impl(ctx, n, matvec, B, X, m, tol, max_restart, rep);
 }

@@ -192,6 +192,15 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
                           int accumulate);
 emfa_status convert_copy(emfa_ctx *ctx, const emfa_mat *src, emfa_mat *dst);
 emfa_status axpy_f64(emfa_ctx *ctx, emfa_mat *X, const emfa_mat *D);  // X += D (D any prec)
+// y = Z x for device vectors of n complex FP64 (K7, FP64 accumulation; gmres.hip)
+emfa_status gemv(emfa_ctx *ctx, const emfa_mat *Z, const double2 *x, double2 *y);
+// BiCGSTAB over an operator on device vectors (bicgstab.hip)
+// work: device workspace of work_bytes >= bicgstab_work_bytes(n), or null
+// for the context's scratch
+emfa_status bicgstab_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(const double2 *, double2 *)> &matvec,
+                          const emfa_mat *B, emfa_mat *X, double tol, int max_iter, int variant,
+                          emfa_bicgstab_report *rep, void *work = nullptr, size_t work_bytes = 0);
+size_t bicgstab_work_bytes(int n);
 // restarted GMRES over an operator on device vectors of n complex FP64 (gmres.hip)
 emfa_status gmres_impl(emfa_ctx *ctx, int n, const std::function<emfa_status(const double2 *, double2 *)> &matvec,
                        const emfa_mat *B, emfa_mat *X, int m, double tol, int max_restart, emfa_gmres_report *rep);

@@ -1048,4 +1048,11 @@ emfa_status emfa_mlfma_gmres(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, em
     return gmres_impl(ctx, m->n, op, B, X, restart, tol, max_restart, rep);
 }
 
+emfa_status emfa_mlfma_bicgstab(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, double tol,
+                                int max_iter, int variant, emfa_bicgstab_report *rep) {
+    EMFA_REQUIRE(ctx && m && B && X, EMFA_E_INVALID, "emfa_mlfma_bicgstab: null argument");
+    auto op = [&](const double2 *in, double2 *out) -> emfa_status { return ml_apply(ctx, m, in, out); };
+    return bicgstab_impl(ctx, m->n, op, B, X, tol, max_iter, variant, rep);
+}
+
 }  // extern "C"

@@ -106,15 +106,21 @@ void DenseMoM::init(component::ConfigLoader *ploader) {
         throw component::ConfigInvalid("precision must be FP64 or FP32");
     quad_ = ploader->getInt("quadrature_order:", 3);
     if (quad_ != 3 && quad_ != 7) throw component::ConfigInvalid("quadrature_order must be 3 or 7");
-    std::string ls = ploader->getString("linear_solver:", "LU");
+    std::string ls = ploader->getString("linear_solver:", defaultSolver());
     if (ls == "LU")
-        use_gmres_ = false;
+        solver_ = LinearSolver::LU;
     else if (ls == "GMRES")
-        use_gmres_ = true;
+        solver_ = LinearSolver::GMRES;
+    else if (ls == "BICGSTAB")
+        solver_ = LinearSolver::BICGSTAB;
     else
-        throw component::ConfigInvalid("linear_solver must be LU or GMRES");
+        throw component::ConfigInvalid("linear_solver must be LU, GMRES or BICGSTAB");
     gmres_m_ = ploader->getInt("gmres_restart:", 30);
     gmres_tol_ = ploader->getDouble("gmres_tolerance:", 1e-6);
+    bicg_tol_ = ploader->getDouble("bicgstab_tolerance:", 0.01);
+    bicg_max_ = ploader->getInt("bicgstab_max_iterations:", 500);
+    if (!(bicg_tol_ > 0) || bicg_max_ < 1)
+        throw component::ConfigInvalid("bicgstab_tolerance and bicgstab_max_iterations must be positive");
     std::string cm = ploader->getString("cfie_self_term:", "reference");
     if (cm == "reference")
         cfie_mode_ = 0;
@@ -146,13 +152,17 @@ void DenseMoM::init(component::ConfigLoader *ploader) {
     timeLog("init");
 }
 
+const char *DenseMoM::solverName(LinearSolver s) {
+    return s == LinearSolver::LU ? "LU" : s == LinearSolver::GMRES ? "GMRES" : "BICGSTAB";
+}
+
 void DenseMoM::reportInfo(std::ostream &strm) const {
     mesh_.reportInfo(strm);
     ce_.reportInfo(strm);
     strm << "Device path\n"
          << "  -> precision: " << (prec_ == EMFA_F64 ? "FP64" : "FP32 + FP64 refinement") << '\n'
          << "  -> quadrature: " << quad_ << "x" << quad_ << '\n'
-         << "  -> linear solver: " << (use_gmres_ ? "GMRES" : "LU") << '\n'
+         << "  -> linear solver: " << solverName(solver_) << '\n'
          << "  -> right-hand sides: " << rhs_angles_.size() << '\n';
 }
 
@@ -207,7 +217,17 @@ void DenseMoM::solve() {
             timeLog("fillZ");
         }
         if (matrix_free) {
-        } else if (use_gmres_) {
+        } else if (solver_ == LinearSolver::BICGSTAB) {
+            // EMSolver/CFIE.cpp:74-80: iml::BICGSTAB(Z, V, I, 0.01f, 500, iter_num)
+            emfa_bicgstab_report rep{};
+            emfa_status st = emfa_bicgstab(ctx_, Z, B, X_, bicg_tol_, bicg_max_, 0, &rep);
+            if (st == EMFA_E_NOCONV)
+                logger_ << "  -> WARNING: Iteration cannot converge\n";
+            else
+                check(st, "BICGSTAB");
+            logger_ << "  -> Iterative number: " << rep.iterations << ", relative residual: " << rep.relres << '\n';
+            std::cout << "  -> Iterative number: " << rep.iterations << std::endl;
+        } else if (solver_ == LinearSolver::GMRES) {
             emfa_gmres_report rep{};
             emfa_status st = emfa_gmres(ctx_, Z, B, X_, gmres_m_, gmres_tol_, 1000, &rep);
             logger_ << "  -> GMRES iterations: " << rep.iterations << ", restarts: " << rep.restarts
@@ -461,6 +481,7 @@ void MLFMA::init(component::ConfigLoader *ploader) {
         throw component::ConfigInvalid("MLFMA: iteration_threshold, max_iteration_number and mlfma_box_threshold "
                                        "must be positive");
     DenseMoM::init(ploader);
+    if (solver_ == LinearSolver::LU) throw component::ConfigInvalid("MLFMA: linear_solver must be BICGSTAB or GMRES");
 }
 
 // EMSolver/MLFMA.cpp:32: "_MLFMA_CFIE_" + to_string(alpha).substr(0, 3)
@@ -470,7 +491,8 @@ void MLFMA::reportInfo(std::ostream &strm) const {
     DenseMoM::reportInfo(strm);
     strm << "MLFMA\n"
          << "  -> box threshold: " << box_threshold_ << " wavelengths\n"
-         << "  -> GMRES(" << gmres_m_ << ") to " << iter_threshold_ << ", at most " << max_iter_ << " iterations\n"
+         << "  -> " << (solver_ == LinearSolver::GMRES ? "GMRES(" + std::to_string(gmres_m_) + ")" : "BICGSTAB")
+         << " to " << iter_threshold_ << ", at most " << max_iter_ << " iterations\n"
          << "  -> preconditioning: " << (precond_requested_ ? "requested, not built (unpreconditioned)" : "off")
          << '\n';
 }
@@ -488,16 +510,28 @@ bool MLFMA::solveMatrixFree(emfa_mat *B) {
     for (int l = 2; l <= info.levels; ++l)
         logger_ << "  -> level " << l << ": " << info.boxes[l] << " boxes, L = " << info.multipoles[l] << '\n';
     timeLog("MLFMA setup");
-    const int m = std::max(1, gmres_m_);
-    emfa_gmres_report rep{};
-    st = emfa_mlfma_gmres(ctx_, op, B, X_, m, iter_threshold_, (max_iter_ + m - 1) / m, &rep);
+    int iters = 0;
+    double relres = 0;
+    if (solver_ == LinearSolver::GMRES) {
+        const int m = std::max(1, gmres_m_);
+        emfa_gmres_report rep{};
+        st = emfa_mlfma_gmres(ctx_, op, B, X_, m, iter_threshold_, (max_iter_ + m - 1) / m, &rep);
+        iters = rep.iterations;
+        relres = rep.relres;
+    } else {
+        // MLFMA::iterativeSolve (MLFMA.cpp:402-452)
+        emfa_bicgstab_report rep{};
+        st = emfa_mlfma_bicgstab(ctx_, op, B, X_, iter_threshold_, max_iter_, 0, &rep);
+        iters = rep.iterations;
+        relres = rep.relres;
+    }
     emfa_mlfma_destroy(op);
-    logger_ << "  -> Iterative number: " << rep.iterations << ", relative residual: " << rep.relres << '\n';
-    std::cout << "  -> Iterative number: " << rep.iterations << std::endl;
+    logger_ << "  -> Iterative number: " << iters << ", relative residual: " << relres << '\n';
+    std::cout << "  -> Iterative number: " << iters << std::endl;
     if (st == EMFA_E_NOCONV)
         logger_ << "  -> WARNING: Iteration cannot converge\n";
     else
-        check(st, "MLFMA GMRES");
+        check(st, solverName(solver_));
     return true;
 }
 

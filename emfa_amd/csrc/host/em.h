@@ -62,9 +62,12 @@ emfa_ctx *device_context(int device);
 //   precision:        FP64 | FP32          [FP64]  FP32 = complex-FP32 fill and
 //                                                  LU with FP64 iterative refinement
 //   quadrature_order: 3 | 7                [3]     triangle rule of the fill
-//   linear_solver:    LU | GMRES           [LU]
+//   linear_solver:    LU | GMRES | BICGSTAB [LU; CFIE: BICGSTAB as CFIE.cpp:76;
+//                                                  MLFMA: BICGSTAB as MLFMA.cpp:402-452]
 //   gmres_restart:    m                    [30]
 //   gmres_tolerance:  relative residual    [1e-6]
+//   bicgstab_tolerance:      ||s|| / ||b|| [0.01, CFIE.cpp:76]
+//   bicgstab_max_iterations: steps         [500, CFIE.cpp:76]
 //   cfie_self_term:   reference | accurate [reference]
 //   write_matrix:     true | false         [false] ASCII Z/V/I as EFIE.cpp:932-950
 //   device:           HIP ordinal          [0]
@@ -88,6 +91,8 @@ class DenseMoM : public EM {
     virtual int rhsKind() const = 0;   // EMFA_EFIE, EMFA_CFIE or EMFA_MFIE
     virtual double alpha() const { return 1.0; }
     virtual std::string folderTag() const = 0;
+    // the reference's solver for this formulation when linear_solver: is absent
+    virtual std::string defaultSolver() const { return "LU"; }
 
     // Matrix-free solvers override this: solve Z X_ = B without a dense Z
     // and return true.  The dense path (fill, then LU or GMRES) runs otherwise.
@@ -106,9 +111,12 @@ class DenseMoM : public EM {
     int unknowns_ = 0;
     double k_ = 0;
     std::complex<double> sigma_{0, 0};
-    int prec_ = EMFA_F64, quad_ = 3, cfie_mode_ = 0, gmres_m_ = 30, device_ = 0;
-    double gmres_tol_ = 1e-6;
-    bool use_gmres_ = false, write_matrix_ = false;
+    enum class LinearSolver { LU, GMRES, BICGSTAB };
+    int prec_ = EMFA_F64, quad_ = 3, cfie_mode_ = 0, gmres_m_ = 30, device_ = 0, bicg_max_ = 500;
+    double gmres_tol_ = 1e-6, bicg_tol_ = 0.01;
+    LinearSolver solver_ = LinearSolver::LU;
+    bool write_matrix_ = false;
+    static const char *solverName(LinearSolver s);
     std::vector<std::array<double, 3>> rhs_angles_;  // (theta, phi, pole) per column
     std::vector<std::array<double, 3>> rcs_dirs_;    // monostatic: direction per column
     int ncols_ = 0;
@@ -135,6 +143,7 @@ class CFIE : public DenseMoM {
     int rhsKind() const override { return EMFA_CFIE; }
     double alpha() const override { return alpha_; }
     std::string folderTag() const override;
+    std::string defaultSolver() const override { return "BICGSTAB"; }
     double alpha_ = 0.5;
 };
 
@@ -150,11 +159,12 @@ class MFIE : public DenseMoM {
 
 // EMSolver/MLFMA.cpp: the CFIE operator applied by the multilevel fast
 // multipole method (near matrix + aggregation / translation /
-// disaggregation on the device) and solved by GMRES(m) to
-// iteration_threshold (the reference iterates with BiCGSTAB,
-// MLFMA.cpp:402-452).  Keys: alpha:, iteration_threshold:,
-// max_iteration_number:, mlfma_box_threshold:; preconditioning: is read and
-// reported but the sparse approximate inverse is not part of this build.
+// disaggregation on the device), solved by BiCGSTAB to iteration_threshold
+// in at most max_iteration_number steps as MLFMA.cpp:402-452 does, or by
+// GMRES(gmres_restart) with linear_solver: GMRES.  Keys: alpha:,
+// iteration_threshold:, max_iteration_number:, mlfma_box_threshold:;
+// preconditioning: is read and reported but the sparse approximate inverse
+// is not part of this build.
 class MLFMA : public DenseMoM {
   public:
     void init(component::ConfigLoader *ploader) override;
@@ -166,6 +176,7 @@ class MLFMA : public DenseMoM {
     double alpha() const override { return alpha_; }
     std::string folderTag() const override;
     bool solveMatrixFree(emfa_mat *B) override;
+    std::string defaultSolver() const override { return "BICGSTAB"; }
     double alpha_ = 0.5, box_threshold_ = 0.5, iter_threshold_ = 1e-3;
     int max_iter_ = 1000;
     bool precond_requested_ = false;

@@ -26,7 +26,7 @@ EXPORTS = [
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
-    "emfa_lu_set_update_mode",
+    "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab",
 ]
 
 _vp = ctypes.c_void_p
@@ -45,6 +45,10 @@ class RefineReport(ctypes.Structure):
 
 class GmresReport(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("relres", ctypes.c_double)]
+
+
+class BicgstabReport(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("max_steps", ctypes.c_int), ("relres", ctypes.c_double)]
 
 
 class LuTuning(ctypes.Structure):
@@ -375,6 +379,18 @@ def gemm_residual(ctx, A, X, C, negate=True, accumulate=True):
     _check(lib().emfa_gemm_residual(ctx.h, A.h, X.h, C.h, int(negate), int(accumulate)))
 
 
+def bicgstab(ctx, Z, B, X, tol=1e-2, max_iter=500, variant=0):
+    """BiCGSTAB from x0 = 0 (variant 0: EMIterative/iml.cpp:3-39, as CFIE.cpp:76
+    calls it; variant 1: the matrix-free overload iml.cpp:88-154).
+    Returns (report, converged)."""
+    rep = BicgstabReport()
+    code = lib().emfa_bicgstab(ctx.h, Z.h, B.h, X.h, ctypes.c_double(tol), int(max_iter), int(variant),
+                               ctypes.byref(rep))
+    if code not in (EMFA_OK, EMFA_E_NOCONV):
+        _check(code)
+    return rep, code == EMFA_OK
+
+
 def gmres(ctx, Z, B, X, m=30, tol=1e-6, max_restart=100):
     rep = GmresReport()
     code = lib().emfa_gmres(ctx.h, Z.h, B.h, X.h, m, ctypes.c_double(tol), max_restart, ctypes.byref(rep))
@@ -402,6 +418,14 @@ class MLFMA:
 
     def matvec(self, X, Y):
         _check(lib().emfa_mlfma_matvec(self.ctx.h, self.h, X.h, Y.h))
+
+    def bicgstab(self, B, X, tol=1e-4, max_iter=500, variant=0):
+        rep = BicgstabReport()
+        code = lib().emfa_mlfma_bicgstab(self.ctx.h, self.h, B.h, X.h, ctypes.c_double(tol), int(max_iter),
+                                         int(variant), ctypes.byref(rep))
+        if code not in (EMFA_OK, EMFA_E_NOCONV):
+            _check(code)
+        return rep, code == EMFA_OK
 
     def gmres(self, B, X, m=30, tol=1e-4, max_restart=50):
         rep = GmresReport()

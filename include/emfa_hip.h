@@ -213,6 +213,24 @@ typedef struct {
 emfa_status emfa_gmres(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, int m, double tol,
                        int max_restart, emfa_gmres_report *rep);
 
+/* ---- BiCGSTAB --------------------------------------------------------- */
+typedef struct {
+    int iterations;  /* steps, summed over the right-hand sides */
+    int max_steps;   /* most steps any right-hand side took */
+    double relres;   /* largest ||b - Z x|| / ||b|| of the returned x */
+} emfa_bicgstab_report;
+/* Solves Z x = b column by column from x0 = 0 (X is overwritten), FP64
+ * vectors and inner products, the matrix in its own precision.
+ * variant 0 replaces iml::BICGSTAB(const Qcx_mat&, ...) (EMIterative/
+ * iml.cpp:3-39; EMSolver/CFIE.cpp:76 calls it with tol 0.01, 500 steps):
+ * stops when ||s|| / ||b|| < tol, EMFA_E_NOCONV (X still written) when a
+ * column ran all max_iter steps, as the reference's `false`.
+ * variant 1 replaces the matrix-free overload (iml.cpp:88-154): stops when
+ * ||r|| / ||b|| <= tol or after max_iter steps, restarts the shadow
+ * residual when rho vanishes, and never reports failure (as the reference). */
+emfa_status emfa_bicgstab(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, double tol, int max_iter,
+                          int variant, emfa_bicgstab_report *rep);
+
 /* ---- far field (K9) --------------------------------------------------- */
 /* RCS in dB for ndir directions dirs[ndir][3]; direction d uses column
  * cols[d] of X (cols may be NULL: column 0).  Replaces getBiRCS
@@ -255,6 +273,12 @@ emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, e
  * the initial guess.  EMFA_E_NOCONV when a column misses tol. */
 emfa_status emfa_mlfma_gmres(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, int restart, double tol,
                              int max_restart, emfa_gmres_report *rep);
+
+/* BiCGSTAB on the MLFMA operator (MLFMA::iterativeSolve, MLFMA.cpp:402-452,
+ * runs variant 0 of emfa_bicgstab; the sparse-approximate-inverse
+ * preconditioner of MLFMA.cpp:1294-1391 is not built); X is overwritten. */
+emfa_status emfa_mlfma_bicgstab(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, double tol,
+                                int max_iter, int variant, emfa_bicgstab_report *rep);
 
 /* When enabled, each launch of a named kernel family is bracketed by HIP
  * events on the context stream; totals are read with emfa_prof_get.

@@ -110,7 +110,7 @@ def test_efie_bistatic_files_match_oracle(tmp_path, precision):
 
 def test_cfie_monostatic_multi_rhs(tmp_path):
     xyz, tri = _run(tmp_path, "CFIE", "MonostaticRCS", freq="6e8", radius=1.0, tto=180, tdelta=30,
-                    extra="    precision: FP32\n    quadrature_order: 3\n")
+                    extra="    precision: FP32\n    quadrature_order: 3\n    linear_solver: LU\n")
     out = tmp_path / "sphere_CFIE_0.5"
     rwg, _ = O.build_rwg(tri)
     k = O.wavenumber(6e8)
@@ -121,6 +121,28 @@ def test_cfie_monostatic_multi_rhs(tmp_path):
         X = np.linalg.solve(Z, B)
         ref = O.rcs(xyz, tri, rwg, k, X, _dirs(th, ph), np.arange(len(th)))
         assert np.abs(val - ref).max() <= 0.1 + 1e-4
+
+
+def test_cfie_default_bicgstab_as_the_reference(tmp_path):
+    """EMSolver: CFIE without linear_solver: solves with BiCGSTAB at the
+    reference's settings (CFIE.cpp:76: eps 0.01, at most 500 steps, x0 = 0),
+    on the device's FP64 matrix.  The oracle's restatement of the same
+    iml::BICGSTAB overload on the oracle's matrix takes the same steps and
+    gives the same RCS to the files' 4 decimals; the log carries the
+    reference's "Iterative number" line."""
+    xyz, tri = _run(tmp_path, "CFIE", "BistaticRCS", freq="6e8", radius=1.0, tto=180, tdelta=10)
+    out = tmp_path / "sphere_CFIE_0.5"
+    log = open(out / "sca_runtime.log").read()
+    assert "linear solver: BICGSTAB" in log and "Iterative number" in log
+    rwg, _ = O.build_rwg(tri)
+    k = O.wavenumber(6e8)
+    Z = O.fill_cfie(xyz, tri, rwg, k, 0.5, 3, 0)
+    b = O.rhs(xyz, tri, rwg, 1, k, 0.5, O.inc_table([(0, 0, 0)]))[:, 0]
+    x, steps, ok = O.bicgstab(Z, b, 0.01, 500)
+    assert ok and f"Iterative number: {steps}," in log
+    th, val = _read_rcs(out / "bistatic_rcs_phi_0.dat")
+    ref = O.rcs(xyz, tri, rwg, k, x, _dirs(th, 0))
+    assert np.abs(val - ref).max() <= 1e-3
 
 
 def test_mfie_and_gmres(tmp_path):

@@ -161,6 +161,25 @@ def test_gmres_solution_and_rcs_near_dense_solve(ctx, built):
         m.close()
 
 
+def test_bicgstab_on_the_operator_matches_gmres(ctx, built):
+    """MLFMA::iterativeSolve's BiCGSTAB (MLFMA.cpp:402-452) on the device
+    operator reaches the operator's own GMRES solution."""
+    v = built["s4_800MHz"]
+    n, k = v["rwg"].shape[0], v["k"]
+    inc = hip.inc_table([(0.0, 0.0, 0.0), (90.0, 0.0, 90.0)])
+    B = hip.Mat(ctx, n, 2, hip.F64)
+    hip.rhs_fill(ctx, v["mesh"], B, hip.CFIE, k, ALPHA, inc)
+    X = hip.Mat(ctx, n, 2, hip.F64)
+    Xg = hip.Mat(ctx, n, 2, hip.F64)
+    rep, ok = v["op"].bicgstab(B, X, tol=1e-6, max_iter=500)
+    assert ok and rep.relres < 1e-5
+    rep2, ok2 = v["op"].gmres(B, Xg, m=30, tol=1e-8, max_restart=20)
+    assert ok2
+    assert rel(X.numpy(), Xg.numpy()) < 1e-3
+    for m in (B, X, Xg):
+        m.close()
+
+
 def test_sibling_grouped_translation_gives_the_same_matvec(ctx, built, monkeypatch):
     # the opt-in kernel visits each child's far boxes in far-list order: same sums
     v = built["s4_800MHz"]

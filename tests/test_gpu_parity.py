@@ -488,6 +488,47 @@ def test_gmres(ctx):
     assert rel(X.numpy(), ref) < 1e-8
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("prec", [hip.F64, hip.F32])
+def test_bicgstab_matches_oracle(ctx, variant, prec):
+    """Device BiCGSTAB against the oracle's restatement of the same overload
+    (variant 0: EMIterative/iml.cpp:3-39 at CFIE.cpp:76's eps 0.01 / 500;
+    variant 1: iml.cpp:88-154) on the same matrix: the same step count per
+    right-hand side and the same iterate to 1e-9 (FP64 vectors and products
+    on both sides; an FP32 matrix is widened exactly).  At eps 1e-10 both
+    variants reach the LU solution."""
+    _say(f"bicgstab variant={variant} prec={prec}")
+    xyz, tri, rwg = sphere(2, 1.0)
+    k = hip.wavenumber(6e8)
+    M = hip.Mesh(ctx, xyz, tri, rwg)
+    Z = hip.Mat(ctx, M.n, M.n, prec)
+    hip.fill(ctx, M, Z, hip.CFIE, k, 3, alpha=0.5)
+    inc = hip.inc_table([(0, 0, 0), (70, 20, 90), (180, 0, 0)])
+    B = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    hip.rhs_fill(ctx, M, B, hip.CFIE, k, 0.5, inc)
+    X = hip.Mat(ctx, M.n, inc.shape[0], hip.F64)
+    Zh, Bh = Z.numpy(), B.numpy()
+    rep, ok = hip.bicgstab(ctx, Z, B, X, tol=1e-2, max_iter=500, variant=variant)
+    assert ok
+    steps = 0
+    for j in range(inc.shape[0]):
+        if variant == 0:
+            xo, it, oko = O.bicgstab(Zh, Bh[:, j], 1e-2, 500)
+        else:
+            xo, it, _ = O.bicgstab_mf(Zh, Bh[:, j], 1e-2, 500)
+        steps += it
+        assert rel(X.numpy()[:, j], xo) < 1e-9
+    assert rep.iterations == steps
+    rep2, ok2 = hip.bicgstab(ctx, Z, B, X, tol=1e-10, max_iter=500, variant=variant)
+    assert ok2 and rep2.relres < 1e-9
+    assert rel(X.numpy(), np.linalg.solve(Zh, Bh)) < 1e-7
+    # too few steps: variant 0 reports it as the reference's `false`
+    rep3, ok3 = hip.bicgstab(ctx, Z, B, X, tol=1e-12, max_iter=2, variant=variant)
+    assert ok3 == (variant == 1) and rep3.max_steps == 2
+    for m in (X, B, Z, M):
+        m.close()
+
+
 # ---------------------------------------------------------------- end to end (C1 mesh)
 @pytest.mark.parametrize("quad", [3, 7])
 def test_end_to_end_efie_sphere(ctx, quad):
