@@ -8,6 +8,8 @@
 #include <iomanip>
 #include <iostream>
 #include <limits>
+#include <map>
+#include <array>
 
 #include "emfa_hip.h"
 
@@ -274,8 +276,95 @@ void Mesh::loadMeshFile(const std::string &filename) {
         loadNasStream(in);
         return;
     }
-    if (suf == "rpt" || suf == "stl") throw FileError("mesh format ." + suf + " is not supported by this build");
+    if (suf == "rpt" || suf == "stl") {
+        std::ifstream in(filename);
+        if (in.fail()) throw FileError("fail loading " + suf + " mesh file: " + filename);
+        if (suf == "rpt")
+            loadRptStream(in);
+        else
+            loadStlStream(in);
+        return;
+    }
     throw FileError("cannot identify mesh file type: " + suf + " (1.rpt, 2.stl(text), 3.nas)");
+}
+
+// A coordinate as the reference keeps it after its temp.dat round trip:
+// parsed as float, written with the default stream precision (6 significant
+// digits), read back as float.
+static double round6(float v) {
+    std::ostringstream o;
+    o << v;
+    float r = 0;
+    std::istringstream(o.str()) >> r;
+    return (double)r;
+}
+
+// .rpt of EMComponent/Mesh.cpp:178-218 + :673-724: two header lines, then
+// "id n1 n2 n3" per triangle (node positions from 1) until a record does
+// not parse, that line and the next skipped, then "id x y z i j" per node in
+// position order; anything after the nodes makes the file invalid.
+void Mesh::loadRptStream(std::istream &in) {
+    const auto maxs = std::numeric_limits<std::streamsize>::max();
+    in.ignore(maxs, '\n');
+    in.ignore(maxs, '\n');
+    std::vector<int> tri;
+    int id = 0, p1 = 0, p2 = 0, p3 = 0;
+    while (in >> id >> p1 >> p2 >> p3) {
+        tri.push_back(p1 - 1);
+        tri.push_back(p2 - 1);
+        tri.push_back(p3 - 1);
+    }
+    in.clear();
+    in.ignore(maxs, '\n');
+    in.ignore(maxs, '\n');
+    std::vector<double> xyz;
+    float c1 = 0, c2 = 0, c3 = 0;
+    while (in >> id >> c1 >> c2 >> c3 >> p1 >> p1) {
+        xyz.push_back(round6(c1));
+        xyz.push_back(round6(c2));
+        xyz.push_back(round6(c3));
+    }
+    if (!in.eof()) throw FileError("fail loading rpt mesh file: unexpected record after the nodes");
+    const int nv = (int)(xyz.size() / 3);
+    for (int v : tri)
+        if (v < 0 || v >= nv) throw FileError("fail loading rpt mesh file: vertex id out of range");
+    setGeometry(std::move(xyz), std::move(tri));
+}
+
+// ASCII .stl of EMComponent/Mesh.cpp:220-259 + :781-829: after the first
+// line, per facet the three numbers after the next 'l' (the facet normal)
+// and the three after each of the next three 'x' (the vertices), until a
+// facet's normal does not parse.  The reference keeps each triangle's three
+// corners and the file's normal without vertex numbers; here corners with
+// identical (rounded) coordinates become one vertex, numbered in order of
+// first appearance, so that RWG functions can be built across facets.  The
+// normal is recomputed from the corner order, as for the other formats.
+void Mesh::loadStlStream(std::istream &in) {
+    const auto maxs = std::numeric_limits<std::streamsize>::max();
+    in.ignore(maxs, '\n');
+    std::vector<double> xyz;
+    std::vector<int> tri;
+    std::map<std::array<double, 3>, int> ids;
+    float c1 = 0, c2 = 0, c3 = 0;
+    while (true) {
+        in.ignore(maxs, 'l');
+        in >> c1 >> c2 >> c3;  // facet normal
+        if (in.fail()) break;
+        for (int v = 0; v < 3; ++v) {
+            in.ignore(maxs, 'x');
+            in >> c1 >> c2 >> c3;
+            std::array<double, 3> p{round6(c1), round6(c2), round6(c3)};
+            auto it = ids.find(p);
+            if (it == ids.end()) {
+                it = ids.emplace(p, (int)(xyz.size() / 3)).first;
+                xyz.insert(xyz.end(), p.begin(), p.end());
+            }
+            tri.push_back(it->second);
+        }
+        in.ignore(maxs, 'l');
+    }
+    if (tri.empty()) throw FileError("fail loading stl mesh file: no facets");
+    setGeometry(std::move(xyz), std::move(tri));
 }
 
 // NASTRAN dialect of EMComponent/Mesh.cpp:261-309: 9 header lines, then
@@ -291,13 +380,6 @@ void Mesh::loadNasStream(std::istream &in) {
     std::vector<double> xyz;
     std::vector<int> tri;
     int nodeNum = 0, triNum = 0;
-    auto round6 = [](float v) {
-        std::ostringstream o;
-        o << v;  // default precision 6
-        float r = 0;
-        std::istringstream(o.str()) >> r;
-        return (double)r;
-    };
     while (true) {
         int ch = in.peek();
         if (ch == EOF) throw FileError("nas file ended before the 'E' record");

@@ -113,9 +113,11 @@ class ConfigLoader {
 // Triangle mesh: vertices (metres) and 0-based triangles.
 class Mesh {
   public:
-    // EMComponent/Mesh.cpp:22-49: chosen by suffix; .nas parsed in memory
+    // EMComponent/Mesh.cpp:22-49: chosen by suffix (.nas, .rpt, .stl), parsed in memory
     void loadMeshFile(const std::string &filename);
     void loadNasStream(std::istream &in);
+    void loadRptStream(std::istream &in);
+    void loadStlStream(std::istream &in);
     void setGeometry(std::vector<double> xyz, std::vector<int> tri);
     size_t getNodeNum() const { return xyz_.size() / 3; }
     size_t getTriangleNum() const { return tri_.size() / 3; }

@@ -17,6 +17,11 @@
 //
 // usage: emfa_ref efie|cfie|mfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin
 //        emfa_ref time cfie|efie mesh.nas freq alpha
+//        emfa_ref mesh file.nas|.rpt|.stl out.bin
+// The third form loads a mesh with Mesh::loadMeshFile (EMComponent/Mesh.cpp:
+// 22-49, any of the three formats) and writes what the reference holds:
+// int32 nv, nt; float xyz[nv*3]; per triangle int32 ids[3] and float
+// corners[9], normal[3].
 // The second form times the reference's own fillZ, one fillV and the far
 // field of 37 directions (getBiRCS) on the mesh, single-threaded as the
 // reference runs, and prints "n fillZ_s fillV_s rcs_s_per_direction".
@@ -342,7 +347,35 @@ void write_output(const char *path, const Out &o) {
 
 }  // namespace
 
+int dump_mesh(const char *file, const char *out) {
+    component::Mesh m;
+    m.loadMeshFile(file);
+    FILE *f = std::fopen(out, "wb");
+    if (!f) return 3;
+    int hdr[2] = {(int)m.getNodeNum(), (int)m.getTriangleNum()};
+    std::fwrite(hdr, sizeof(int), 2, f);
+    for (int i = 0; i < hdr[0]; ++i) {
+        const VectorR3 &v = m.getVertex(i);
+        float x[3] = {v.x, v.y, v.z};
+        std::fwrite(x, sizeof(float), 3, f);
+    }
+    for (int t = 0; t < hdr[1]; ++t) {
+        const auto &tr = m.getTriangleRef(t);
+        int ids[3];
+        tr.getVertex(ids[0], ids[1], ids[2]);
+        VectorR3 a, b, c;
+        tr.getVertex(a, b, c);
+        const VectorR3 &n = tr.getNormal();
+        float x[12] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, n.x, n.y, n.z};
+        std::fwrite(ids, sizeof(int), 3, f);
+        std::fwrite(x, sizeof(float), 12, f);
+    }
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc == 4 && std::string(argv[1]) == "mesh") return dump_mesh(argv[2], argv[3]);
     if (argc == 6 && std::string(argv[1]) == "time") {
         std::string kind = argv[2];
         float freq = std::strtof(argv[4], nullptr), alpha = std::strtof(argv[5], nullptr);

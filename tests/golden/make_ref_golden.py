@@ -36,6 +36,68 @@ CASES = {
 }
 
 
+def rpt_text(xyz, tri):
+    """.rpt as EMComponent/Mesh.cpp:178-218 reads it: two header lines,
+    triangles, a separator and a header line, then nodes (id x y z i j)."""
+    out = ["RPT mesh", "TRIANGLES"]
+    out += [f"{i + 1} {t[0] + 1} {t[1] + 1} {t[2] + 1}" for i, t in enumerate(tri)]
+    out += ["END TRIANGLES", "NODES"]
+    out += [f"{i + 1} {p[0]:.9e} {p[1]:.9e} {p[2]:.9e} 0 0" for i, p in enumerate(xyz)]
+    return "\n".join(out) + "\n"
+
+
+def stl_text(xyz, tri):
+    """ASCII STL as EMComponent/Mesh.cpp:220-259 reads it."""
+    out = ["solid emfa"]
+    for t in tri:
+        a, b, c = xyz[t[0]], xyz[t[1]], xyz[t[2]]
+        n = np.cross(b - a, c - b)
+        n = n / np.linalg.norm(n)
+        out.append(f"  facet normal {n[0]:.9e} {n[1]:.9e} {n[2]:.9e}")
+        out.append("    outer loop")
+        for p in (a, b, c):
+            out.append(f"      vertex {p[0]:.9e} {p[1]:.9e} {p[2]:.9e}")
+        out.append("    endloop")
+        out.append("  endfacet")
+    out.append("endsolid emfa")
+    return "\n".join(out) + "\n"
+
+
+# name: (format, mesh builder)
+MESH_CASES = {
+    "rpt_sphere_s1": ("rpt", lambda: meshgen.icosphere(1, 0.731)),
+    "rpt_plate3": ("rpt", lambda: meshgen.plate(3, 1.3)),
+    "stl_sphere_s1": ("stl", lambda: meshgen.icosphere(1, 0.731)),
+    "stl_ellipsoid_s1": ("stl", lambda: meshgen.ellipsoid(1, 1.1, 0.7, 0.45)),
+}
+
+
+def read_mesh_dump(path):
+    with open(path, "rb") as f:
+        nv, nt = np.frombuffer(f.read(8), dtype=np.int32)
+        xyz = np.frombuffer(f.read(12 * nv), dtype=np.float32).reshape(nv, 3)
+        rec = np.frombuffer(f.read(60 * nt), dtype=np.uint8).reshape(nt, 60)
+    ids = rec[:, :12].copy().view(np.int32).reshape(nt, 3)
+    fl = rec[:, 12:].copy().view(np.float32).reshape(nt, 12)
+    return xyz, ids, fl[:, :9].reshape(nt, 3, 3), fl[:, 9:]
+
+
+def make_mesh_fixtures():
+    for name, (fmt, build) in MESH_CASES.items():
+        xyz, tri = build()
+        text = rpt_text(xyz, tri) if fmt == "rpt" else stl_text(xyz, tri)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "mesh." + fmt)
+            with open(path, "w") as f:
+                f.write(text)
+            out = os.path.join(d, "out.bin")
+            subprocess.run([EXE, "mesh", path, out], cwd=d, check=True, stdout=subprocess.DEVNULL)
+            rx, ids, corners, normals = read_mesh_dump(out)
+        np.savez_compressed(os.path.join(HERE, f"ref_mesh_{name}.npz"), format=fmt, text=np.array(text),
+                            xyz=rx, ids=ids, corners=corners, normals=normals)
+        print("mesh", name, "vertices", rx.shape[0], "triangles", ids.shape[0])
+
+
 def read_out(path):
     with open(path, "rb") as f:
         nv, nt, n, na = np.frombuffer(f.read(16), dtype=np.int32)
@@ -88,3 +150,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    make_mesh_fixtures()

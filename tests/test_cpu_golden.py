@@ -19,7 +19,7 @@ from golden.make_ref_golden import CASES
 from test_cpu_host import _read_mesh, write_nas
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_*.npz")))
+FIXTURES = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "ref_*.npz")) if "ref_mesh_" not in p)
 
 
 def _name(path):
@@ -115,3 +115,39 @@ def test_oracle_bicgstab_matches_reference_golden(path):
     x2, it2, rr = O.bicgstab_mf(Z, b, 0.01, 500)
     assert rr <= 0.01
     assert np.abs(x2 - g["bicgstab_mf_x"]).max() < 2e-5 * np.abs(x2).max()
+
+
+MESHES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_mesh_*.npz")))
+
+
+@pytest.mark.parametrize("path", MESHES, ids=lambda p: os.path.basename(p)[9:-4])
+def test_rpt_stl_readers_read_what_the_reference_reads(path, tmp_path):
+    """The .rpt and .stl readers against the reference's own Mesh::loadMeshFile
+    (EMComponent/Mesh.cpp:22-49, 178-259, 673-829) on the same file text: the
+    same coordinates to the bit (6-digit temp-file rounding included), the same
+    triangles.  The reference keeps .stl facets as corner triples without
+    vertex numbers; the reader joins equal corners into numbered vertices, so
+    there every triangle's corners must be the reference's corners."""
+    g = np.load(path)
+    fmt = str(g["format"])
+    f = tmp_path / f"mesh.{fmt}"
+    f.write_text(str(g["text"]))
+    rc, xyz, tri = _read_mesh(str(f))
+    assert rc == 0
+    corners = xyz[tri]  # nt x 3 x 3
+    assert np.array_equal(corners.astype(np.float32), g["corners"])
+    assert np.array_equal(corners, g["corners"].astype(np.float64))
+    if fmt == "rpt":
+        assert np.array_equal(tri, g["ids"])
+        assert np.array_equal(xyz.astype(np.float32), g["xyz"])
+    else:
+        # equal corners share a vertex, distinct corners do not
+        assert len(np.unique(xyz, axis=0)) == xyz.shape[0]
+        assert xyz.shape[0] == len(np.unique(g["corners"].reshape(-1, 3), axis=0))
+        # the file's facet normals agree with the corner order's right-hand normal
+        n = np.cross(corners[:, 1] - corners[:, 0], corners[:, 2] - corners[:, 1])
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        assert (np.sum(n * g["normals"], axis=1) > 0.9999).all()
+        # a closed surface: every edge shared by exactly two facets (RWG count 1.5 T)
+        rwg, nb = O.build_rwg(tri)
+        assert nb == 0 and rwg.shape[0] == 3 * tri.shape[0] // 2
