@@ -69,6 +69,34 @@ int emfa_core_read_mesh(const char *path, double *xyz, int nv_cap, int *tri, int
     }
 }
 
+// Test hook for the result-file formats: writes the bistatic files and
+// current.sc of fixed synthetic values (an RCS of 10 z + 5 x - 30 dB in
+// direction (x, y, z); five triangles of set corners and magnitudes) with the
+// writers the solvers use, for comparison with the reference's writers.
+int emfa_core_write_formats(const char *dir, float th0, float th1, float thd, float ph0, float ph1, float phd) {
+    try {
+        component::Scattering sca{th0, th1, thd, ph0, ph1, phd};
+        component::result::writeBistatic(dir, sca, [](int nd, const double *d, double *out) {
+            for (int i = 0; i < nd; ++i) out[i] = 10.0f * (float)d[3 * i + 2] + 5.0f * (float)d[3 * i] - 30.0f;
+        });
+        std::vector<component::CurrentData> cd(5);
+        for (int t = 0; t < 5; ++t) {
+            auto &c = cd[t];
+            c.v1 = {0.1f * t, -0.25f, 1.0f / 3.0f};
+            c.v2 = {-1.5f + t, 2.0f / 7.0f, 1e-3f * t};
+            c.v3 = {123.456f, -0.000123f * t, 7.0f};
+            c.magn1 = 1e-3f * (t + 1);
+            c.magn2 = 0.5f / (t + 1);
+            c.magn3 = 3.25f * t;
+            c.magnc = 1.0f / 3.0f + t;
+        }
+        component::result::writeCurrent(dir, cd);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(3, e.what());
+    }
+}
+
 // Runs solver::loadSolver on a config file.  Returns 0, or the launcher's
 // exit codes (2 config, 3 file, 4 invalid argument, 5 runtime, 6 other).
 int emfa_core_run(const char *config_path) {

@@ -5,6 +5,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <limits>
@@ -465,6 +466,41 @@ void prefixSCData(std::ostream &strm) {
          << std::setw(18) << "v2.x" << std::setw(18) << "v2.y" << std::setw(18) << "v2.z" << std::setw(18) << "v3.x"
          << std::setw(18) << "v3.y" << std::setw(18) << "v3.z" << std::setw(18) << "magn1" << std::setw(18)
          << "magn2" << std::setw(18) << "magn3" << std::setw(18) << "magn center" << '\n';
+}
+
+void writeBistatic(const std::string &dir, const Scattering &sca,
+                   const std::function<void(int, const double *, double *)> &rcs) {
+    const float d2r = (float)(3.14159265358979323846 / 180.0);
+    std::vector<double> dirs;
+    std::vector<std::pair<std::string, std::vector<float>>> files;  // name, angle column
+    int num = static_cast<int>((sca.phi_to - sca.phi_from) / sca.phi_delta + 1);
+    for (int i = 0; i < num; ++i) {
+        float phiDegree = sca.phi_from + i * sca.phi_delta;
+        float phi = phiDegree * d2r;
+        std::string str = std::to_string(phiDegree);
+        files.push_back({"/bistatic_rcs_phi_" + str.substr(0, str.find('.')) + ".dat", {}});
+        for (float thetaDegree = sca.theta_from; thetaDegree <= sca.theta_to; thetaDegree += sca.theta_delta) {
+            float theta = thetaDegree * d2r;
+            dirs.insert(dirs.end(), {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)});
+            files.back().second.push_back(thetaDegree);
+        }
+    }
+    files.push_back({"/bistatic_rcs_theta_90.dat", {}});
+    for (float phiDegree = 0; phiDegree < 360; phiDegree += 1) {
+        float phi = phiDegree * d2r;
+        dirs.insert(dirs.end(), {std::cos(phi), std::sin(phi), 0.0});
+        files.back().second.push_back(phiDegree);
+    }
+    const int nd = (int)(dirs.size() / 3);
+    std::vector<double> db(nd);
+    rcs(nd, dirs.data(), db.data());
+    size_t o = 0;
+    for (auto &f : files) {
+        std::ofstream out(dir + f.first);
+        if (out.fail()) throw FileError("cannot write " + dir + f.first);
+        prefixRCSData(out);
+        for (float ang : f.second) out << std::setw(10) << ang << '\t' << std::setw(10) << (float)db[o++] << '\n';
+    }
 }
 
 void writeCurrent(const std::string &dir, const std::vector<CurrentData> &data) {

@@ -10,6 +10,7 @@
 #include <ostream>
 #include <sstream>
 #include <stdexcept>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -169,6 +170,11 @@ namespace result {
 void prefixRCSData(std::ostream &s);
 void prefixSCData(std::ostream &s);
 void writeCurrent(const std::string &dir, const std::vector<CurrentData> &data);
+// The bistatic files of EMComponent/Result.h:48-104 (bistatic_rcs_phi_<phi>.dat
+// per phi cut, then bistatic_rcs_theta_90.dat); rcs(nd, dirs[nd][3], out_db)
+// gives the values of all directions at once, in file order.
+void writeBistatic(const std::string &dir, const Scattering &sca,
+                   const std::function<void(int, const double *, double *)> &rcs);
 }  // namespace result
 
 }  // namespace component

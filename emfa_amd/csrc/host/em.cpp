@@ -318,38 +318,9 @@ void DenseMoM::clear() {
 
 // EMComponent/Result.h:48-104: phi cuts over theta, then the theta = 90 cut.
 void DenseMoM::writeBistatic() {
-    const auto &sca = scattering_;
-    const float d2r = (float)(kPi / 180.0);
-    std::vector<double> dirs;
-    std::vector<std::pair<std::string, std::vector<float>>> files;  // name, angle column
-    int num = static_cast<int>((sca.phi_to - sca.phi_from) / sca.phi_delta + 1);
-    for (int i = 0; i < num; ++i) {
-        float phiDegree = sca.phi_from + i * sca.phi_delta;
-        float phi = phiDegree * d2r;
-        std::string str = std::to_string(phiDegree);
-        files.push_back({"/bistatic_rcs_phi_" + str.substr(0, str.find('.')) + ".dat", {}});
-        for (float thetaDegree = sca.theta_from; thetaDegree <= sca.theta_to; thetaDegree += sca.theta_delta) {
-            float theta = thetaDegree * d2r;
-            dirs.insert(dirs.end(), {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)});
-            files.back().second.push_back(thetaDegree);
-        }
-    }
-    files.push_back({"/bistatic_rcs_theta_90.dat", {}});
-    for (float phiDegree = 0; phiDegree < 360; phiDegree += 1) {
-        float phi = phiDegree * d2r;
-        dirs.insert(dirs.end(), {std::cos(phi), std::sin(phi), 0.0});
-        files.back().second.push_back(phiDegree);
-    }
-    int nd = (int)(dirs.size() / 3);
-    std::vector<double> db(nd);
-    check(emfa_rcs(ctx_, dmesh_, k_, X_, nd, dirs.data(), nullptr, db.data()), "RCS");
-    size_t o = 0;
-    for (auto &f : files) {
-        std::ofstream out(dir_ + f.first);
-        if (out.fail()) throw component::FileError("cannot write " + dir_ + f.first);
-        component::result::prefixRCSData(out);
-        for (float ang : f.second) out << std::setw(10) << ang << '\t' << std::setw(10) << (float)db[o++] << '\n';
-    }
+    component::result::writeBistatic(dir_, scattering_, [&](int nd, const double *dirs, double *db) {
+        check(emfa_rcs(ctx_, dmesh_, k_, X_, nd, dirs, nullptr, db), "RCS");
+    });
 }
 
 // EMComponent/Result.h:107-155: one incidence per direction (backscatter).

@@ -18,10 +18,18 @@ const char *emfa_core_last_error(void);
  * or the text itself when is_text != 0.  Returns 0, 1 (missing key) or 2. */
 int emfa_core_config_get(const char *src, int is_text, const char *key, char *out, int cap);
 
-/* Mesh::loadMeshFile for .nas files (EMComponent/Mesh.cpp:261-309): 0-based
- * triangles, vertex coordinates as the reader stores them.  With null
- * buffers only nv/nt are returned.  Returns 0, 1 (buffers too small), 3. */
+/* Mesh::loadMeshFile (EMComponent/Mesh.cpp:22-49) for .nas (:261-309),
+ * .rpt (:178-218) and ASCII .stl (:220-259) files: 0-based triangles, vertex
+ * coordinates as the reader stores them (.stl: equal corners joined into one
+ * vertex).  With null buffers only nv/nt are returned.  Returns 0, 1
+ * (buffers too small), 3. */
 int emfa_core_read_mesh(const char *path, double *xyz, int nv_cap, int *tri, int nt_cap, int *nv, int *nt);
+
+/* Test hook of the result-file writers (bistatic_rcs_*.dat of
+ * EMComponent/Result.h:48-104 and current.sc of :234-255) on fixed synthetic
+ * values, for a byte comparison with the reference's writers.  Returns 0 or 3. */
+int emfa_core_write_formats(const char *dir, float theta_from, float theta_to, float theta_delta, float phi_from,
+                            float phi_to, float phi_delta);
 
 /* solver::loadSolver(config_path) (EMSolver/Solver.cpp:45-95): reads the
  * config, runs the named solver, writes its result files.  Returns 0 or the

@@ -18,6 +18,7 @@
 // usage: emfa_ref efie|cfie|mfie mesh.nas freq alpha sigma_re sigma_im theta phi pole out.bin
 //        emfa_ref time cfie|efie mesh.nas freq alpha
 //        emfa_ref mesh file.nas|.rpt|.stl out.bin
+//        emfa_ref formats dir theta_from theta_to theta_delta phi_from phi_to phi_delta
 // The third form loads a mesh with Mesh::loadMeshFile (EMComponent/Mesh.cpp:
 // 22-49, any of the three formats) and writes what the reference holds:
 // int32 nv, nt; float xyz[nv*3]; per triangle int32 ids[3] and float
@@ -36,6 +37,7 @@
 #include "CommonEdge.h"
 #include "EFIE.h"
 #include "Mesh.h"
+#include "Result.h"
 #include "iml.h"
 #define private protected
 #include "MFIE.h"
@@ -347,6 +349,38 @@ void write_output(const char *path, const Out &o) {
 
 }  // namespace
 
+// Result-file format probe: the reference's own writers (EMComponent/
+// Result.h:48-104 getBistaticRCS, :234-255 getCurrentDistribution, with the
+// prefixes of Result.cpp:26-51) fed with fixed synthetic values.
+struct FmtProbe {
+    std::string dir_;
+    component::Scattering scattering_;
+    value_t rcs(const VectorR3 &k) const { return 10.0f * k.z + 5.0f * k.x - 30.0f; }
+    void cur(std::vector<component::CurrentData> *d) const {
+        d->resize(5);
+        for (int t = 0; t < 5; ++t) {
+            auto &c = (*d)[t];
+            c.v1 = VectorR3(0.1f * t, -0.25f, 1.0f / 3.0f);
+            c.v2 = VectorR3(-1.5f + t, 2.0f / 7.0f, 1e-3f * t);
+            c.v3 = VectorR3(123.456f, -0.000123f * t, 7.0f);
+            c.magn1 = 1e-3f * (t + 1);
+            c.magn2 = 0.5f / (t + 1);
+            c.magn3 = 3.25f * t;
+            c.magnc = 1.0f / 3.0f + t;
+        }
+    }
+};
+
+int write_formats(const char *dir, float th0, float th1, float thd, float ph0, float ph1, float phd) {
+    FmtProbe p;
+    p.dir_ = dir;
+    p.scattering_ = {th0, th1, thd, ph0, ph1, phd};
+    component::Result r;
+    r.getBistaticRCS(&p, &FmtProbe::rcs);
+    r.getCurrentDistribution(&p, &FmtProbe::cur);
+    return 0;
+}
+
 int dump_mesh(const char *file, const char *out) {
     component::Mesh m;
     m.loadMeshFile(file);
@@ -376,6 +410,10 @@ int dump_mesh(const char *file, const char *out) {
 
 int main(int argc, char **argv) {
     if (argc == 4 && std::string(argv[1]) == "mesh") return dump_mesh(argv[2], argv[3]);
+    if (argc == 9 && std::string(argv[1]) == "formats")
+        return write_formats(argv[2], std::strtof(argv[3], nullptr), std::strtof(argv[4], nullptr),
+                             std::strtof(argv[5], nullptr), std::strtof(argv[6], nullptr),
+                             std::strtof(argv[7], nullptr), std::strtof(argv[8], nullptr));
     if (argc == 6 && std::string(argv[1]) == "time") {
         std::string kind = argv[2];
         float freq = std::strtof(argv[4], nullptr), alpha = std::strtof(argv[5], nullptr);

@@ -151,3 +151,31 @@ def test_rpt_stl_readers_read_what_the_reference_reads(path, tmp_path):
         # a closed surface: every edge shared by exactly two facets (RWG count 1.5 T)
         rwg, nb = O.build_rwg(tri)
         assert nb == 0 and rwg.shape[0] == 3 * tri.shape[0] // 2
+
+
+def test_result_files_byte_identical_to_the_reference_writers(tmp_path):
+    """bistatic_rcs_phi_<phi>.dat, bistatic_rcs_theta_90.dat and current.sc as
+    the reference writes them (EMComponent/Result.h:48-104, :234-255, headers
+    of Result.cpp:26-51): the reference's own writers (oracle/_ref, compiled
+    from its sources) and this build's writers, fed the same synthetic values,
+    produce the same bytes."""
+    import ctypes
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "emfa_ref")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/emfa_ref is not built (make -C oracle -f ref.mk)")
+    from test_cpu_host import CORE
+    L = ctypes.CDLL(CORE)
+    L.emfa_core_write_formats.argtypes = [ctypes.c_char_p] + [ctypes.c_float] * 6
+    for sweep in [(0, 360, 5, 0, 90, 90), (10, 170, 2.5, 15, 75, 30)]:
+        ref, mine = tmp_path / f"ref{sweep[2]}", tmp_path / f"mine{sweep[2]}"
+        ref.mkdir()
+        mine.mkdir()
+        subprocess.run([exe, "formats", str(ref)] + [repr(float(v)) for v in sweep], check=True,
+                       stdout=subprocess.DEVNULL)
+        assert L.emfa_core_write_formats(str(mine).encode(), *[float(v) for v in sweep]) == 0
+        names = sorted(os.listdir(ref))
+        assert names == sorted(os.listdir(mine)) and "current.sc" in names and len(names) >= 3
+        for name in names:
+            assert (ref / name).read_bytes() == (mine / name).read_bytes(), name
