@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="copy a buffer of this many MiB once (a plain copy kernel of known size, to scale "
                          "hardware traffic counters)")
-    ap.add_argument("--mixed-requests", type=int, default=32)
+    ap.add_argument("--mixed-requests", type=int, default=64)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=96.0)
     ap.add_argument("--served-passes", type=int, default=5,
@@ -176,6 +176,45 @@ def cpu_baseline_reference(kind, radius, freq, alpha, n, nrhs, sample_subdiv=3, 
     }
 
 
+def mlfma_cpu_baseline(radius, freq, alpha, near_nnz, n, reps=3, sample_subdiv=3):
+    """A lower bound on the reference's CPU time for one MLFMA solve: its
+    near-field matrix alone.  MLFMA::calculateNearZ (EMSolver/MLFMA.cpp:
+    1004-1131) evaluates the CFIE kernels of Miscellaneous.cpp for every near
+    pair, the cost per pair that the reference's own CFIE::fillZ shows; that
+    cost is timed here (oracle/_ref/emfa_ref, one thread, median of `reps`)
+    on a sample icosphere and multiplied by the operator's near entries.
+    Far-field setup, the translations and the iterations are left out, so the
+    rate is an upper bound on the reference's solves per second.  The
+    reference's MLFMA.cpp itself does not compile here (an Armadillo
+    constructor at MLFMA.cpp:1033,1317)."""
+    import subprocess
+    import tempfile
+    from emfa_amd import hip
+    exe = os.path.join(ROOT, "oracle", "_ref", "emfa_ref")
+    if not os.path.exists(exe):
+        return None
+    xyz, tri = hip.icosphere(sample_subdiv, radius)
+    runs = []
+    with tempfile.TemporaryDirectory() as d:
+        nas = os.path.join(d, "sample.nas")
+        write_nas(nas, xyz, tri)
+        for _ in range(reps):
+            r = subprocess.run([exe, "time", "cfie", nas, repr(float(freq)), repr(float(alpha))], cwd=d,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, check=True)
+            ns, tz = r.stderr.split()[-4:-2]
+            runs.append(float(tz) / float(ns) ** 2)
+    per_pair = float(np.median(runs))
+    near_s = per_pair * near_nnz
+    return {
+        "value": 1.0 / near_s, "unit": "solves/s", "cores": 1, "kind": "reference",
+        "sample": (f"reference CFIE kernels (oracle/_ref/emfa_ref, one thread): {per_pair * 1e6:.2f} us per RWG pair "
+                   f"from CFIE::fillZ on icosphere s={sample_subdiv} at the same frequency (median of {reps}) x "
+                   f"{near_nnz} near entries = {near_s:.0f}s for MLFMA::calculateNearZ alone; far-field setup, "
+                   f"translations and iterations not included (an upper bound on the reference's rate)"),
+        "host": host_cpu(),
+    }
+
+
 def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
     """Parallel port: the oracle's FP64 fill (the reference's RWG-pair loop,
     OpenMP over test functions on every host core) on a row sample + LAPACK
@@ -224,7 +263,10 @@ def cpu_baseline(xyz, tri, rwg, kind, k, alpha, quad, n_rhs, budget_s):
     }
 
 
-def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5):
+RATE_SWEEP = (0.25, 0.5, 1.0, 2.0, 4.0, 8.0)
+
+
+def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5, rates=RATE_SWEEP):
     """Answer `reqs` on the plain FP64 path (sequential, nothing cached) and
     through the solve server (FP32+refine, `workers` contexts, resident
     factors): `passes` times with every request queued at t=0 (capacity: the
@@ -256,6 +298,14 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5
         lat, st2 = workload.run_served(srv, reqs, precision, rate=latency_rate * cap)
     err = max(float(np.abs(p - r["rcs_db"]).max()) for out, _ in runs for p, r in zip(plain, out))
     err2 = max(float(np.abs(p - r["rcs_db"]).max()) for p, r in zip(plain, lat))
+    # Poisson arrivals at multiples of the measured capacity, each on a fresh server
+    sweep = []
+    for f in rates:
+        with serve.Server(workers=workers, hbm_budget=budget) as srv:
+            out_r, st_r = workload.run_served(srv, reqs, precision, rate=f * cap)
+        err2 = max(err2, max(float(np.abs(p - r["rcs_db"]).max()) for p, r in zip(plain, out_r)))
+        sweep.append({"rate_x_capacity": f, "arrival_rate_per_s": f * cap, "p50_ms": st_r["p50_ms"],
+                      "p99_ms": st_r["p99_ms"], "mean_ms": st_r["mean_ms"], "solves_per_s": st_r["solves_per_s"]})
     fb = max(int(sum(r["fp64_fallback"] for r in out)) for out, _ in runs)
     ns = [workload.unknowns(r) for r in reqs]
     pl = np.sort(plain_lat)
@@ -277,6 +327,7 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5
         "speedup_vs_plain_fp64": cap / (len(reqs) / plain_wall),
         "latency_run": {"arrival_rate_per_s": latency_rate * cap, "p50_ms": st2["p50_ms"], "p99_ms": st2["p99_ms"],
                         "mean_ms": st2["mean_ms"], "max_ms": st2["max_ms"], "solves_per_s": st2["solves_per_s"]},
+        "rate_sweep": sweep,
         "saturated_p50_ms": st["p50_ms"], "saturated_p99_ms": st["p99_ms"],
         "rcs_max_abs_diff_db_vs_fp64": max(err, err2),
     }
@@ -515,6 +566,14 @@ def main_mlfma(a, subdiv=6, radius=1.0, freq=1.5e9, alpha=0.5, threshold=0.25, t
     info = out["info"]
     nf = fams["mlfma_near"]
     near_gbs = nf["work_per_step"] / (nf["ms_per_step"] * 1e-3) / 1e9 if nf["ms_per_step"] > 0 else 0.0
+    # dominant kernel: the downward pass (translation + anterpolation, k_ml_down)
+    dn = fams["mlfma_down"]
+    down_gbs = dn["work_per_step"] / (dn["ms_per_step"] * 1e-3) / 1e9 if dn["ms_per_step"] > 0 else 0.0
+    # what its gathers move through the caches: each far pair reads one
+    # translation entry (8 B) and one radiation pattern (24 B) per sample
+    K = [m * 2 * m for m in info["multipoles"]]
+    gather_b = sum(info["far_pairs"][l] * K[l] * 32.0 for l in range(2, info["levels"] + 1))
+    down_launch_ms = dn["ms_per_step"] / max(dn["launches_per_step"], 1)
     matvecs = out["iterations"] + out["restarts"] + 1  # Arnoldi steps + restart residuals + final residual
     mv_ms = sum(fams[f]["ms_per_step"] for f in ["mlfma_near", "mlfma_agg", "mlfma_up", "mlfma_down", "mlfma_out"])
     res = {
@@ -538,11 +597,19 @@ def main_mlfma(a, subdiv=6, radius=1.0, freq=1.5e9, alpha=0.5, threshold=0.25, t
                   "setup_s": out["setup_s"], "solve_s": out["solve_s"], "gmres_iterations": out["iterations"],
                   "relres": out["relres"], "converged": out["converged"],
                   "matvec_ms": mv_ms / max(matvecs, 1)},
-        "roofline": {"bound": "hbm", "kernel": "mlfma_near (k_ml_near, CSR near-field SpMV)",
-                     "achieved": near_gbs, "peak": PEAK["hbm"], "unit": "GB/s", "frac": near_gbs / PEAK["hbm"],
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "mlfma_down (k_ml_down: far-box translation + anterpolation)",
+                     "achieved": down_gbs, "peak": PEAK["hbm"], "unit": "GB/s", "frac": down_gbs / PEAK["hbm"],
+                     "traffic": None,
+                     "avg_launch_ms": down_launch_ms, "launches_per_step": dn["launches_per_step"],
+                     "gather_bytes_per_matvec": gather_b,
+                     "gather_gb_per_s": gather_b * dn["launches_per_step"] / max(matvecs, 1)
+                     / max(info["levels"] - 1, 1) / (dn["ms_per_step"] * 1e-3) / 1e9 if dn["ms_per_step"] > 0 else 0,
+                     "note": "achieved counts each level's patterns, translation rows in use and parent patterns "
+                             "once (algorithmic bytes); the kernel reads far-pair patterns and translation entries "
+                             "through L2 (gather_gb_per_s), which bounds it"},
+        "near_spmv": {"achieved_gb_per_s": near_gbs, "frac": near_gbs / PEAK["hbm"]},
         "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
-        "cpu_baseline": None,
+        "cpu_baseline": None if a.no_cpu else mlfma_cpu_baseline(radius, freq, alpha, info["near_nnz"], n),
     }
     print(json.dumps(res))
 
@@ -556,8 +623,8 @@ def main_stream(a):
         name = "256 PEC ellipsoids, semi-axes U[0.3,1.2] m, icosphere s=2..4, 100-600 MHz, EFIE, 1-64 RHS"
     else:
         reqs = workload.mixed_stream(a.steps)
-        name = ("mixed stream: spheres/ellipsoids (s=3,4) and 1x1 m plates, 500-15000 RWG, EFIE/CFIE, 1-64 RHS, "
-                "30% repeat meshes")
+        name = ("mixed stream: spheres/ellipsoids (s=3,4) and 1x1 m plates (sizes stratified over 500-15000 RWG), "
+                "EFIE/CFIE, 1-64 RHS, 30% repeat meshes")
     leg = stream_leg(reqs, a.workers, int(a.budget_gb * 2**30), passes=a.served_passes)
     lr = leg["latency_run"]
     out = {

@@ -581,6 +581,173 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
 }
 
 // ---------------------------------------------------------------------------
+// The same product with one wave per SIMD and register-double-buffered
+// operand fragments.  Workgroup tile 256 x 256, 4 waves as 2 x 2 with wave
+// tiles 128 x 128 (4 x 4 MFMA 32x32x16 tiles: 256 accumulators per lane,
+// which the compiler keeps in the accumulation registers), LDS ring of four
+// K = 16 stages filled by direct global -> LDS loads (wave w loads plane w:
+// Ah, Al, Bh, Bl).  At step kt the wave multiplies the fragments of stage kt,
+// already in registers, while it reads the fragments of stage kt + 1 from
+// LDS into the other register set (one LDS read per three MFMAs, the order
+// pinned with scheduling barriers); the one barrier per step publishes stage
+// kt + 1 and frees the slot of stage kt - 1 for the loads of stage kt + 3.  Per
+// accumulator the products are added in bf3p's order (Al Bh, Ah Bl, Ah Bh per
+// stage, stages in order), so the two kernels give the same bits.
+// ---------------------------------------------------------------------------
+namespace bf3q {
+constexpr int BM = 256, BN = 256, BK = 16, NS = 4, PLANE = 256 * BK;
+}
+
+__global__ __launch_bounds__(256, 1) void k_gemm_bf3q(Bf3Planes g) {
+    using namespace bf3q;
+    __shared__ __attribute__((aligned(16))) __bf16 sm[NS][4][PLANE];  // stage, plane (Ah Al Bh Bl)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int TX = (g.Nr + BN - 1) / BN, TY = (g.M + BM - 1) / BM, TT = TX * TY;
+    const int t0 = g.persistent ? (int)blockIdx.x : (int)((blockIdx.z * TY + blockIdx.y) * gridDim.x + blockIdx.x);
+    const int tstep = g.persistent ? (int)gridDim.x : TT * g.bt.count;
+    const int r32 = lane & 31, kg = lane >> 5;
+    for (int tl = t0; tl < TT * g.bt.count; tl += tstep) {
+        if (tl != t0) __syncthreads();  // the previous tile's LDS reads are done
+        const int zb = tl / TT;
+        int ty, tx;
+        tile_of_linear(tl - zb * TT, TX, TY, &ty, &tx);
+        const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
+        const int nk = g.Kp / BK;
+
+        // this wave's direct loads: plane `wave`, rows 32 i + lane / 2 (i < 8),
+        // chunk (lane & 1) stored at its swizzled position
+        const __bf16 *src = batch_scr(g.bt, wave == 0 ? g.Ah : wave == 1 ? g.Al : wave == 2 ? g.Bh : g.Bl, zb);
+        const long long ld = wave < 2 ? g.lda : g.ldb;
+        const long long row0 = wave < 2 ? bm : bn;
+        const int lr = lane >> 1;
+        const __bf16 *gbase = src + (row0 + lr) * ld + 8 * ((lane & 1) ^ ((lr >> 2) & 1));
+        const long long rstep = 32 * ld;
+        auto issue = [&](int kt) {
+            const int st = kt & (NS - 1);
+            const __bf16 *gp = gbase + (long long)kt * BK;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                __builtin_amdgcn_global_load_lds((const void *)(gp + i * rstep), (lds_void *)&sm[st][wave][32 * i * BK],
+                                                 16, 0, 0);
+        };
+        int offa[4], offb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            offa[t] = bf3p_off(wm * 128 + t * 32 + r32, kg);
+            offb[t] = bf3p_off(wn * 128 + t * 32 + r32, kg);
+        }
+        bf16x8 ah[2][4], al[2][4], bh[2][4], bl[2][4];
+        auto read = [&](int kt, int buf) {
+            const int st = kt & (NS - 1);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                ah[buf][t] = *(const bf16x8 *)&sm[st][0][offa[t]];
+                al[buf][t] = *(const bf16x8 *)&sm[st][1][offa[t]];
+                bh[buf][t] = *(const bf16x8 *)&sm[st][2][offb[t]];
+                bl[buf][t] = *(const bf16x8 *)&sm[st][3][offb[t]];
+            }
+        };
+        floatx16 acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+
+        issue(0);
+        if (nk > 1) issue(1);
+        if (nk > 2) issue(2);
+        if (nk > 2)
+            asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+        else if (nk > 1)
+            asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        read(0, 0);
+        // one step: fragments of stage kt in set cur, stage kt + 1 read into set nxt
+        auto step = [&](int kt, int cur, int nxt) {
+            if (kt + 2 < nk)
+                asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+            else
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (kt + 3 < nk) issue(kt + 3);
+            read(kt + 1 < nk ? kt + 1 : kt, nxt);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[cur][mt], bh[cur][nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[cur][mt], bl[cur][nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[cur][mt], bh[cur][nt], acc[mt][nt], 0, 0, 0);
+            // one LDS read of the next set, then three MFMAs, sixteen times
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+            }
+        };
+        // set 1 is read while set 0 is multiplied, then becomes set 0
+        for (int kt = 0; kt < nk; ++kt) {
+            step(kt, 0, 1);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                ah[0][t] = ah[1][t];
+                al[0][t] = al[1][t];
+                bh[0][t] = bh[1][t];
+                bl[0][t] = bl[1][t];
+            }
+        }
+
+        // epilogue: all C loads of a 32-row slab are issued before its stores
+        const float sgn = g.alpha_neg ? -1.f : 1.f;
+        float *__restrict__ C =
+            batch_mat(g.bt, g.C, zb) + (bm + wm * 128 + 4 * (lane >> 5)) * g.ldc + bn + wn * 128 + r32;
+        const int mrem = (int)min((long long)BM, g.M - bm) - wm * 128 - 4 * (lane >> 5);
+        const int nrem = (int)min((long long)BN, (long long)g.Nr - bn) - wn * 128 - r32;
+        const bool full = bm + BM <= g.M && bn + BN <= g.Nr;
+        auto epilogue = [&](auto guarded) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                float cv[4][16];
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int i = mt * 32 + 8 * (v >> 2) + (v & 3), j = nt * 32;
+                        const bool in = !guarded.value || (i < mrem && j < nrem);
+                        cv[nt][v] = (g.beta && in) ? C[i * g.ldc + j] : 0.f;
+                    }
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int i = mt * 32 + 8 * (v >> 2) + (v & 3), j = nt * 32;
+                        if (!guarded.value || (i < mrem && j < nrem)) C[i * g.ldc + j] = cv[nt][v] + sgn * acc[mt][nt][v];
+                    }
+            }
+        };
+        if (full)
+            epilogue(std::false_type{});
+        else
+            epilogue(std::true_type{});
+    }  // tiles
+}
+
+// ---------------------------------------------------------------------------
 namespace f64k {
 constexpr int BM = 128, BN = 128, BK = 16, PAD = 2;
 }
@@ -797,12 +964,25 @@ emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g) {
     const int TX = (g.Nr + bf3p::BN - 1) / bf3p::BN, TY = (g.M + bf3p::BM - 1) / bf3p::BM;
     EMFA_REQUIRE(g.bt.count >= 1 && g.bt.count <= 65535, EMFA_E_INVALID, "gemm bf16x3 planes: batch %d", g.bt.count);
     Bf3Planes gg = g;
+    // EMFA_BF3_KERNEL=q selects the one-wave-per-SIMD kernel (same bits)
+    static const bool default_p = [] {
+        const char *e = getenv("EMFA_BF3_KERNEL");
+        return !(e && e[0] == 'q');
+    }();
+    const bool use_p = g.kernel == 1 || (g.kernel == 0 && default_p);
+    const int threads = use_p ? 512 : 256;
+    auto launch = [&](dim3 grid) {
+        if (use_p)
+            k_gemm_bf3p<<<grid, threads, 0, s>>>(gg);
+        else
+            k_gemm_bf3q<<<grid, threads, 0, s>>>(gg);
+    };
     if (g.max_wgs >= 8 && (long long)TX * TY * g.bt.count > g.max_wgs) {
         gg.persistent = 1;
-        k_gemm_bf3p<<<(g.max_wgs / 8) * 8, 512, 0, s>>>(gg);
+        launch(dim3((g.max_wgs / 8) * 8));
     } else {
         gg.persistent = 0;
-        k_gemm_bf3p<<<dim3(TX, TY, g.bt.count), 512, 0, s>>>(gg);
+        launch(dim3(TX, TY, g.bt.count));
     }
     EMFA_HIP_CHECK(hipGetLastError());
     return EMFA_OK;

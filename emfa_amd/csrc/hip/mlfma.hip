@@ -44,6 +44,7 @@ using cf = cx<float>;
 
 struct MlLevel {
     int nbox = 0, L = 0, K = 0;
+    int offsets_used = 0;  // box-grid offsets that occur among the far pairs
     double a = 0;
     std::vector<double> cen;  // nbox x 3
     std::vector<int> ix;      // nbox x 3 integer box coordinates
@@ -792,6 +793,7 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
         std::vector<int> codes;
         for (int code = 0; code < 343; ++code)
             if (used[code]) codes.push_back(code);
+        C.offsets_used = (int)codes.size();
         std::vector<cf> tr((size_t)343 * C.K, cf{0, 0});
         {
             const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -957,7 +959,13 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
     for (int l = 2; l <= lvl; ++l) {
         MlLevel &C = M->lv[l];
         MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;
-        ProfScope ps(ctx, "mlfma_down", 0);
+        // algorithmic bytes: the level's radiation patterns read and receive
+        // patterns written once (3 complex FP32 per box and sample), its
+        // translation table (one row of K per box-grid offset in use) and
+        // the parent's receive patterns with the anterpolation weights
+        const double ws = (double)C.nbox * C.K * 24.0 * 2 + (double)C.offsets_used * C.K * 8.0 +
+                          (P ? (double)P->nbox * P->K * 24.0 + (double)C.K * 9 * 12.0 : 0.0);
+        ProfScope ps(ctx, "mlfma_down", ws);
         if (P && sib) {
             k_ml_down_sib<<<(unsigned)(((long long)P->nbox * C.K + 255) / 256), 256, 0, s>>>(
                 P->near_ptr_d, P->near_idx_d, P->ch_ptr_d, P->ch_idx_d, P->nbox, C.ix_d, C.oct_d, C.trans, C.up,
@@ -1009,6 +1017,7 @@ emfa_status emfa_mlfma_get_info(const emfa_mlfma *m, emfa_mlfma_info *info) {
         info->boxes[l] = m->lv[l].nbox;
         info->multipoles[l] = m->lv[l].L;
         info->box_edge[l] = m->lv[l].a;
+        info->far_pairs[l] = (long long)m->lv[l].far_idx.size();
     }
     return EMFA_OK;
 }

@@ -66,7 +66,8 @@ class MlfmaParams(ctypes.Structure):
 
 class MlfmaInfo(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("near_nnz", ctypes.c_longlong), ("bytes", ctypes.c_size_t),
-                ("boxes", ctypes.c_int * 16), ("multipoles", ctypes.c_int * 16), ("box_edge", ctypes.c_double * 16)]
+                ("boxes", ctypes.c_int * 16), ("multipoles", ctypes.c_int * 16), ("box_edge", ctypes.c_double * 16),
+                ("far_pairs", ctypes.c_longlong * 16)]
 
 
 class EmfaError(RuntimeError):
@@ -414,7 +415,8 @@ class MLFMA:
         _check(lib().emfa_mlfma_get_info(self.h, ctypes.byref(i)))
         lv = i.levels
         return {"levels": lv, "near_nnz": i.near_nnz, "bytes": i.bytes, "boxes": list(i.boxes[:lv + 1]),
-                "multipoles": list(i.multipoles[:lv + 1]), "box_edge": list(i.box_edge[:lv + 1])}
+                "multipoles": list(i.multipoles[:lv + 1]), "box_edge": list(i.box_edge[:lv + 1]),
+                "far_pairs": list(i.far_pairs[:lv + 1])}
 
     def matvec(self, X, Y):
         _check(lib().emfa_mlfma_matvec(self.ctx.h, self.h, X.h, Y.h))

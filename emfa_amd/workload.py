@@ -4,9 +4,10 @@
   U[0.3, 1.2] m, icosphere subdivisions {2, 3, 4} (320..5120 triangles),
   100..600 MHz, 1..64 incidences each (theta spread over 0..180 deg).
 * `mixed_stream` — spheres, ellipsoids and 1 x 1 m plates (2 triangles per
-  cell) with 500..15000 RWG unknowns, EFIE/CFIE (plates EFIE only: CFIE
-  needs a closed surface), 1..64 incidences, 30 % of the requests repeat an
-  earlier (mesh, frequency, formulation) with new incidences.
+  cell) with 500..15000 RWG unknowns (plate sizes stratified over the range),
+  EFIE/CFIE (plates EFIE only: CFIE needs a closed surface), 1..64
+  incidences, 30 % of the requests repeat an earlier (mesh, frequency,
+  formulation) with new incidences.
 
 `run_plain` answers a list of requests one after another through the C ABI
 with a fresh FP64 fill and LU each (the drop-in path, nothing cached);
@@ -42,9 +43,18 @@ def batch_problems(count=256, seed=SEED):
     return out
 
 
+def _plate_cells(n_unknowns):
+    """Cells per side of the square plate with about n_unknowns RWGs (3c^2 - 2c)."""
+    return int(round((2.0 + np.sqrt(4.0 + 12.0 * n_unknowns)) / 6.0))
+
+
 def mixed_stream(count, seed=SEED, repeat=0.3):
+    """New plates take their size from four equal strata of 500..15000
+    unknowns in turn (a size drawn uniformly inside the stratum), so that any
+    prefix of the stream covers the whole range."""
     rng = np.random.default_rng(seed)
     out = []
+    new_plates = 0
     for _ in range(count):
         nrhs = int(rng.integers(1, 65))
         if out and rng.random() < repeat:
@@ -55,8 +65,10 @@ def mixed_stream(count, seed=SEED, repeat=0.3):
             continue
         fam = int(rng.integers(3))
         f = float(rng.uniform(100e6, 600e6))
-        if fam == 2:  # plate: 3c^2 - 2c unknowns in [560, 14981]
-            c = int(rng.integers(14, 72))
+        if fam == 2:  # plate: 3c^2 - 2c unknowns, 500..15000
+            lo = 500.0 + 3625.0 * (new_plates % 4)
+            c = min(71, max(14, _plate_cells(rng.uniform(lo, lo + 3625.0))))
+            new_plates += 1
             out.append(dict(kind=serve.PLATE, subdiv=c, axes=(1.0, 1.0, 1.0), formulation=hip.EFIE, alpha=1.0,
                             freq=f, angles=_angles(rng, nrhs, 80.0)))
         else:  # sphere / ellipsoid, subdivisions 3..4 (1920 / 7680 unknowns)

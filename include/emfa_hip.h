@@ -263,6 +263,7 @@ typedef struct {
     int boxes[16];        /* non-empty boxes per level */
     int multipoles[16];   /* L per level (levels >= 2) */
     double box_edge[16];  /* box edge per level (m) */
+    long long far_pairs[16]; /* (box, far box) translations per level */
 } emfa_mlfma_info;
 emfa_status emfa_mlfma_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out);
 emfa_status emfa_mlfma_destroy(emfa_mlfma *m);
