@@ -231,6 +231,19 @@ typedef struct {
 emfa_status emfa_bicgstab(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, double tol, int max_iter,
                           int variant, emfa_bicgstab_report *rep);
 
+/* emfa_bicgstab without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The solver's vectors and partial sums live in a fixed 8 MiB
+ * device array inside the library (one per device, held until the process
+ * ends), so Z may have at most EMFA_BICGSTAB_NOALLOC_MAX_N rows; larger
+ * systems get EMFA_E_INVALID.  Calls on one device share that array: calls
+ * that overlap in time on one device race and must be serialised by the
+ * caller.  Same results as emfa_bicgstab. */
+#define EMFA_BICGSTAB_NOALLOC_MAX_N 58000
+emfa_status emfa_bicgstab_noalloc(emfa_ctx *ctx, const emfa_mat *Z, const emfa_mat *B, emfa_mat *X, double tol,
+                                  int max_iter, int variant, emfa_bicgstab_report *rep);
+
 /* ---- far field (K9) --------------------------------------------------- */
 /* RCS in dB for ndir directions dirs[ndir][3]; direction d uses column
  * cols[d] of X (cols may be NULL: column 0).  Replaces getBiRCS
