@@ -300,6 +300,12 @@ size_t emfa_ctx_bytes_in_use(const emfa_ctx *ctx) { return ctx ? ctx->bytes : 0;
 
 int emfa_ctx_cu_count(const emfa_ctx *ctx) { return ctx ? ctx->num_cu : 0; }
 
+emfa_status emfa_ctx_set_panel_cap(emfa_ctx *ctx, int wgs) {
+    EMFA_REQUIRE(ctx && wgs >= 0, EMFA_E_INVALID, "emfa_ctx_set_panel_cap: bad argument");
+    ctx->panel_cap = wgs;
+    return EMFA_OK;
+}
+
 emfa_status emfa_ctx_set_cu_share(emfa_ctx *ctx, int cus) {
     EMFA_REQUIRE(ctx && cus >= 0, EMFA_E_INVALID, "emfa_ctx_set_cu_share: bad argument");
     ctx->cu_share = cus;

@@ -439,8 +439,11 @@ __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ sr
 // base + 16 l), four stages of K = 16 in flight: stage kt + 3 is issued while
 // stage kt is multiplied.  No registers stage the operands and no VALU work
 // sits in the loop.  An LDS plane row holds 16 bf16 (two 16-byte chunks);
-// chunk c of row r is stored at position c ^ ((r >> 2) & 1), so the eight
-// rows served per ds_read_b128 cycle hit eight distinct 4-bank groups.
+// chunk c of row r is stored at position c ^ swz(r), swz(r) = bit 2 ^ bit 3
+// of r: the sixteen rows that one ds_read_b128 cycle serves (sixteen lanes,
+// 64 banks of 4 bytes) fall on sixteen distinct 4-bank groups.  (With bit 2
+// alone, rows r and r + 8 shared banks: the counters showed four conflict
+// cycles per LDS read.)
 // Each wave waits (s_waitcnt vmcnt) for its own loads of the stage, then the
 // workgroup barrier publishes every wave's loads; the same barrier tells the
 // issuing waves that all reads of the buffer being refilled are done.
@@ -449,7 +452,8 @@ namespace bf3p {
 constexpr int BM = 256, BN = 256, BK = 16, NS = 4, PLANE = 256 * BK;
 }
 
-__device__ __forceinline__ int bf3p_off(int r, int kg) { return r * bf3p::BK + ((kg ^ ((r >> 2) & 1)) << 3); }
+__device__ __forceinline__ int bf3_swz(int r) { return ((r >> 2) ^ (r >> 3)) & 1; }
+__device__ __forceinline__ int bf3p_off(int r, int kg) { return r * bf3p::BK + ((kg ^ bf3_swz(r)) << 3); }
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -480,7 +484,7 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int r = half * 128 + 32 * i + (lane >> 1);
-        const int c = (lane & 1) ^ ((r >> 2) & 1);
+        const int c = (lane & 1) ^ bf3_swz(r);
         gp[i] = src + (row0 + r) * ld + 8 * c;
     }
     auto issue = [&](int kt) {
@@ -621,7 +625,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf3q(Bf3Planes g) {
         const long long ld = wave < 2 ? g.lda : g.ldb;
         const long long row0 = wave < 2 ? bm : bn;
         const int lr = lane >> 1;
-        const __bf16 *gbase = src + (row0 + lr) * ld + 8 * ((lane & 1) ^ ((lr >> 2) & 1));
+        const __bf16 *gbase = src + (row0 + lr) * ld + 8 * ((lane & 1) ^ bf3_swz(lr));
         const long long rstep = 32 * ld;
         auto issue = [&](int kt) {
             const int st = kt & (NS - 1);

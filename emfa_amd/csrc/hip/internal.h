@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <functional>
@@ -88,7 +89,14 @@ struct emfa_ctx {
     hipStream_t aux = nullptr;  // second stream (LU look-ahead), default priority
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int num_cu = 0;
-    int cu_share = 0;  // cap on the cooperative panel's workgroups (0: num_cu)
+    // compute units this context may count on while other contexts share
+    // the device (0: all); a solve server changes it from its own thread as
+    // workers start and finish, so it is read at each launch
+    std::atomic<int> cu_share{0};
+    // bound on the workgroups of a panel whose workgroups wait on each other
+    // (0: cu_share); fixed while contexts share the device, so that such
+    // panels of all contexts fit on the device together
+    int panel_cap = 0;
     // LU parameters (0 / -1: built-in choice); set by emfa_lu_set_params or the autotuner
     int lu_nb = 0, panel_g = 0, panel_x = -1;  // panel_x 2: FP32 panels without interchanges
     int force_pivoting = 0;                    // set while a grown pivot-free factor is redone

@@ -1008,7 +1008,8 @@ struct UpdatePlanes {
             int cap = 1;
             if (const char *e = getenv("EMFA_GEMM_SHARE_CAP")) cap = atoi(e);
             // a batch of factorisations does the work of that many workers
-            if (cap > 0) g.max_wgs = std::min(ctx->num_cu, cap * ctx->cu_share * std::max(1, ctx->batch.count));
+            const int share = ctx->cu_share;
+            if (cap > 0) g.max_wgs = std::min(ctx->num_cu, cap * share * std::max(1, ctx->batch.count));
         }
         return gemm_launch_bf3_planes(s, g);
     }
@@ -1979,7 +1980,8 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     // panel geometry: co-resident workgroups, rows per workgroup in LDS
     size_t lds_cap = 150 * 1024;
     if (const char *s = getenv("EMFA_PANEL_LDS_KB")) lds_cap = (size_t)std::max(16, std::min(150, atoi(s))) * 1024;
-    int G_max = std::max(1, ctx->cu_share > 0 ? std::min(ctx->cu_share, ctx->num_cu) : ctx->num_cu);
+    const int share = ctx->panel_cap > 0 ? ctx->panel_cap : (int)ctx->cu_share;
+    int G_max = std::max(1, share > 0 ? std::min(share, ctx->num_cu) : ctx->num_cu);
     if (ctx->panel_g > 0) G_max = std::max(1, std::min(G_max, ctx->panel_g));
     if (const char *s = getenv("EMFA_PANEL_G")) G_max = std::max(1, std::min(G_max, atoi(s)));
     int panel_rows_min = 16;

@@ -112,8 +112,13 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
             for (auto x : ctxs_) emfa_ctx_destroy(x);
             throw std::runtime_error(std::string("serve: ") + emfa_last_error());
         }
-        // concurrent factorisations split the compute units between them
-        if (workers > 1) emfa_ctx_set_cu_share(c, std::max(1, emfa_ctx_cu_count(c) / workers));
+        // concurrent factorisations split the compute units between them; the
+        // panels whose workgroups wait on each other keep the static split
+        if (workers > 1) {
+            emfa_ctx_set_cu_share(c, std::max(1, emfa_ctx_cu_count(c) / workers));
+            emfa_ctx_set_panel_cap(c, std::max(1, emfa_ctx_cu_count(c) / workers));
+        }
+        dynamic_share_ = env_flag("EMFA_SERVE_DYNAMIC_SHARE", true);
         // FP32 factors are only ever used under FP64 refinement: factor them
         // without interchanges (redone with pivoting when the multipliers
         // grow) and with the bf16-split trailing update; FP64 factors pivot
@@ -177,6 +182,17 @@ bool Server::wait(int64_t id, Result *out) {
         }
         cv_done_.wait(l);
     }
+}
+
+// The compute units are split between the workers that are running a
+// request: with all of them busy each context gets its static share, and a
+// request left alone (the end of a burst, a quiet period) gets the whole
+// device for its launches.  A context reads its share at each launch.
+void Server::rebalance_locked() {
+    if (!dynamic_share_ || ctxs_.size() < 2) return;
+    const int ncu = emfa_ctx_cu_count(ctxs_[0]);
+    const int share = std::max(1, ncu / std::max(1, busy_workers_));
+    for (auto c : ctxs_) emfa_ctx_set_cu_share(c, share);
 }
 
 void Server::drain() {
@@ -312,6 +328,8 @@ void Server::worker(int w) {
                 }
             }
             for (auto &g : groups) busy_ += (int)g.size();
+            ++busy_workers_;
+            rebalance_locked();
         }
         const auto start = clock_t_::now();
         if (factor_batch) {
@@ -371,6 +389,9 @@ void Server::worker(int w) {
             }
             cv_done_.notify_all();
         }
+        std::lock_guard<std::mutex> l(mu_);
+        --busy_workers_;
+        rebalance_locked();
     }
 }
 

@@ -133,6 +133,9 @@ class Server {
     std::set<int64_t> pending_;  // submitted and not yet returned by wait()
     int64_t next_id_ = 0;
     int busy_ = 0;
+    int busy_workers_ = 0;   // workers running a request (mu_)
+    bool dynamic_share_ = true;
+    void rebalance_locked();  // CU shares of all contexts from busy_workers_
     bool stop_ = false;
     size_t reserved_ = 0, peak_ = 0;
     // factor cache (LRU)

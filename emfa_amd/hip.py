@@ -19,7 +19,7 @@ EFIE, CFIE = 0, 1
 # every symbol include/emfa_hip.h declares
 EXPORTS = [
     "emfa_last_error", "emfa_version", "emfa_ctx_create", "emfa_ctx_destroy", "emfa_ctx_sync", "emfa_ctx_stream",
-    "emfa_ctx_bytes_in_use", "emfa_ctx_cu_count", "emfa_ctx_set_cu_share", "emfa_geom_icosphere", "emfa_geom_plate", "emfa_rwg_build", "emfa_mesh_upload",
+    "emfa_ctx_bytes_in_use", "emfa_ctx_cu_count", "emfa_ctx_set_cu_share", "emfa_ctx_set_panel_cap", "emfa_geom_icosphere", "emfa_geom_plate", "emfa_rwg_build", "emfa_mesh_upload",
     "emfa_mesh_destroy", "emfa_mesh_unknowns", "emfa_mat_create", "emfa_mat_destroy", "emfa_mat_upload",
     "emfa_mat_download", "emfa_mat_copy", "emfa_mat_rows", "emfa_mat_cols", "emfa_mat_prec", "emfa_mat_device_ptr",
     "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_factor_batch", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",

@@ -59,12 +59,18 @@ void *emfa_ctx_stream(emfa_ctx *ctx);
 size_t emfa_ctx_bytes_in_use(const emfa_ctx *ctx);
 /* number of compute units of the context's device */
 int emfa_ctx_cu_count(const emfa_ctx *ctx);
-/* Caps the workgroups of this context's cooperative LU panel at `cus`
- * (0: all compute units).  Contexts that factor concurrently on one device
- * keep the sum of their caps within emfa_ctx_cu_count so every panel's
- * workgroups can be resident together.  No counterpart in the reference
- * (it has no device). */
+/* The compute units this context may count on while other contexts share
+ * the device (0: all): its trailing updates run at most that many
+ * workgroups, and (unless emfa_ctx_set_panel_cap set a bound) so does its
+ * LU panel, whose workgroups wait on each other.  May be changed from
+ * another thread while the context works; each launch reads the current
+ * value.  No counterpart in the reference (it has no device). */
 emfa_status emfa_ctx_set_cu_share(emfa_ctx *ctx, int cus);
+/* Caps the workgroups of this context's LU panel (whose workgroups wait on
+ * each other) at `wgs` (0: the CU share).  Contexts that factor concurrently
+ * keep the sum of their caps within emfa_ctx_cu_count so every panel's
+ * workgroups can be resident together. */
+emfa_status emfa_ctx_set_panel_cap(emfa_ctx *ctx, int wgs);
 
 /* ---- host-side geometry (no device work) ---------------------------- */
 /* Synthetic meshes of the workloads: icosahedron subdivided `subdiv` times
