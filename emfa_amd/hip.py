@@ -26,7 +26,7 @@ EXPORTS = [
     "emfa_lu_matrix", "emfa_lu_info", "emfa_refine", "emfa_gemm_residual", "emfa_gmres", "emfa_rcs",
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
-    "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab",
+    "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab", "emfa_bicgstab_noalloc",
 ]
 
 _vp = ctypes.c_void_p

@@ -20,16 +20,21 @@ EXPORTS = [
     "emfa_core_last_error", "emfa_core_config_get", "emfa_core_read_mesh", "emfa_core_run", "emfa_serve_create",
     "emfa_serve_destroy", "emfa_serve_submit", "emfa_serve_wait", "emfa_serve_fetch", "emfa_serve_drain",
     "emfa_serve_get_stats", "emfa_serve_reset_stats", "emfa_serve_cached_factors", "emfa_serve_footprint",
+    "emfa_core_write_formats",
 ]
 
-ICOSPHERE, PLATE = 0, 1
+ICOSPHERE, PLATE, USER = 0, 1, 2
+DENSE, MLFMA = 0, 1
 
 
 class Request(ctypes.Structure):
     _fields_ = [("mesh_kind", ctypes.c_int), ("subdiv", ctypes.c_int), ("ax", ctypes.c_double),
                 ("ay", ctypes.c_double), ("az", ctypes.c_double), ("formulation", ctypes.c_int),
                 ("alpha", ctypes.c_double), ("freq", ctypes.c_double), ("precision", ctypes.c_int),
-                ("quad", ctypes.c_int), ("nrhs", ctypes.c_int), ("angles", ctypes.POINTER(ctypes.c_double))]
+                ("quad", ctypes.c_int), ("nrhs", ctypes.c_int), ("angles", ctypes.POINTER(ctypes.c_double)),
+                ("nv", ctypes.c_int), ("nt", ctypes.c_int), ("xyz", ctypes.POINTER(ctypes.c_double)),
+                ("tri", ctypes.POINTER(ctypes.c_int)), ("solver", ctypes.c_int), ("mlfma_box", ctypes.c_double),
+                ("iter_tol", ctypes.c_double), ("iter_max", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -115,16 +120,29 @@ class Server:
         self._lock = threading.Lock()  # submit and wait may run on different threads
 
     def submit(self, angles, kind=ICOSPHERE, subdiv=2, axes=(1.0, 1.0, 1.0), formulation=hip.EFIE,
-               alpha=1.0, freq=3e8, precision=hip.F64, quad=3):
-        """angles: (nrhs, 3) array of (theta, phi, pole) in degrees."""
+               alpha=1.0, freq=3e8, precision=hip.F64, quad=3, mesh=None, solver=DENSE, mlfma_box=0.25,
+               iter_tol=1e-4, iter_max=600):
+        """angles: (nrhs, 3) array of (theta, phi, pole) in degrees.  mesh:
+        (xyz (nv, 3), tri (nt, 3) 0-based) of the caller's own mesh (kind
+        becomes USER).  solver=MLFMA keeps the multipole operator resident and
+        solves by GMRES(30) to iter_tol."""
         a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64).reshape(-1, 3))
         r = Request(kind, subdiv, axes[0], axes[1], axes[2], formulation, alpha, freq, precision, quad, a.shape[0],
                     a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        r.solver, r.mlfma_box, r.iter_tol, r.iter_max = solver, mlfma_box, iter_tol, iter_max
+        n = unknowns(kind, subdiv)
+        if mesh is not None:
+            xyz = np.ascontiguousarray(mesh[0], dtype=np.float64).reshape(-1, 3)
+            tri = np.ascontiguousarray(mesh[1], dtype=np.int32).reshape(-1, 3)
+            r.mesh_kind, r.nv, r.nt = USER, xyz.shape[0], tri.shape[0]
+            r.xyz = xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            r.tri = tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            n = hip.rwg_build(tri)[0].shape[0]
         with self._lock:
             rid = core().emfa_serve_submit(self.h, ctypes.byref(r))
             if rid < 0:
                 raise _err(hip.EMFA_E_INVALID)
-            self._nrhs[rid] = (a.shape[0], unknowns(kind, subdiv))
+            self._nrhs[rid] = (a.shape[0], n)
         return rid
 
     def wait(self, rid=-1):
