@@ -18,18 +18,41 @@ static double ms_between(clock_t_::time_point a, clock_t_::time_point b) {
 std::string MeshSpec::key() const {
     std::ostringstream o;
     o.precision(17);
+    if (kind == USER) {
+        o << "u:" << std::hex << hash << std::dec << ':' << (xyz ? xyz->size() / 3 : 0) << ':'
+          << (tri ? tri->size() / 3 : 0);
+        return o.str();
+    }
     o << kind << ':' << subdiv << ':' << ax << ':' << ay << ':' << az;
     return o.str();
 }
 
+uint64_t mesh_hash(const std::vector<double> &xyz, const std::vector<int> &tri) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t bytes) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t i = 0; i < bytes; ++i) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    mix(xyz.data(), xyz.size() * sizeof(double));
+    mix(tri.data(), tri.size() * sizeof(int));
+    return h;
+}
+
+// What stays resident for a (mesh, frequency, formulation, ...) key: the
+// device mesh with Z and its LU, or with the MLFMA operator.
 struct Server::Factor {
     emfa_mesh *mesh = nullptr;
     emfa_mat *Z = nullptr;
     emfa_lu *lu = nullptr;
+    emfa_mlfma *op = nullptr;
     int n = 0, prec = EMFA_F64;
     size_t bytes = 0;
     double k = 0;
     ~Factor() {
+        if (op) emfa_mlfma_destroy(op);
         if (lu) emfa_lu_destroy(lu);
         if (Z) emfa_mat_destroy(Z);
         if (mesh) emfa_mesh_destroy(mesh);
@@ -49,7 +72,12 @@ static void build_mesh(const MeshSpec &m, std::vector<double> &xyz, std::vector<
                        int &n) {
     int nv = 0, nt = 0;
     emfa_status st;
-    if (m.kind == PLATE) {
+    if (m.kind == USER) {
+        xyz = *m.xyz;
+        tri = *m.tri;
+        nt = (int)(tri.size() / 3);
+        st = EMFA_OK;
+    } else if (m.kind == PLATE) {
         st = emfa_geom_plate(m.subdiv, m.ax, nullptr, 0, nullptr, 0, &nv, &nt);
         if (st) throw StatusError(st, emfa_last_error());
         xyz.resize(3 * (size_t)nv);
@@ -79,6 +107,7 @@ static bool env_flag(const char *name, bool dflt) {
 // RWG count of a mesh spec without building it (closed icosphere: 1.5 T =
 // 30 4^s edges; plate of c x c cells: 3 c^2 - 2 c interior edges)
 static int spec_unknowns(const MeshSpec &m) {
+    if (m.kind == USER) return m.nrwg;
     if (m.kind == PLATE) return 3 * m.subdiv * m.subdiv - 2 * m.subdiv;
     return 30 * (1 << (2 * m.subdiv));
 }
@@ -255,8 +284,15 @@ static std::string factor_key(const Request &r, int prec) {
     std::ostringstream ks;
     ks.precision(17);
     ks << r.mesh.key() << '|' << r.freq << '|' << r.formulation << '|' << r.alpha << '|' << prec << '|' << r.quad;
+    if (r.solver == MLFMA_OP) ks << "|mlfma|" << r.mlfma_box;
     return ks.str();
 }
+
+// Device bytes an MLFMA operator is reserved before it is built (near
+// matrix, patterns and tables; the 122880-unknown sphere at 1.5 GHz holds
+// 1.09 GB, 9 KB per unknown); the cache then accounts the operator's own
+// figure.
+static size_t mlfma_estimate(int n) { return (size_t)n * 32 * 1024 + (64u << 20); }
 
 // Continuous batching: a worker that takes a request also takes the queued
 // requests that share its factor (mesh, frequency, formulation, alpha,
@@ -295,7 +331,8 @@ void Server::worker(int w) {
                 }
             }
             const Request &r0 = jobs[0].req;
-            if (factor_batch_max_ > 1 && r0.precision == EMFA_F32 && !cache_.count(key) && !building_.count(key)) {
+            if (factor_batch_max_ > 1 && r0.solver == DENSE && r0.precision == EMFA_F32 && !cache_.count(key) &&
+                !building_.count(key)) {
                 const int n0 = spec_unknowns(r0.mesh);
                 std::map<std::string, size_t> group_of{{key, 0}};
                 std::vector<int> group_rhs{rhs};
@@ -303,7 +340,8 @@ void Server::worker(int w) {
                 for (auto &q : queue_) {
                     const int m = (int)(q.req.angles.size() / 3);
                     bool taken = false;
-                    if (q.req.precision == EMFA_F32 && m > 0 && spec_unknowns(q.req.mesh) == n0) {
+                    if (q.req.solver == DENSE && q.req.precision == EMFA_F32 && m > 0 &&
+                        spec_unknowns(q.req.mesh) == n0) {
                         const std::string k = factor_key(q.req, EMFA_F32);
                         auto g = group_of.find(k);
                         if (g != group_of.end() && g->second > 0 && group_rhs[g->second] + m <= batch_max_rhs_) {
@@ -480,6 +518,15 @@ std::shared_ptr<Server::Factor> Server::make_factor(emfa_ctx *ctx, const Request
     check(emfa_mesh_upload(ctx, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3), rwg.data(), n,
                            &nf->mesh),
           "mesh");
+    if (r.solver == MLFMA_OP) {
+        // MLFMA::init .. calculateNearZ (EMSolver/MLFMA.cpp:23-65, 688-1422) on the device
+        emfa_mlfma_params mp{r.formulation, k, r.alpha, r.mlfma_box, 0};
+        check(emfa_mlfma_create(ctx, nf->mesh, &mp, &nf->op), "MLFMA setup");
+        emfa_mlfma_info info{};
+        check(emfa_mlfma_get_info(nf->op, &info), "MLFMA info");
+        nf->bytes = info.bytes;
+        return nf;
+    }
     check(emfa_mat_create(ctx, n, n, prec, &nf->Z), "Z");
     emfa_fill_params fp{r.formulation, r.quad, k, r.alpha, 0.0, 0.0, 0};
     check(emfa_fill(ctx, nf->mesh, &fp, nf->Z), "fill");
@@ -528,7 +575,9 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
         n = f->n;
     else
         build_mesh(r.mesh, xyz, tri, rwg, n);
-    const size_t need = footprint(n, nrhs, prec, (bool)f);
+    const bool mlfma = r.solver == MLFMA_OP;
+    const size_t need = mlfma ? (size_t)n * nrhs * 16 * 4 + (f ? 0 : mlfma_estimate(n))
+                              : footprint(n, nrhs, prec, (bool)f);
     {
         mu_.lock();
         bool ok = reserve(need, *held + pinned);
@@ -539,8 +588,8 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
     if (f) return f;
 
     auto nf = make_factor(ctx, r, prec, xyz, tri, rwg, n);
-    check(emfa_lu_factor(ctx, nf->Z, prec, &nf->lu), "LU");
-    {
+    if (!mlfma) {
+        check(emfa_lu_factor(ctx, nf->Z, prec, &nf->lu), "LU");
         int inter = 1, fallback = 0;
         check(emfa_lu_get_pivoting(nf->lu, &inter, &fallback), "LU");
         if (fallback) {
@@ -554,10 +603,13 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
         std::lock_guard<std::mutex> l(mu_);
         cache_[key] = nf;
         lru_.push_back(key);
-        // the factor's bytes move from this request's reservation to the cache
+        // the factor's bytes move from this request's reservation to the
+        // cache (an MLFMA operator larger than its estimate moves what the
+        // request holds and is counted in full by the cache)
+        const size_t moved = std::min(nf->bytes, *held);
         cache_bytes_ += nf->bytes;
-        reserved_ -= nf->bytes;
-        *held -= nf->bytes;
+        reserved_ -= moved;
+        *held -= moved;
         building_.erase(key);
         bguard.on = false;
     }
@@ -625,7 +677,15 @@ void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result
     int iters = 0;
     double relres = 0;
     bool fallback = false;
-    if (r.precision == EMFA_F64) {
+    if (f->op) {
+        // GMRES(30) on the resident operator (the C5 solve), X from zero
+        emfa_gmres_report rep{};
+        const int restarts = std::max(1, (r.iter_max + 29) / 30);
+        emfa_status st = emfa_mlfma_gmres(ctx, f->op, B, X, 30, r.iter_tol, restarts, &rep);
+        iters = rep.iterations;
+        relres = rep.relres;
+        check(st, "MLFMA GMRES");
+    } else if (r.precision == EMFA_F64) {
         check(emfa_mat_copy(X, B), "copy");
         check(emfa_lu_solve(ctx, f->lu, X), "solve");
     } else {

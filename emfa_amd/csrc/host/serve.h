@@ -27,14 +27,24 @@
 
 namespace serve {
 
-enum MeshKind { ICOSPHERE = 0, PLATE = 1 };
+enum MeshKind { ICOSPHERE = 0, PLATE = 1, USER = 2 };
+enum SolverKind { DENSE = 0, MLFMA_OP = 1 };
 
 struct MeshSpec {
     int kind = ICOSPHERE;
     int subdiv = 2;               // icosphere subdivisions, or plate cells per side
     double ax = 1, ay = 1, az = 1;  // semi-axes (icosphere) / side length in ax (plate)
+    // USER: the caller's geometry (copied at submit, shared by the requests
+    // that name it), its content hash and RWG count
+    std::shared_ptr<const std::vector<double>> xyz;
+    std::shared_ptr<const std::vector<int>> tri;
+    uint64_t hash = 0;
+    int nrwg = 0;
     std::string key() const;
 };
+
+// 64-bit FNV-1a over the vertex coordinates and triangle ids: the cache key of a caller's mesh
+uint64_t mesh_hash(const std::vector<double> &xyz, const std::vector<int> &tri);
 
 struct Request {
     MeshSpec mesh;
@@ -44,6 +54,12 @@ struct Request {
     int precision = EMFA_F64;     // EMFA_F64: FP64 LU; EMFA_F32: FP32 LU + FP64 refinement
     int quad = 3;
     std::vector<double> angles;   // nrhs x 3 (theta, phi, pole) degrees; monostatic RCS per column
+    // MLFMA_OP: the multilevel fast multipole operator stays resident instead
+    // of Z and its LU; each request is solved by GMRES(30) on it
+    int solver = DENSE;
+    double mlfma_box = 0.25;      // finest box edge, wavelengths
+    double iter_tol = 1e-4;       // GMRES relative residual
+    int iter_max = 600;           // GMRES iterations at most
 };
 
 struct Result {

@@ -48,10 +48,11 @@ int emfa_core_run(const char *config_path);
  * and a request runs on one context's streams only. */
 typedef struct emfa_server emfa_server;
 
-enum { EMFA_MESH_ICOSPHERE = 0, EMFA_MESH_PLATE = 1 };
+enum { EMFA_MESH_ICOSPHERE = 0, EMFA_MESH_PLATE = 1, EMFA_MESH_USER = 2 };
+enum { EMFA_SERVE_DENSE = 0, EMFA_SERVE_MLFMA = 1 };
 
 typedef struct {
-    int mesh_kind;      /* EMFA_MESH_ICOSPHERE (ellipsoid) or EMFA_MESH_PLATE */
+    int mesh_kind;      /* EMFA_MESH_ICOSPHERE (ellipsoid), EMFA_MESH_PLATE or EMFA_MESH_USER */
     int subdiv;         /* icosphere subdivisions, or plate cells per side */
     double ax, ay, az;  /* semi-axes (m); plate: side length in ax */
     int formulation;    /* EMFA_EFIE or EMFA_CFIE */
@@ -61,6 +62,20 @@ typedef struct {
     int quad;           /* 3 or 7 */
     int nrhs;           /* monostatic incidences */
     const double *angles; /* nrhs x (theta, phi, pole), degrees */
+    /* EMFA_MESH_USER: the caller's mesh, copied at submit; requests whose
+     * vertices and triangles are equal share resident data (the key is a
+     * hash of their content) */
+    int nv, nt;
+    const double *xyz;  /* nv x 3 (m) */
+    const int *tri;     /* nt x 3, 0-based vertex ids */
+    /* EMFA_SERVE_DENSE: Z and its LU stay resident; EMFA_SERVE_MLFMA: the
+     * multilevel fast multipole operator stays resident and each request is
+     * solved by GMRES(30) on it (EMSolver/MLFMA.cpp; quad and precision are
+     * not used) */
+    int solver;
+    double mlfma_box;   /* finest box edge in wavelengths (0: 0.25) */
+    double iter_tol;    /* GMRES relative residual (0: 1e-4) */
+    int iter_max;       /* GMRES iterations at most (0: 600) */
 } emfa_serve_request;
 
 typedef struct {

@@ -181,3 +181,69 @@ def test_new_factors_of_one_size_are_factored_together_and_match_alone(monkeypat
         assert g["refine_iterations"] == a["refine_iterations"]
         assert rel(g["current"], a["current"]) < 1e-12
         assert np.abs(g["rcs_db"] - a["rcs_db"]).max() < 1e-9
+
+
+def test_caller_mesh_served_like_the_same_synthetic_mesh():
+    """A request carrying its own vertices and triangles (EMFA_MESH_USER) gets
+    the same bits as the request naming the same mesh synthetically; a second
+    request with equal mesh content reuses the resident factor (the key is a
+    hash of the content)."""
+    r = _req(serve.ICOSPHERE, 2, (0.7, 0.5, 0.6), hip.CFIE, 3.5e8, 6)
+    xyz, tri = hip.icosphere(2, 0.7, 0.5, 0.6)
+    with serve.Server(workers=2, hbm_budget=8 * GIB, keep_current=True) as srv:
+        syn = srv.wait(_submit(srv, r, hip.F32))
+        usr = srv.wait(srv.submit(r["angles"], formulation=r["formulation"], alpha=r["alpha"], freq=r["freq"],
+                                  precision=hip.F32, mesh=(xyz, tri)))
+        again = srv.wait(srv.submit(r["angles"][:2], formulation=r["formulation"], alpha=r["alpha"],
+                                    freq=r["freq"], precision=hip.F32, mesh=(xyz.copy(), tri.copy())))
+    for g in (syn, usr, again):
+        assert g["status"] == 0, g["error"]
+    assert not usr["cache_hit"] and again["cache_hit"]
+    assert np.array_equal(usr["rcs_db"], syn["rcs_db"])
+    assert np.array_equal(usr["current"], syn["current"])
+    assert np.array_equal(again["rcs_db"], syn["rcs_db"][:2])
+
+
+def test_mlfma_operator_resident_and_same_alone_and_in_the_mix():
+    """solver=MLFMA keeps the multipole operator resident: the answer equals
+    GMRES(30) on the same operator built directly (hip.MLFMA) bit for bit, a
+    repeat reuses the operator, and the answer is the same in a mix of dense
+    requests."""
+    xyz, tri = hip.icosphere(3, 0.5)
+    freq, alpha, box, tol = 1.2e9, 0.5, 0.25, 1e-5
+    ang = np.array([[0.0, 0.0, 0.0], [60.0, 30.0, 0.0], [150.0, 0.0, 90.0]])
+    ctx = hip.Context(0)
+    try:
+        rwg, _ = hip.rwg_build(tri)
+        mesh = hip.Mesh(ctx, xyz, tri, rwg)
+        k = hip.wavenumber(freq)
+        op = hip.MLFMA(ctx, mesh, k, alpha, box)
+        inc = hip.inc_table(ang)
+        B = hip.Mat(ctx, mesh.n, 3, hip.F64)
+        hip.rhs_fill(ctx, mesh, B, hip.CFIE, k, alpha, inc)
+        X = hip.Mat(ctx, mesh.n, 3, hip.F64)
+        rep, ok = op.gmres(B, X, m=30, tol=tol, max_restart=20)
+        assert ok
+        direct = hip.rcs(ctx, mesh, k, X, -inc[:, 0:3], np.arange(3, dtype=np.int32))
+        xd = X.numpy()
+        for m in (X, B, op, mesh):
+            m.close()
+    finally:
+        ctx.close()
+    kw = dict(formulation=hip.CFIE, alpha=alpha, freq=freq, mesh=(xyz, tri), solver=serve.MLFMA, mlfma_box=box,
+              iter_tol=tol, iter_max=600)
+    with serve.Server(workers=3, hbm_budget=16 * GIB, keep_current=True) as srv:
+        alone = srv.wait(srv.submit(ang, **kw))
+        ids = [_submit(srv, x, p) for x, p in zip(OTHERS[:2], [hip.F32, hip.F64])]
+        tid = srv.submit(ang, **kw)
+        ids.append(_submit(srv, OTHERS[2], hip.F32))
+        got = {}
+        for _ in range(len(ids) + 1):
+            res = srv.wait(-1)
+            got[res["id"]] = res
+    assert alone["status"] == 0, alone["error"]
+    assert not alone["cache_hit"] and got[tid]["cache_hit"]
+    assert all(got[i]["status"] == 0 for i in ids + [tid])
+    assert np.array_equal(alone["rcs_db"], direct)
+    assert np.array_equal(alone["current"], xd)
+    assert np.array_equal(got[tid]["rcs_db"], alone["rcs_db"])
