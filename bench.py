@@ -72,6 +72,7 @@ def parse():
     ap.add_argument("--mixed-requests", type=int, default=64)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=96.0)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the Poisson rate sweep of the serving legs")
     ap.add_argument("--served-passes", type=int, default=5,
                     help="saturated server passes over the request stream; the median is reported")
     return ap.parse_args()
@@ -328,6 +329,11 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5
         "latency_run": {"arrival_rate_per_s": latency_rate * cap, "p50_ms": st2["p50_ms"], "p99_ms": st2["p99_ms"],
                         "mean_ms": st2["mean_ms"], "max_ms": st2["max_ms"], "solves_per_s": st2["solves_per_s"]},
         "rate_sweep": sweep,
+        # the median capacity pass, per request: unknowns, RHS, factor reused, batch size, service ms
+        "served_requests": [[workload.unknowns(q), len(q["angles"]), int(bool(o["cache_hit"])), int(o["batch_size"]),
+                             round(float(o["service_ms"]), 2)]
+                            for q, o in zip(reqs, runs[sorted(range(len(runs)), key=lambda i: per_pass[i])[len(runs) // 2]][0])],
+        "plain_requests_ms": [round(float(x), 2) for x in plain_lat],
         "saturated_p50_ms": st["p50_ms"], "saturated_p99_ms": st["p99_ms"],
         "rcs_max_abs_diff_db_vs_fp64": max(err, err2),
     }
@@ -466,7 +472,7 @@ def main():
     if not a.no_mixed:
         from emfa_amd import workload
         mixed = stream_leg(workload.mixed_stream(a.mixed_requests), a.workers, int(a.budget_gb * 2**30),
-                           passes=a.served_passes)
+                           passes=a.served_passes, rates=() if a.no_sweep else RATE_SWEEP)
 
     cpu = cpu_par = None
     if not a.no_cpu:
@@ -625,7 +631,8 @@ def main_stream(a):
         reqs = workload.mixed_stream(a.steps)
         name = ("mixed stream: spheres/ellipsoids (s=3,4) and 1x1 m plates (sizes stratified over 500-15000 RWG), "
                 "EFIE/CFIE, 1-64 RHS, 30% repeat meshes")
-    leg = stream_leg(reqs, a.workers, int(a.budget_gb * 2**30), passes=a.served_passes)
+    leg = stream_leg(reqs, a.workers, int(a.budget_gb * 2**30), passes=a.served_passes,
+                     rates=() if a.no_sweep else RATE_SWEEP)
     lr = leg["latency_run"]
     out = {
         "metric": "solves/sec (served FP32+refine) + p50/p99 latency",
