@@ -82,6 +82,10 @@ struct emfa_mlfma {
     long long nnz = 0;
     std::vector<std::pair<void *, size_t>> allocs;
     size_t bytes = 0;
+    // single-level operator (emfa_fmm_create): its one level of boxes is
+    // lv[2] (no interpolation, every non-near box far); grid = boxes per axis
+    bool single = false;
+    int grid[3] = {0, 0, 0};
 };
 
 namespace emfa {
@@ -635,6 +639,9 @@ static void ml_free(emfa_mlfma *M) {
         }                                \
     } while (0)
 
+static emfa_status ml_leaf_setup(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *mesh, const emfa_mlfma_params *prm,
+                                 const std::vector<int> &order, emfa_mlfma **out);
+
 static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *prm, emfa_mlfma **out) {
     const int n = mesh->n, T = mesh->nt, nv = mesh->nv;
     const double k = prm->k, lam = 2 * kPi / k;
@@ -861,6 +868,17 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
             ML_TRY(ml_upload(M, &N.an_w, anw));
         }
     }
+    return ml_leaf_setup(ctx, M, mesh, prm, order, out);
+}
+
+// Leaf part shared by both operators: permutation to box order, leaf of each
+// position, near matrix (CSR, box order), radiation / receive patterns and the
+// near entries.  M->lv[M->levels] is the leaf level with its boxes, unknown
+// ranges, near lists and samples set.  On failure M is freed.
+static emfa_status ml_leaf_setup(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *mesh, const emfa_mlfma_params *prm,
+                                 const std::vector<int> &order, emfa_mlfma **out) {
+    const int n = mesh->n, T = mesh->nt, levels = M->levels;
+    const double k = prm->k;
     // leaf data: permutation, leaf of each position, centres
     MlLevel &F = M->lv[levels];
     std::vector<int> leaf_of(n);
@@ -930,6 +948,190 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
     EMFA_HIP_CHECK(hipStreamSynchronize(s));
     *out = M;
     return EMFA_OK;
+}
+
+// Single-level operator (EMSolver/FMM.cpp).  The box grid follows
+// EMComponent/FMM_Boundary.cpp:15-53 in FP32 arithmetic, as the reference
+// computes it, so that the grid, the box of every RWG and L agree with it:
+//   delta = box length * lambda; num = int(extent / delta + 0.5);
+//   delta = extent / num; the vertex bounds widened by 0.1 delta; boxes per
+//   axis int((size + 0.9 delta) / delta); RWG u in box
+//   ((x * ny + y) * nz + z) of its edge midpoint; box centre
+//   ((x + 0.5) delta, (y + 0.5) delta, (z + 0.5) delta).
+// That centre leaves out the lower bound (FMM_Boundary.cpp:47): every centre
+// is shifted by the same vector, which changes each pattern by a phase that
+// cancels between radiation and receive, so the operator is the same; the
+// shift is kept so that the patterns are the reference's.
+// Non-empty boxes in grid order, unknowns of a box in RWG order
+// (FMM.cpp:157-206); near = centres closer than 1.75 delta (the 27 grid
+// neighbours), far = every other box; L = int(kd + 6 cbrt(kd)),
+// kd = 1.732051 k delta (FMM.cpp:47-50); translation per far pair
+// (FMM.cpp:349-381), tabulated here per grid offset in use.
+static emfa_status fmm_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *prm, emfa_mlfma **out) {
+#pragma clang fp contract(off)
+    const int n = mesh->n, nv = mesh->nv;
+    const double k = prm->k;
+    const std::vector<double> &xyz = mesh->xyz_host;
+    const std::vector<int> &rwg = mesh->rwg_host;
+    float mn[3], mx[3];
+    for (int c = 0; c < 3; ++c) mn[c] = mx[c] = (float)xyz[c];
+    for (int v = 1; v < nv; ++v)
+        for (int c = 0; c < 3; ++c) {
+            const float x = (float)xyz[3 * v + c];
+            mn[c] = std::min(mn[c], x);
+            mx[c] = std::max(mx[c], x);
+        }
+    const float lam = (float)(2 * kPi / k);
+    float delta = (float)prm->box_threshold;
+    delta *= lam;
+    float ext[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    const float max_len = ext[0] < ext[1] ? (ext[1] < ext[2] ? ext[2] : ext[1]) : (ext[0] < ext[2] ? ext[2] : ext[0]);
+    const float ratio = max_len / delta;
+    const int num = (int)(ratio + 0.5f);
+    EMFA_REQUIRE(num >= 1, EMFA_E_INVALID,
+                 "fmm: the box length (%.3g wavelengths) exceeds twice the mesh extent; no box grid", prm->box_threshold);
+    delta = max_len / (float)num;
+    const float off = 0.1f * delta;
+    int nb[3];
+    for (int c = 0; c < 3; ++c) {
+        mn[c] = mn[c] - off;
+        mx[c] = mx[c] + off;
+        const float size = mx[c] - mn[c];
+        const float d9 = 0.9f * delta;
+        nb[c] = (int)((size + d9) / delta);
+    }
+    const long long cells = (long long)nb[0] * nb[1] * nb[2];
+    EMFA_REQUIRE(cells > 0 && cells < (1LL << 31), EMFA_E_INVALID, "fmm: box grid %d x %d x %d out of range", nb[0],
+                 nb[1], nb[2]);
+    // box of every RWG (edge midpoint, FP32)
+    std::vector<long long> cell(n);
+    for (int u = 0; u < n; ++u) {
+        int ixc[3];
+        for (int c = 0; c < 3; ++c) {
+            const float s = (float)xyz[3 * rwg[6 * u] + c] + (float)xyz[3 * rwg[6 * u + 1] + c];
+            const float mid = s / 2.0f;
+            ixc[c] = (int)((mid - mn[c]) / delta);
+            EMFA_REQUIRE(ixc[c] >= 0 && ixc[c] < nb[c], EMFA_E_INVALID, "fmm: RWG %d outside the box grid", u);
+        }
+        cell[u] = ((long long)ixc[0] * nb[1] + ixc[1]) * nb[2] + ixc[2];
+    }
+    std::vector<int> order(n);
+    for (int u = 0; u < n; ++u) order[u] = u;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell[a] < cell[b]; });
+
+    emfa_mlfma *M = new emfa_mlfma();
+    M->ctx = ctx;
+    M->n = n;
+    M->levels = 2;
+    M->single = true;
+    for (int c = 0; c < 3; ++c) M->grid[c] = nb[c];
+    M->k = k;
+    M->alpha = prm->kind == EMFA_EFIE ? 1.0 : prm->alpha;
+    M->kind = prm->kind;
+    M->lv.resize(3);
+    MlLevel &C = M->lv[2];
+    C.a = (double)delta;
+    for (int i = 0; i < n;) {
+        const long long id = cell[order[i]];
+        int j = i;
+        while (j < n && cell[order[j]] == id) ++j;
+        const int x = (int)(id / ((long long)nb[1] * nb[2])), y = (int)((id / nb[2]) % nb[1]), z = (int)(id % nb[2]);
+        C.ix.push_back(x);
+        C.ix.push_back(y);
+        C.ix.push_back(z);
+        C.cen.push_back((double)(((float)x + 0.5f) * delta));
+        C.cen.push_back((double)(((float)y + 0.5f) * delta));
+        C.cen.push_back((double)(((float)z + 0.5f) * delta));
+        C.parent.push_back(-1);
+        C.octant.push_back(0);
+        C.ubeg.push_back(i);
+        C.uend.push_back(j);
+        ++C.nbox;
+        i = j;
+    }
+    const int B = C.nbox;
+    // near / far lists in box order; far codes = dense index of the grid offset
+    const long long sx = 2LL * nb[0] - 1, sy = 2LL * nb[1] - 1, sz = 2LL * nb[2] - 1;
+    const long long noff = sx * sy * sz;
+    EMFA_REQUIRE(noff < (1LL << 28), EMFA_E_INVALID, "fmm: %lld grid offsets (box grid too fine)", noff);
+    EMFA_REQUIRE((long long)B * B < (1LL << 31), EMFA_E_INVALID, "fmm: %d boxes, too many box pairs", B);
+    std::vector<int> dense((size_t)noff, -1);
+    std::vector<long long> codes;
+    C.near_ptr.assign(B + 1, 0);
+    C.far_ptr.assign(B + 1, 0);
+    C.far_idx.reserve((size_t)B * B);
+    C.far_code.reserve((size_t)B * B);
+    for (int b = 0; b < B; ++b) {
+        for (int f = 0; f < B; ++f) {
+            const int dx = C.ix[3 * b] - C.ix[3 * f], dy = C.ix[3 * b + 1] - C.ix[3 * f + 1],
+                      dz = C.ix[3 * b + 2] - C.ix[3 * f + 2];
+            if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) <= 1) {
+                C.near_idx.push_back(f);
+                continue;
+            }
+            const long long code = ((dx + nb[0] - 1) * sy + (dy + nb[1] - 1)) * sz + (dz + nb[2] - 1);
+            if (dense[code] < 0) {
+                dense[code] = (int)codes.size();
+                codes.push_back(code);
+            }
+            C.far_idx.push_back(f);
+            C.far_code.push_back(dense[code]);
+        }
+        C.near_ptr[b + 1] = (int)C.near_idx.size();
+        C.far_ptr[b + 1] = (int)C.far_idx.size();
+    }
+    // samples (FMM.cpp:47-50,322-347)
+    {
+        const float kf = (float)k;
+        const float kd = 1.732051f * kf * delta;
+        C.L = std::max(1, (int)(kd + 6 * cbrtf(kd)));
+    }
+    C.K = 2 * C.L * C.L;
+    gauss_legendre(C.L, C.xt, C.wt);
+    gauss_legendre(2 * C.L, C.xp, C.wp);
+    C.kh.resize(3 * (size_t)C.K);
+    C.W.resize(C.K);
+    C.theta.resize(C.K);
+    for (int p = 0; p < 2 * C.L; ++p)
+        for (int t = 0; t < C.L; ++t) {
+            const int q = p * C.L + t;
+            const double th = kPi / 2 * (C.xt[t] + 1), ph = kPi * (C.xp[p] + 1);
+            C.kh[3 * q] = std::sin(th) * std::cos(ph);
+            C.kh[3 * q + 1] = std::sin(th) * std::sin(ph);
+            C.kh[3 * q + 2] = std::cos(th);
+            C.W[q] = C.wp[p] * C.wt[t];
+            C.theta[q] = th;
+        }
+    ML_TRY(ml_alloc(M, &C.up, (size_t)B * C.K * 3));
+    ML_TRY(ml_alloc(M, &C.down, (size_t)B * C.K * 3));
+    ML_TRY(ml_upload(M, &C.far_ptr_d, C.far_ptr));
+    ML_TRY(ml_upload(M, &C.far_idx_d, C.far_idx.empty() ? std::vector<int>{0} : C.far_idx));
+    ML_TRY(ml_upload(M, &C.far_code_d, C.far_code.empty() ? std::vector<int>{0} : C.far_code));
+    ML_TRY(ml_upload(M, &C.parent_d, C.parent));
+    ML_TRY(ml_upload(M, &C.oct_d, C.octant));
+    ML_TRY(ml_upload(M, &C.ix_d, C.ix));
+    ML_TRY(ml_upload(M, &C.near_ptr_d, C.near_ptr));
+    ML_TRY(ml_upload(M, &C.near_idx_d, C.near_idx));
+    // translation per grid offset in use, D = receiving minus source centre
+    C.offsets_used = (int)codes.size();
+    std::vector<cf> tr(std::max<size_t>(1, codes.size()) * C.K, cf{0, 0});
+    {
+        const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t)
+            pool.emplace_back([&, t] {
+                for (size_t i = t; i < codes.size(); i += nth) {
+                    const long long code = codes[i];
+                    const long long ox = code / (sy * sz) - (nb[0] - 1), oy = (code / sz) % sy - (nb[1] - 1),
+                                    oz = code % sz - (nb[2] - 1);
+                    const double D[3] = {C.a * (double)ox, C.a * (double)oy, C.a * (double)oz};
+                    translation(k, D, C.kh, C.L, &tr[i * C.K]);
+                }
+            });
+        for (auto &th : pool) th.join();
+    }
+    ML_TRY(ml_upload(M, &C.trans, tr));
+    return ml_leaf_setup(ctx, M, mesh, prm, order, out);
 }
 #undef ML_TRY
 
@@ -1010,16 +1212,42 @@ emfa_status emfa_mlfma_destroy(emfa_mlfma *m) {
 emfa_status emfa_mlfma_get_info(const emfa_mlfma *m, emfa_mlfma_info *info) {
     EMFA_REQUIRE(m && info, EMFA_E_INVALID, "emfa_mlfma_get_info: null argument");
     std::memset(info, 0, sizeof(*info));
-    info->levels = m->levels;
+    info->levels = m->single ? 1 : m->levels;
     info->near_nnz = m->nnz;
     info->bytes = m->bytes;
-    for (int l = 0; l <= m->levels && l < 16; ++l) {
-        info->boxes[l] = m->lv[l].nbox;
-        info->multipoles[l] = m->lv[l].L;
-        info->box_edge[l] = m->lv[l].a;
-        info->far_pairs[l] = (long long)m->lv[l].far_idx.size();
+    for (int c = 0; c < 3; ++c) info->grid[c] = m->grid[c];
+    for (int l = 0; l <= info->levels && l < 16; ++l) {
+        const MlLevel &v = m->lv[m->single && l == 1 ? 2 : m->single ? 0 : l];
+        if (m->single && l == 0) continue;
+        info->boxes[l] = v.nbox;
+        info->multipoles[l] = v.L;
+        info->box_edge[l] = v.a;
+        info->far_pairs[l] = (long long)v.far_idx.size();
     }
     return EMFA_OK;
+}
+
+emfa_status emfa_mlfma_boxes(const emfa_mlfma *m, int level, int *nnear, int *nfar, int *nunk) {
+    EMFA_REQUIRE(m, EMFA_E_INVALID, "emfa_mlfma_boxes: null operator");
+    const int top = m->single ? 1 : m->levels;
+    EMFA_REQUIRE(level >= 1 && level <= top, EMFA_E_INVALID, "emfa_mlfma_boxes: level %d outside 1..%d", level, top);
+    const MlLevel &v = m->lv[m->single ? 2 : level];
+    for (int b = 0; b < v.nbox; ++b) {
+        if (nnear) nnear[b] = v.near_ptr[b + 1] - v.near_ptr[b];
+        if (nfar) nfar[b] = v.far_ptr.empty() ? 0 : v.far_ptr[b + 1] - v.far_ptr[b];
+        if (nunk) nunk[b] = v.ubeg.empty() ? 0 : v.uend[b] - v.ubeg[b];
+    }
+    return EMFA_OK;
+}
+
+emfa_status emfa_fmm_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out) {
+    EMFA_REQUIRE(ctx && mesh && p && out, EMFA_E_INVALID, "emfa_fmm_create: null argument");
+    *out = nullptr;
+    EMFA_REQUIRE(p->kind == EMFA_EFIE || p->kind == EMFA_CFIE, EMFA_E_INVALID, "emfa_fmm_create: kind");
+    EMFA_REQUIRE(p->k > 0 && p->box_threshold > 0, EMFA_E_INVALID, "emfa_fmm_create: k and box length must be > 0");
+    EMFA_REQUIRE(!mesh->rwg_host.empty(), EMFA_E_INVALID, "emfa_fmm_create: mesh without host copy");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    return fmm_build(ctx, mesh, p, out);
 }
 
 emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, emfa_mat *Y) {

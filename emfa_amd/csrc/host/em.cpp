@@ -468,19 +468,28 @@ void MLFMA::reportInfo(std::ostream &strm) const {
          << '\n';
 }
 
-bool MLFMA::solveMatrixFree(emfa_mat *B) {
+emfa_status MLFMA::createOperator(emfa_mlfma **op) {
     emfa_mlfma_params mp{EMFA_CFIE, k_, alpha_, box_threshold_, cfie_mode_};
-    emfa_mlfma *op = nullptr;
-    emfa_status st = emfa_mlfma_create(ctx_, dmesh_, &mp, &op);
-    if (st == EMFA_E_INVALID) throw std::runtime_error(std::string("MLFMA: ") + emfa_last_error());
-    check(st, "MLFMA setup");
+    return emfa_mlfma_create(ctx_, dmesh_, &mp, op);
+}
+
+void MLFMA::logOperator(emfa_mlfma *op) {
     emfa_mlfma_info info{};
     emfa_mlfma_get_info(op, &info);
     logger_ << "MLFMA\n  -> levels: " << info.levels << ", near entries: " << info.near_nnz
             << ", device bytes: " << info.bytes << '\n';
     for (int l = 2; l <= info.levels; ++l)
         logger_ << "  -> level " << l << ": " << info.boxes[l] << " boxes, L = " << info.multipoles[l] << '\n';
-    timeLog("MLFMA setup");
+}
+
+bool MLFMA::solveMatrixFree(emfa_mat *B) {
+    const std::string name = operatorName();
+    emfa_mlfma *op = nullptr;
+    emfa_status st = createOperator(&op);
+    if (st == EMFA_E_INVALID) throw std::runtime_error(name + ": " + emfa_last_error());
+    check(st, (name + " setup").c_str());
+    logOperator(op);
+    timeLog(name + " setup");
     int iters = 0;
     double relres = 0;
     if (solver_ == LinearSolver::GMRES) {
@@ -490,7 +499,7 @@ bool MLFMA::solveMatrixFree(emfa_mat *B) {
         iters = rep.iterations;
         relres = rep.relres;
     } else {
-        // MLFMA::iterativeSolve (MLFMA.cpp:402-452)
+        // MLFMA::iterativeSolve (MLFMA.cpp:402-452), FMM::iterativeSolve (FMM.cpp:222-268)
         emfa_bicgstab_report rep{};
         st = emfa_mlfma_bicgstab(ctx_, op, B, X_, iter_threshold_, max_iter_, 0, &rep);
         iters = rep.iterations;
@@ -504,6 +513,79 @@ bool MLFMA::solveMatrixFree(emfa_mat *B) {
     else
         check(st, solverName(solver_));
     return true;
+}
+
+// ------------------------------------------------------------------ FMM
+void FMM::init(component::ConfigLoader *ploader) {
+    alpha_ = ploader->getAlpha();
+    iter_threshold_ = ploader->getDouble("iteration_threshold:", 1e-3);
+    max_iter_ = ploader->getInt("max_iteration_number:", 1000);
+    // FMM.cpp:43 reads fmm_box_length: unconditionally (ConfigLoader.cpp:536-541)
+    box_threshold_ = ploader->getDouble("fmm_box_length:", -1.0);
+    if (!(box_threshold_ > 0)) throw component::ConfigInvalid("FMM: fmm_box_length: must be given and positive");
+    if (!(iter_threshold_ > 0) || max_iter_ < 1)
+        throw component::ConfigInvalid("FMM: iteration_threshold and max_iteration_number must be positive");
+    DenseMoM::init(ploader);
+    if (solver_ == LinearSolver::LU) throw component::ConfigInvalid("FMM: linear_solver must be BICGSTAB or GMRES");
+}
+
+// EMSolver/FMM.cpp:33: "_FMM_CFIE_" + to_string(alpha).substr(0, 3)
+std::string FMM::folderTag() const { return "_FMM_CFIE_" + std::to_string((float)alpha_).substr(0, 3); }
+
+void FMM::reportInfo(std::ostream &strm) const {
+    DenseMoM::reportInfo(strm);
+    strm << "FMM\n"
+         << "  -> box length: " << box_threshold_ << " wavelengths\n"
+         << "  -> " << (solver_ == LinearSolver::GMRES ? "GMRES(" + std::to_string(gmres_m_) + ")" : "BICGSTAB")
+         << " to " << iter_threshold_ << ", at most " << max_iter_ << " iterations\n";
+}
+
+emfa_status FMM::createOperator(emfa_mlfma **op) {
+    emfa_mlfma_params mp{EMFA_CFIE, k_, alpha_, box_threshold_, cfie_mode_};
+    return emfa_fmm_create(ctx_, dmesh_, &mp, op);
+}
+
+// FMM::reportFFMInfo / reportBoxInfo (FMM.cpp:729-772): the operator's sizes in
+// the log and one line per box in ffm_box_log.txt
+void FMM::logOperator(emfa_mlfma *op) {
+    emfa_mlfma_info info{};
+    emfa_mlfma_get_info(op, &info);
+    const int nb = info.boxes[1], L = info.multipoles[1];
+    const long long K = 2LL * L * L, n = unknowns_;
+    std::vector<int> nn(nb), nf(nb), nu(nb);
+    emfa_mlfma_boxes(op, 1, nn.data(), nf.data(), nu.data());
+    const size_t mb = 1024 * 1024;
+    const size_t near_mem = (size_t)info.near_nnz * 2 * sizeof(float) / mb;
+    const size_t rdcv_mem = 12 * (size_t)n * K * sizeof(float) / mb;
+    const size_t tran_mem = 2 * (size_t)info.far_pairs[1] * K * sizeof(float) / mb;
+    const size_t orig_mem = 2 * sizeof(float) * (size_t)n * n / mb;
+    const size_t total_mem = near_mem + rdcv_mem + tran_mem;
+    auto info_lines = [&](std::ostream &o) {
+        o << "FMM Information:\n"
+          << "  -> Total boxes: " << (long long)info.grid[0] * info.grid[1] * info.grid[2] << '\n'
+          << "  -> Valid boxes: " << nb << '\n'
+          << "  -> Average per box: " << (nb ? n / nb : 0) << '\n'
+          << "  -> Number of near: " << info.near_nnz << '\n'
+          << "  -> Sparse rate: " << 100.0f * (float)info.near_nnz / (float)n / (float)n << "%\n"
+          << "  -> Number of multipoles: " << L << '\n'
+          << "  -> Near impedance: " << near_mem << " MB\n"
+          << "  -> Transfer (per box pair, as the reference stores it): " << tran_mem << " MB\n"
+          << "  -> Rad and Recv: " << rdcv_mem << " MB\n"
+          << "  -> Total memory" << total_mem << " MB\n"
+          << "  -> Saved memory: " << (total_mem > orig_mem ? "-" : "")
+          << (total_mem > orig_mem ? total_mem - orig_mem : orig_mem - total_mem) << " MB\n"
+          << "  -> Grid: " << info.grid[0] << 'x' << info.grid[1] << 'x' << info.grid[2] << " boxes of "
+          << info.box_edge[1] << " m, device bytes: " << info.bytes << '\n';
+    };
+    info_lines(logger_);
+    std::ofstream box_log(dir_ + "/ffm_box_log.txt");
+    info_lines(box_log);
+    char line[128];
+    for (int b = 0; b < nb; ++b) {
+        std::snprintf(line, sizeof(line), "    -> Box %4d:    near: %3d    far: %5d    unknowns: %5d\n", b + 1, nn[b],
+                      nf[b], nu[b]);
+        box_log << line;
+    }
 }
 
 }  // namespace mom
@@ -522,6 +604,8 @@ void loadSolver(const std::string &conf_file) {
         invoke_solver<mom::MFIE>(&loader);
     else if (solver_type == "MLFMA")
         invoke_solver<mom::MLFMA>(&loader);
+    else if (solver_type == "FMM")
+        invoke_solver<mom::FMM>(&loader);
     else
         throw component::ConfigInvalid("undefined EM solver type (" + solver_type + ')');
 }

@@ -177,9 +177,27 @@ class MLFMA : public DenseMoM {
     std::string folderTag() const override;
     bool solveMatrixFree(emfa_mat *B) override;
     std::string defaultSolver() const override { return "BICGSTAB"; }
+    // the fast operator of this solver and what its set-up reports
+    virtual emfa_status createOperator(emfa_mlfma **op);
+    virtual void logOperator(emfa_mlfma *op);
+    virtual const char *operatorName() const { return "MLFMA"; }
     double alpha_ = 0.5, box_threshold_ = 0.5, iter_threshold_ = 1e-3;
     int max_iter_ = 1000;
     bool precond_requested_ = false;
+};
+
+// Single-level FMM (EMSolver/FMM.cpp): CFIE on a uniform box grid of
+// `fmm_box_length:` wavelengths, solved with BiCGSTAB (FMM.cpp:222-268).
+class FMM : public MLFMA {
+  public:
+    void init(component::ConfigLoader *ploader) override;
+    void reportInfo(std::ostream &strm) const override;
+
+  protected:
+    std::string folderTag() const override;
+    emfa_status createOperator(emfa_mlfma **op) override;
+    void logOperator(emfa_mlfma *op) override;
+    const char *operatorName() const override { return "FMM"; }
 };
 
 }  // namespace mom

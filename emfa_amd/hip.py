@@ -27,6 +27,7 @@ EXPORTS = [
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
     "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab", "emfa_bicgstab_noalloc",
+    "emfa_mlfma_boxes", "emfa_fmm_create",
 ]
 
 _vp = ctypes.c_void_p
@@ -67,7 +68,7 @@ class MlfmaParams(ctypes.Structure):
 class MlfmaInfo(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("near_nnz", ctypes.c_longlong), ("bytes", ctypes.c_size_t),
                 ("boxes", ctypes.c_int * 16), ("multipoles", ctypes.c_int * 16), ("box_edge", ctypes.c_double * 16),
-                ("far_pairs", ctypes.c_longlong * 16)]
+                ("far_pairs", ctypes.c_longlong * 16), ("grid", ctypes.c_int * 3)]
 
 
 class EmfaError(RuntimeError):
@@ -403,11 +404,13 @@ def gmres(ctx, Z, B, X, m=30, tol=1e-6, max_restart=100):
 class MLFMA:
     """Multilevel fast multipole operator of a mesh (emfa_mlfma_*)."""
 
+    _create = "emfa_mlfma_create"
+
     def __init__(self, ctx, mesh, k, alpha=0.5, box_threshold=0.25, kind=CFIE, cfie_mode=0):
         self.ctx = ctx
         self.h = _vp()
         p = MlfmaParams(kind, float(k), float(alpha), float(box_threshold), int(cfie_mode))
-        _check(lib().emfa_mlfma_create(ctx.h, mesh.h, ctypes.byref(p), ctypes.byref(self.h)))
+        _check(getattr(lib(), self._create)(ctx.h, mesh.h, ctypes.byref(p), ctypes.byref(self.h)))
         self.n = mesh.rwg.shape[0]
 
     def info(self):
@@ -416,7 +419,15 @@ class MLFMA:
         lv = i.levels
         return {"levels": lv, "near_nnz": i.near_nnz, "bytes": i.bytes, "boxes": list(i.boxes[:lv + 1]),
                 "multipoles": list(i.multipoles[:lv + 1]), "box_edge": list(i.box_edge[:lv + 1]),
-                "far_pairs": list(i.far_pairs[:lv + 1])}
+                "far_pairs": list(i.far_pairs[:lv + 1]), "grid": list(i.grid)}
+
+    def boxes(self, level):
+        """(near boxes, far boxes, unknowns) per box of a level, box order."""
+        nb = self.info()["boxes"][level]
+        a, b, c = (np.zeros(nb, dtype=np.int32) for _ in range(3))
+        _check(lib().emfa_mlfma_boxes(self.h, int(level), a.ctypes.data_as(_i), b.ctypes.data_as(_i),
+                                      c.ctypes.data_as(_i)))
+        return a, b, c
 
     def matvec(self, X, Y):
         _check(lib().emfa_mlfma_matvec(self.ctx.h, self.h, X.h, Y.h))
@@ -441,6 +452,15 @@ class MLFMA:
         if self.h and self.ctx.h:
             lib().emfa_mlfma_destroy(self.h)
         self.h = _vp()
+
+
+class FMM(MLFMA):
+    """Single-level fast multipole operator (emfa_fmm_create; box_threshold is
+    the box length in wavelengths, `fmm_box_length:`)."""
+    _create = "emfa_fmm_create"
+
+    def __init__(self, ctx, mesh, k, alpha=0.5, box_length=0.4, kind=CFIE, cfie_mode=0):
+        super().__init__(ctx, mesh, k, alpha, box_length, kind, cfie_mode)
 
 
 def rcs(ctx, mesh, k, X, dirs, cols=None):

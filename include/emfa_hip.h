@@ -283,10 +283,29 @@ typedef struct {
     int multipoles[16];   /* L per level (levels >= 2) */
     double box_edge[16];  /* box edge per level (m) */
     long long far_pairs[16]; /* (box, far box) translations per level */
+    int grid[3];          /* single-level operator: boxes per axis of its grid (0 for the MLFMA) */
 } emfa_mlfma_info;
 emfa_status emfa_mlfma_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out);
 emfa_status emfa_mlfma_destroy(emfa_mlfma *m);
 emfa_status emfa_mlfma_get_info(const emfa_mlfma *m, emfa_mlfma_info *info);
+/* Per box of a level (1..info.levels, box order): near boxes, far boxes and
+ * unknowns; each output may be null.  FMM::reportBoxInfo (FMM.cpp:759-772)
+ * writes these counts to ffm_box_log.txt. */
+emfa_status emfa_mlfma_boxes(const emfa_mlfma *m, int level, int *nnear, int *nfar, int *nunk);
+
+/* Single-level fast multipole operator: replaces FMM::init's box set-up,
+ * preCalculateKArray / preCalculateTrans / preCalculateRadAndRecv and
+ * calculateNearZ (EMSolver/FMM.cpp:23-58,157-206,322-573,
+ * EMComponent/FMM_Boundary.cpp:15-53).  p->box_threshold is the box length
+ * in wavelengths (`fmm_box_length:`); the grid, the box of each RWG and L
+ * are computed in FP32 as the reference computes them.  Every box that is
+ * not one of the 27 grid neighbours of a box is far from it (one level, no
+ * interpolation).  The handle is an emfa_mlfma: emfa_mlfma_matvec
+ * (FMM::matrixVectorMultiply, FMM.cpp:270-320), emfa_mlfma_bicgstab (variant
+ * 0 is FMM::iterativeSolve, FMM.cpp:222-268), emfa_mlfma_gmres,
+ * emfa_mlfma_get_info (levels = 1, the grid at index 1) and
+ * emfa_mlfma_destroy apply to it. */
+emfa_status emfa_fmm_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out);
 /* Y = Z_mlfma X for F64 n x ncols matrices (column by column). */
 emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, emfa_mat *Y);
 /* GMRES(restart) with FP64 orthogonalisation on the MLFMA operator; X holds

@@ -194,6 +194,27 @@ def test_mlfma_solver_files_near_dense_cfie(tmp_path):
     assert np.linalg.norm(p - q) / np.linalg.norm(q) < 5e-2
 
 
+def test_fmm_solver_files_near_dense_cfie(tmp_path):
+    # EMSolver: FMM (single level, fmm_box_length: 0.4 wavelengths) on the
+    # 1920-unknown sphere at 600 MHz, BiCGSTAB as FMM.cpp:222-268
+    xyz, tri = _run(tmp_path, "FMM", "BistaticRCS", freq="6e8", radius=0.5, subdiv=3, tto=180, tdelta=10,
+                    extra="    iteration_threshold: 1e-4\n    max_iteration_number: 600\n"
+                          "    fmm_box_length: 0.4\n")
+    out = tmp_path / "sphere_FMM_CFIE_0.5"
+    log = open(out / "sca_runtime.log").read()
+    assert "Valid boxes" in log and "Iterative number" in log and "cannot converge" not in log
+    boxes = open(out / "ffm_box_log.txt").read().splitlines()
+    assert sum(1 for l in boxes if l.startswith("    -> Box ")) > 1
+    rwg, _ = O.build_rwg(tri)
+    k = O.wavenumber(6e8)
+    Z = O.fill_cfie(xyz, tri, rwg, k, 0.5, 3, 0)
+    X = np.linalg.solve(Z, O.rhs(xyz, tri, rwg, 1, k, 0.5, O.inc_table([(0, 0, 0)])))
+    th, val = _read_rcs(out / "bistatic_rcs_phi_0.dat")
+    ref = O.rcs(xyz, tri, rwg, k, X, _dirs(th, 0))
+    lit = ref > ref.max() - 15
+    assert np.abs(val - ref)[lit].max() <= 0.5
+
+
 def test_launcher_beside_a_process_that_launched_cooperatively(tmp_path):
     # This process factors a matrix on a plain context first (cooperative
     # panel launches) and keeps the context open; the launcher process that
