@@ -277,6 +277,9 @@ emfa_status emfa_ctx_destroy(emfa_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     prof_collect(ctx);
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto &g : ctx->lu_graph) (void)hipGraphExecDestroy(g.second);
+    ctx->lu_graph.clear();
+    if (ctx->bmat) ctx_free(ctx, ctx->bmat, ctx->bmat_cap * sizeof(void *));
     if (ctx->scratch) ctx_free(ctx, ctx->scratch, ctx->scratch_bytes);
     ctx_free(ctx, ctx->flags, 64);
     (void)hipStreamSynchronize(ctx->aux);
@@ -303,6 +306,19 @@ int emfa_ctx_cu_count(const emfa_ctx *ctx) { return ctx ? ctx->num_cu : 0; }
 emfa_status emfa_ctx_set_panel_cap(emfa_ctx *ctx, int wgs) {
     EMFA_REQUIRE(ctx && wgs >= 0, EMFA_E_INVALID, "emfa_ctx_set_panel_cap: bad argument");
     ctx->panel_cap = wgs;
+    return EMFA_OK;
+}
+
+emfa_status emfa_ctx_set_graphs(emfa_ctx *ctx, int on) {
+    EMFA_REQUIRE(ctx, EMFA_E_INVALID, "emfa_ctx_set_graphs: null ctx");
+    ctx->lu_graphs = on != 0;
+    return EMFA_OK;
+}
+
+emfa_status emfa_ctx_graph_stats(const emfa_ctx *ctx, long long *captures, long long *launches) {
+    EMFA_REQUIRE(ctx, EMFA_E_INVALID, "emfa_ctx_graph_stats: null ctx");
+    if (captures) *captures = ctx->graph_captures;
+    if (launches) *launches = ctx->graph_launches;
     return EMFA_OK;
 }
 

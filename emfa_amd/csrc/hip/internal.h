@@ -111,6 +111,15 @@ struct emfa_ctx {
     size_t scratch_bytes = 0;
     int *flags = nullptr;  // [0] spin-timeout flag, [1] singular info, [2] pivot-free growth
     emfa::Batch batch;     // set while a batch of factorisations runs (the LU helpers launch over it)
+    // LU launch sequences captured as HIP graphs (emfa_ctx_set_graphs): one
+    // executable graph per launch configuration, replayed for every batch of
+    // that configuration; the matrices are reached through `bmat`, a device
+    // array of their bases whose address stays fixed while its capacity does
+    bool lu_graphs = false;
+    std::map<std::string, hipGraphExec_t> lu_graph;
+    void *bmat = nullptr;
+    size_t bmat_cap = 0;  // entries
+    long long graph_captures = 0, graph_launches = 0;
 };
 
 struct emfa_mesh {

@@ -16,6 +16,7 @@
 // factor and pivots match ?getrf's (the routine behind arma::solve at
 // EMSolver/MFIE.cpp:64) up to rounding.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -1961,6 +1962,44 @@ static bool coop_launch(hipStream_t s, const void *fn, int G, void **args, size_
     return false;
 }
 
+// Run `body` (a launch sequence on ctx->stream, with the look-ahead stream
+// joined back) as the context's graph for `key`: captured and instantiated
+// the first time, replayed afterwards.
+static emfa_status lu_graph_run(emfa_ctx *ctx, const std::string &key, const std::function<emfa_status()> &body) {
+    auto it = ctx->lu_graph.find(key);
+    if (it == ctx->lu_graph.end()) {
+        EMFA_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        emfa_status st = body();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            if (st) return st;
+            set_error("lu: capturing the factorisation failed: %s", hipGetErrorString(e));
+            return EMFA_E_HIP;
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("lu: instantiating the factorisation graph failed: %s", hipGetErrorString(ei));
+            return EMFA_E_HIP;
+        }
+        if (ctx->lu_graph.size() >= 64) {  // bounded: drop them all once the stream is idle
+            EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            for (auto &o : ctx->lu_graph) (void)hipGraphExecDestroy(o.second);
+            ctx->lu_graph.clear();
+        }
+        it = ctx->lu_graph.emplace(key, x).first;
+        ++ctx->graph_captures;
+    }
+    EMFA_HIP_CHECK(hipGraphLaunch(it->second, ctx->stream));
+    ++ctx->graph_launches;
+    return EMFA_OK;
+}
+
 template <typename R>
 // With ctx->batch set (count matrices of Am's size, matrix 0 = Am), the
 // pivot-free FP32 path with the split update factors all of them with one
@@ -2041,14 +2080,17 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     // growth flags and pivots: the context's for one matrix, each slab's for a batch
     int *flagsp = batched ? (int *)take(256) : ctx->flags;
     int *ipivp = batched ? (int *)take((size_t)N * sizeof(int)) : ipiv;
-    {
+    // slots, barrier and flags start at zero (fill kernels, so that the
+    // sequence can be captured in a graph)
+    auto zero_state = [&]() -> emfa_status {
         const int nw = (int)((x_pk_b + x_rw_b + x_cw_b) / 4);
         k_zero_u32<<<(nw + 255) / 256, 256, 0, ctx->stream>>>((unsigned *)x_pk, nw);
-    }
-    k_zero_u32<<<1, 64, 0, ctx->stream>>>(bar, 64);
-    EMFA_HIP_CHECK(hipGetLastError());
-    k_zero_u32<<<dim3(1, 1, ctx->batch.count), 64, 0, ctx->stream>>>((unsigned *)flagsp, 16, ctx->batch);
-    EMFA_HIP_CHECK(hipGetLastError());
+        k_zero_u32<<<1, 64, 0, ctx->stream>>>(bar, 64);
+        EMFA_HIP_CHECK(hipGetLastError());
+        k_zero_u32<<<dim3(1, 1, ctx->batch.count), 64, 0, ctx->stream>>>((unsigned *)flagsp, 16, ctx->batch);
+        EMFA_HIP_CHECK(hipGetLastError());
+        return EMFA_OK;
+    };
     unsigned bar_base = 0;
     // Cooperative launches of concurrent contexts are serialised by the
     // runtime, so contexts sharing the device (cu_share > 0) launch the
@@ -2165,6 +2207,10 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         return EMFA_OK;
     };
 
+    // the whole launch sequence of the factorisation
+    auto body = [&]() -> emfa_status {
+    if ((st = zero_state())) return st;
+    bar_base = 0;
     int jb0 = std::min(nb, N);
     if ((st = factor_outer(0, jb0))) return st;
     if ((st = outer_swap_trsm(0, jb0))) return st;
@@ -2212,6 +2258,20 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
         }
         if ((st = outer_swap_trsm(next, nbn))) return st;
     }
+    return EMFA_OK;
+    };
+    if (batched && ctx->lu_graphs && !ctx->prof) {
+        // everything the launch sequence depends on; the matrices themselves
+        // are read through the batch's device array of bases
+        char key[320];
+        std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %zu %p %d", N, nb, plan.w,
+                      ctx->batch.count, G_max, (int)split_update, (int)nopiv, (int)xchg, (int)lookahead, (int)coop,
+                      (int)ctx->cu_share, ctx->num_cu, scr, slab, (const void *)ctx->batch.mat, panel_rows_min);
+        st = lu_graph_run(ctx, key, body);
+    } else {
+        st = body();
+    }
+    if (st) return st;
     if (batched) {
         // per-matrix growth flags from each slab
         std::vector<int> hf(3 * (size_t)ctx->batch.count, 0);
@@ -2579,6 +2639,14 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
 
 using namespace emfa;
 
+// The configuration that factors a batch with one launch sequence: FP32,
+// pivot-free panels, the split update, inverse-block solves.
+static bool lu_batch_config(const emfa_ctx *ctx, int prec) {
+    return prec == EMFA_F32 && ctx->panel_x == 2 && ctx->lu_update_mode == 1 && !ctx->force_pivoting &&
+           !getenv("EMFA_SOLVE_BLOCKED") && !getenv("EMFA_LU_NOPIV") && !getenv("EMFA_LU_BF16X3") &&
+           !getenv("EMFA_LU_NOBATCH");
+}
+
 extern "C" {
 
 // Compose the interchanges of lu->ipiv into one row permutation (used by the
@@ -2617,6 +2685,8 @@ emfa_status emfa_lu_factor(emfa_ctx *ctx, const emfa_mat *Z, int prec, emfa_lu *
     EMFA_REQUIRE(Z->rows == Z->cols, EMFA_E_INVALID, "emfa_lu_factor: matrix is %dx%d, not square", Z->rows,
                  Z->cols);
     EMFA_REQUIRE(prec == EMFA_F32 || prec == EMFA_F64, EMFA_E_INVALID, "emfa_lu_factor: bad precision");
+    // graph-captured factorisations go through the batch path as batches of one
+    if (ctx->lu_graphs && lu_batch_config(ctx, prec)) return emfa_lu_factor_batch(ctx, &Z, 1, prec, out);
     *out = nullptr;
     EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     emfa_lu *lu = new emfa_lu();
@@ -2666,9 +2736,8 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
     EMFA_REQUIRE(prec == EMFA_F32 || prec == EMFA_F64, EMFA_E_INVALID, "emfa_lu_factor_batch: bad precision");
     const int n = Z[0]->rows;
     // one launch sequence for the batch: FP32, pivot-free panels, split update, inverse-block solves
-    const bool batchable = count > 1 && prec == EMFA_F32 && ctx->panel_x == 2 && ctx->lu_update_mode == 1 &&
-                           !ctx->force_pivoting && !getenv("EMFA_SOLVE_BLOCKED") && !getenv("EMFA_LU_NOPIV") &&
-                           !getenv("EMFA_LU_BF16X3") && !getenv("EMFA_LU_NOBATCH");
+    // (a batch of one when the context runs factorisations as graphs)
+    const bool batchable = (count > 1 || ctx->lu_graphs) && lu_batch_config(ctx, prec);
     if (!batchable) {
         emfa_status worst = EMFA_OK;
         for (int z = 0; z < count; ++z) {
@@ -2691,7 +2760,7 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
     auto fail = [&](emfa_status st) {
         (void)hipStreamSynchronize(ctx->stream);
         ctx->batch = Batch();
-        if (dmat) ctx_free(ctx, dmat, sizeof(void *) * count);
+        if (dmat && dmat != ctx->bmat) ctx_free(ctx, dmat, sizeof(void *) * count);
         if (ddinv) ctx_free(ctx, ddinv, sizeof(void *) * count);
         if (dgh) ctx_free(ctx, dgh, sizeof(void *) * count);
         for (auto l : lus)
@@ -2718,7 +2787,26 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
     // device array of the factors' bases; host copies stay alive until the stream syncs below
     std::vector<const char *> hmat(count), hdinv(count), hgh(count);
     for (int z = 0; z < count; ++z) hmat[z] = (const char *)lus[z]->a->d;
-    if (ctx_alloc(ctx, &dmat, sizeof(void *) * count) != hipSuccess) {
+    if (ctx->lu_graphs) {
+        // the context's array of bases: its address is part of the captured graphs
+        if (ctx->bmat_cap < (size_t)count) {
+            if (ctx->bmat) {
+                EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+                ctx_free(ctx, ctx->bmat, ctx->bmat_cap * sizeof(void *));
+                ctx->bmat = nullptr;
+                ctx->bmat_cap = 0;
+            }
+            const size_t cap = std::max<size_t>(16, (size_t)count);
+            if (ctx_alloc(ctx, &ctx->bmat, cap * sizeof(void *)) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->bmat = nullptr;
+                set_error("emfa_lu_factor_batch: allocation failed");
+                return fail(EMFA_E_OOM);
+            }
+            ctx->bmat_cap = cap;
+        }
+        dmat = ctx->bmat;
+    } else if (ctx_alloc(ctx, &dmat, sizeof(void *) * count) != hipSuccess) {
         (void)hipGetLastError();
         set_error("emfa_lu_factor_batch: allocation failed");
         return fail(EMFA_E_OOM);
@@ -2811,7 +2899,7 @@ emfa_status emfa_lu_factor_batch(emfa_ctx *ctx, const emfa_mat *const *Z, int co
         }
     }
     EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    ctx_free(ctx, dmat, sizeof(void *) * count);
+    if (dmat != ctx->bmat) ctx_free(ctx, dmat, sizeof(void *) * count);
     ctx_free(ctx, ddinv, sizeof(void *) * count);
     if (dgh) ctx_free(ctx, dgh, sizeof(void *) * count);
     emfa_status worst = EMFA_OK;

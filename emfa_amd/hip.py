@@ -27,7 +27,7 @@ EXPORTS = [
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
     "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab", "emfa_bicgstab_noalloc",
-    "emfa_mlfma_boxes", "emfa_fmm_create",
+    "emfa_mlfma_boxes", "emfa_fmm_create", "emfa_ctx_set_graphs", "emfa_ctx_graph_stats",
 ]
 
 _vp = ctypes.c_void_p
@@ -216,6 +216,16 @@ class Context:
 
     def set_cu_share(self, cus):
         _check(lib().emfa_ctx_set_cu_share(self.h, int(cus)))
+
+    def graphs(self, on=True):
+        """Run pivot-free bf16-split FP32 factorisations as captured HIP graphs."""
+        _check(lib().emfa_ctx_set_graphs(self.h, int(bool(on))))
+
+    def graph_stats(self):
+        """(graphs captured, graph launches) of this context."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().emfa_ctx_graph_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def prof(self, on=True):
         _check(lib().emfa_prof_enable(self.h, int(on)))

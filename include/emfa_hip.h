@@ -71,6 +71,18 @@ emfa_status emfa_ctx_set_cu_share(emfa_ctx *ctx, int cus);
  * keep the sum of their caps within emfa_ctx_cu_count so every panel's
  * workgroups can be resident together. */
 emfa_status emfa_ctx_set_panel_cap(emfa_ctx *ctx, int wgs);
+/* on != 0: FP32 factorisations without interchanges and with the bf16-split
+ * update (panel variant 2, update mode 1) run as HIP graphs.  The launch
+ * sequence of a factorisation depends only on its size, the batch count and
+ * the context's LU configuration (block, panel caps, CU share), so it is
+ * captured once per such configuration and replayed for every later matrix
+ * or batch of matrices of that configuration; the matrices are reached
+ * through a device array of their bases that is rewritten before each
+ * replay.  Single factorisations take this path as batches of one.  No
+ * counterpart in the reference (it has no device). */
+emfa_status emfa_ctx_set_graphs(emfa_ctx *ctx, int on);
+/* graphs captured and graph launches made by this context so far */
+emfa_status emfa_ctx_graph_stats(const emfa_ctx *ctx, long long *captures, long long *launches);
 
 /* ---- host-side geometry (no device work) ---------------------------- */
 /* Synthetic meshes of the workloads: icosahedron subdivided `subdiv` times
