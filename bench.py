@@ -19,6 +19,7 @@ with rank K), timed with HIP events on the library's stream.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -73,6 +74,12 @@ def parse():
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=96.0)
     ap.add_argument("--no-sweep", action="store_true", help="skip the Poisson rate sweep of the serving legs")
+    ap.add_argument("--no-fp32-update", action="store_true",
+                    help="skip the extra factorisation with the FP32-MFMA trailing update")
+    ap.add_argument("--no-library", action="store_true",
+                    help="skip the vendor-GEMM comparison on the trailing-update shapes")
+    ap.add_argument("--library-gemm", type=int, nargs=2, metavar=("N", "NB"), default=None,
+                    help="(child process) time the vendor GEMM on the trailing-update shapes of an N-unknown LU")
     ap.add_argument("--served-passes", type=int, default=5,
                     help="saturated server passes over the request stream; the median is reported")
     return ap.parse_args()
@@ -339,8 +346,63 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5
     }
 
 
+def library_gemm_ceiling(n, nb):
+    """Runs library_gemm_child in a child process (torch carries its own HIP
+    runtime, which cannot initialise beside this process's) and returns its
+    result."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--library-gemm", str(n), str(nb)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"failed": r.stdout[-300:]}
+    return json.loads(lines[-1])
+
+
+def library_gemm_child(n, nb):
+    """The outer trailing-update products of one n-unknown LU (M = n - (i+1) nb
+    rows, 2 M real columns, K = 2 nb reals for i = 0, 1, ...) through the vendor
+    GEMM (torch.mm, hipBLASLt on ROCm), timed on torch's stream: bf16 operands
+    with the three split products stacked along K (one product with K = 6 nb,
+    FP32 output, no C read), and the plain FP32 product.  Returns the total ms
+    and the algorithmic TFLOP/s (8 M^2 nb per update, counted once) of each,
+    for comparison with the build's kernels on the same shapes."""
+    import torch
+    Ms = [n - (i + 1) * nb for i in range(n // nb + 1) if n - (i + 1) * nb > 0]
+    if not Ms:
+        return None
+    Kr, Mx = 2 * nb, Ms[0]
+    flops = sum(8.0 * M * M * nb for M in Ms)
+    dev = torch.device("cuda")
+    out = {"updates": len(Ms), "nb": nb, "shape_first": [Mx, 2 * Mx, Kr]}
+    for name, dt, kmul in (("bf16x3", torch.bfloat16, 3), ("fp32", torch.float32, 1)):
+        A = torch.randn(Mx, kmul * Kr, device=dev).to(dt)
+        B = torch.randn(2 * Mx, kmul * Kr, device=dev).to(dt)
+        kw = {"out_dtype": torch.float32} if dt == torch.bfloat16 else {}
+
+        def run():
+            for M in Ms:
+                torch.mm(A[:M], B[:2 * M].t(), **kw)
+        run()  # warm (library heuristics, allocator)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        tf = flops / (ms * 1e-3) / 1e12
+        peak = PEAK["bf16_mfma"] if dt == torch.bfloat16 else PEAK["f32_mfma"]
+        out[name] = {"ms": ms, "achieved": tf, "frac": tf / peak, "peak": peak}
+        del A, B
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     a = parse()
+    if a.library_gemm:
+        print(json.dumps(library_gemm_child(*a.library_gemm)))
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
@@ -440,6 +502,35 @@ def main():
                 "frac": gemm_tflops / PEAK["f32_mfma"], "traffic": None,
                 "avg_launch_ms": avg_launch_ms, "launches_per_step": g["launches_per_step"]}
 
+    # the same factorisation with the FP32-MFMA trailing update (k_gemm_f32),
+    # and the vendor GEMM on the same update shapes
+    nb_used = tuning["nb"] if tuning else int(os.environ.get("EMFA_LU_NB", "256") or 256)
+    fp32_update = None
+    if split and not a.no_fp32_update:
+        ctx.lu_update_mode(0)
+        hip.fill(ctx, mesh, Z, kind, k, quad, alpha=alpha)
+        ctx.sync()
+        ctx.prof_reset()
+        ctx.prof(True)
+        lu0 = hip.LU(ctx, Z, hip.F32)
+        ctx.sync()
+        ms0, cnt0, work0 = ctx.prof_get("lu_gemm")
+        ctx.prof(False)
+        lu0.close()
+        ctx.lu_update_mode(1)
+        tf0 = work0 / (ms0 * 1e-3) / 1e12 if ms0 > 0 else 0.0
+        fp32_update = {"bound": "mfma", "kernel": "lu_gemm (k_gemm_f32, V_MFMA_F32_32X32X2_F32), same LU with the "
+                                                  "FP32 trailing update",
+                       "achieved": tf0, "peak": PEAK["f32_mfma"], "unit": "TFLOP/s", "frac": tf0 / PEAK["f32_mfma"],
+                       "avg_launch_ms": ms0 / max(cnt0, 1), "launches": cnt0, "ms": ms0}
+    library = None
+    if not a.no_library:
+        try:
+            library = library_gemm_ceiling(n, nb_used)
+        except Exception as e:  # a comparison, never required
+            library = {"failed": str(e)[:200]}
+    roof["library_same_shapes"] = library
+
     extra = {}
     if not a.no_fp64:
         Z.close()
@@ -508,6 +599,7 @@ def main():
                        "min": min(step_s) * 1e3, "max": max(step_s) * 1e3,
                        "note": f"percentiles over the {a.steps} timed steps (one solve each, back to back)"},
         "roofline": roof,
+        "roofline_fp32_update": fp32_update,
         "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
         "lu_tuning": tuning,
         "refine": {"iterations": reports[-1][0], "relres_first": reports[-1][1], "relres_final": reports[-1][2],

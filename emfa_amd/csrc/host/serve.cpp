@@ -156,6 +156,10 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
         if (const char *e = std::getenv("EMFA_SERVE_BATCH_RHS")) batch_max_rhs_ = std::max(0, std::atoi(e));
         if (const char *e = std::getenv("EMFA_SERVE_FACTOR_BATCH")) factor_batch_max_ = std::max(0, std::atoi(e));
         if (env_flag("EMFA_SERVE_SPLIT", true)) emfa_lu_set_update_mode(c, 1);
+        largest_first_ = env_flag("EMFA_SERVE_LARGEST_FIRST", true);
+        if (const char *e = std::getenv("EMFA_SERVE_MAX_WAIT_MS")) max_wait_ms_ = std::max(0.0, std::atof(e));
+        // new FP32 factors (alone or batched) as replayed HIP graphs
+        if (env_flag("EMFA_SERVE_GRAPHS", false)) emfa_ctx_set_graphs(c, 1);
         ctxs_.push_back(c);
     }
     for (int w = 0; w < workers; ++w) threads_.emplace_back(&Server::worker, this, w);
@@ -314,6 +318,32 @@ void Server::worker(int w) {
             cv_work_.wait(l, [&] { return stop_ || !queue_.empty(); });
             if (stop_) return;
             std::vector<Job> &jobs = groups[0];
+            // the queued request with the most work first (a new dense factor
+            // costs ~n^3, a resident one ~n^2 per right-hand side), unless
+            // the oldest one has waited max_wait_ms_
+            const bool overdue = ms_between(queue_.front().arrival, clock_t_::now()) >= max_wait_ms_;
+            if (largest_first_ && !overdue) {
+                auto cost = [&](const Job &j) {
+                    const double n = spec_unknowns(j.req.mesh);
+                    const std::string k = factor_key(j.req, j.req.precision);
+                    const bool fresh = j.req.solver == DENSE && !cache_.count(k) && !building_.count(k);
+                    return fresh ? n * n * n : n * n * (double)(j.req.angles.size() / 3 + 1);
+                };
+                auto best = queue_.begin();
+                double bc = cost(*best);
+                for (auto it = std::next(queue_.begin()); it != queue_.end(); ++it) {
+                    const double c = cost(*it);
+                    if (c > bc) {
+                        bc = c;
+                        best = it;
+                    }
+                }
+                if (best != queue_.begin()) {
+                    Job j = std::move(*best);
+                    queue_.erase(best);
+                    queue_.push_front(std::move(j));
+                }
+            }
             jobs.push_back(std::move(queue_.front()));
             queue_.pop_front();
             int rhs = (int)(jobs[0].req.angles.size() / 3);

@@ -151,6 +151,10 @@ class Server {
     int busy_ = 0;
     int busy_workers_ = 0;   // workers running a request (mu_)
     bool dynamic_share_ = true;
+    // EMFA_SERVE_LARGEST_FIRST (default on): take the queued request with the
+    // most work, or the oldest once it has waited EMFA_SERVE_MAX_WAIT_MS
+    bool largest_first_ = true;
+    double max_wait_ms_ = 1000.0;
     void rebalance_locked();  // CU shares of all contexts from busy_workers_
     bool stop_ = false;
     size_t reserved_ = 0, peak_ = 0;
