@@ -86,6 +86,13 @@ struct emfa_mlfma {
     // lv[2] (no interpolation, every non-near box far); grid = boxes per axis
     bool single = false;
     int grid[3] = {0, 0, 0};
+    // sparse approximate inverse P (emfa_mlfma_precondition): CSR over the
+    // box-order rows; with it the operator is P Z and right-hand sides P B
+    long long *sai_ptr = nullptr;
+    int *sai_col = nullptr;
+    emfa::cf *sai_val = nullptr;
+    long long sai_nnz = 0;
+    double2 *ybox = nullptr;  // Z x in box order, before P
 };
 
 namespace emfa {
@@ -561,7 +568,8 @@ __global__ __launch_bounds__(256) void k_ml_down_sib(
 // receive + near part, back to RWG order: y[perm[i]] = yn[i] + sum_q rec[i][q] . down[leaf(i)][q]
 __global__ __launch_bounds__(256) void k_ml_out(const cf *__restrict__ rec, const cf *__restrict__ down,
                                                 const int *__restrict__ leaf_of, const int *__restrict__ perm,
-                                                const cf *__restrict__ yn, int n, int K, double2 *__restrict__ y) {
+                                                const cf *__restrict__ yn, int n, int K, double2 *__restrict__ y,
+                                                double2 *__restrict__ ybox) {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= n) return;
@@ -582,8 +590,42 @@ __global__ __launch_bounds__(256) void k_ml_out(const cf *__restrict__ rec, cons
     }
     if (lane == 0) {
         cf v = yn[i];
-        y[perm[i]] = make_double2((double)v.re + (double)sr, (double)v.im + (double)si);
+        const double2 o = make_double2((double)v.re + (double)sr, (double)v.im + (double)si);
+        if (ybox)
+            ybox[i] = o;
+        else
+            y[perm[i]] = o;
     }
+}
+
+// y[perm[i]] = sum_j P(i, j) x[j]: the sparse approximate inverse (CSR over
+// box-order rows, FP32 entries) applied to a box-order FP64 vector, one
+// wavefront per row, FP64 sums
+__global__ __launch_bounds__(256) void k_ml_sai(const long long *__restrict__ ptr, const int *__restrict__ col,
+                                                const cf *__restrict__ val, const double2 *__restrict__ x,
+                                                const int *__restrict__ perm, int n, double2 *__restrict__ y) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double sr = 0, si = 0;
+    for (long long j = ptr[row] + lane; j < ptr[row + 1]; j += 64) {
+        const cf a = val[j];
+        const double2 b = x[col[j]];
+        sr += (double)a.re * b.x - (double)a.im * b.y;
+        si += (double)a.re * b.y + (double)a.im * b.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    if (lane == 0) y[perm[row]] = make_double2(sr, si);
+}
+
+// box-order copy of an FP64 vector in RWG order
+__global__ __launch_bounds__(256) void k_ml_gather64(const double2 *__restrict__ x, const int *__restrict__ perm,
+                                                     int n, double2 *__restrict__ xb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) xb[i] = x[perm[i]];
 }
 
 // ------------------------------------------------------------------ host
@@ -1180,9 +1222,200 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
     }
     {
         ProfScope ps(ctx, "mlfma_out", 0);
-        k_ml_out<<<(n + 3) / 4, 256, 0, s>>>(M->rec, F.down, M->leaf_of, M->perm, M->yn, n, F.K, y);
+        k_ml_out<<<(n + 3) / 4, 256, 0, s>>>(M->rec, F.down, M->leaf_of, M->perm, M->yn, n, F.K, y,
+                                             M->sai_val ? M->ybox : nullptr);
+    }
+    if (M->sai_val) {
+        ProfScope ps(ctx, "mlfma_precond", (double)M->sai_nnz * 12);
+        k_ml_sai<<<(n + 3) / 4, 256, 0, s>>>(M->sai_ptr, M->sai_col, M->sai_val, M->ybox, M->perm, n, y);
     }
     EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+// out = P b (RWG order in and out) for a preconditioned operator
+static emfa_status ml_sai_rhs(emfa_ctx *ctx, emfa_mlfma *M, const double2 *b, double2 *out) {
+    const int n = M->n;
+    k_ml_gather64<<<(n + 255) / 256, 256, 0, ctx->stream>>>(b, M->perm, n, M->ybox);
+    k_ml_sai<<<(n + 3) / 4, 256, 0, ctx->stream>>>(M->sai_ptr, M->sai_col, M->sai_val, M->ybox, M->perm, n, out);
+    EMFA_HIP_CHECK(hipGetLastError());
+    return EMFA_OK;
+}
+
+// Householder QR of A (m x k, column-major, m >= k), then for each own row j in
+// [own, own + nu): x = R^-1 (Q^H e_j)[0:k], the least-squares solution of
+// A x = e_j, written to out (k x nu, column-major).  A zero column of R
+// gives zero coefficients for that column.
+static void sai_box(std::vector<std::complex<double>> &A, int m, int k, int own, int nu,
+                    std::vector<std::complex<double>> &out) {
+    using cd = std::complex<double>;
+    std::vector<cd> v((size_t)m * k, cd(0, 0)), rdiag(k);
+    std::vector<double> vn(k, 0.0);
+    for (int j = 0; j < k; ++j) {
+        cd *a = &A[(size_t)j * m];
+        double nx = 0;
+        for (int i = j; i < m; ++i) nx += std::norm(a[i]);
+        nx = std::sqrt(nx);
+        if (nx == 0) {
+            rdiag[j] = 0;
+            continue;
+        }
+        const cd ph = std::abs(a[j]) > 0 ? a[j] / std::abs(a[j]) : cd(1, 0);
+        const cd alpha = -ph * nx;
+        cd *vj = &v[(size_t)j * m];
+        for (int i = j; i < m; ++i) vj[i] = a[i];
+        vj[j] -= alpha;
+        double vv = 0;
+        for (int i = j; i < m; ++i) vv += std::norm(vj[i]);
+        vn[j] = vv;
+        // apply H = I - 2 v v^H / (v^H v) to the remaining columns
+        for (int c = j + 1; c < k; ++c) {
+            cd *ac = &A[(size_t)c * m];
+            cd d(0, 0);
+            for (int i = j; i < m; ++i) d += std::conj(vj[i]) * ac[i];
+            d *= 2.0 / vv;
+            for (int i = j; i < m; ++i) ac[i] -= d * vj[i];
+        }
+        rdiag[j] = alpha;
+    }
+    std::vector<cd> e(m);
+    out.assign((size_t)k * nu, cd(0, 0));
+    for (int u = 0; u < nu; ++u) {
+        std::fill(e.begin(), e.end(), cd(0, 0));
+        e[own + u] = cd(1, 0);
+        for (int j = 0; j < k; ++j) {  // Q^H e = H_k ... H_1 e
+            if (vn[j] == 0) continue;
+            const cd *vj = &v[(size_t)j * m];
+            cd d(0, 0);
+            for (int i = j; i < m; ++i) d += std::conj(vj[i]) * e[i];
+            d *= 2.0 / vn[j];
+            for (int i = j; i < m; ++i) e[i] -= d * vj[i];
+        }
+        cd *x = &out[(size_t)u * k];
+        for (int j = k - 1; j >= 0; --j) {  // R x = (Q^H e)[0:k]
+            cd t = e[j];
+            for (int c = j + 1; c < k; ++c) t -= A[(size_t)c * m + j] * x[c];
+            x[j] = rdiag[j] == cd(0, 0) ? cd(0, 0) : t / rdiag[j];
+        }
+    }
+}
+
+// MLFMA::calculatePreconditioner (MLFMA.cpp:714-741,1294-1391): thresholds in
+// leaf edges; distances of RWG edge midpoints from the leaf-box centre.
+static emfa_status ml_precondition(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *mesh, double row_threshold,
+                                   double col_threshold, long long *entries) {
+    using cd = std::complex<double>;
+    const int n = M->n;
+    MlLevel &F = M->lv[M->levels];
+    const double rt = row_threshold * F.a, ct = col_threshold * F.a;
+    hipStream_t s = ctx->stream;
+    std::vector<long long> rp(n + 1);
+    std::vector<int> col((size_t)M->nnz), perm(n);
+    std::vector<cf> val((size_t)M->nnz);
+    EMFA_HIP_CHECK(hipMemcpyAsync(rp.data(), M->row_ptr, rp.size() * sizeof(long long), hipMemcpyDeviceToHost, s));
+    EMFA_HIP_CHECK(hipMemcpyAsync(col.data(), M->col, col.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    EMFA_HIP_CHECK(hipMemcpyAsync(val.data(), M->val, val.size() * sizeof(cf), hipMemcpyDeviceToHost, s));
+    EMFA_HIP_CHECK(hipMemcpyAsync(perm.data(), M->perm, perm.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    EMFA_HIP_CHECK(hipStreamSynchronize(s));
+    const std::vector<double> &xyz = mesh->xyz_host;
+    const std::vector<int> &rwg = mesh->rwg_host;
+    std::vector<double> mid(3 * (size_t)n);  // box order
+    for (int i = 0; i < n; ++i) {
+        const int u = perm[i];
+        for (int c = 0; c < 3; ++c) mid[3 * i + c] = 0.5 * (xyz[3 * rwg[6 * u] + c] + xyz[3 * rwg[6 * u + 1] + c]);
+    }
+    struct Entry {
+        int row, col;
+        cf v;
+    };
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<Entry>> part(nth);
+    std::vector<int> bad(nth, -1);
+    {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t)
+            pool.emplace_back([&, t] {
+                std::vector<int> pos(n, -1), rows, cols;
+                std::vector<cd> A, res;
+                for (int b = t; b < F.nbox; b += nth) {
+                    rows.clear();
+                    cols.clear();
+                    int own = -1;
+                    const double *c0 = &F.cen[3 * b];
+                    for (int e = F.near_ptr[b]; e < F.near_ptr[b + 1]; ++e) {
+                        const int nb = F.near_idx[e];
+                        if (nb == b) own = (int)rows.size();
+                        for (int i = F.ubeg[nb]; i < F.uend[nb]; ++i) {
+                            const double dx = mid[3 * i] - c0[0], dy = mid[3 * i + 1] - c0[1], dz = mid[3 * i + 2] - c0[2];
+                            const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+                            if (d <= rt) rows.push_back(i);
+                            if (d <= ct) cols.push_back(i);
+                        }
+                    }
+                    const int m = (int)rows.size(), k = (int)cols.size(), nu = F.uend[b] - F.ubeg[b];
+                    // the box's own unknowns are the rows from `own` on
+                    if (own < 0 || k > m || own + nu > m || rows[own] != F.ubeg[b] || rows[own + nu - 1] != F.uend[b] - 1) {
+                        bad[t] = b;
+                        return;
+                    }
+                    A.assign((size_t)m * k, cd(0, 0));
+                    for (int c = 0; c < k; ++c) pos[cols[c]] = c;
+                    for (int f = 0; f < m; ++f)
+                        for (long long q = rp[rows[f]]; q < rp[rows[f] + 1]; ++q) {
+                            const int c = pos[col[q]];
+                            if (c >= 0) A[(size_t)c * m + f] = cd(val[q].re, val[q].im);
+                        }
+                    for (int c = 0; c < k; ++c) pos[cols[c]] = -1;
+                    sai_box(A, m, k, own, nu, res);
+                    for (int u = 0; u < nu; ++u)
+                        for (int c = 0; c < k; ++c) {
+                            const cd x = res[(size_t)u * k + c];
+                            part[t].push_back({cols[c], F.ubeg[b] + u, cf{(float)x.real(), (float)x.imag()}});
+                        }
+                }
+            });
+        for (auto &th : pool) th.join();
+    }
+    for (int t = 0; t < nth; ++t)
+        EMFA_REQUIRE(bad[t] < 0, EMFA_E_INVALID,
+                     "mlfma preconditioner: box %d: thresholds (row %.3g, col %.3g leaf edges) must keep the box's "
+                     "own unknowns among the rows and no more columns than rows",
+                     bad[t], row_threshold, col_threshold);
+    // CSR over rows, columns ascending
+    std::vector<long long> ptr(n + 1, 0);
+    for (auto &pp : part)
+        for (auto &e : pp) ++ptr[e.row + 1];
+    for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+    const long long nnz = ptr[n];
+    std::vector<int> pc((size_t)nnz);
+    std::vector<cf> pv((size_t)nnz);
+    {
+        std::vector<long long> fill(ptr.begin(), ptr.end() - 1);
+        for (auto &pp : part)
+            for (auto &e : pp) {
+                pc[(size_t)fill[e.row]] = e.col;
+                pv[(size_t)fill[e.row]] = e.v;
+                ++fill[e.row];
+            }
+        for (int i = 0; i < n; ++i) {
+            std::vector<std::pair<int, cf>> r;
+            for (long long q = ptr[i]; q < ptr[i + 1]; ++q) r.push_back({pc[q], pv[q]});
+            std::sort(r.begin(), r.end(), [](const std::pair<int, cf> &a, const std::pair<int, cf> &b) {
+                return a.first < b.first;
+            });
+            for (long long q = ptr[i]; q < ptr[i + 1]; ++q) {
+                pc[q] = r[q - ptr[i]].first;
+                pv[q] = r[q - ptr[i]].second;
+            }
+        }
+    }
+    emfa_status st;
+    if ((st = ml_upload(M, &M->sai_ptr, ptr))) return st;
+    if ((st = ml_upload(M, &M->sai_col, pc))) return st;
+    if ((st = ml_upload(M, &M->sai_val, pv))) return st;
+    if ((st = ml_alloc(M, &M->ybox, (size_t)n))) return st;
+    M->sai_nnz = nnz;
+    if (entries) *entries = nnz;
     return EMFA_OK;
 }
 
@@ -1278,18 +1511,77 @@ emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, e
     return EMFA_OK;
 }
 
+// With a preconditioner the system solved is P Z X = P B (MLFMA.cpp:406-407):
+// returns P B in a new matrix (null without one)
+static emfa_status ml_rhs(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat **Bp) {
+    *Bp = nullptr;
+    if (!m->sai_val) return EMFA_OK;
+    EMFA_REQUIRE(B->rows == m->n && B->prec == EMFA_F64, EMFA_E_INVALID, "mlfma: B must be F64 n x m");
+    const int n = m->n, nc = B->cols;
+    emfa_status st = emfa_mat_create(ctx, n, nc, EMFA_F64, Bp);
+    if (st) return st;
+    void *scr = nullptr;
+    if ((st = ctx_scratch(ctx, 2 * (size_t)n * sizeof(double2), &scr))) {
+        emfa_mat_destroy(*Bp);
+        *Bp = nullptr;
+        return st;
+    }
+    double2 *xin = (double2 *)scr, *yout = xin + n;
+    for (int c = 0; c < nc && !st; ++c) {
+        if (hipMemcpy2DAsync(xin, sizeof(double2), (const double2 *)B->d + c, nc * sizeof(double2), sizeof(double2),
+                             n, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            st = EMFA_E_HIP;
+        if (!st) st = ml_sai_rhs(ctx, m, xin, yout);
+        if (!st && hipMemcpy2DAsync((double2 *)(*Bp)->d + c, nc * sizeof(double2), yout, sizeof(double2),
+                                    sizeof(double2), n, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            st = EMFA_E_HIP;
+    }
+    if (st) {
+        (void)hipGetLastError();
+        emfa_mat_destroy(*Bp);
+        *Bp = nullptr;
+        set_error("mlfma: preconditioning the right-hand sides failed");
+    }
+    return st;
+}
+
 emfa_status emfa_mlfma_gmres(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, int restart, double tol,
                              int max_restart, emfa_gmres_report *rep) {
     EMFA_REQUIRE(ctx && m && B && X, EMFA_E_INVALID, "emfa_mlfma_gmres: null argument");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     auto op = [&](const double2 *in, double2 *out) -> emfa_status { return ml_apply(ctx, m, in, out); };
-    return gmres_impl(ctx, m->n, op, B, X, restart, tol, max_restart, rep);
+    emfa_mat *Bp = nullptr;
+    emfa_status st = ml_rhs(ctx, m, B, &Bp);
+    if (st) return st;
+    st = gmres_impl(ctx, m->n, op, Bp ? Bp : B, X, restart, tol, max_restart, rep);
+    if (Bp) emfa_mat_destroy(Bp);
+    return st;
 }
 
 emfa_status emfa_mlfma_bicgstab(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, double tol,
                                 int max_iter, int variant, emfa_bicgstab_report *rep) {
     EMFA_REQUIRE(ctx && m && B && X, EMFA_E_INVALID, "emfa_mlfma_bicgstab: null argument");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
     auto op = [&](const double2 *in, double2 *out) -> emfa_status { return ml_apply(ctx, m, in, out); };
-    return bicgstab_impl(ctx, m->n, op, B, X, tol, max_iter, variant, rep);
+    emfa_mat *Bp = nullptr;
+    emfa_status st = ml_rhs(ctx, m, B, &Bp);
+    if (st) return st;
+    st = bicgstab_impl(ctx, m->n, op, Bp ? Bp : B, X, tol, max_iter, variant, rep);
+    if (Bp) emfa_mat_destroy(Bp);
+    return st;
+}
+
+emfa_status emfa_mlfma_precondition(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mesh *mesh, double row_threshold,
+                                    double col_threshold, long long *entries) {
+    EMFA_REQUIRE(ctx && m && mesh, EMFA_E_INVALID, "emfa_mlfma_precondition: null argument");
+    EMFA_REQUIRE(!m->single, EMFA_E_INVALID, "emfa_mlfma_precondition: the single-level operator has none");
+    EMFA_REQUIRE(mesh->n == m->n && !mesh->rwg_host.empty(), EMFA_E_INVALID,
+                 "emfa_mlfma_precondition: not the operator's mesh");
+    EMFA_REQUIRE(row_threshold > 0 && col_threshold > 0, EMFA_E_INVALID,
+                 "emfa_mlfma_precondition: thresholds must be > 0");
+    EMFA_REQUIRE(!m->sai_val, EMFA_E_INVALID, "emfa_mlfma_precondition: the operator is preconditioned already");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    return ml_precondition(ctx, m, mesh, row_threshold, col_threshold, entries);
 }
 
 }  // extern "C"

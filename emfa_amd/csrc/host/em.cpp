@@ -448,6 +448,13 @@ void MLFMA::init(component::ConfigLoader *ploader) {
     max_iter_ = ploader->getInt("max_iteration_number:", 1000);
     box_threshold_ = ploader->getDouble("mlfma_box_threshold:", 0.5);
     precond_requested_ = ploader->getString("preconditioning:", "false") == "true";
+    if (precond_requested_) {
+        // MLFMA.cpp:44-45 reads both thresholds (ConfigLoader.cpp:632-665)
+        row_threshold_ = ploader->getDouble("row_threshold:", -1.0);
+        col_threshold_ = ploader->getDouble("col_threshold:", -1.0);
+        if (!(row_threshold_ > 0) || !(col_threshold_ > 0))
+            throw component::ConfigInvalid("MLFMA: preconditioning needs positive row_threshold: and col_threshold:");
+    }
     if (!(iter_threshold_ > 0) || max_iter_ < 1 || !(box_threshold_ > 0))
         throw component::ConfigInvalid("MLFMA: iteration_threshold, max_iteration_number and mlfma_box_threshold "
                                        "must be positive");
@@ -464,13 +471,27 @@ void MLFMA::reportInfo(std::ostream &strm) const {
          << "  -> box threshold: " << box_threshold_ << " wavelengths\n"
          << "  -> " << (solver_ == LinearSolver::GMRES ? "GMRES(" + std::to_string(gmres_m_) + ")" : "BICGSTAB")
          << " to " << iter_threshold_ << ", at most " << max_iter_ << " iterations\n"
-         << "  -> preconditioning: " << (precond_requested_ ? "requested, not built (unpreconditioned)" : "off")
+         << "  -> preconditioning: "
+         << (precond_requested_ ? "sparse approximate inverse, row/col thresholds " + std::to_string(row_threshold_) +
+                                      " / " + std::to_string(col_threshold_) + " leaf edges"
+                                : std::string("off"))
          << '\n';
 }
 
 emfa_status MLFMA::createOperator(emfa_mlfma **op) {
     emfa_mlfma_params mp{EMFA_CFIE, k_, alpha_, box_threshold_, cfie_mode_};
-    return emfa_mlfma_create(ctx_, dmesh_, &mp, op);
+    emfa_status st = emfa_mlfma_create(ctx_, dmesh_, &mp, op);
+    if (st || !precond_requested_) return st;
+    // MLFMA::solve calls calculatePreconditioner when preconditioning: true
+    long long entries = 0;
+    st = emfa_mlfma_precondition(ctx_, *op, dmesh_, row_threshold_, col_threshold_, &entries);
+    if (st) {
+        emfa_mlfma_destroy(*op);
+        *op = nullptr;
+        return st;
+    }
+    logger_ << "  -> Number of precondition: " << entries << '\n';
+    return EMFA_OK;
 }
 
 void MLFMA::logOperator(emfa_mlfma *op) {

@@ -182,6 +182,7 @@ class MLFMA : public DenseMoM {
     virtual void logOperator(emfa_mlfma *op);
     virtual const char *operatorName() const { return "MLFMA"; }
     double alpha_ = 0.5, box_threshold_ = 0.5, iter_threshold_ = 1e-3;
+    double row_threshold_ = 0.9, col_threshold_ = 0.87;  // leaf edges (preconditioner)
     int max_iter_ = 1000;
     bool precond_requested_ = false;
 };

@@ -28,6 +28,7 @@ EXPORTS = [
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
     "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab", "emfa_bicgstab_noalloc",
     "emfa_mlfma_boxes", "emfa_fmm_create", "emfa_ctx_set_graphs", "emfa_ctx_graph_stats",
+    "emfa_mlfma_precondition",
 ]
 
 _vp = ctypes.c_void_p
@@ -441,6 +442,15 @@ class MLFMA:
 
     def matvec(self, X, Y):
         _check(lib().emfa_mlfma_matvec(self.ctx.h, self.h, X.h, Y.h))
+
+    def precondition(self, mesh, row_threshold=0.9, col_threshold=0.87):
+        """Sparse approximate inverse (emfa_mlfma_precondition); the operator
+        becomes P Z and the solvers precondition their right-hand sides.
+        Returns P's nonzero count."""
+        e = ctypes.c_longlong()
+        _check(lib().emfa_mlfma_precondition(self.ctx.h, self.h, mesh.h, ctypes.c_double(row_threshold),
+                                             ctypes.c_double(col_threshold), ctypes.byref(e)))
+        return e.value
 
     def bicgstab(self, B, X, tol=1e-4, max_iter=500, variant=0):
         rep = BicgstabReport()

@@ -326,10 +326,23 @@ emfa_status emfa_mlfma_gmres(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, em
                              int max_restart, emfa_gmres_report *rep);
 
 /* BiCGSTAB on the MLFMA operator (MLFMA::iterativeSolve, MLFMA.cpp:402-452,
- * runs variant 0 of emfa_bicgstab; the sparse-approximate-inverse
- * preconditioner of MLFMA.cpp:1294-1391 is not built); X is overwritten. */
+ * runs variant 0 of emfa_bicgstab); X is overwritten.  With a
+ * preconditioner (emfa_mlfma_precondition) this and emfa_mlfma_gmres solve
+ * P Z X = P B and report residuals relative to ||P B||, as the reference
+ * does. */
 emfa_status emfa_mlfma_bicgstab(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *B, emfa_mat *X, double tol,
                                 int max_iter, int variant, emfa_bicgstab_report *rep);
+/* Sparse approximate inverse of MLFMA::calculatePreconditioner
+ * (MLFMA.cpp:714-741,1294-1391; `preconditioning: true`): for each leaf box,
+ * the near entries between the unknowns of its near boxes whose RWG edge
+ * midpoints lie within row_threshold / col_threshold leaf edges of its
+ * centre (`row_threshold:`, `col_threshold:`), solved in the least-squares
+ * sense by Householder QR (FP64, on the host's threads) for each of the
+ * box's own unknowns; P is kept on the device (FP32, CSR).  Afterwards
+ * emfa_mlfma_matvec returns P Z X (MLFMA.cpp:506-508).  `entries` (may be
+ * null) receives P's nonzero count.  The MLFMA operator only. */
+emfa_status emfa_mlfma_precondition(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mesh *mesh, double row_threshold,
+                                    double col_threshold, long long *entries);
 
 /* When enabled, each launch of a named kernel family is bracketed by HIP
  * events on the context stream; totals are read with emfa_prof_get.

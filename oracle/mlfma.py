@@ -20,6 +20,12 @@ Steps, each following the reference routine named beside it:
   coefficients      MLFMA.cpp:382-400
   matvec            MLFMA.cpp:454-686 (leaf aggregation, upward pass,
                     translation, downward pass, receive, + near matrix)
+  preconditioner    MLFMA.cpp:714-741,1294-1391 (sparse approximate inverse:
+                    per leaf box, the near entries between the near-box
+                    unknowns within row_threshold and col_threshold leaf
+                    edges of its centre, least squares by QR, one column of
+                    P per own unknown; the operator becomes P Z, the
+                    right-hand side P V, MLFMA.cpp:406-407,506-508)
 Unknowns are kept in the original RWG order at the interface; `perm` gives
 the reference's box order.
 """
@@ -260,3 +266,37 @@ class Tree:
             u = b["unk"]
             y[u] = np.einsum("ukc,kc,k->u", rec[u], down[lv][bi], coeff)
         return y
+
+
+def sai(boxes, mid, Znear, row_threshold, col_threshold, box_len):
+    """Sparse approximate inverse of MLFMA::calculatePreconditioner
+    (MLFMA.cpp:1294-1391) as a dense n x n matrix in RWG order.  `boxes` are
+    leaf boxes (dicts with center, unk, near: indices into boxes) in box
+    order, `mid` the RWG edge midpoints, Znear the near matrix (RWG order,
+    zero outside the near pattern); the thresholds are in leaf edges.  For
+    leaf box b: rows / cols = unknowns of its near boxes (near-list order)
+    whose midpoints lie within row / col threshold of b's centre; A =
+    Znear[rows][:, cols]; A = Q R (economy); column u of P (u an unknown of
+    b) holds R^-1 Q[own row of u, :]^H at rows cols."""
+    n = len(mid)
+    P = np.zeros((n, n), dtype=complex)
+    rt, ct = row_threshold * box_len, col_threshold * box_len
+    for b in boxes:
+        rows, cols, own = [], [], None
+        for nb in b["near"]:
+            nbx = boxes[nb]
+            if nbx is b:
+                own = len(rows)
+            for u in nbx["unk"]:
+                d = np.linalg.norm(mid[u] - b["center"])
+                if d <= rt:
+                    rows.append(u)
+                if d <= ct:
+                    cols.append(u)
+        A = Znear[np.ix_(rows, cols)]
+        Q, R = np.linalg.qr(A)
+        nu = len(b["unk"])
+        res = np.linalg.solve(R, Q[own:own + nu, :].conj().T)
+        for j, u in enumerate(b["unk"]):
+            P[cols, u] = res[:, j]
+    return P

@@ -101,3 +101,50 @@ def test_far_interactions_of_separated_boxes_match_dense_entries():
         errs.append(abs(zm - zd) / abs(zd))
     # RWG supports reach past their leaf boxes, so a few per cent at worst
     assert max(errs) < 3e-2 and np.median(errs) < 1e-2
+
+
+def test_sparse_approximate_inverse_columns_are_least_squares_solutions():
+    """MLFMA::calculatePreconditioner (MLFMA.cpp:1294-1391) restated with QR:
+    each column of P equals the least-squares solution (SVD-based lstsq) of
+    A m = e_u over the box's rows and columns, and the row / column sets
+    follow the thresholds."""
+    xyz, tri, rwg = sphere(2, 0.5)
+    f, alpha = 1.2e9, 0.5
+    T = OM.Tree(xyz, tri, rwg, f, 0.25)
+    lv = T.level
+    boxes = T.boxes[lv]
+    n = len(rwg)
+    Z = O.fill_cfie(xyz, tri, rwg, T.k, alpha, 3, 0)
+    rows, cols = T.near_pairs()
+    Zn = np.zeros_like(Z)
+    Zn[rows, cols] = Z[rows, cols]
+    mid = (xyz[rwg[:, 0]] + xyz[rwg[:, 1]]) / 2
+    a = T.blen[lv]
+    P = OM.sai(boxes, mid, Zn, 0.9, 0.87, a)
+    checked = 0
+    for b in boxes[::3]:
+        near_u = [u for nb in b["near"] for u in boxes[nb]["unk"]]
+        d = np.linalg.norm(mid[near_u] - b["center"], axis=1)
+        r = [u for u, x in zip(near_u, d) if x <= 0.9 * a]
+        c = [u for u, x in zip(near_u, d) if x <= 0.87 * a]
+        A = Zn[np.ix_(r, c)]
+        for u in b["unk"]:
+            e = np.zeros(len(r), dtype=complex)
+            e[r.index(u)] = 1
+            m = np.linalg.lstsq(A, e, rcond=None)[0]
+            assert np.abs(P[c, u] - m).max() <= 1e-8 * max(1.0, np.abs(m).max())
+            # nothing outside the column set
+            outside = np.setdiff1d(np.arange(n), c)
+            assert np.all(P[outside, u] == 0)
+            checked += 1
+    assert checked > 20
+    # a transposed instead of conjugated Q (a plausible slip) is not the least-squares solution
+    b = boxes[0]
+    near_u = [u for nb in b["near"] for u in boxes[nb]["unk"]]
+    d = np.linalg.norm(mid[near_u] - b["center"], axis=1)
+    r = [u for u, x in zip(near_u, d) if x <= 0.9 * a]
+    c = [u for u, x in zip(near_u, d) if x <= 0.87 * a]
+    Q, R = np.linalg.qr(Zn[np.ix_(r, c)])
+    own = r.index(b["unk"][0])
+    bad = np.linalg.solve(R, Q[own, :].T)
+    assert np.abs(bad - P[c, b["unk"][0]]).max() > 1e-3 * np.abs(P[c, b["unk"][0]]).max()

@@ -194,6 +194,26 @@ def test_mlfma_solver_files_near_dense_cfie(tmp_path):
     assert np.linalg.norm(p - q) / np.linalg.norm(q) < 5e-2
 
 
+def test_mlfma_preconditioned_solver_files(tmp_path):
+    # preconditioning: true with the shipped thresholds: same RCS as the
+    # unpreconditioned operator (dense CFIE check above), fewer iterations
+    xyz, tri = _run(tmp_path, "MLFMA", "BistaticRCS", freq="8e8", radius=0.5, subdiv=4, tto=180, tdelta=10,
+                    extra="    iteration_threshold: 1e-4\n    max_iteration_number: 600\n"
+                          "    mlfma_box_threshold: 0.25\n    preconditioning: true\n"
+                          "    row_threshold: 0.9\n    col_threshold: 0.87\n")
+    out = tmp_path / "sphere_MLFMA_CFIE_0.5"
+    log = open(out / "sca_runtime.log").read()
+    assert "Number of precondition" in log and "cannot converge" not in log
+    rwg, _ = O.build_rwg(tri)
+    k = O.wavenumber(8e8)
+    Z = O.fill_cfie(xyz, tri, rwg, k, 0.5, 3, 0)
+    X = np.linalg.solve(Z, O.rhs(xyz, tri, rwg, 1, k, 0.5, O.inc_table([(0, 0, 0)])))
+    th, val = _read_rcs(out / "bistatic_rcs_phi_0.dat")
+    ref = O.rcs(xyz, tri, rwg, k, X, _dirs(th, 0))
+    lit = ref > ref.max() - 15
+    assert np.abs(val - ref)[lit].max() <= 1.0
+
+
 def test_fmm_solver_files_near_dense_cfie(tmp_path):
     # EMSolver: FMM (single level, fmm_box_length: 0.4 wavelengths) on the
     # 1920-unknown sphere at 600 MHz, BiCGSTAB as FMM.cpp:222-268
