@@ -38,7 +38,7 @@ PEAK = {"f32_mfma": 157.3, "f64_mfma": 78.6, "bf16_mfma": 2500.0, "hbm": 8000.0}
 # workload (nb 512), from the FETCH_SIZE / WRITE_SIZE counter runs recorded in
 # profiles/lu_gemm_split_cfie_sphere_30720.md (fetch scaled by 2.0 after a 1 GiB
 # copy calibration), beside the algorithmic bytes of the same launches
-TRAFFIC_SPLIT_30720 = {"measured": 4.718e9, "algorithmic": 2.644e9}
+TRAFFIC_SPLIT_30720 = {"measured": 4.752e9, "algorithmic": 2.644e9}
 
 CONFIGS = {
     # name: (kind, subdiv, radius, freq, alpha, quad, nrhs-angles)
@@ -315,6 +315,8 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5
         sweep.append({"rate_x_capacity": f, "arrival_rate_per_s": f * cap, "p50_ms": st_r["p50_ms"],
                       "p99_ms": st_r["p99_ms"], "mean_ms": st_r["mean_ms"], "solves_per_s": st_r["solves_per_s"]})
     fb = max(int(sum(r["fp64_fallback"] for r in out)) for out, _ in runs)
+    errors = sorted({r["error"] for out, _ in runs for r in out if r.get("status", 0) != 0} |
+                    {r["error"] for r in lat if r.get("status", 0) != 0})
     ns = [workload.unknowns(r) for r in reqs]
     pl = np.sort(plain_lat)
     return {
@@ -324,6 +326,7 @@ def stream_leg(reqs, workers, budget, latency_rate=0.5, precision=None, passes=5
         "served_solves_per_s": cap, "served_passes_solves_per_s": per_pass,
         "served_rhs_per_s": st["rhs_per_s"], "served_wall_s": st["wall_s"],
         "served_cache_hits": st["cache_hits"], "served_failed": max(st_["failed"] for _, st_ in runs),
+        "served_errors": errors[:5],
         "served_fp64_fallbacks": fb,
         "served_pivot_fallbacks": int(st.get("pivot_fallbacks", 0)),
         "served_batched_requests": int(st.get("batched", 0)),
@@ -481,13 +484,13 @@ def main():
         fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
     ctx.prof(False)
     # the dominant kernel: the trailing update of the outer panels -- bf16-split
-    # planes (k_gemm_bf3p, bf16 MFMA) when the tuner chose them, else FP32 MFMA
+    # planes (k_gemm_bf3r, bf16 MFMA) when the tuner chose them, else FP32 MFMA
     split = fams["lu_gemm_split"]["launches_per_step"] > 0
     g = fams["lu_gemm_split" if split else "lu_gemm"]
     gemm_tflops = g["work_per_step"] / (g["ms_per_step"] * 1e-3) / 1e12 if g["ms_per_step"] > 0 else 0.0
     avg_launch_ms = g["ms_per_step"] / max(g["launches_per_step"], 1)
     if split:
-        roof = {"bound": "mfma", "kernel": "lu_gemm_split (k_gemm_bf3p, V_MFMA_F32_32X32X16_BF16, 3 products)",
+        roof = {"bound": "mfma", "kernel": "lu_gemm_split (k_gemm_bf3r, V_MFMA_F32_32X32X16_BF16, 3 products)",
                 "achieved": gemm_tflops, "peak": PEAK["bf16_mfma"], "unit": "TFLOP/s",
                 "frac": gemm_tflops / PEAK["bf16_mfma"],
                 "traffic": TRAFFIC_SPLIT_30720["measured"] if n == 30720 else None,

@@ -53,7 +53,7 @@ struct Bf3Planes {
     int alpha_neg, beta;
     int max_wgs = 0;     // > 0: at most this many workgroups (rounded down to a multiple of 8), looping over tiles
     int persistent = 0;  // set by the launcher
-    int kernel = 0;      // 1: k_gemm_bf3p, 2: k_gemm_bf3q, 0: EMFA_BF3_KERNEL (q) or k_gemm_bf3p
+    int kernel = 0;      // 1: k_gemm_bf3p, 2: k_gemm_bf3q, 4: k_gemm_bf3r, 0: EMFA_BF3_KERNEL (p, q) or bf3r
     Batch bt;            // C in the batch's matrices, the planes in its scratch slabs
 };
 

@@ -585,6 +585,160 @@ __global__ __launch_bounds__(512) void k_gemm_bf3p(Bf3Planes g) {
 }
 
 // ---------------------------------------------------------------------------
+// The same product with register-staged operand loads and K = 32 per stage.
+// Workgroup tile 256 x 256, 8 waves of 64 x 128 as in k_gemm_bf3p; two LDS
+// buffers of 4 planes x 256 rows x 32 K (128 KB).  Each thread loads eight
+// 16-byte chunks per stage (two per plane; four threads cover the 64 bytes of
+// a row's stage) into registers with ordinary global loads, and stores them
+// to the other LDS buffer after multiplying the current one; one barrier per
+// stage publishes the stores and frees the buffer just read.  A row's stage
+// holds 4 chunks; chunk c sits at position c ^ ((r >> 2) & 3), so the
+// sixteen rows one ds_read_b128 cycle serves cover all 64 banks.  The K = 16
+// halves of a stage are multiplied in order with bf3p's product order, so the
+// bits equal k_gemm_bf3p's.  Needs Kp % 32 == 0.
+// ---------------------------------------------------------------------------
+namespace bf3r {
+constexpr int BM = 256, BN = 256, BK = 32, PLANE = 256 * BK;
+}
+
+__device__ __forceinline__ int bf3r_off(int r, int c) { return r * bf3r::BK + ((c ^ ((r >> 2) & 3)) << 3); }
+
+__global__ __launch_bounds__(512) void k_gemm_bf3r(Bf3Planes g) {
+    using namespace bf3r;
+    __shared__ __attribute__((aligned(16))) __bf16 sm[2][4][PLANE];  // buffer, plane (Ah Al Bh Bl)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int TX = (g.Nr + BN - 1) / BN, TY = (g.M + BM - 1) / BM, TT = TX * TY;
+    const int t0 = g.persistent ? (int)blockIdx.x : (int)((blockIdx.z * TY + blockIdx.y) * gridDim.x + blockIdx.x);
+    const int tstep = g.persistent ? (int)gridDim.x : TT * g.bt.count;
+    const int r32 = lane & 31, kg = lane >> 5;
+    // this thread's chunks: q = i * 512 + tid, row q >> 2, chunk q & 3 (same in every plane)
+    const int q0 = tid, q1 = 512 + tid;
+    const int oa0 = (q0 >> 2) * (int)g.lda + 8 * (q0 & 3), oa1 = (q1 >> 2) * (int)g.lda + 8 * (q1 & 3);
+    const int ob0 = (q0 >> 2) * (int)g.ldb + 8 * (q0 & 3), ob1 = (q1 >> 2) * (int)g.ldb + 8 * (q1 & 3);
+    const int ol0 = bf3r_off(q0 >> 2, q0 & 3), ol1 = bf3r_off(q1 >> 2, q1 & 3);
+    for (int tl = t0; tl < TT * g.bt.count; tl += tstep) {
+        if (tl != t0) __syncthreads();  // the previous tile's LDS reads are done
+        const int zb = tl / TT;
+        int ty, tx;
+        tile_of_linear(tl - zb * TT, TX, TY, &ty, &tx);
+        const long long bm = (long long)ty * BM, bn = (long long)tx * BN;
+        const int nk = g.Kp / BK;
+        const __bf16 *pah = batch_scr(g.bt, g.Ah, zb) + bm * g.lda, *pal = batch_scr(g.bt, g.Al, zb) + bm * g.lda;
+        const __bf16 *pbh = batch_scr(g.bt, g.Bh, zb) + bn * g.ldb, *pbl = batch_scr(g.bt, g.Bl, zb) + bn * g.ldb;
+        // staging registers (named, so that they stay in registers)
+        uint4 s00, s01, s10, s11, s20, s21, s30, s31;
+#define BF3R_GLOAD(kt)                                            \
+    do {                                                          \
+        const int k0_ = (kt) * BK;                                \
+        s00 = *(const uint4 *)(pah + k0_ + oa0);                  \
+        s01 = *(const uint4 *)(pah + k0_ + oa1);                  \
+        s10 = *(const uint4 *)(pal + k0_ + oa0);                  \
+        s11 = *(const uint4 *)(pal + k0_ + oa1);                  \
+        s20 = *(const uint4 *)(pbh + k0_ + ob0);                  \
+        s21 = *(const uint4 *)(pbh + k0_ + ob1);                  \
+        s30 = *(const uint4 *)(pbl + k0_ + ob0);                  \
+        s31 = *(const uint4 *)(pbl + k0_ + ob1);                  \
+    } while (0)
+#define BF3R_LSTORE(b)                                            \
+    do {                                                          \
+        *(uint4 *)&sm[(b)][0][ol0] = s00;                         \
+        *(uint4 *)&sm[(b)][0][ol1] = s01;                         \
+        *(uint4 *)&sm[(b)][1][ol0] = s10;                         \
+        *(uint4 *)&sm[(b)][1][ol1] = s11;                         \
+        *(uint4 *)&sm[(b)][2][ol0] = s20;                         \
+        *(uint4 *)&sm[(b)][2][ol1] = s21;                         \
+        *(uint4 *)&sm[(b)][3][ol0] = s30;                         \
+        *(uint4 *)&sm[(b)][3][ol1] = s31;                         \
+    } while (0)
+        floatx16 acc[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+        BF3R_GLOAD(0);
+        BF3R_LSTORE(0);
+        if (nk > 1) BF3R_GLOAD(1);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const int buf = kt & 1;
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                bf16x8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    const int o = bf3r_off(wm * 64 + mt * 32 + r32, 2 * kk + kg);
+                    ah[mt] = *(const bf16x8 *)&sm[buf][0][o];
+                    al[mt] = *(const bf16x8 *)&sm[buf][1][o];
+                }
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const int o = bf3r_off(wn * 128 + nt * 32 + r32, 2 * kk + kg);
+                    bh[nt] = *(const bf16x8 *)&sm[buf][2][o];
+                    bl[nt] = *(const bf16x8 *)&sm[buf][3][o];
+                }
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bl[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+            }
+            if (kt + 1 < nk) {
+                BF3R_LSTORE(buf ^ 1);  // last read in step kt - 1, before the previous barrier
+                if (kt + 2 < nk) BF3R_GLOAD(kt + 2);
+            }
+            __syncthreads();
+        }
+#undef BF3R_GLOAD
+#undef BF3R_LSTORE
+
+        const float sgn = g.alpha_neg ? -1.f : 1.f;
+        float *__restrict__ C =
+            batch_mat(g.bt, g.C, zb) + (bm + wm * 64 + 4 * (lane >> 5)) * g.ldc + bn + wn * 128 + r32;
+        const int mrem = (int)min((long long)BM, g.M - bm) - wm * 64 - 4 * (lane >> 5);
+        const int nrem = (int)min((long long)BN, (long long)g.Nr - bn) - wn * 128 - r32;
+        const bool full = bm + BM <= g.M && bn + BN <= g.Nr;
+        auto epilogue = [&](auto guarded) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                float cv[4][16];
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int i = mt * 32 + 8 * (v >> 2) + (v & 3), j = nt * 32;
+                        const bool in = !guarded.value || (i < mrem && j < nrem);
+                        cv[nt][v] = (g.beta && in) ? C[i * g.ldc + j] : 0.f;
+                    }
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int i = mt * 32 + 8 * (v >> 2) + (v & 3), j = nt * 32;
+                        if (!guarded.value || (i < mrem && j < nrem)) C[i * g.ldc + j] = cv[nt][v] + sgn * acc[mt][nt][v];
+                    }
+            }
+        };
+        if (full)
+            epilogue(std::false_type{});
+        else
+            epilogue(std::true_type{});
+    }  // tiles
+}
+
+// ---------------------------------------------------------------------------
 // The same product with one wave per SIMD and register-double-buffered
 // operand fragments.  Workgroup tile 256 x 256, 4 waves as 2 x 2 with wave
 // tiles 128 x 128 (4 x 4 MFMA 32x32x16 tiles: 256 accumulators per lane,
@@ -968,18 +1122,25 @@ emfa_status gemm_launch_bf3_planes(hipStream_t s, const Bf3Planes &g) {
     const int TX = (g.Nr + bf3p::BN - 1) / bf3p::BN, TY = (g.M + bf3p::BM - 1) / bf3p::BM;
     EMFA_REQUIRE(g.bt.count >= 1 && g.bt.count <= 65535, EMFA_E_INVALID, "gemm bf16x3 planes: batch %d", g.bt.count);
     Bf3Planes gg = g;
-    // EMFA_BF3_KERNEL=q selects the one-wave-per-SIMD kernel (same bits)
-    static const bool default_p = [] {
+    // the register-staged kernel when K is a multiple of 32, else the
+    // direct-load one; EMFA_BF3_KERNEL=p / q select the direct-load or the
+    // one-wave-per-SIMD kernel (all give the same bits)
+    static const int env_kernel = [] {
         const char *e = getenv("EMFA_BF3_KERNEL");
-        return !(e && e[0] == 'q');
+        return e && e[0] == 'q' ? 2 : e && e[0] == 'p' ? 1 : 4;
     }();
-    const bool use_p = g.kernel == 1 || (g.kernel == 0 && default_p);
-    const int threads = use_p ? 512 : 256;
+    int kern = g.kernel ? g.kernel : env_kernel;
+    if (kern == 4 && g.Kp % bf3r::BK != 0) kern = 1;
+    EMFA_REQUIRE(kern == 4 ? (g.lda <= (1LL << 23) && g.ldb <= (1LL << 23)) : true, EMFA_E_INVALID,
+                 "gemm bf16x3 planes: plane stride too large");
+    const int threads = kern == 2 ? 256 : 512;
     auto launch = [&](dim3 grid) {
-        if (use_p)
-            k_gemm_bf3p<<<grid, threads, 0, s>>>(gg);
-        else
+        if (kern == 2)
             k_gemm_bf3q<<<grid, threads, 0, s>>>(gg);
+        else if (kern == 4)
+            k_gemm_bf3r<<<grid, threads, 0, s>>>(gg);
+        else
+            k_gemm_bf3p<<<grid, threads, 0, s>>>(gg);
     };
     if (g.max_wgs >= 8 && (long long)TX * TY * g.bt.count > g.max_wgs) {
         gg.persistent = 1;
