@@ -279,6 +279,8 @@ emfa_status emfa_ctx_destroy(emfa_ctx *ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto &g : ctx->lu_graph) (void)hipGraphExecDestroy(g.second);
     ctx->lu_graph.clear();
+    if (ctx->cap_main) (void)hipStreamDestroy(ctx->cap_main);
+    if (ctx->cap_aux) (void)hipStreamDestroy(ctx->cap_aux);
     if (ctx->bmat) ctx_free(ctx, ctx->bmat, ctx->bmat_cap * sizeof(void *));
     if (ctx->scratch) ctx_free(ctx, ctx->scratch, ctx->scratch_bytes);
     ctx_free(ctx, ctx->flags, 64);

@@ -120,6 +120,7 @@ struct emfa_ctx {
     void *bmat = nullptr;
     size_t bmat_cap = 0;  // entries
     long long graph_captures = 0, graph_launches = 0;
+    hipStream_t cap_main = nullptr, cap_aux = nullptr;  // streams the graphs are captured on
 };
 
 struct emfa_mesh {

@@ -1968,10 +1968,26 @@ static bool coop_launch(hipStream_t s, const void *fn, int G, void **args, size_
 static emfa_status lu_graph_run(emfa_ctx *ctx, const std::string &key, const std::function<emfa_status()> &body) {
     auto it = ctx->lu_graph.find(key);
     if (it == ctx->lu_graph.end()) {
-        EMFA_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        emfa_status st = body();
+        // Captured on two streams of its own: a capture may not depend on
+        // work outside it, and the device pool leaves waits on other
+        // contexts' events on the context's streams when it hands out their
+        // freed blocks.  Relaxed mode: other threads' contexts allocate and
+        // launch meanwhile.
+        if (!ctx->cap_main) {
+            EMFA_HIP_CHECK(hipStreamCreateWithFlags(&ctx->cap_main, hipStreamNonBlocking));
+            EMFA_HIP_CHECK(hipStreamCreateWithFlags(&ctx->cap_aux, hipStreamNonBlocking));
+        }
+        std::swap(ctx->stream, ctx->cap_main);
+        std::swap(ctx->aux, ctx->cap_aux);
+        emfa_status st = EMFA_OK;
         hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            st = body();
+            e = hipStreamEndCapture(ctx->stream, &g);
+        }
+        std::swap(ctx->stream, ctx->cap_main);
+        std::swap(ctx->aux, ctx->cap_aux);
         if (st || e != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
             (void)hipGetLastError();
@@ -2263,10 +2279,12 @@ static emfa_status lu_impl(emfa_ctx *ctx, emfa_mat *Am, int *ipiv, int *info_out
     if (batched && ctx->lu_graphs && !ctx->prof) {
         // everything the launch sequence depends on; the matrices themselves
         // are read through the batch's device array of bases
+        // (the CU share enters only as shared / alone: the workgroup caps it
+        // sets are baked in at capture, and any cap gives the same results)
         char key[320];
         std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %zu %p %d", N, nb, plan.w,
                       ctx->batch.count, G_max, (int)split_update, (int)nopiv, (int)xchg, (int)lookahead, (int)coop,
-                      (int)ctx->cu_share, ctx->num_cu, scr, slab, (const void *)ctx->batch.mat, panel_rows_min);
+                      (int)(ctx->cu_share > 0), ctx->num_cu, scr, slab, (const void *)ctx->batch.mat, panel_rows_min);
         st = lu_graph_run(ctx, key, body);
     } else {
         st = body();
