@@ -38,7 +38,7 @@ PEAK = {"f32_mfma": 157.3, "f64_mfma": 78.6, "bf16_mfma": 2500.0, "hbm": 8000.0}
 # workload (nb 512), from the FETCH_SIZE / WRITE_SIZE counter runs recorded in
 # profiles/lu_gemm_split_cfie_sphere_30720.md (fetch scaled by 2.0 after a 1 GiB
 # copy calibration), beside the algorithmic bytes of the same launches
-TRAFFIC_SPLIT_30720 = {"measured": 4.752e9, "algorithmic": 2.644e9}
+TRAFFIC_SPLIT_30720 = {"measured": 4.681e9, "algorithmic": 2.644e9}
 
 CONFIGS = {
     # name: (kind, subdiv, radius, freq, alpha, quad, nrhs-angles)
