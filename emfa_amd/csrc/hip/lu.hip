@@ -2563,9 +2563,18 @@ emfa_status gemm_residual(emfa_ctx *ctx, const emfa_mat *A, const emfa_mat *X, e
     const int N = A->rows, K = A->cols, m = X->cols;
     EMFA_REQUIRE(X->rows == K && C->rows == N && C->cols == m && X->prec == EMFA_F64 && C->prec == EMFA_F64,
                  EMFA_E_INVALID, "gemm_residual: shape/precision mismatch");
-    int skinny_max = kSkinnyMaxCols;
-    if (const char *e = getenv("EMFA_SKINNY_MAX")) skinny_max = std::min(kSkinnyMaxCols, atoi(e));
-    if (m <= skinny_max) {
+    // k_smallmm on the VALU up to 64 columns, the FP64 MFMA GEMM above.
+    // Contexts that share the device take the GEMM from 17 columns: the
+    // served 64-request stream (1045-14981 unknowns, 1-64 excitations per
+    // request) measured 116.3 against 107.2 solves/s (median of 3 passes),
+    // the VALU residual having held 18 % of the CUs' time.
+    static const char *resid_env = [] {
+        const char *e = getenv("EMFA_RESID_SKINNY_MAX");
+        return e ? e : getenv("EMFA_SKINNY_MAX");
+    }();
+    const int resid_skinny_max =
+        resid_env ? std::min(kSkinnyMaxCols, atoi(resid_env)) : ctx->cu_share > 0 ? 16 : kSkinnyMaxCols;
+    if (m <= resid_skinny_max) {
         // row blocks x K slices (summed in slice order) on the VALU
         const int mc = m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
         // tall residuals with wide column tiles: 8 rows per lane (X re-read 4x less)
