@@ -1,5 +1,9 @@
-// Host-side geometry for the synthetic workloads and the RWG builder.
+// Host-side geometry for the synthetic workloads, the RWG builder and the
+// half-RWG (contour edge) builder of the IP-SIE solver.
+#include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -120,20 +124,18 @@ extern "C" emfa_status emfa_geom_plate(int cells, double size, double *xyz, int 
     return EMFA_OK;
 }
 
-// RWG list with EMComponent/CommonEdge.cpp:704-743 semantics in O(E):
+// Edge list with EMComponent/CommonEdge.cpp:704-743 semantics in O(E):
 // triangles in order; candidate edges (A,B),(B,C),(C,A); a candidate matches
 // every still-unmatched stored edge that is its reverse; unmatched candidates
-// are appended with vxp = opposite vertex, tp = triangle.  Output keeps the
-// stored order of the matched edges (CommonEdge.cpp:86-95).
-extern "C" emfa_status emfa_rwg_build(int nt, const int *tri, int *rwg, int cap, int *n, int *n_boundary) {
-    if (nt <= 0 || !tri || !n) {
-        emfa::set_error("emfa_rwg_build: bad arguments");
-        return EMFA_E_INVALID;
-    }
-    struct E {
-        int v1, v2, vp, vm, tp, tm;
-    };
-    std::vector<E> e;
+// are appended with vxp = opposite vertex, tp = triangle.  Boundary edges
+// stay in the list with tm = -1.
+namespace {
+struct Edge {
+    int v1, v2, vp, vm, tp, tm;
+};
+
+void common_edges(int nt, const int *tri, std::vector<Edge> &e) {
+    e.clear();
     e.reserve(3 * (size_t)nt / 2 + 16);
     std::unordered_map<long long, std::vector<int>> open;  // directed (v1,v2) -> unmatched stored edges
     open.reserve(3 * (size_t)nt);
@@ -158,6 +160,18 @@ extern "C" emfa_status emfa_rwg_build(int nt, const int *tri, int *rwg, int cap,
             open[key(cand[c][0], cand[c][1])].push_back((int)e.size() - 1);
         }
     }
+}
+}  // namespace
+
+// RWG list: the shared edges of common_edges in stored order
+// (CommonEdge.cpp:86-95).
+extern "C" emfa_status emfa_rwg_build(int nt, const int *tri, int *rwg, int cap, int *n, int *n_boundary) {
+    if (nt <= 0 || !tri || !n) {
+        emfa::set_error("emfa_rwg_build: bad arguments");
+        return EMFA_E_INVALID;
+    }
+    std::vector<Edge> e;
+    common_edges(nt, tri, e);
     int cnt = 0, nb = 0;
     for (auto &x : e) {
         if (x.tm == -1) {
@@ -181,5 +195,173 @@ extern "C" emfa_status emfa_rwg_build(int nt, const int *tri, int *rwg, int cap,
     }
     *n = cnt;
     if (n_boundary) *n_boundary = nb;
+    return EMFA_OK;
+}
+
+// Half-RWG list of the IP-SIE solver (EMComponent/ContourEdge.cpp:15-95,
+// buildContourEdge), in FP32 vertex arithmetic as the reference's VectorR3:
+//   1. every shared edge of common_edges, in order, gives two conformal
+//      half-RWGs, (v1, v2, vp, tp) and (v2, v1, vm, tm), edge number e++;
+//   2. a boundary edge whose end points coincide (coordinates, reversed) with
+//      those of a LATER boundary edge gives the conformal pair (itself, the
+//      first such later edge); a boundary edge without a later partner goes
+//      on to step 3 -- the later partner of a pair included, as the
+//      reference leaves it in its list;
+//   3. a step-3 edge (v1, v2) gets a contour line from every other step-3
+//      edge (v3, v4) running against it over an overlap: v1v2.(v3 - v1) > 0,
+//      v1v2.(v4 - v2) < 0, and the cosines of (v4 - v1) and (v3 - v2) with
+//      v1v2 above 0.95 in modulus (1 where the points coincide); the line
+//      runs from (cos(v4 - v1) < 0 ? v1 : v4) to (cos(v3 - v2) < 0 ? v3 : v2)
+//      and belongs to the triangle of (v3, v4).  Edges with at least one line
+//      are the nonconformal half-RWGs, numbered from 0.
+// The pairing of step 2 is found through a hash of the end-point coordinates
+// (the first later partner, as the reference's forward scan finds it); step 3
+// keeps the reference's all-pairs scan, whose order the line lists follow.
+namespace {
+#pragma clang fp contract(off)
+struct F3 {
+    float x, y, z;
+};
+inline F3 fsub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float fdot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float fnorm(F3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+inline bool feq(F3 a, F3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+}  // namespace
+
+extern "C" emfa_status emfa_contour_build(int nv, const double *xyz, int nt, const int *tri, int *hrwg,
+                                          double *length, int cap, int *n, int *line_ptr, int *line_tri,
+                                          double *line_seg, int line_cap, int *n_nc, int *n_lines) {
+    if (nv <= 0 || nt <= 0 || !xyz || !tri || !n || !n_nc || !n_lines) {
+        emfa::set_error("emfa_contour_build: bad arguments");
+        return EMFA_E_INVALID;
+    }
+    for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+        if (tri[i] < 0 || tri[i] >= nv) {
+            emfa::set_error("emfa_contour_build: triangle %zu names vertex %d of %d", i / 3, tri[i], nv);
+            return EMFA_E_INVALID;
+        }
+    std::vector<F3> v(nv);
+    for (int i = 0; i < nv; ++i) v[i] = {(float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2]};
+    std::vector<Edge> e;
+    common_edges(nt, tri, e);
+    std::vector<float> elen(e.size());
+    for (size_t i = 0; i < e.size(); ++i) elen[i] = fnorm(fsub(v[e[i].v2], v[e[i].v1]));
+    struct H {
+        int v1, v2, vx, t, idx, type;
+        float len;
+    };
+    std::vector<H> out;
+    out.reserve(3 * (size_t)nt);
+    std::vector<int> left;  // boundary edges in list order
+    int no = 0;
+    for (size_t i = 0; i < e.size(); ++i) {
+        const Edge &x = e[i];
+        if (x.tm != -1) {
+            out.push_back({x.v1, x.v2, x.vp, x.tp, no, 0, elen[i]});
+            out.push_back({x.v2, x.v1, x.vm, x.tm, no, 0, elen[i]});
+            ++no;
+        } else {
+            left.push_back((int)i);
+        }
+    }
+    // step 2: coordinates of (v1, v2) -> positions in `left` (ascending)
+    auto bits = [](float f) {
+        f += 0.0f;  // -0 and +0 compare equal
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return (uint64_t)u;
+    };
+    struct KeyHash {
+        size_t operator()(const std::array<uint64_t, 6> &k) const {
+            uint64_t h = 1469598103934665603ull;
+            for (uint64_t x : k) h = (h ^ x) * 1099511628211ull;
+            return (size_t)h;
+        }
+    };
+    auto key = [&](F3 a, F3 b) {
+        return std::array<uint64_t, 6>{bits(a.x), bits(a.y), bits(a.z), bits(b.x), bits(b.y), bits(b.z)};
+    };
+    std::unordered_map<std::array<uint64_t, 6>, std::vector<int>, KeyHash> pos;
+    pos.reserve(2 * left.size() + 1);
+    for (size_t p = 0; p < left.size(); ++p) {
+        const Edge &x = e[left[p]];
+        pos[key(v[x.v1], v[x.v2])].push_back((int)p);
+    }
+    std::vector<int> rest;
+    for (size_t p = 0; p < left.size(); ++p) {
+        const Edge &c = e[left[p]];
+        auto it = pos.find(key(v[c.v2], v[c.v1]));
+        int q = -1;
+        if (it != pos.end()) {
+            auto nx = std::upper_bound(it->second.begin(), it->second.end(), (int)p);
+            if (nx != it->second.end()) q = *nx;
+        }
+        if (q >= 0) {
+            const Edge &d = e[left[q]];
+            out.push_back({c.v1, c.v2, c.vp, c.tp, no, 0, elen[left[p]]});
+            out.push_back({d.v1, d.v2, d.vp, d.tp, no, 0, elen[left[q]]});
+            ++no;
+        } else {
+            rest.push_back(left[p]);
+        }
+    }
+    // step 3
+    const double threshold = 0.95f;
+    std::vector<int> lptr{0}, ltri;
+    std::vector<float> lseg;
+    int nc = 0;
+    for (size_t a = 0; a < rest.size(); ++a) {
+        const Edge &c = e[rest[a]];
+        const F3 v1 = v[c.v1], v2 = v[c.v2], v1v2 = fsub(v2, v1);
+        const double nv1v2 = fnorm(v1v2);
+        const size_t before = ltri.size();
+        for (size_t b = 0; b < rest.size(); ++b) {
+            if (b == a) continue;
+            const Edge &d = e[rest[b]];
+            const F3 v3 = v[d.v1], v4 = v[d.v2];
+            const F3 t1 = fsub(v4, v1), t2 = fsub(v3, v1), t3 = fsub(v4, v2), t4 = fsub(v3, v2);
+            if (fdot(v1v2, t2) <= 0 || fdot(v1v2, t3) >= 0) continue;
+            const double t1s = feq(v1, v4) ? 1.0 : fdot(v1v2, t1) / (nv1v2 * fnorm(t1));
+            const double t4s = feq(v2, v3) ? 1.0 : fdot(v1v2, t4) / (nv1v2 * fnorm(t4));
+            if (std::fabs(t1s) > threshold && std::fabs(t4s) > threshold) {
+                const F3 st = t1s < 0 ? v1 : v4, en = t4s < 0 ? v3 : v2;
+                ltri.push_back(d.tp);
+                lseg.insert(lseg.end(), {st.x, st.y, st.z, en.x, en.y, en.z});
+            }
+        }
+        if (ltri.size() > before) {
+            out.push_back({c.v1, c.v2, c.vp, c.tp, nc, 1, elen[rest[a]]});
+            ++nc;
+            lptr.push_back((int)ltri.size());
+        }
+    }
+    *n = (int)out.size();
+    *n_nc = nc;
+    *n_lines = (int)ltri.size();
+    if (hrwg || length) {
+        if (!hrwg || !length || cap < *n) {
+            emfa::set_error("emfa_contour_build: capacity %d for %d half-RWGs", cap, *n);
+            return EMFA_E_INVALID;
+        }
+        for (size_t i = 0; i < out.size(); ++i) {
+            int *o = hrwg + 6 * i;
+            o[0] = out[i].v1;
+            o[1] = out[i].v2;
+            o[2] = out[i].vx;
+            o[3] = out[i].t;
+            o[4] = out[i].idx;
+            o[5] = out[i].type;
+            length[i] = out[i].len;
+        }
+    }
+    if (line_ptr || line_tri || line_seg) {
+        if (!line_ptr || !line_tri || !line_seg || line_cap < *n_lines) {
+            emfa::set_error("emfa_contour_build: line capacity %d for %d lines", line_cap, *n_lines);
+            return EMFA_E_INVALID;
+        }
+        std::memcpy(line_ptr, lptr.data(), lptr.size() * sizeof(int));
+        for (size_t i = 0; i < ltri.size(); ++i) line_tri[i] = ltri[i];
+        for (size_t i = 0; i < lseg.size(); ++i) line_seg[i] = lseg[i];
+    }
     return EMFA_OK;
 }

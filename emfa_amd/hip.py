@@ -19,7 +19,7 @@ EFIE, CFIE = 0, 1
 # every symbol include/emfa_hip.h declares
 EXPORTS = [
     "emfa_last_error", "emfa_version", "emfa_ctx_create", "emfa_ctx_destroy", "emfa_ctx_sync", "emfa_ctx_stream",
-    "emfa_ctx_bytes_in_use", "emfa_ctx_cu_count", "emfa_ctx_set_cu_share", "emfa_ctx_set_panel_cap", "emfa_geom_icosphere", "emfa_geom_plate", "emfa_rwg_build", "emfa_mesh_upload",
+    "emfa_ctx_bytes_in_use", "emfa_ctx_cu_count", "emfa_ctx_set_cu_share", "emfa_ctx_set_panel_cap", "emfa_geom_icosphere", "emfa_geom_plate", "emfa_rwg_build", "emfa_contour_build", "emfa_mesh_upload",
     "emfa_mesh_destroy", "emfa_mesh_unknowns", "emfa_mat_create", "emfa_mat_destroy", "emfa_mat_upload",
     "emfa_mat_download", "emfa_mat_copy", "emfa_mat_rows", "emfa_mat_cols", "emfa_mat_prec", "emfa_mat_device_ptr",
     "emfa_fill", "emfa_rhs_fill", "emfa_lu_factor", "emfa_lu_factor_batch", "emfa_lu_destroy", "emfa_lu_solve", "emfa_lu_pivots",
@@ -144,6 +144,45 @@ def rwg_build(tri):
     out = np.zeros((max(n.value, 1), 6), dtype=np.int32)
     _check(lib().emfa_rwg_build(tri.shape[0], _pi(tri), _pi(out), n.value, ctypes.byref(n), ctypes.byref(nb)))
     return out[:n.value].copy(), nb.value
+
+
+class Contour:
+    """Half-RWG list of the IP-SIE solver (emfa_contour_build; the
+    reference's ContourEdge with policy::HALF_RWG).  hrwg: n x 6 (v1, v2, vx,
+    triangle, edge number, type 0/1), length: n, and the contour lines of
+    the nonconformal half-RWGs as CSR (line_ptr, line_tri, line_seg n_lines x 6)."""
+
+    def __init__(self, hrwg, length, line_ptr, line_tri, line_seg):
+        self.hrwg, self.length = hrwg, length
+        self.line_ptr, self.line_tri, self.line_seg = line_ptr, line_tri, line_seg
+
+    @property
+    def n(self):
+        return self.hrwg.shape[0]
+
+    def line_length(self, i):
+        """FP32 lengths of the contour lines of nonconformal edge i."""
+        seg = self.line_seg[self.line_ptr[i]:self.line_ptr[i + 1]].astype(np.float32)
+        d = seg[:, 3:] - seg[:, :3]
+        return np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+
+
+def contour_build(xyz, tri):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tri = np.ascontiguousarray(tri, dtype=np.int32)
+    n, nnc, nl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L = lib()
+    _check(L.emfa_contour_build(xyz.shape[0], _pd(xyz), tri.shape[0], _pi(tri), None, None, 0, ctypes.byref(n),
+                                None, None, None, 0, ctypes.byref(nnc), ctypes.byref(nl)))
+    h = np.zeros((max(n.value, 1), 6), dtype=np.int32)
+    ln = np.zeros(max(n.value, 1))
+    ptr = np.zeros(nnc.value + 1, dtype=np.int32)
+    lt = np.zeros(max(nl.value, 1), dtype=np.int32)
+    seg = np.zeros((max(nl.value, 1), 6))
+    _check(L.emfa_contour_build(xyz.shape[0], _pd(xyz), tri.shape[0], _pi(tri), _pi(h), _pd(ln), h.shape[0],
+                                ctypes.byref(n), _pi(ptr), _pi(lt), _pd(seg), lt.shape[0], ctypes.byref(nnc),
+                                ctypes.byref(nl)))
+    return Contour(h[:n.value].copy(), ln[:n.value].copy(), ptr, lt[:nl.value].copy(), seg[:nl.value].copy())
 
 
 def wavenumber(freq):

@@ -97,6 +97,23 @@ emfa_status emfa_geom_plate(int cells, double size, double *xyz, int nv_cap, int
  * 704-743: first-appearance order of shared edges, v1->v2 as in tp),
  * built in O(E) with a hash map.  Returns the RWG count in *n. */
 emfa_status emfa_rwg_build(int nt, const int *tri, int *rwg, int cap, int *n, int *n_boundary);
+/* Half-RWG list of the IP-SIE solver (FastWT): replaces
+ * component::ContourEdge::buildEdge(mesh, policy::HALF_RWG)
+ * (EMComponent/ContourEdge.cpp:15-95; the edge list of :216-252).  Conformal
+ * pairs from shared edges and from boundary edges whose end points coincide
+ * with a later boundary edge's (reversed), then the nonconformal half-RWGs:
+ * boundary edges overlapped by anti-parallel, nearly collinear (cosines >
+ * 0.95) boundary edges, each with its contour lines, all in FP32 vertex
+ * arithmetic as the reference's.  Outputs (pass NULL to get the counts only):
+ *   hrwg     cap x 6 ints per half-RWG: v1, v2, vx (free vertex), triangle,
+ *            edge number (conformal pair / nonconformal index), type (0
+ *            conformal, 1 nonconformal); length[cap]: FP32 edge length;
+ *   line_ptr (n_nc + 1) over the nonconformal half-RWGs, line_tri[line_cap]
+ *            (the triangle of the overlapping edge), line_seg[line_cap x 6]
+ *            (start xyz, end xyz), as ContourEdge::line_list_. */
+emfa_status emfa_contour_build(int nv, const double *xyz, int nt, const int *tri, int *hrwg, double *length,
+                               int cap, int *n, int *line_ptr, int *line_tri, double *line_seg, int line_cap,
+                               int *n_nc, int *n_lines);
 
 /* ---- mesh upload ---------------------------------------------------- */
 emfa_status emfa_mesh_upload(emfa_ctx *ctx, const double *xyz, int nv, const int *tri, int nt, const int *rwg, int n,

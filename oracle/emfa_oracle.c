@@ -505,6 +505,36 @@ void orc_fill_cfie(const double *xyz, const int *tri, int n, const int *rwg, dou
     }
 }
 
+/* Surface part of the IP-SIE near entry between half-RWGs,
+ * EMSolver/FastWT.cpp:753-762: z(f,s) = l_f l_s K(f,s), K the CFIE
+ * triangle-pair kernel with both signs +1 (cfieZSingular when the two
+ * half-RWGs share their triangle).  hrwg: n x 6 ints (v1, v2, vx, triangle,
+ * ...), len[n] the edge lengths as the reference holds them. */
+void orc_fill_hrwg(const double *xyz, const int *tri, int n, const int *hrwg, const double *len, double k,
+                   double alpha, int q, int mode, int row0, int row1, double *Zout)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int f = row0; f < row1; ++f) {
+        const int *hf = hrwg + 6 * f;
+        v3 tf[3];
+        tri_vtx(xyz, tri, hf[3], tf);
+        v3 nf = tri_normal(tf[0], tf[1], tf[2]);
+        v3 vf = vtx(xyz, hf[2]);
+        for (int s = 0; s < n; ++s) {
+            const int *hs = hrwg + 6 * s;
+            v3 ts[3];
+            tri_vtx(xyz, tri, hs[3], ts);
+            v3 vs = vtx(xyz, hs[2]);
+            cplx zz = hf[3] == hs[3] ? cfie_singular(q, k, alpha, tf, vf, vs, 1.0, 1.0, mode)
+                                     : cfie_regular(q, k, alpha, tf, ts, vf, vs, nf, 1.0, 1.0);
+            cplx z = len[f] * len[s] * zz;
+            size_t o = 2 * ((size_t)(f - row0) * n + s);
+            Zout[o] = creal(z);
+            Zout[o + 1] = cimag(z);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Excitation.  EFIE: EMSolver/EFIE.cpp:568-601 + 698-722 (7-pt rule). */
 /* CFIE: EMSolver/Miscellaneous.cpp:152-189 + CFIE.cpp:570-597 (3-pt). */

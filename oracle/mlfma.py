@@ -232,13 +232,15 @@ class Tree:
         return np.array(rows), np.array(cols)
 
     def far_matvec(self, x, rad, rec):
-        """Far-field part of Z x (original RWG order)."""
+        """Far-field part of Z x (original RWG order); the patterns may carry
+        any number of channels (3 vector components, a fourth scalar one)."""
         lv, k = self.level, self.k
         boxes = self.boxes
+        nc = rad.shape[2]
         up = {lv: np.array([np.einsum("u,ukc->kc", x[b["unk"]], rad[b["unk"]]) for b in boxes[lv]])}
         for l in range(lv - 1, 1, -1):
             K = len(self.kh[l])
-            out = np.zeros((len(boxes[l]), K, 3), dtype=complex)
+            out = np.zeros((len(boxes[l]), K, nc), dtype=complex)
             for ci, c in enumerate(boxes[l + 1]):
                 p = c["parent"]
                 r = boxes[l][p]["center"] - c["center"]
@@ -248,7 +250,7 @@ class Tree:
         down = {}
         for l in range(2, lv + 1):
             K = len(self.kh[l])
-            dn = np.zeros((len(boxes[l]), K, 3), dtype=complex)
+            dn = np.zeros((len(boxes[l]), K, nc), dtype=complex)
             for bi, b in enumerate(boxes[l]):
                 for fi in b["far"]:
                     T = translation(k, b["center"] - boxes[l][fi]["center"], self.kh[l], self.L[l])

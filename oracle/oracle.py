@@ -35,6 +35,8 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, _d]
         L.orc_fill_cfie.argtypes = [_d, _i, ctypes.c_int, _i, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
+        L.orc_fill_hrwg.argtypes = [_d, _i, ctypes.c_int, _i, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
         L.orc_rhs.argtypes = [_d, _i, ctypes.c_int, _i, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, _d, _d]
         L.orc_rcs.argtypes = [_d, _i, ctypes.c_int, _i, ctypes.c_double, _d, ctypes.c_int, ctypes.c_int, _d, _i, _d]
@@ -147,6 +149,21 @@ def fill_cfie(xyz, tri, rwg, k, alpha, quad=3, mode=0, rows=None):
     Z = np.zeros((r1 - r0, n), dtype=np.complex128)
     lib().orc_fill_cfie(_pd(xyz), _pi(tri), n, _pi(rwg), float(k), float(alpha), int(quad), int(mode), r0, r1,
                         Z.ctypes.data_as(_d))
+    return Z
+
+
+def fill_hrwg(xyz, tri, hrwg, length, k, alpha, quad=3, mode=0, rows=None):
+    """Surface part l_f l_s K(f, s) of the IP-SIE near entries between
+    half-RWGs (EMSolver/FastWT.cpp:753-762 with Miscellaneous.cpp:6-77)."""
+    xyz = _c(xyz, np.float64)
+    tri = _c(tri, np.int32)
+    hrwg = _c(hrwg, np.int32)
+    length = _c(length, np.float64)
+    n = hrwg.shape[0]
+    r0, r1 = (0, n) if rows is None else rows
+    Z = np.zeros((r1 - r0, n), dtype=np.complex128)
+    lib().orc_fill_hrwg(_pd(xyz), _pi(tri), n, _pi(hrwg), _pd(length), float(k), float(alpha), int(quad), int(mode),
+                        r0, r1, Z.ctypes.data_as(_d))
     return Z
 
 

@@ -1,4 +1,4 @@
-# Builds oracle/_ref/emfa_ref: the reference's own EFIE/CFIE sources, compiled
+# Builds oracle/_ref/emfa_ref: the reference's own EFIE/CFIE/contour-edge sources, compiled
 # where they lie under /root/reference (never copied), plus oracle/ref_driver.cpp.
 # Test infrastructure only (golden vectors); the product never links it.
 #   make -C oracle -f ref.mk
@@ -15,7 +15,7 @@ CXXFLAGS := -std=c++14 -O2 -fpermissive -w -DARMA_DONT_USE_LAPACK -DARMA_DONT_US
 
 SRCS := EMComponent/Quadrature.cpp EMComponent/Mesh.cpp EMComponent/CommonEdge.cpp EMComponent/ConfigLoader.cpp \
   EMComponent/Triangle.cpp EMComponent/Tetrahedron.cpp EMComponent/Result.cpp EMComponent/ThreadPool.cpp \
-  EMComponent/tools.cpp EMIterative/iml.cpp EMSolver/EM.cpp EMSolver/EFIE.cpp EMSolver/CFIE.cpp \
+  EMComponent/tools.cpp EMComponent/ContourEdge.cpp EMIterative/iml.cpp EMSolver/EM.cpp EMSolver/EFIE.cpp EMSolver/CFIE.cpp \
   EMSolver/Miscellaneous.cpp EMSolver/MFIE.cpp
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.cpp=.o)))
 

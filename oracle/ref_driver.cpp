@@ -19,6 +19,16 @@
 //        emfa_ref time cfie|efie mesh.nas freq alpha
 //        emfa_ref mesh file.nas|.rpt|.stl out.bin
 //        emfa_ref formats dir theta_from theta_to theta_delta phi_from phi_to phi_delta
+//        emfa_ref contour mesh.nas freq alpha out.bin
+// The contour form builds the half-RWG list of the IP-SIE solver with the
+// reference's ContourEdge::buildEdge(mesh, HALF_RWG) (EMComponent/
+// ContourEdge.cpp:15-95) and writes it with the contour lines of every
+// nonconformal edge and, for every pair of half-RWGs, the surface part of
+// the near entry that EMSolver/FastWT.cpp:753-762 forms from the reference's
+// integral::cfieZKernel / cfieZSingular (Miscellaneous.cpp:6-77); FastWT.cpp
+// itself does not compile with this toolchain (its size_t location matrices
+// meet Armadillo's 64-bit uword as unsigned long long), so its own loop is
+// restated here around the reference's kernels.
 // The third form loads a mesh with Mesh::loadMeshFile (EMComponent/Mesh.cpp:
 // 22-49, any of the three formats) and writes what the reference holds:
 // int32 nv, nt; float xyz[nv*3]; per triangle int32 ids[3] and float
@@ -41,7 +51,9 @@
 #include "iml.h"
 #define private protected
 #include "MFIE.h"
+#include "ContourEdge.h"
 #undef private
+#include "Miscellaneous.h"
 
 using component::VectorR3;
 
@@ -408,8 +420,87 @@ int dump_mesh(const char *file, const char *out) {
     return 0;
 }
 
+
+// IP-SIE contour edges (ContourEdge keeps its lists private; `private` is read
+// as `protected` above, which leaves the class layout unchanged).
+class RefContour : public component::ContourEdge {
+  public:
+    const std::vector<HRWG> &hrwgs() const { return ce_list_; }
+    const std::vector<ContourLine> &lines() const { return line_list_; }
+};
+
+// layout: int32 nv, nt, n, nl; float xyz[nv*3]; int32 tri[nt*3];
+//         per half-RWG int32 v1, v2, vx, tid, idx, type and float length;
+//         per nonconformal edge int32 count, then count x (int32 tri, float start[3], end[3]);
+//         float zfs[n*n*2] (row = field half-RWG, column = source)
+int dump_contour(const char *file, float freq, float alpha, const char *out) {
+    auto mesh = std::make_shared<component::Mesh>();
+    mesh->loadMeshFile(file);
+    auto ce = std::make_shared<RefContour>();
+    ce->buildEdge(mesh, policy::HALF_RWG);
+    const auto &h = ce->hrwgs();
+    const auto &ln = ce->lines();
+    FILE *f = std::fopen(out, "wb");
+    if (!f) return 3;
+    int hdr[4] = {(int)mesh->getNodeNum(), (int)mesh->getTriangleNum(), (int)h.size(), (int)ln.size()};
+    std::fwrite(hdr, sizeof(int), 4, f);
+    for (int i = 0; i < hdr[0]; ++i) {
+        const VectorR3 &v = mesh->getVertex(i);
+        float x[3] = {v.x, v.y, v.z};
+        std::fwrite(x, sizeof(float), 3, f);
+    }
+    for (int t = 0; t < hdr[1]; ++t) {
+        int ids[3];
+        mesh->getTriangleRef(t).getVertex(ids[0], ids[1], ids[2]);
+        std::fwrite(ids, sizeof(int), 3, f);
+    }
+    for (const auto &e : h) {
+        int r[6] = {e.v1, e.v2, e.vx, e.tid, e.idx, (int)e.type};
+        std::fwrite(r, sizeof(int), 6, f);
+        std::fwrite(&e.length, sizeof(float), 1, f);
+    }
+    for (const auto &lst : ln) {
+        int c = (int)lst.size();
+        std::fwrite(&c, sizeof(int), 1, f);
+        for (const auto &p : lst) {
+            float x[6] = {p.second.start.x, p.second.start.y, p.second.start.z,
+                          p.second.end.x,   p.second.end.y,   p.second.end.z};
+            std::fwrite(&p.first, sizeof(int), 1, f);
+            std::fwrite(x, sizeof(float), 6, f);
+        }
+    }
+    // k as EMSolver/FastWT.cpp:50-51 computes it
+    auto wave_length = component::cc / freq;
+    value_t k = component::PI2 / wave_length;
+    const int n = (int)h.size();
+    std::vector<float> z((size_t)n * n * 2);
+    for (int fi = 0; fi < n; ++fi) {
+        const auto &fh = h[fi];
+        const auto &ft = mesh->getTriangleRef(fh.tid);
+        VectorR3 vf3[3];
+        ft.getVertex(vf3[0], vf3[1], vf3[2]);
+        const VectorR3 vfx = mesh->getVertex(fh.vx), nml = ft.getNormal();
+        for (int si = 0; si < n; ++si) {
+            const auto &sh = h[si];
+            VectorR3 vs3[3];
+            mesh->getTriangleRef(sh.tid).getVertex(vs3[0], vs3[1], vs3[2]);
+            const VectorR3 vsx = mesh->getVertex(sh.vx);
+            Complex zfs = sh.tid == fh.tid
+                              ? sh.length * fh.length * mom::integral::cfieZSingular(alpha, k, vf3, vs3, vfx, vsx, 1, 1)
+                              : sh.length * fh.length * mom::integral::cfieZKernel(alpha, k, vf3, vs3, vfx, vsx, nml, 1, 1);
+            z[2 * ((size_t)fi * n + si)] = zfs.real();
+            z[2 * ((size_t)fi * n + si) + 1] = zfs.imag();
+        }
+    }
+    std::fwrite(z.data(), sizeof(float), z.size(), f);
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc == 4 && std::string(argv[1]) == "mesh") return dump_mesh(argv[2], argv[3]);
+    if (argc == 6 && std::string(argv[1]) == "contour")
+        return dump_contour(argv[2], std::strtof(argv[3], nullptr), std::strtof(argv[4], nullptr), argv[5]);
     if (argc == 9 && std::string(argv[1]) == "formats")
         return write_formats(argv[2], std::strtof(argv[3], nullptr), std::strtof(argv[4], nullptr),
                              std::strtof(argv[5], nullptr), std::strtof(argv[6], nullptr),

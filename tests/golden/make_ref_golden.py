@@ -7,7 +7,11 @@ rows, V, the LU solution I, the RCS cut, and the per-triangle current
 magnitudes of CFIE::calculateSurfaceCurrent computed from that I; for
 CFIE and MFIE also the reference's BiCGSTAB solutions of its own Z, V).
 
-  python tests/golden/make_ref_golden.py
+The ref_contour_*.npz fixtures hold the reference's half-RWG lists of the
+IP-SIE solver with their contour lines and the surface part of every
+half-RWG pair's near entry.
+
+  python tests/golden/make_ref_golden.py [contour]
 """
 import os
 import subprocess
@@ -121,6 +125,56 @@ def read_out(path):
     return xyz, tri, rwg, c(Z), c(V), c(I), th, rcs, cur, bicg
 
 
+# name: (mesh builder, freq, alpha, rows of the half-RWG surface entries kept)
+CONTOUR_CASES = {
+    "sphere_s1": (lambda: meshgen.icosphere(1, 0.5), 3e8, 0.5, 64),
+    "cube_232322": (lambda: meshgen.cube_faces([2, 3, 2, 3, 2, 2], 1.0), 3e8, 0.5, None),
+    "plate_pair_34": (lambda: meshgen.plate_pair(3, 4, 1.0), 4e8, 0.3, None),
+}
+
+
+def read_contour(path):
+    with open(path, "rb") as f:
+        nv, nt, n, nl = np.frombuffer(f.read(16), dtype=np.int32)
+        xyz = np.frombuffer(f.read(12 * nv), dtype=np.float32).reshape(nv, 3)
+        tri = np.frombuffer(f.read(12 * nt), dtype=np.int32).reshape(nt, 3)
+        rec = np.frombuffer(f.read(28 * n), dtype=np.uint8).reshape(n, 28)
+        hrwg = rec[:, :24].copy().view(np.int32).reshape(n, 6)
+        length = rec[:, 24:].copy().view(np.float32).reshape(n)
+        ptr, ltri, seg = [0], [], []
+        for _ in range(nl):
+            (c,) = np.frombuffer(f.read(4), dtype=np.int32)
+            for _ in range(c):
+                r = f.read(28)
+                ltri.append(int(np.frombuffer(r[:4], dtype=np.int32)[0]))
+                seg.append(np.frombuffer(r[4:], dtype=np.float32))
+            ptr.append(len(ltri))
+        z = np.frombuffer(f.read(8 * n * n), dtype=np.float32).reshape(n, n, 2)
+    seg = np.array(seg, dtype=np.float32).reshape(-1, 6)
+    return xyz, tri, hrwg, length, np.array(ptr, dtype=np.int32), np.array(ltri, dtype=np.int32), seg, \
+        z[..., 0].astype(np.float64) + 1j * z[..., 1].astype(np.float64)
+
+
+def make_contour_fixtures():
+    """Half-RWG lists of ContourEdge::buildEdge(mesh, HALF_RWG) with their
+    contour lines, and the surface part of every half-RWG pair's near entry
+    (EMSolver/FastWT.cpp:753-762 around Miscellaneous.cpp's kernels)."""
+    for name, (build, freq, alpha, rows) in CONTOUR_CASES.items():
+        xyz, tri = build()
+        with tempfile.TemporaryDirectory() as d:
+            nas = os.path.join(d, "mesh.nas")
+            write_nas(nas, xyz, tri)
+            out = os.path.join(d, "out.bin")
+            subprocess.run([EXE, "contour", nas, repr(freq), repr(alpha), out], cwd=d, check=True,
+                           stdout=subprocess.DEVNULL)
+            rx, rt, hrwg, length, ptr, ltri, seg, z = read_contour(out)
+        keep = z if rows is None else z[:rows]
+        np.savez_compressed(os.path.join(HERE, f"ref_contour_{name}.npz"), freq=freq, alpha=alpha, xyz=rx, tri=rt,
+                            hrwg=hrwg, length=length, line_ptr=ptr, line_tri=ltri, line_seg=seg,
+                            zfs=keep.astype(np.complex64))
+        print("contour", name, "half-RWGs", hrwg.shape[0], "nonconformal", len(ptr) - 1, "lines", len(ltri))
+
+
 def main():
     if not os.path.exists(EXE):
         sys.exit("oracle/_ref/emfa_ref is not built (make -C oracle -f ref.mk)")
@@ -149,5 +203,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
-    make_mesh_fixtures()
+    if sys.argv[1:] == ["contour"]:
+        make_contour_fixtures()
+    else:
+        main()
+        make_mesh_fixtures()
+        make_contour_fixtures()

@@ -74,3 +74,52 @@ def plate(n, size=1.0):
             tri.append((v00, v10, v11))
             tri.append((v00, v11, v01))
     return xyz, np.array(tri, dtype=np.int32)
+
+
+def rect(nx, ny, origin, ex, ey):
+    """Parallelogram origin + s ex + t ey (s, t in [0, 1]) with nx x ny cells,
+    two triangles per cell, counter-clockwise about ex x ey."""
+    origin, ex, ey = (np.asarray(v, dtype=np.float64) for v in (origin, ex, ey))
+    xyz = np.array([origin + (i / nx) * ex + (j / ny) * ey for j in range(ny + 1) for i in range(nx + 1)])
+    tri = []
+    for j in range(ny):
+        for i in range(nx):
+            v00 = j * (nx + 1) + i
+            v10, v01 = v00 + 1, v00 + nx + 1
+            tri.append((v00, v10, v01 + 1))
+            tri.append((v00, v01 + 1, v01))
+    return np.round(xyz, 9), np.array(tri, dtype=np.int32)
+
+
+def join(parts):
+    """Concatenate meshes; every part keeps its own vertices (no merging)."""
+    xyz, tri, off = [], [], 0
+    for x, t in parts:
+        xyz.append(x)
+        tri.append(t + off)
+        off += len(x)
+    return np.concatenate(xyz), np.concatenate(tri).astype(np.int32)
+
+
+def cube_faces(res, size=1.0):
+    """Closed cube surface of edge `size` centred at the origin whose six faces
+    (+x, -x, +y, -y, +z, -z) are meshed separately with res[f] x res[f]
+    cells and their own vertices, triangles counter-clockwise seen from
+    outside.  Faces of equal resolution meet with coincident vertices under
+    distinct ids, faces of different resolution nonconformally."""
+    h = size / 2
+    s = 2 * h
+    faces = [((h, -h, -h), (0, s, 0), (0, 0, s)), ((-h, -h, -h), (0, 0, s), (0, s, 0)),
+             ((-h, h, -h), (0, 0, s), (s, 0, 0)), ((-h, -h, -h), (s, 0, 0), (0, 0, s)),
+             ((-h, -h, h), (s, 0, 0), (0, s, 0)), ((-h, -h, -h), (0, s, 0), (s, 0, 0))]
+    return join([rect(r, r, o, a, b) for r, (o, a, b) in zip(res, faces)])
+
+
+def plate_pair(nl, nr, size=1.0):
+    """size x size plate in z = 0 cut along x = 0 into two halves meshed
+    separately: the left half with nl cells along y (and nl // 2 + 1 along
+    x), the right half with nr; the halves share no vertex ids."""
+    h = size / 2
+    left = rect(max(1, nl // 2 + 1), nl, (-h, -h, 0), (h, 0, 0), (0, size, 0))
+    right = rect(max(1, nr // 2 + 1), nr, (0, -h, 0), (h, 0, 0), (0, size, 0))
+    return join([left, right])

@@ -1,0 +1,216 @@
+"""The IP-SIE fast solver's host part and its restatement (oracle/fastwt.py)
+without a device: the product's half-RWG builder against the reference's
+ContourEdge (compiled from its sources, tests/golden/ref_contour_*.npz), bit
+for bit; the restatement's surface entries against the reference's kernels;
+and what the mathematics fixes for the parts the reference build cannot
+reach (EMSolver/FastWT.cpp does not compile with this toolchain): on a
+conformal mesh the two halves of every edge recombine into the RWG CFIE
+matrix exactly (line and penalty terms cancel); the plane-wave expansion of
+the Green's function, with the vector patterns and the scalar line pattern,
+reproduces the near-entry formula of separated pairs, line term included;
+the scalar channel alone reproduces -alpha Z0 / (4 pi j k) l_f l_s <G> over
+the two edges; the 3 x 3 line rule converges to the integral; and the
+IP-SIE answer on a nonconformal cube converges to the RWG CFIE answer on
+the conformal one, while a flipped line term misses it by many dB."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import meshgen
+from conftest import sphere
+from emfa_amd import hip
+from oracle import fastwt as W
+from oracle import mlfma as OM
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "ref_contour_*.npz")))
+
+
+def _contour(xyz, tri):
+    c = hip.contour_build(xyz, tri)
+    return W.Contour(c.hrwg, c.length, c.line_ptr, c.line_tri, c.line_seg)
+
+
+def _merge(xyz, tri):
+    key = np.round(xyz, 7)
+    u, inv = np.unique(key, axis=0, return_inverse=True)
+    return u, inv.ravel()[tri].astype(np.int32)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[12:-4] for p in GOLDEN])
+def test_builder_equals_reference_contour_edges(path):
+    g = np.load(path)
+    c = hip.contour_build(g["xyz"].astype(np.float64), g["tri"])
+    assert np.array_equal(c.hrwg, g["hrwg"])
+    assert np.array_equal(c.length.astype(np.float32), g["length"])
+    assert np.array_equal(c.line_ptr, g["line_ptr"])
+    assert np.array_equal(c.line_tri, g["line_tri"])
+    assert np.array_equal(c.line_seg.astype(np.float32), g["line_seg"])
+
+
+def test_golden_meshes_cover_every_kind_of_edge():
+    kinds = {}
+    for path in GOLDEN:
+        g = np.load(path)
+        h = g["hrwg"]
+        kinds[os.path.basename(path)] = (int((h[:, 5] == 0).sum()), int((h[:, 5] == 1).sum()), len(g["line_tri"]))
+    assert kinds["ref_contour_sphere_s1.npz"] == (240, 0, 0)      # shared edges only
+    assert kinds["ref_contour_cube_232322.npz"][1] > 0             # coincident and nonconformal face seams
+    assert kinds["ref_contour_plate_pair_34.npz"] == (84, 7, 12)   # 3 against 4 cells on the seam
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[12:-4] for p in GOLDEN])
+def test_surface_entries_match_reference_kernels(path):
+    g = np.load(path)
+    k = O.wavenumber(float(g["freq"]))
+    zr = g["zfs"].astype(np.complex128)
+    z = O.fill_hrwg(g["xyz"].astype(np.float64), g["tri"], g["hrwg"], g["length"].astype(np.float64), k,
+                    float(g["alpha"]), rows=(0, zr.shape[0]))
+    assert np.abs(z - zr).max() / np.abs(zr).max() < 3e-6
+    assert np.median(np.abs(z - zr) / np.abs(zr)) < 1e-6
+
+
+def test_conformal_halves_recombine_into_the_rwg_matrix():
+    xyz, tri, rwg = sphere(1, 0.5)
+    C = _contour(xyz, tri)
+    k, alpha = O.wavenumber(3e8), 0.5
+    Zs, line, ip = W.dense_parts(xyz, tri, C, k, W.beta(0.1, 0.07503), alpha)
+    P = W.expand(C, rwg)
+    Zr = O.fill_cfie(xyz, tri, rwg, k, alpha, 3, 0)
+    # the half-RWG lengths are the reference's FP32 ones
+    assert np.abs(P.T @ Zs @ P - Zr).max() / np.abs(Zr).max() < 1e-6
+    assert np.abs(P.T @ line @ P).max() / np.abs(line).max() < 1e-12
+    assert np.abs(P.T @ ip @ P).max() / np.abs(ip).max() < 1e-12
+    assert np.abs(line).max() > 1e-2 * np.abs(Zs).max()
+
+
+def test_dense_and_pairwise_restatements_agree():
+    xyz, tri = meshgen.cube_faces([2, 3, 2, 3, 2, 2], 1.0)
+    C = _contour(xyz, tri)
+    k, b = O.wavenumber(3e8), W.beta(0.1, 0.07503)
+    D = W.dense(xyz, tri, C, k, 0.5, b)
+    nc = np.where(C.hrwg[:, 5] == 1)[0]
+    rng = np.random.default_rng(0)
+    pairs = [(int(rng.integers(C.n)), int(rng.integers(C.n))) for _ in range(100)]
+    pairs += [(u, u) for u in range(8)] + [(int(f), int(s)) for f in nc[:6] for s in nc]
+    e = W.entries(xyz, tri, C, k, 0.5, b, pairs)
+    d = np.array([D[f, s] for f, s in pairs])
+    assert np.abs(e - d).max() / np.abs(d).max() < 1e-13
+
+
+@pytest.fixture(scope="module")
+def cube_far():
+    xyz, tri = meshgen.cube_faces([6, 7, 6, 8, 6, 7], 1.0)
+    C = _contour(xyz, tri)
+    f, alpha = 6e8, 0.5
+    T = W.tree(xyz, tri, C, f, 0.2)
+    rad, rec = W.patterns(T, xyz, tri, C, alpha)
+    lv, k = T.level, T.k
+    boxes = T.boxes[lv]
+    coeff = T.w[lv] * np.sin(T.theta[lv]) * k * k * OM.Z0 / 32
+    rng = np.random.default_rng(1)
+    pairs = []
+    while len(pairs) < 24:
+        bi = int(rng.integers(len(boxes)))
+        far = [fi for fi in boxes[bi]["far"]
+               if np.linalg.norm(boxes[bi]["center"] - boxes[fi]["center"]) >= 3.9 * T.blen[lv]]
+        if not far:
+            continue
+        fi = far[int(rng.integers(len(far)))]
+        m = boxes[bi]["unk"][int(rng.integers(len(boxes[bi]["unk"])))]
+        s = boxes[fi]["unk"][int(rng.integers(len(boxes[fi]["unk"])))]
+        Tk = OM.translation(k, boxes[bi]["center"] - boxes[fi]["center"], T.kh[lv], T.L[lv])
+        pairs.append((m, s, Tk))
+    return xyz, tri, C, T, rad, rec, coeff, alpha, pairs
+
+
+def test_far_expansion_reproduces_the_near_formula(cube_far):
+    """Vector and line patterns through the translation against the
+    near-entry formula (line term, no penalty: the pairs are far apart).
+    A flipped or dropped line term misses by order one."""
+    xyz, tri, C, T, rad, rec, coeff, alpha, pairs = cube_far
+    k = T.k
+    ok, flip, drop = [], [], []
+    for m, s, Tk in pairs:
+        zm = np.sum(coeff[:, None] * rec[m] * Tk[:, None] * rad[s])
+        zs = O.fill_hrwg(xyz, tri, C.hrwg, C.length, k, alpha, 3, 0, rows=(m, m + 1))[0, s]
+        lt = alpha * W.line_term(xyz, tri, C, m, s, k)
+        ok.append(abs(zm - (zs + lt)) / abs(zs + lt))
+        flip.append(abs(zm - (zs - lt)) / abs(zs - lt))
+        drop.append(abs(zm - zs) / abs(zs))
+    assert np.median(ok) < 1.5e-2 and np.percentile(ok, 75) < 3e-2
+    assert np.median(flip) > 0.5 and np.median(drop) > 0.5
+
+
+def test_scalar_channel_is_the_edge_to_edge_potential(cube_far):
+    xyz, tri, C, T, rad, rec, coeff, alpha, pairs = cube_far
+    k = T.k
+    errs, noconj = [], []
+    for m, s, Tk in pairs:
+        zs = np.sum(coeff * rec[m, :, 3] * Tk * rad[s, :, 3])
+        hm, hs = C.hrwg[m], C.hrwg[s]
+        pm = xyz[hm[0]] + W.GL3[:, None] * (xyz[hm[1]] - xyz[hm[0]])
+        ps = xyz[hs[0]] + W.GL3[:, None] * (xyz[hs[1]] - xyz[hs[0]])
+        R = np.linalg.norm(pm[:, None] - ps[None], axis=2)
+        G = np.sum(W.GLW3[:, None] * W.GLW3[None, :] * np.exp(-1j * k * R) / R)
+        zc = -alpha * OM.Z0 / (4 * np.pi * 1j * k) * C.length[m] * C.length[s] * G
+        errs.append(abs(zs - zc) / abs(zc))
+        bad = np.sum(coeff * np.conj(rec[m, :, 3]) * Tk * rad[s, :, 3])
+        noconj.append(abs(bad - zc) / abs(zc))
+    assert np.median(errs) < 5e-3
+    assert np.median(noconj) > 0.3
+
+
+def test_line_rule_converges_to_the_integral():
+    """3 x 3 points against 24-point Gauss-Legendre on the edge and a 16 x 16
+    Duffy-free subdivision of a well-separated triangle."""
+    k = O.wavenumber(3e8)
+    T = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.02]])
+    a, b = np.array([0.3, 0.2, 0.05]), np.array([0.35, 0.28, 0.0])
+    x, w = np.polynomial.legendre.leggauss(24)
+    x, w = (x + 1) / 2, w / 2
+    m = 16
+    pts, wts = [], []
+    for i in range(m):
+        for j in range(m - i):
+            for (p, q), wt in (((i + 1 / 3, j + 1 / 3), 1.0), ((i + 2 / 3, j + 2 / 3), 1.0)):
+                if p + q < m:
+                    pts.append(T[0] + (p / m) * (T[1] - T[0]) + (q / m) * (T[2] - T[0]))
+                    wts.append(wt)
+    pts, wts = np.array(pts), np.array(wts) / len(wts)
+    e = a[None, :] + x[:, None] * (b - a)[None, :]
+    r = np.linalg.norm(e[:, None, :] - pts[None, :, :], axis=2)
+    exact = np.linalg.norm(b - a) * np.sum(w[:, None] * wts[None, :] * np.exp(-1j * k * r) / r)
+    got = W.line_integral(T, a, b, k)
+    assert abs(got - exact) / abs(exact) < 2e-3
+    # a rule without the edge length misses by the length itself
+    assert abs(got / np.linalg.norm(b - a) - exact) / abs(exact) > 1.0
+
+
+def test_nonconformal_cube_converges_to_the_conformal_rwg_answer():
+    """IP-SIE (dense restatement) on a cube whose faces meet nonconformally
+    against RWG CFIE on a conformal 6 x 6 per face cube: within 0.75 dB over
+    the theta cut; with the line term flipped the answer is many dB off."""
+    f, alpha = 3e8, 0.5
+    k = O.wavenumber(f)
+    inc = O.inc_table([(0.0, 0.0, 0.0)])
+    th = np.radians(np.arange(0, 181, 10.0))
+    dirs = np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+    xm, tm = _merge(*meshgen.cube_faces([6] * 6, 0.6))
+    rwg, nb = hip.rwg_build(tm)
+    assert nb == 0
+    Zr = O.fill_cfie(xm, tm, rwg, k, alpha, 3, 0)
+    Ir = np.linalg.solve(Zr, O.rhs(xm, tm, rwg, 1, k, alpha, inc)[:, 0])
+    r_rwg = O.rcs(xm, tm, rwg, k, Ir[:, None], dirs)
+    xyz, tri = meshgen.cube_faces([4, 5, 4, 5, 4, 5], 0.6)
+    C = _contour(xyz, tri)
+    assert (C.hrwg[:, 5] == 1).sum() > 50
+    Zs, line, ip = W.dense_parts(xyz, tri, C, k, W.beta(0.1, 0.07503), alpha)
+    B = W.rhs(xyz, tri, C, k, alpha, inc)[:, 0]
+    r = W.rcs(xyz, tri, C, k, np.linalg.solve(Zs + alpha * (line + ip), B), dirs)
+    assert np.abs(r - r_rwg).max() < 0.75
+    r_bad = W.rcs(xyz, tri, C, k, np.linalg.solve(Zs + alpha * (-line + ip), B), dirs)
+    assert np.abs(r_bad - r_rwg).max() > 5.0

@@ -19,7 +19,8 @@ from golden.make_ref_golden import CASES
 from test_cpu_host import _read_mesh, write_nas
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIXTURES = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "ref_*.npz")) if "ref_mesh_" not in p)
+FIXTURES = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "ref_*.npz"))
+                  if "ref_mesh_" not in p and "ref_contour_" not in p)
 
 
 def _name(path):
