@@ -48,6 +48,7 @@ CONFIGS = {
     "c3": None,  # 256 uneven EFIE ellipsoid problems (emfa_amd.workload.batch_problems)
     "c5": None,  # MLFMA CFIE, icosphere s=6 r=1 m at 1.5 GHz, GMRES(30) to 1e-4
     "c4": None,  # mixed serving stream (emfa_amd.workload.mixed_stream)
+    "fastwt": None,  # IP-SIE (FastWT) on a nonconformal cube, 1.5 GHz, BiCGSTAB to 1e-4
 }
 
 
@@ -421,6 +422,10 @@ def main():
         if a.steps is None:
             a.steps = 2
         return main_mlfma(a)
+    if a.config == "fastwt":
+        if a.steps is None:
+            a.steps = 2
+        return main_fastwt(a)
     if a.steps is None:
         a.steps = 3
 
@@ -711,6 +716,138 @@ def main_mlfma(a, subdiv=6, radius=1.0, freq=1.5e9, alpha=0.5, threshold=0.25, t
         "near_spmv": {"achieved_gb_per_s": near_gbs, "frac": near_gbs / PEAK["hbm"]},
         "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
         "cpu_baseline": None if a.no_cpu else mlfma_cpu_baseline(radius, freq, alpha, info["near_nnz"], n),
+    }
+    print(json.dumps(res))
+
+
+def fastwt_cpu_baseline(near_nnz, freq, alpha, beta, reps=3, sample_res=(4, 5, 4, 5, 4, 5)):
+    """A lower bound on the CPU time of one IP-SIE solve: its near matrix
+    alone.  FastWT::calculateNearZ (EMSolver/FastWT.cpp:689-784) evaluates
+    for every near pair the CFIE kernel of Miscellaneous.cpp, two line
+    integrals and the penalty; the restatement (oracle/fastwt.py, the C
+    oracle's kernels and numpy, one thread) is timed per pair on every pair
+    of a small nonconformal cube at the same frequency (median of `reps`)
+    and multiplied by the operator's near entries.  The reference's
+    FastWT.cpp does not compile here."""
+    from emfa_amd import hip, workload
+    from oracle import fastwt as W
+    from oracle import oracle as O
+    xyz, tri = workload.nonconformal_cube(sample_res, 0.6)
+    c = hip.contour_build(xyz, tri)
+    C = W.Contour(c.hrwg, c.length, c.line_ptr, c.line_tri, c.line_seg)
+    k = hip.wavenumber(freq)
+    threads = O.max_threads()
+    O.set_threads(1)
+    try:
+        runs = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            W.dense_parts(xyz, tri, C, k, beta, alpha)
+            runs.append((time.perf_counter() - t0) / C.n ** 2)
+    finally:
+        O.set_threads(threads)
+    per_pair = float(np.median(runs))
+    near_s = per_pair * near_nnz
+    return {
+        "value": 1.0 / near_s, "unit": "solves/s", "cores": 1, "kind": "port",
+        "sample": (f"restated IP-SIE near entries (oracle/fastwt.py, one thread): {per_pair * 1e6:.2f} us per "
+                   f"half-RWG pair over all {C.n}^2 pairs of a nonconformal cube (median of {reps}) x {near_nnz} "
+                   f"near entries = {near_s:.0f}s for FastWT::calculateNearZ alone; far-field set-up, translations "
+                   f"and iterations not included (an upper bound on a CPU rate)"),
+        "host": host_cpu(),
+    }
+
+
+def main_fastwt(a, res=(40, 48, 40, 48, 40, 48), size=1.0, freq=1.5e9, alpha=0.5, threshold=0.25, tol=1e-4,
+                stab_factor=0.1, avg_size=0.07503):
+    """fastwt: one step = contour edges + IP-SIE operator set-up (tree,
+    tables, near matrix, patterns) + excitation + BiCGSTAB to 1e-4 + bistatic
+    RCS at 361 angles, on a cube whose faces are meshed separately with 40
+    and 48 cells per edge (nonconformal seams)."""
+    from emfa_amd import hip, workload
+    xyz, tri = workload.nonconformal_cube(res, size)
+    rwg, _ = hip.rwg_build(tri)
+    k = hip.wavenumber(freq)
+    beta = stab_factor * abs(np.log10(avg_size))
+    inc = hip.inc_table([(0.0, 0.0, 0.0)])
+    th = np.radians(np.arange(0.0, 361.0, 1.0))
+    dirs = np.stack([np.sin(th), 0 * th, np.cos(th)], 1)
+    ctx = hip.Context(0)
+    mesh = hip.Mesh(ctx, xyz, tri, rwg)
+    out = {}
+
+    def step():
+        t0 = time.perf_counter()
+        c = hip.contour_build(xyz, tri)
+        op = hip.FastWT(ctx, mesh, c, k, alpha, threshold, beta)
+        ctx.sync()
+        t1 = time.perf_counter()
+        n = op.n
+        B = hip.Mat(ctx, n, 1, hip.F64)
+        X = hip.Mat(ctx, n, 1, hip.F64)
+        op.rhs(B, inc)
+        rep, ok = op.bicgstab(B, X, tol=tol, max_iter=2000, variant=0)
+        r = op.rcs(X, dirs)
+        ctx.sync()
+        t2 = time.perf_counter()
+        out.update(setup_s=t1 - t0, solve_s=t2 - t1, iterations=rep.iterations, relres=rep.relres, converged=ok,
+                   info=op.info(), rcs=r, n=n, nonconformal=int((c.hrwg[:, 5] == 1).sum()))
+        B.close()
+        X.close()
+        op.close()
+
+    for _ in range(a.warmup):
+        step()
+    ctx.prof_reset()
+    ctx.prof(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / a.steps
+    fams = {}
+    for f in ["mlfma_setup", "mlfma_near", "mlfma_agg", "mlfma_up", "mlfma_down", "mlfma_out"]:
+        ms, cnt, work = ctx.prof_get(f)
+        fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
+    ctx.prof(False)
+    info, n = out["info"], out["n"]
+    dn = fams["mlfma_down"]
+    down_gbs = dn["work_per_step"] / (dn["ms_per_step"] * 1e-3) / 1e9 if dn["ms_per_step"] > 0 else 0.0
+    nf = fams["mlfma_near"]
+    near_gbs = nf["work_per_step"] / (nf["ms_per_step"] * 1e-3) / 1e9 if nf["ms_per_step"] > 0 else 0.0
+    matvecs = 2 * out["iterations"] + 1
+    mv_ms = sum(fams[f]["ms_per_step"] for f in ["mlfma_near", "mlfma_agg", "mlfma_up", "mlfma_down", "mlfma_out"])
+    res = {
+        "metric": "solves/sec (FastWT IP-SIE, BiCGSTAB to 1e-4)",
+        "value": 1.0 / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 near/far, f64 self terms and BiCGSTAB",
+        "data": "synthetic nonconformal cube mesh",
+        "config": {"workload": f"FastWT IP-SIE alpha={alpha} PEC cube {size} m, faces of {res[0]} / {res[1]} cells "
+                               f"per edge meshed separately, f={freq / 1e9:.1f} GHz, box threshold {threshold} "
+                               f"lambda, BiCGSTAB to {tol}",
+                   "unknowns": n, "nonconformal": out["nonconformal"], "triangles": int(tri.shape[0]),
+                   "parallelism": "single GPU"},
+        "fastwt": {"levels": info["levels"], "boxes": info["boxes"], "multipoles": info["multipoles"],
+                   "near_nnz": info["near_nnz"], "device_gb": info["bytes"] / 2**30,
+                   "setup_s": out["setup_s"], "solve_s": out["solve_s"], "bicgstab_iterations": out["iterations"],
+                   "relres": out["relres"], "converged": out["converged"], "matvec_ms": mv_ms / max(matvecs, 1)},
+        "roofline": {"bound": "hbm", "kernel": "mlfma_down (k_ml_down<4>: far-box translation + anterpolation)",
+                     "achieved": down_gbs, "peak": PEAK["hbm"], "unit": "GB/s", "frac": down_gbs / PEAK["hbm"],
+                     "traffic": None, "avg_launch_ms": dn["ms_per_step"] / max(dn["launches_per_step"], 1),
+                     "launches_per_step": dn["launches_per_step"],
+                     "note": "achieved counts each level's four-channel patterns, translation rows in use and "
+                             "parent patterns once (algorithmic bytes)"},
+        "near_spmv": {"achieved_gb_per_s": near_gbs, "frac": near_gbs / PEAK["hbm"]},
+        "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
+        "cpu_baseline": None if a.no_cpu else fastwt_cpu_baseline(info["near_nnz"], freq, alpha, beta),
     }
     print(json.dumps(res))
 

@@ -93,6 +93,22 @@ struct emfa_mlfma {
     emfa::cf *sai_val = nullptr;
     long long sai_nnz = 0;
     double2 *ybox = nullptr;  // Z x in box order, before P
+    // channels per pattern sample: 3 (vector), 4 for the IP-SIE operator
+    // (emfa_fastwt_create: a scalar line channel besides the vector ones)
+    int nc = 3;
+    std::vector<double> mid;  // unknown centres (n x 3, unknown order): the binning points
+    // IP-SIE half-RWG tables (unknown order): triangle, local index of the
+    // free vertex, FP32 / FP64 length
+    int *wt_tri = nullptr, *wt_fv = nullptr;
+    float *wt_len32 = nullptr;
+    double *wt_len64 = nullptr;
+    // edge number * 2 + type (0 conformal, 1 nonconformal) per half-RWG; the
+    // contour lines of the nonconformal edges (CSR by nonconformal index:
+    // triangle, FP32 length); the stabilization factor beta
+    int *wt_edge = nullptr, *wt_lptr = nullptr, *wt_ltri = nullptr;
+    float *wt_llen = nullptr;
+    double wt_beta = 0;
+    int nt = 0;  // triangles of the mesh the operator was built on
 };
 
 namespace emfa {
@@ -400,28 +416,32 @@ __global__ __launch_bounds__(256) void k_ml_near(const long long *__restrict__ r
     if (lane == 0) yn[row] = {sr, si};
 }
 
+// Patterns carry NC channels per sample: the three vector components (and,
+// for the IP-SIE operator, a fourth scalar line channel), stored together.
 // leaf aggregation: up[b][q] = sum_{i in b} x_i rad[i][q]
+template <int NC>
 __global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, const int *__restrict__ uend,
                                                 const cf *__restrict__ rad, const cf *__restrict__ xb, int K,
                                                 int nbox, cf *__restrict__ up) {
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
     const int b = (int)(idx / K), q = (int)(idx % K);
     if (b >= nbox) return;
-    cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    cf a[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = {0, 0};
     for (int i = ubeg[b]; i < uend[b]; ++i) {
         const cf x = xb[i];
-        const cf *r = rad + ((size_t)i * K + q) * 3;
-        a0 = cfma(a0, x, r[0]);
-        a1 = cfma(a1, x, r[1]);
-        a2 = cfma(a2, x, r[2]);
+        const cf *r = rad + ((size_t)i * K + q) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], x, r[c]);
     }
-    cf *o = up + ((size_t)b * K + q) * 3;
-    o[0] = a0;
-    o[1] = a1;
-    o[2] = a2;
+    cf *o = up + ((size_t)b * K + q) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o[c] = a[c];
 }
 
 // upward: up_p[q] = sum_children ph[oct][q] * sum_9 w * up_c[col]
+template <int NC>
 __global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, const int *__restrict__ ch_idx,
                                                const int *__restrict__ ch_oct, const int *__restrict__ ip_col,
                                                const float *__restrict__ ip_w, const cf *__restrict__ ph, int K,
@@ -436,31 +456,35 @@ __global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, c
         cols[j] = ip_col[q * 9 + j];
         ws[j] = ip_w[q * 9 + j];
     }
-    cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    cf a[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = {0, 0};
     for (int e = ch_ptr[p]; e < ch_ptr[p + 1]; ++e) {
-        const int c = ch_idx[e];
-        const cf *src = upc + (size_t)c * Kc * 3;
-        cf s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0};
+        const int ch = ch_idx[e];
+        const cf *src = upc + (size_t)ch * Kc * NC;
+        cf sm[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c] = {0, 0};
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
-            const cf *v = src + (size_t)cols[j] * 3;
-            const cf v0 = v[0], v1 = v[1], v2 = v[2];
-            s0 = {s0.re + ws[j] * v0.re, s0.im + ws[j] * v0.im};
-            s1 = {s1.re + ws[j] * v1.re, s1.im + ws[j] * v1.im};
-            s2 = {s2.re + ws[j] * v2.re, s2.im + ws[j] * v2.im};
+            const cf *v = src + (size_t)cols[j] * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const cf vc = v[c];
+                sm[c] = {sm[c].re + ws[j] * vc.re, sm[c].im + ws[j] * vc.im};
+            }
         }
-        const cf f = ph[ch_oct[c] * K + q];
-        a0 = cfma(a0, f, s0);
-        a1 = cfma(a1, f, s1);
-        a2 = cfma(a2, f, s2);
+        const cf f = ph[ch_oct[ch] * K + q];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], f, sm[c]);
     }
-    cf *o = upp + ((size_t)p * K + q) * 3;
-    o[0] = a0;
-    o[1] = a1;
-    o[2] = a2;
+    cf *o = upp + ((size_t)p * K + q) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o[c] = a[c];
 }
 
 // downward at one level: translation from far boxes + anterpolation from the parent
+template <int NC>
 __global__ __launch_bounds__(256) void k_ml_down(const int *__restrict__ far_ptr, const int *__restrict__ far_idx,
                                                  const int *__restrict__ far_code, const cf *__restrict__ trans,
                                                  const cf *__restrict__ up, int K, const int *__restrict__ parent,
@@ -471,31 +495,30 @@ __global__ __launch_bounds__(256) void k_ml_down(const int *__restrict__ far_ptr
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
     const int b = (int)(idx / K), q = (int)(idx % K);
     if (b >= nbox) return;
-    cf a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    cf a[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = {0, 0};
     for (int e = far_ptr[b]; e < far_ptr[b + 1]; ++e) {
         const cf t = trans[(size_t)far_code[e] * K + q];
-        const cf *u = up + ((size_t)far_idx[e] * K + q) * 3;
-        a0 = cfma(a0, t, u[0]);
-        a1 = cfma(a1, t, u[1]);
-        a2 = cfma(a2, t, u[2]);
+        const cf *u = up + ((size_t)far_idx[e] * K + q) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], t, u[c]);
     }
     if (downp) {
         const int p = parent[b], o = oct[b];
-        const cf *dp = downp + (size_t)p * Kp * 3;
+        const cf *dp = downp + (size_t)p * Kp * NC;
         for (int e = an_ptr[q]; e < an_ptr[q + 1]; ++e) {
             const int kp = an_kp[e];
             const cf f0 = php[o * Kp + kp];
             const cf f = {an_w[e] * f0.re, -an_w[e] * f0.im};  // conj: exp(+j k khat.(c_p - c_b))
-            const cf *v = dp + (size_t)kp * 3;
-            a0 = cfma(a0, f, v[0]);
-            a1 = cfma(a1, f, v[1]);
-            a2 = cfma(a2, f, v[2]);
+            const cf *v = dp + (size_t)kp * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], f, v[c]);
         }
     }
-    cf *o = down + ((size_t)b * K + q) * 3;
-    o[0] = a0;
-    o[1] = a1;
-    o[2] = a2;
+    cf *o = down + ((size_t)b * K + q) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o[c] = a[c];
 }
 
 // Downward pass at level l >= 3 grouped by parent: the children of parent P
@@ -566,6 +589,7 @@ __global__ __launch_bounds__(256) void k_ml_down_sib(
 }
 
 // receive + near part, back to RWG order: y[perm[i]] = yn[i] + sum_q rec[i][q] . down[leaf(i)][q]
+template <int NC>
 __global__ __launch_bounds__(256) void k_ml_out(const cf *__restrict__ rec, const cf *__restrict__ down,
                                                 const int *__restrict__ leaf_of, const int *__restrict__ perm,
                                                 const cf *__restrict__ yn, int n, int K, double2 *__restrict__ y,
@@ -573,13 +597,13 @@ __global__ __launch_bounds__(256) void k_ml_out(const cf *__restrict__ rec, cons
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= n) return;
-    const cf *d = down + (size_t)leaf_of[i] * K * 3;
-    const cf *r = rec + (size_t)i * K * 3;
+    const cf *d = down + (size_t)leaf_of[i] * K * NC;
+    const cf *r = rec + (size_t)i * K * NC;
     float sr = 0, si = 0;
     for (int q = lane; q < K; q += 64) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            cf a = r[3 * q + c], b = d[3 * q + c];
+        for (int c = 0; c < NC; ++c) {
+            cf a = r[NC * q + c], b = d[NC * q + c];
             sr += a.re * b.re - a.im * b.im;
             si += a.re * b.im + a.im * b.re;
         }
@@ -626,6 +650,296 @@ __global__ __launch_bounds__(256) void k_ml_gather64(const double2 *__restrict__
                                                      int n, double2 *__restrict__ xb) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) xb[i] = x[perm[i]];
+}
+
+
+// ------------------------------------------------------------------ IP-SIE
+// Half-RWG interior-penalty operator (EMSolver/FastWT.cpp).  A half-RWG u
+// lives on one triangle wt_tri[u] with free vertex wt_fv[u] (local index) and
+// the edge opposite it; its basis is l_u / (2 A) (r - v_free) there.
+
+// 3-point Gauss-Legendre on [0, 1] (EMComponent/Quadrature.h:17-18)
+__device__ __forceinline__ double gl3(int p) { return p == 0 ? 0.1127016654 : p == 1 ? 0.5 : 0.8872983346; }
+__device__ __forceinline__ double glw3(int p) { return p == 1 ? 0.444444444444445 : 0.2777777777775; }
+
+// len sum_p sum_q glw3[p] w3[q] e^{-jkr} / r between the three points of the
+// edge opposite local vertex b of triangle ts and the three Gauss points of
+// triangle tt (FastWT.cpp:1494-1511 without the factor Z0 / (4 pi j k)), FP32
+__device__ cf wt_line(const float *__restrict__ tg, int T, int tt, int ts, int b, float len, float k) {
+    const int i0 = b == 2 ? 0 : b + 1, i1 = b == 0 ? 2 : b - 1;
+    float ct[3], cs[3], vt[3][3], e0[3], e1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ct[c] = tg[(size_t)c * T + tt];
+        cs[c] = tg[(size_t)c * T + ts];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) vt[i][c] = tg[(size_t)(3 + 3 * i + c) * T + tt];
+        e0[c] = tg[(size_t)(3 + 3 * i0 + c) * T + ts];
+        e1[c] = tg[(size_t)(3 + 3 * i1 + c) * T + ts];
+    }
+    float ar = 0, ai = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const float g = (float)gl3(p), wp = (float)(glw3(p) / 3.0);
+        float ep[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ep[c] = e0[c] + g * (e1[c] - e0[c]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            float u[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                u[c] = ct[c] - cs[c] +
+                       ((float)rule<3>(q, 0) * vt[0][c] + (float)rule<3>(q, 1) * vt[1][c] +
+                        (float)rule<3>(q, 2) * vt[2][c]) -
+                       ep[c];
+            const float r = sqrtf(dot3(u, u)), ir = 1.0f / r;
+            float sn, cs_;
+            sincos_r(k * r, &sn, &cs_);
+            ar += wp * cs_ * ir;
+            ai -= wp * sn * ir;
+        }
+    }
+    return {len * ar, len * ai};
+}
+
+// Near entries of the IP-SIE operator (FastWT.cpp:734-784), one wavefront per
+// row of the near matrix (field f = perm[row]) over its columns (source s):
+//   z    = l_f l_s K(f, s): K the CFIE triangle-pair kernel with both signs
+//          +1, FP32 regular / FP64 self term on a shared triangle;
+//   line = -(l_f C(T_f; edge s) + l_s C(T_s; edge f)), C = Z0 / (4 pi j k) x
+//          wt_line;
+//   ip   = stab l_f for f = s and for the two conformal halves of one edge on
+//          different triangles; stab x (length of f's first contour line on
+//          s's triangle) between nonconformal edges; stab = 2 beta Z0 / k;
+//   entry = z + alpha (line + ip).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_wt_near_fill(
+    const float *__restrict__ tg, const double *__restrict__ tg64, int T, const int *__restrict__ wtri,
+    const int *__restrict__ wfv, const float *__restrict__ wlen, const int *__restrict__ wedge,
+    const int *__restrict__ lptr, const int *__restrict__ ltri, const float *__restrict__ llen,
+    const int *__restrict__ perm, const long long *__restrict__ row_ptr, const int *__restrict__ col, int n, float k,
+    float ik2, float cE, float cM, double kd, double alpha, int cfie_mode, float cl, float stab, cf *__restrict__ val) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const int f = perm[row];
+    const int tf = wtri[f], af = wfv[f], ef = wedge[f];
+    const float lf = wlen[f], a32 = (float)alpha;
+    for (long long j = row_ptr[row] + lane; j < row_ptr[row + 1]; j += 64) {
+        const int s = perm[col[j]];
+        const int ts = wtri[s], as = wfv[s], es = wedge[s];
+        const float ls = wlen[s];
+        float zr, zi, ip = 0.0f;
+        if (tf == ts) {
+            d3 t[3];
+            for (int v = 0; v < 3; ++v)
+                t[v] = mk3(tg64[(size_t)(3 + 3 * v) * T + tf], tg64[(size_t)(4 + 3 * v) * T + tf],
+                           tg64[(size_t)(5 + 3 * v) * T + tf]);
+            double sr, si, gr, gi;
+            self_term<3>(KIND, kd, alpha, 0.0, 0.0, cfie_mode, t, tg64[(size_t)15 * T + tf], af, as, &sr, &si, &gr,
+                         &gi);
+            zr = (float)sr;
+            zi = (float)si;
+            if (f == s) ip = stab * lf;
+        } else {
+            const cf z = pair_regular<KIND>(tg, T, tf, af, ts, as, k, ik2, cE, cM);
+            zr = z.re;
+            zi = z.im;
+            if (!(ef & 1) && ef == es) {
+                ip = stab * lf;
+            } else if ((ef & 1) && (es & 1)) {
+                const int e = ef >> 1;
+                for (int q = lptr[e]; q < lptr[e + 1]; ++q)
+                    if (ltri[q] == ts) {
+                        ip = stab * llen[q];
+                        break;
+                    }
+            }
+        }
+        const float ll = lf * ls;
+        const cf pa = wt_line(tg, T, tf, ts, as, ls, k), pb = wt_line(tg, T, ts, tf, af, lf, k);
+        const float sr = lf * pa.re + ls * pb.re, si = lf * pa.im + ls * pb.im;
+        // line = -Z0 / (4 pi j k) (sr + j si) = cl (-si + j sr), cl = Z0 / (4 pi k)
+        val[j] = {ll * zr + a32 * (-cl * si + ip), ll * zi + a32 * (cl * sr)};
+    }
+}
+
+// Radiation / receive patterns of half-RWG perm[i] about its leaf-box centre
+// (FastWT.cpp:583-653), FP64, stored FP32 with four channels: the vector
+// patterns as the MLFMA's with one triangle, and the line pattern
+// prad = l sum_p glw3 e^{+jk khat.(gl_p - c)} on the edge, prec = conj(prad)
+// times coeff alpha / k^2 (= pcoeff, FastWT.cpp:986-989).
+__global__ __launch_bounds__(256) void k_wt_radrec(const double *__restrict__ tg, int T, const int *__restrict__ wtri,
+                                                   const int *__restrict__ wfv, const double *__restrict__ wlen,
+                                                   const int *__restrict__ perm, const int *__restrict__ leaf_of,
+                                                   const double *__restrict__ leaf_cen,
+                                                   const double *__restrict__ kh, const double *__restrict__ coeff,
+                                                   int n, int K, double k, double alpha, cf *__restrict__ rad,
+                                                   cf *__restrict__ rec) {
+    const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= (long long)n * K) return;
+    const int i = (int)(id / K), q = (int)(id % K);
+    const int u = perm[i], b = leaf_of[i];
+    const int t = wtri[u], a = wfv[u];
+    const double kx = kh[3 * q], ky = kh[3 * q + 1], kz = kh[3 * q + 2];
+    double c[3], v[3][3], nn[3];
+    for (int cc = 0; cc < 3; ++cc) {
+        c[cc] = tg[(size_t)cc * T + t] - leaf_cen[3 * b + cc];
+        nn[cc] = tg[(size_t)(12 + cc) * T + t];
+        for (int j = 0; j < 3; ++j) v[j][cc] = tg[(size_t)(3 + 3 * j + cc) * T + t];
+    }
+    double vr[3] = {0, 0, 0}, vi[3] = {0, 0, 0}, hr[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int p = 0; p < 3; ++p) {
+        double x[3], rho[3];
+        for (int cc = 0; cc < 3; ++cc) {
+            x[cc] = rule<3>(p, 0) * v[0][cc] + rule<3>(p, 1) * v[1][cc] + rule<3>(p, 2) * v[2][cc];
+            rho[cc] = x[cc] - v[a][cc];
+        }
+        const double ph = k * (kx * (c[0] + x[0]) + ky * (c[1] + x[1]) + kz * (c[2] + x[2]));
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        const double w = 1.0 / 3.0;
+        const double nr[3] = {nn[1] * rho[2] - nn[2] * rho[1], nn[2] * rho[0] - nn[0] * rho[2],
+                              nn[0] * rho[1] - nn[1] * rho[0]};
+        for (int cc = 0; cc < 3; ++cc) {
+            vr[cc] += w * rho[cc] * cs;
+            vi[cc] += w * rho[cc] * sn;
+            hr[cc] += w * nr[cc] * cs;
+            hi[cc] -= w * nr[cc] * sn;
+        }
+    }
+    const double pr = kx * vr[0] + ky * vr[1] + kz * vr[2], pi = kx * vi[0] + ky * vi[1] + kz * vi[2];
+    const double kv[3] = {kx, ky, kz};
+    for (int cc = 0; cc < 3; ++cc) {
+        vr[cc] -= pr * kv[cc];
+        vi[cc] -= pi * kv[cc];
+    }
+    const double cr[3] = {ky * hr[2] - kz * hr[1], kz * hr[0] - kx * hr[2], kx * hr[1] - ky * hr[0]};
+    const double ci[3] = {ky * hi[2] - kz * hi[1], kz * hi[0] - kx * hi[2], kx * hi[1] - ky * hi[0]};
+    const double len = wlen[u], hl = 0.5 * len, cq = coeff[q];
+    cf *ro = rad + ((size_t)i * K + q) * 4;
+    cf *co = rec + ((size_t)i * K + q) * 4;
+    for (int cc = 0; cc < 3; ++cc) {
+        ro[cc] = {(float)(hl * vr[cc]), (float)(hl * vi[cc])};
+        const double er = alpha * vr[cc] + (1 - alpha) * cr[cc];
+        const double ei = -alpha * vi[cc] + (1 - alpha) * ci[cc];
+        co[cc] = {(float)(cq * hl * er), (float)(cq * hl * ei)};
+    }
+    const int i0 = a == 2 ? 0 : a + 1, i1 = a == 0 ? 2 : a - 1;
+    double lr = 0, li = 0;
+    for (int p = 0; p < 3; ++p) {
+        double e[3];
+        for (int cc = 0; cc < 3; ++cc) e[cc] = c[cc] + v[i0][cc] + gl3(p) * (v[i1][cc] - v[i0][cc]);
+        double sn, cs;
+        sincos(k * (kx * e[0] + ky * e[1] + kz * e[2]), &sn, &cs);
+        lr += glw3(p) * cs;
+        li += glw3(p) * sn;
+    }
+    const double sc = cq * alpha / (k * k) * len;
+    ro[3] = {(float)(len * lr), (float)(len * li)};
+    co[3] = {(float)(sc * lr), (float)(-sc * li)};
+}
+
+// Excitation of every half-RWG (FastWT.cpp:786-806 fillVbyBox with cVKernel
+// :1475-1492): V = l/2 sum_a w3 (alpha rho.E + (1 - alpha) rho.(n x H))
+// e^{-jk g_a.k_inc}, FP64, half-RWG order; inc: nrhs x 9 (k, e, h).
+__global__ __launch_bounds__(256) void k_wt_rhs(const double *__restrict__ tg, int T, const int *__restrict__ wtri,
+                                                const int *__restrict__ wfv, const double *__restrict__ wlen, int n,
+                                                const double *__restrict__ inc, int nrhs, double k, double alpha,
+                                                double2 *__restrict__ B) {
+    const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= (long long)n * nrhs) return;
+    const int u = (int)(id / nrhs), j = (int)(id % nrhs);
+    const int t = wtri[u], a = wfv[u];
+    double c[3], v[3][3], nn[3];
+    for (int cc = 0; cc < 3; ++cc) {
+        c[cc] = tg[(size_t)cc * T + t];
+        nn[cc] = tg[(size_t)(12 + cc) * T + t];
+        for (int i = 0; i < 3; ++i) v[i][cc] = tg[(size_t)(3 + 3 * i + cc) * T + t];
+    }
+    const double *in = inc + 9 * (size_t)j;
+    const double nh[3] = {nn[1] * in[8] - nn[2] * in[7], nn[2] * in[6] - nn[0] * in[8], nn[0] * in[7] - nn[1] * in[6]};
+    double er = 0, ei = 0, mr = 0, mi = 0;
+    for (int p = 0; p < 3; ++p) {
+        double x[3], rho[3];
+        for (int cc = 0; cc < 3; ++cc) {
+            x[cc] = rule<3>(p, 0) * v[0][cc] + rule<3>(p, 1) * v[1][cc] + rule<3>(p, 2) * v[2][cc];
+            rho[cc] = x[cc] - v[a][cc];
+        }
+        double sn, cs;
+        sincos(-k * ((c[0] + x[0]) * in[0] + (c[1] + x[1]) * in[1] + (c[2] + x[2]) * in[2]), &sn, &cs);
+        const double de = (rho[0] * in[3] + rho[1] * in[4] + rho[2] * in[5]) / 3.0;
+        const double dm = (rho[0] * nh[0] + rho[1] * nh[1] + rho[2] * nh[2]) / 3.0;
+        er += de * cs;
+        ei += de * sn;
+        mr += dm * cs;
+        mi += dm * sn;
+    }
+    const double hl = 0.5 * wlen[u];
+    B[(size_t)u * nrhs + j] = make_double2(hl * (alpha * er + (1 - alpha) * mr), hl * (alpha * ei + (1 - alpha) * mi));
+}
+
+// Bistatic RCS of column `col` of X (half-RWG order) per direction
+// (FastWT.cpp:1393-1420): Es = sum_u I_u l_u sum_g w3 (g - v_free)
+// e^{+jk g.s}; sigma = Z0^2 k^2 |s x (s x Es) / 2|^2 / 4 pi, in dB.  One
+// workgroup per direction, FP64 sums reduced in a fixed order.
+__global__ __launch_bounds__(256) void k_wt_rcs(const double *__restrict__ tg, int T, const int *__restrict__ wtri,
+                                                const int *__restrict__ wfv, const double *__restrict__ wlen, int n,
+                                                const double2 *__restrict__ X, int ncols, int col,
+                                                const double *__restrict__ dirs, double k,
+                                                double *__restrict__ out_db) {
+    const int d = blockIdx.x;
+    const double sx = dirs[3 * d], sy = dirs[3 * d + 1], sz = dirs[3 * d + 2];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int u = threadIdx.x; u < n; u += 256) {
+        const int t = wtri[u], a = wfv[u];
+        double c[3], v[3][3];
+        for (int cc = 0; cc < 3; ++cc) {
+            c[cc] = tg[(size_t)cc * T + t];
+            for (int i = 0; i < 3; ++i) v[i][cc] = tg[(size_t)(3 + 3 * i + cc) * T + t];
+        }
+        double er[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
+        for (int p = 0; p < 3; ++p) {
+            double x[3];
+            for (int cc = 0; cc < 3; ++cc)
+                x[cc] = rule<3>(p, 0) * v[0][cc] + rule<3>(p, 1) * v[1][cc] + rule<3>(p, 2) * v[2][cc];
+            double sn, cs;
+            sincos(k * ((c[0] + x[0]) * sx + (c[1] + x[1]) * sy + (c[2] + x[2]) * sz), &sn, &cs);
+            for (int cc = 0; cc < 3; ++cc) {
+                const double r = (x[cc] - v[a][cc]) / 3.0;
+                er[cc] += r * cs;
+                ei[cc] += r * sn;
+            }
+        }
+        const double2 I = X[(size_t)u * ncols + col];
+        const double l = wlen[u], ir = l * I.x, ii = l * I.y;
+        for (int cc = 0; cc < 3; ++cc) {
+            acc[2 * cc] += er[cc] * ir - ei[cc] * ii;
+            acc[2 * cc + 1] += er[cc] * ii + ei[cc] * ir;
+        }
+    }
+    __shared__ double red[6][256];
+    for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double Er[3] = {red[0][0], red[2][0], red[4][0]}, Ei[3] = {red[1][0], red[3][0], red[5][0]};
+        const double s[3] = {sx, sy, sz};
+        double fr[3], fi[3];
+        // f = s x (s x E) / 2 = (s (s.E) - E) / 2
+        const double pr = s[0] * Er[0] + s[1] * Er[1] + s[2] * Er[2], pi = s[0] * Ei[0] + s[1] * Ei[1] + s[2] * Ei[2];
+        for (int c = 0; c < 3; ++c) {
+            fr[c] = 0.5 * (s[c] * pr - Er[c]);
+            fi[c] = 0.5 * (s[c] * pi - Ei[c]);
+        }
+        const double nrm = fr[0] * fr[0] + fi[0] * fi[0] + fr[1] * fr[1] + fi[1] * fi[1] + fr[2] * fr[2] + fi[2] * fi[2];
+        out_db[d] = 10.0 * log10(kZ0 * kZ0 * k * k * nrm / (4.0 * kPi));
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -684,11 +998,14 @@ static void ml_free(emfa_mlfma *M) {
 static emfa_status ml_leaf_setup(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *mesh, const emfa_mlfma_params *prm,
                                  const std::vector<int> &order, emfa_mlfma **out);
 
-static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *prm, emfa_mlfma **out) {
-    const int n = mesh->n, T = mesh->nt, nv = mesh->nv;
+// Octree, samples and per-level tables of the multilevel operator over n
+// unknowns binned by their centres mid (n x 3, unknown order); on success *Mout
+// holds the operator without its leaf part and `order` the box order.
+static emfa_status ml_tree(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *prm, int n,
+                           const std::vector<double> &mid, int nc, emfa_mlfma **Mout, std::vector<int> &order) {
+    const int nv = mesh->nv;
     const double k = prm->k, lam = 2 * kPi / k;
     const std::vector<double> &xyz = mesh->xyz_host;
-    const std::vector<int> &rwg = mesh->rwg_host;
     // octree (MLFMA_Boundary.cpp:15-35)
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
     for (int v = 0; v < nv; ++v)
@@ -708,19 +1025,16 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
                  "mlfma: %d level(s) only; the box threshold (%.3g wavelengths) is too large for this mesh", levels,
                  prm->box_threshold);
     EMFA_REQUIRE(levels <= 14, EMFA_E_INVALID, "mlfma: %d levels (box threshold too small)", levels);
-    std::vector<double> mid(3 * (size_t)n);
-    for (int u = 0; u < n; ++u)
-        for (int c = 0; c < 3; ++c) mid[3 * u + c] = 0.5 * (xyz[3 * rwg[6 * u] + c] + xyz[3 * rwg[6 * u + 1] + c]);
-
     emfa_mlfma *M = new emfa_mlfma();
     M->ctx = ctx;
     M->n = n;
+    M->nc = nc;
     M->levels = levels;
     M->k = k;
     M->alpha = prm->kind == EMFA_EFIE ? 1.0 : prm->alpha;
     M->kind = prm->kind;
     M->lv.resize(levels + 1);
-    std::vector<int> order(n);
+    order.assign(n, 0);
     for (int u = 0; u < n; ++u) order[u] = u;
     {
         MlLevel &r = M->lv[0];
@@ -826,8 +1140,8 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
     // device tables per level
     for (int l = 2; l <= levels; ++l) {
         MlLevel &C = M->lv[l];
-        ML_TRY(ml_alloc(M, &C.up, (size_t)C.nbox * C.K * 3));
-        ML_TRY(ml_alloc(M, &C.down, (size_t)C.nbox * C.K * 3));
+        ML_TRY(ml_alloc(M, &C.up, (size_t)C.nbox * C.K * nc));
+        ML_TRY(ml_alloc(M, &C.down, (size_t)C.nbox * C.K * nc));
         ML_TRY(ml_upload(M, &C.far_ptr_d, C.far_ptr));
         ML_TRY(ml_upload(M, &C.far_idx_d, C.far_idx.empty() ? std::vector<int>{0} : C.far_idx));
         ML_TRY(ml_upload(M, &C.far_code_d, C.far_code.empty() ? std::vector<int>{0} : C.far_code));
@@ -910,6 +1224,105 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
             ML_TRY(ml_upload(M, &N.an_w, anw));
         }
     }
+    *Mout = M;
+    return EMFA_OK;
+}
+
+// RWG edge midpoints (n x 3)
+static std::vector<double> rwg_mid(const emfa_mesh *mesh) {
+    const int n = mesh->n;
+    const std::vector<double> &xyz = mesh->xyz_host;
+    const std::vector<int> &rwg = mesh->rwg_host;
+    std::vector<double> mid(3 * (size_t)n);
+    for (int u = 0; u < n; ++u)
+        for (int c = 0; c < 3; ++c) mid[3 * u + c] = 0.5 * (xyz[3 * rwg[6 * u] + c] + xyz[3 * rwg[6 * u + 1] + c]);
+    return mid;
+}
+
+// IP-SIE operator (EMSolver/FastWT.cpp:26-70,167-400,583-784): the MLFMA's
+// octree over the half-RWG edge midpoints (FastWT.cpp:217-226), four pattern
+// channels, the half-RWG near entries.  The half-RWG tables are checked
+// against the mesh: the free vertex is a vertex of the half-RWG's triangle and
+// v1, v2 are its other two.
+static emfa_status wt_build(emfa_ctx *ctx, const emfa_mesh *mesh, int n, const int *hrwg, const double *length,
+                            int n_nc, const int *line_ptr, const int *line_tri, const double *line_seg,
+                            const emfa_mlfma_params *prm, double beta, emfa_mlfma **out) {
+    const std::vector<double> &xyz = mesh->xyz_host;
+    const std::vector<int> &tri = mesh->tri_host;
+    const int nt = mesh->nt;
+    std::vector<int> wtri(n), wfv(n), wedge(n);
+    std::vector<float> len32(n);
+    std::vector<double> len64(n), mid(3 * (size_t)n);
+    for (int u = 0; u < n; ++u) {
+        const int *h = hrwg + 6 * (size_t)u;
+        const int t = h[3];
+        EMFA_REQUIRE(t >= 0 && t < nt, EMFA_E_INVALID, "fastwt: half-RWG %d: triangle %d of %d", u, t, nt);
+        int a = -1;
+        for (int c = 0; c < 3; ++c)
+            if (tri[3 * (size_t)t + c] == h[2]) a = c;
+        EMFA_REQUIRE(a >= 0, EMFA_E_INVALID, "fastwt: half-RWG %d: free vertex %d not on triangle %d", u, h[2], t);
+        const int o1 = tri[3 * (size_t)t + (a + 1) % 3], o2 = tri[3 * (size_t)t + (a + 2) % 3];
+        EMFA_REQUIRE((h[0] == o1 && h[1] == o2) || (h[0] == o2 && h[1] == o1), EMFA_E_INVALID,
+                     "fastwt: half-RWG %d: edge (%d, %d) is not the edge of triangle %d opposite vertex %d", u, h[0],
+                     h[1], t, h[2]);
+        EMFA_REQUIRE(h[4] >= 0 && (h[5] == 0 || (h[5] == 1 && h[4] < n_nc)), EMFA_E_INVALID,
+                     "fastwt: half-RWG %d: edge number %d / type %d", u, h[4], h[5]);
+        EMFA_REQUIRE(length[u] > 0, EMFA_E_INVALID, "fastwt: half-RWG %d: length %g", u, length[u]);
+        wtri[u] = t;
+        wfv[u] = a;
+        wedge[u] = 2 * h[4] + h[5];
+        len32[u] = (float)length[u];
+        len64[u] = length[u];
+        for (int c = 0; c < 3; ++c) mid[3 * (size_t)u + c] = 0.5 * (xyz[3 * (size_t)h[0] + c] + xyz[3 * (size_t)h[1] + c]);
+    }
+    std::vector<int> lptr(n_nc + 1, 0), ltri;
+    std::vector<float> llen;
+    if (n_nc > 0) {
+        EMFA_REQUIRE(line_ptr && line_tri && line_seg && line_ptr[0] == 0, EMFA_E_INVALID, "fastwt: contour lines");
+        for (int e = 0; e < n_nc; ++e) {
+            EMFA_REQUIRE(line_ptr[e + 1] >= line_ptr[e], EMFA_E_INVALID, "fastwt: line_ptr not ascending");
+            lptr[e + 1] = line_ptr[e + 1];
+        }
+        const int nl = line_ptr[n_nc];
+        ltri.resize(nl);
+        llen.resize(nl);
+        for (int q = 0; q < nl; ++q) {
+            EMFA_REQUIRE(line_tri[q] >= 0 && line_tri[q] < nt, EMFA_E_INVALID, "fastwt: line %d: triangle", q);
+            ltri[q] = line_tri[q];
+            // FP32 length, as ContourEdge::getContourLineLength computes it
+            const double *g = line_seg + 6 * (size_t)q;
+            const float dx = (float)g[3] - (float)g[0], dy = (float)g[4] - (float)g[1], dz = (float)g[5] - (float)g[2];
+            llen[q] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    if (ltri.empty()) {
+        ltri.push_back(0);
+        llen.push_back(0.0f);
+    }
+    std::vector<int> order;
+    emfa_mlfma *M = nullptr;
+    emfa_status st = ml_tree(ctx, mesh, prm, n, mid, 4, &M, order);
+    if (st) return st;
+    M->mid = std::move(mid);
+    M->wt_beta = beta;
+    ML_TRY(ml_upload(M, &M->wt_tri, wtri));
+    ML_TRY(ml_upload(M, &M->wt_fv, wfv));
+    ML_TRY(ml_upload(M, &M->wt_edge, wedge));
+    ML_TRY(ml_upload(M, &M->wt_len32, len32));
+    ML_TRY(ml_upload(M, &M->wt_len64, len64));
+    ML_TRY(ml_upload(M, &M->wt_lptr, lptr));
+    ML_TRY(ml_upload(M, &M->wt_ltri, ltri));
+    ML_TRY(ml_upload(M, &M->wt_llen, llen));
+    return ml_leaf_setup(ctx, M, mesh, prm, order, out);
+}
+
+static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *prm, emfa_mlfma **out) {
+    std::vector<double> mid = rwg_mid(mesh);
+    std::vector<int> order;
+    emfa_mlfma *M = nullptr;
+    emfa_status st = ml_tree(ctx, mesh, prm, mesh->n, mid, 3, &M, order);
+    if (st) return st;
+    M->mid = std::move(mid);
     return ml_leaf_setup(ctx, M, mesh, prm, order, out);
 }
 
@@ -919,7 +1332,8 @@ static emfa_status ml_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
 // ranges, near lists and samples set.  On failure M is freed.
 static emfa_status ml_leaf_setup(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *mesh, const emfa_mlfma_params *prm,
                                  const std::vector<int> &order, emfa_mlfma **out) {
-    const int n = mesh->n, T = mesh->nt, levels = M->levels;
+    const int n = M->n, T = mesh->nt, levels = M->levels;
+    M->nt = T;
     const double k = prm->k;
     // leaf data: permutation, leaf of each position, centres
     MlLevel &F = M->lv[levels];
@@ -957,8 +1371,8 @@ static emfa_status ml_leaf_setup(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *
     ML_TRY(ml_upload(M, &M->row_ptr, rp));
     ML_TRY(ml_upload(M, &M->col, cols));
     ML_TRY(ml_alloc(M, &M->val, cols.size()));
-    ML_TRY(ml_alloc(M, &M->rad, (size_t)n * F.K * 3));
-    ML_TRY(ml_alloc(M, &M->rec, (size_t)n * F.K * 3));
+    ML_TRY(ml_alloc(M, &M->rad, (size_t)n * F.K * M->nc));
+    ML_TRY(ml_alloc(M, &M->rec, (size_t)n * F.K * M->nc));
     ML_TRY(ml_alloc(M, &M->xb, n));
     ML_TRY(ml_alloc(M, &M->yn, n));
     // coefficients (MLFMA.cpp:382-400)
@@ -970,21 +1384,33 @@ static emfa_status ml_leaf_setup(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *
     {
         ProfScope ps(ctx, "mlfma_setup", 0);
         const long long tot = (long long)n * F.K;
-        k_ml_radrec<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap,
-                                                                   mesh->ram, mesh->rlen64, M->perm, M->leaf_of,
-                                                                   M->leaf_cen, M->kh_leaf, coeff_d, n, F.K, k,
-                                                                   M->alpha, M->rad, M->rec);
         const double ce = k * kZ0 / (4.0 * kPi);
         const float cE = (float)(M->alpha * ce), cM = (float)((1.0 - M->alpha) * kZ0 / (16.0 * kPi));
-        if (M->kind == EMFA_EFIE)
-            k_ml_near_fill<EMFA_EFIE><<<(n + 3) / 4, 256, 0, s>>>(
-                mesh->tg32, mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap, mesh->ram, mesh->rlen32, M->perm,
-                M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), (float)ce, 0.0f, k, 1.0, 0, M->val);
-        else
-            k_ml_near_fill<EMFA_CFIE><<<(n + 3) / 4, 256, 0, s>>>(
-                mesh->tg32, mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap, mesh->ram, mesh->rlen32, M->perm,
-                M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), cE, cM, k, M->alpha, prm->cfie_mode,
-                M->val);
+        if (M->nc == 4) {
+            k_wt_radrec<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(mesh->tg64, T, M->wt_tri, M->wt_fv,
+                                                                       M->wt_len64, M->perm, M->leaf_of, M->leaf_cen,
+                                                                       M->kh_leaf, coeff_d, n, F.K, k, M->alpha,
+                                                                       M->rad, M->rec);
+            const float cl = (float)(kZ0 / (4.0 * kPi * k)), stab = (float)(2.0 * M->wt_beta * kZ0 / k);
+            k_wt_near_fill<EMFA_CFIE><<<(n + 3) / 4, 256, 0, s>>>(
+                mesh->tg32, mesh->tg64, T, M->wt_tri, M->wt_fv, M->wt_len32, M->wt_edge, M->wt_lptr, M->wt_ltri,
+                M->wt_llen, M->perm, M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), cE, cM, k, M->alpha,
+                prm->cfie_mode, cl, stab, M->val);
+        } else {
+            k_ml_radrec<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(mesh->tg64, T, mesh->rtp, mesh->rtm,
+                                                                       mesh->rap, mesh->ram, mesh->rlen64, M->perm,
+                                                                       M->leaf_of, M->leaf_cen, M->kh_leaf, coeff_d,
+                                                                       n, F.K, k, M->alpha, M->rad, M->rec);
+            if (M->kind == EMFA_EFIE)
+                k_ml_near_fill<EMFA_EFIE><<<(n + 3) / 4, 256, 0, s>>>(
+                    mesh->tg32, mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap, mesh->ram, mesh->rlen32, M->perm,
+                    M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), (float)ce, 0.0f, k, 1.0, 0, M->val);
+            else
+                k_ml_near_fill<EMFA_CFIE><<<(n + 3) / 4, 256, 0, s>>>(
+                    mesh->tg32, mesh->tg64, T, mesh->rtp, mesh->rtm, mesh->rap, mesh->ram, mesh->rlen32, M->perm,
+                    M->row_ptr, M->col, n, (float)k, (float)(1.0 / (k * k)), cE, cM, k, M->alpha, prm->cfie_mode,
+                    M->val);
+        }
         EMFA_HIP_CHECK(hipGetLastError());
     }
     EMFA_HIP_CHECK(hipStreamSynchronize(s));
@@ -1144,8 +1570,8 @@ static emfa_status fmm_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_ml
             C.W[q] = C.wp[p] * C.wt[t];
             C.theta[q] = th;
         }
-    ML_TRY(ml_alloc(M, &C.up, (size_t)B * C.K * 3));
-    ML_TRY(ml_alloc(M, &C.down, (size_t)B * C.K * 3));
+    ML_TRY(ml_alloc(M, &C.up, (size_t)B * C.K * M->nc));
+    ML_TRY(ml_alloc(M, &C.down, (size_t)B * C.K * M->nc));
     ML_TRY(ml_upload(M, &C.far_ptr_d, C.far_ptr));
     ML_TRY(ml_upload(M, &C.far_idx_d, C.far_idx.empty() ? std::vector<int>{0} : C.far_idx));
     ML_TRY(ml_upload(M, &C.far_code_d, C.far_code.empty() ? std::vector<int>{0} : C.far_code));
@@ -1173,6 +1599,7 @@ static emfa_status fmm_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_ml
         for (auto &th : pool) th.join();
     }
     ML_TRY(ml_upload(M, &C.trans, tr));
+    M->mid = rwg_mid(mesh);
     return ml_leaf_setup(ctx, M, mesh, prm, order, out);
 }
 #undef ML_TRY
@@ -1188,27 +1615,36 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
     }
     {
         ProfScope ps(ctx, "mlfma_agg", 0);
-        k_ml_agg<<<(unsigned)(((long long)F.nbox * F.K + 255) / 256), 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb,
-                                                                                   F.K, F.nbox, F.up);
+        const unsigned g = (unsigned)(((long long)F.nbox * F.K + 255) / 256);
+        if (M->nc == 4)
+            k_ml_agg<4><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
+        else
+            k_ml_agg<3><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
     }
     for (int l = lvl - 1; l >= 2; --l) {
         MlLevel &C = M->lv[l], &N = M->lv[l + 1];
         ProfScope ps(ctx, "mlfma_up", 0);
-        k_ml_up<<<(unsigned)(((long long)C.nbox * C.K + 255) / 256), 256, 0, s>>>(
-            C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox, N.up, C.up);
+        const unsigned g = (unsigned)(((long long)C.nbox * C.K + 255) / 256);
+        if (M->nc == 4)
+            k_ml_up<4><<<g, 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox,
+                                         N.up, C.up);
+        else
+            k_ml_up<3><<<g, 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox,
+                                         N.up, C.up);
     }
     // sibling-grouped translation measured slower than one thread per (box, sample): opt-in
     const char *sib_env = getenv("EMFA_MLFMA_SIBLING");
-    const bool sib = sib_env && atoi(sib_env) != 0;
+    const bool sib = sib_env && atoi(sib_env) != 0 && M->nc == 3;
     for (int l = 2; l <= lvl; ++l) {
         MlLevel &C = M->lv[l];
         MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;
         // algorithmic bytes: the level's radiation patterns read and receive
-        // patterns written once (3 complex FP32 per box and sample), its
+        // patterns written once (nc complex FP32 per box and sample), its
         // translation table (one row of K per box-grid offset in use) and
         // the parent's receive patterns with the anterpolation weights
-        const double ws = (double)C.nbox * C.K * 24.0 * 2 + (double)C.offsets_used * C.K * 8.0 +
-                          (P ? (double)P->nbox * P->K * 24.0 + (double)C.K * 9 * 12.0 : 0.0);
+        const double pb = 8.0 * M->nc;
+        const double ws = (double)C.nbox * C.K * pb * 2 + (double)C.offsets_used * C.K * 8.0 +
+                          (P ? (double)P->nbox * P->K * pb + (double)C.K * 9 * 12.0 : 0.0);
         ProfScope ps(ctx, "mlfma_down", ws);
         if (P && sib) {
             k_ml_down_sib<<<(unsigned)(((long long)P->nbox * C.K + 255) / 256), 256, 0, s>>>(
@@ -1216,14 +1652,23 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
                 C.K, C.an_ptr, C.an_kp, C.an_w, P->ph, P->K, P->down, C.down);
             continue;
         }
-        k_ml_down<<<(unsigned)(((long long)C.nbox * C.K + 255) / 256), 256, 0, s>>>(
-            C.far_ptr_d, C.far_idx_d, C.far_code_d, C.trans, C.up, C.K, C.parent_d, C.oct_d, C.an_ptr, C.an_kp,
-            C.an_w, P ? P->ph : nullptr, P ? P->K : 0, P ? P->down : nullptr, C.nbox, C.down);
+        const unsigned g = (unsigned)(((long long)C.nbox * C.K + 255) / 256);
+        if (M->nc == 4)
+            k_ml_down<4><<<g, 256, 0, s>>>(C.far_ptr_d, C.far_idx_d, C.far_code_d, C.trans, C.up, C.K, C.parent_d,
+                                           C.oct_d, C.an_ptr, C.an_kp, C.an_w, P ? P->ph : nullptr, P ? P->K : 0,
+                                           P ? P->down : nullptr, C.nbox, C.down);
+        else
+            k_ml_down<3><<<g, 256, 0, s>>>(C.far_ptr_d, C.far_idx_d, C.far_code_d, C.trans, C.up, C.K, C.parent_d,
+                                           C.oct_d, C.an_ptr, C.an_kp, C.an_w, P ? P->ph : nullptr, P ? P->K : 0,
+                                           P ? P->down : nullptr, C.nbox, C.down);
     }
     {
         ProfScope ps(ctx, "mlfma_out", 0);
-        k_ml_out<<<(n + 3) / 4, 256, 0, s>>>(M->rec, F.down, M->leaf_of, M->perm, M->yn, n, F.K, y,
-                                             M->sai_val ? M->ybox : nullptr);
+        double2 *yb = M->sai_val ? M->ybox : nullptr;
+        if (M->nc == 4)
+            k_ml_out<4><<<(n + 3) / 4, 256, 0, s>>>(M->rec, F.down, M->leaf_of, M->perm, M->yn, n, F.K, y, yb);
+        else
+            k_ml_out<3><<<(n + 3) / 4, 256, 0, s>>>(M->rec, F.down, M->leaf_of, M->perm, M->yn, n, F.K, y, yb);
     }
     if (M->sai_val) {
         ProfScope ps(ctx, "mlfma_precond", (double)M->sai_nnz * 12);
@@ -1300,8 +1745,9 @@ static void sai_box(std::vector<std::complex<double>> &A, int m, int k, int own,
     }
 }
 
-// MLFMA::calculatePreconditioner (MLFMA.cpp:714-741,1294-1391): thresholds in
-// leaf edges; distances of RWG edge midpoints from the leaf-box centre.
+// MLFMA::calculatePreconditioner (MLFMA.cpp:714-741,1294-1391; FastWT.cpp:
+// 815-914 is the same): thresholds in leaf edges; distances of the unknowns'
+// centres (RWG or half-RWG edge midpoints) from the leaf-box centre.
 static emfa_status ml_precondition(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh *mesh, double row_threshold,
                                    double col_threshold, long long *entries) {
     using cd = std::complex<double>;
@@ -1317,12 +1763,12 @@ static emfa_status ml_precondition(emfa_ctx *ctx, emfa_mlfma *M, const emfa_mesh
     EMFA_HIP_CHECK(hipMemcpyAsync(val.data(), M->val, val.size() * sizeof(cf), hipMemcpyDeviceToHost, s));
     EMFA_HIP_CHECK(hipMemcpyAsync(perm.data(), M->perm, perm.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     EMFA_HIP_CHECK(hipStreamSynchronize(s));
-    const std::vector<double> &xyz = mesh->xyz_host;
-    const std::vector<int> &rwg = mesh->rwg_host;
+    (void)mesh;
+    EMFA_REQUIRE(M->mid.size() == 3 * (size_t)n, EMFA_E_INVALID, "mlfma preconditioner: operator without centres");
     std::vector<double> mid(3 * (size_t)n);  // box order
     for (int i = 0; i < n; ++i) {
         const int u = perm[i];
-        for (int c = 0; c < 3; ++c) mid[3 * i + c] = 0.5 * (xyz[3 * rwg[6 * u] + c] + xyz[3 * rwg[6 * u + 1] + c]);
+        for (int c = 0; c < 3; ++c) mid[3 * i + c] = M->mid[3 * (size_t)u + c];
     }
     struct Entry {
         int row, col;
@@ -1483,6 +1929,63 @@ emfa_status emfa_fmm_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlf
     return fmm_build(ctx, mesh, p, out);
 }
 
+emfa_status emfa_fastwt_create(emfa_ctx *ctx, const emfa_mesh *mesh, int n, const int *hrwg, const double *length,
+                               int n_nc, const int *line_ptr, const int *line_tri, const double *line_seg,
+                               const emfa_mlfma_params *p, double beta, emfa_mlfma **out) {
+    EMFA_REQUIRE(ctx && mesh && hrwg && length && p && out, EMFA_E_INVALID, "emfa_fastwt_create: null argument");
+    *out = nullptr;
+    EMFA_REQUIRE(n > 0 && n_nc >= 0, EMFA_E_INVALID, "emfa_fastwt_create: %d half-RWGs, %d nonconformal", n, n_nc);
+    EMFA_REQUIRE(p->kind == EMFA_CFIE && p->alpha >= 0 && p->alpha <= 1, EMFA_E_INVALID,
+                 "emfa_fastwt_create: the IP-SIE operator is the combined-field one (kind CFIE, alpha in [0, 1])");
+    EMFA_REQUIRE(p->k > 0 && p->box_threshold > 0 && beta >= 0, EMFA_E_INVALID,
+                 "emfa_fastwt_create: k, threshold must be > 0 and beta >= 0");
+    EMFA_REQUIRE(!mesh->tri_host.empty(), EMFA_E_INVALID, "emfa_fastwt_create: mesh without host copy");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    return wt_build(ctx, mesh, n, hrwg, length, n_nc, line_ptr, line_tri, line_seg, p, beta, out);
+}
+
+emfa_status emfa_fastwt_rhs(emfa_ctx *ctx, const emfa_mlfma *m, const emfa_mesh *mesh, emfa_mat *B, int nrhs,
+                            const double *inc) {
+    EMFA_REQUIRE(ctx && m && mesh && B && inc, EMFA_E_INVALID, "emfa_fastwt_rhs: null argument");
+    EMFA_REQUIRE(m->nc == 4 && mesh->nt == m->nt, EMFA_E_INVALID,
+                 "emfa_fastwt_rhs: needs an IP-SIE operator and the mesh it was built on");
+    EMFA_REQUIRE(B->rows == m->n && B->cols == nrhs && nrhs > 0 && B->prec == EMFA_F64, EMFA_E_INVALID,
+                 "emfa_fastwt_rhs: B must be F64 n x nrhs");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    void *d = nullptr;
+    emfa_status st = ctx_scratch(ctx, 9 * sizeof(double) * (size_t)nrhs, &d);
+    if (st) return st;
+    EMFA_HIP_CHECK(hipMemcpyAsync(d, inc, 9 * sizeof(double) * (size_t)nrhs, hipMemcpyHostToDevice, ctx->stream));
+    const long long tot = (long long)m->n * nrhs;
+    k_wt_rhs<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(mesh->tg64, mesh->nt, m->wt_tri, m->wt_fv,
+                                                                     m->wt_len64, m->n, (const double *)d, nrhs,
+                                                                     m->k, m->alpha, (double2 *)B->d);
+    EMFA_HIP_CHECK(hipGetLastError());
+    EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return EMFA_OK;
+}
+
+emfa_status emfa_fastwt_rcs(emfa_ctx *ctx, const emfa_mlfma *m, const emfa_mesh *mesh, const emfa_mat *X, int col,
+                            int ndir, const double *dirs, double *rcs_db) {
+    EMFA_REQUIRE(ctx && m && mesh && X && dirs && rcs_db, EMFA_E_INVALID, "emfa_fastwt_rcs: null argument");
+    EMFA_REQUIRE(m->nc == 4 && mesh->nt == m->nt, EMFA_E_INVALID,
+                 "emfa_fastwt_rcs: needs an IP-SIE operator and the mesh it was built on");
+    EMFA_REQUIRE(X->rows == m->n && X->prec == EMFA_F64 && col >= 0 && col < X->cols && ndir > 0, EMFA_E_INVALID,
+                 "emfa_fastwt_rcs: X must be F64 n x m, 0 <= col < m");
+    EMFA_HIP_CHECK(hipSetDevice(ctx->device));
+    void *d = nullptr;
+    emfa_status st = ctx_scratch(ctx, 4 * sizeof(double) * (size_t)ndir, &d);
+    if (st) return st;
+    double *dd = (double *)d, *od = dd + 3 * (size_t)ndir;
+    EMFA_HIP_CHECK(hipMemcpyAsync(dd, dirs, 3 * sizeof(double) * (size_t)ndir, hipMemcpyHostToDevice, ctx->stream));
+    k_wt_rcs<<<ndir, 256, 0, ctx->stream>>>(mesh->tg64, mesh->nt, m->wt_tri, m->wt_fv, m->wt_len64, m->n,
+                                            (const double2 *)X->d, X->cols, col, dd, m->k, od);
+    EMFA_HIP_CHECK(hipGetLastError());
+    EMFA_HIP_CHECK(hipMemcpyAsync(rcs_db, od, sizeof(double) * (size_t)ndir, hipMemcpyDeviceToHost, ctx->stream));
+    EMFA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return EMFA_OK;
+}
+
 emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, emfa_mat *Y) {
     EMFA_REQUIRE(ctx && m && X && Y, EMFA_E_INVALID, "emfa_mlfma_matvec: null argument");
     EMFA_REQUIRE(X->rows == m->n && Y->rows == m->n && X->cols == Y->cols && X->prec == EMFA_F64 &&
@@ -1575,7 +2078,7 @@ emfa_status emfa_mlfma_precondition(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mes
                                     double col_threshold, long long *entries) {
     EMFA_REQUIRE(ctx && m && mesh, EMFA_E_INVALID, "emfa_mlfma_precondition: null argument");
     EMFA_REQUIRE(!m->single, EMFA_E_INVALID, "emfa_mlfma_precondition: the single-level operator has none");
-    EMFA_REQUIRE(mesh->n == m->n && !mesh->rwg_host.empty(), EMFA_E_INVALID,
+    EMFA_REQUIRE(mesh->nt == m->nt && (m->nc == 4 || mesh->n == m->n), EMFA_E_INVALID,
                  "emfa_mlfma_precondition: not the operator's mesh");
     EMFA_REQUIRE(row_threshold > 0 && col_threshold > 0, EMFA_E_INVALID,
                  "emfa_mlfma_precondition: thresholds must be > 0");

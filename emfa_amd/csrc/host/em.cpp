@@ -609,6 +609,276 @@ void FMM::logOperator(emfa_mlfma *op) {
     }
 }
 
+// ------------------------------------------------------------------ FastWT
+FastWT::~FastWT() { clear(); }
+
+void FastWT::check(emfa_status st, const char *what) const {
+    if (st == EMFA_OK) return;
+    std::string msg = std::string(what) + ": " + emfa_last_error();
+    if (st == EMFA_E_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+void FastWT::init(component::ConfigLoader *ploader) {
+    EM::init(ploader);
+    if (rtype_ == policy::Rad)
+        throw component::ConfigInvalid("result_type Radiation (delta-gap .rad excitation) is not part of this build");
+    alpha_ = ploader->getAlpha();
+    // FastWT.cpp:35: "_FastWT_" + to_string(alpha).substr(0, 3)
+    dir_ = tool::creatFolder(dir_, folder_name_ + "_FastWT_" + std::to_string((float)alpha_).substr(0, 3));
+    mesh_.loadMeshFile(ploader->getMeshPath());
+    ce_.buildCommonEdge(mesh_);
+    const auto &xyz = mesh_.xyz();
+    const auto &tri = mesh_.tri();
+    const int nv = (int)(xyz.size() / 3), nt = (int)(tri.size() / 3);
+    int n = 0, n_nc = 0, n_lines = 0;
+    check(emfa_contour_build(nv, xyz.data(), nt, tri.data(), nullptr, nullptr, 0, &n, nullptr, nullptr, nullptr, 0,
+                             &n_nc, &n_lines),
+          "contour edges");
+    if (n <= 0) throw component::FileError("mesh has no half-RWG unknowns");
+    hrwg_.assign(6 * (size_t)n, 0);
+    hlen_.assign(n, 0.0);
+    line_ptr_.assign(n_nc + 1, 0);
+    line_tri_.assign(std::max(1, n_lines), 0);
+    line_seg_.assign(6 * (size_t)std::max(1, n_lines), 0.0);
+    check(emfa_contour_build(nv, xyz.data(), nt, tri.data(), hrwg_.data(), hlen_.data(), n, &n, line_ptr_.data(),
+                             line_tri_.data(), line_seg_.data(), (int)line_tri_.size(), &n_nc, &n_lines),
+          "contour edges");
+    unknowns_ = n;
+    n_nc_ = n_nc;
+    iter_threshold_ = ploader->getDouble("iteration_threshold:", 1e-3);
+    max_iter_ = ploader->getInt("max_iteration_number:", 1000);
+    precond_ = ploader->getString("preconditioning:", "false") == "true";
+    if (precond_) {
+        row_threshold_ = ploader->getDouble("row_threshold:", -1.0);
+        col_threshold_ = ploader->getDouble("col_threshold:", -1.0);
+        if (!(row_threshold_ > 0) || !(col_threshold_ > 0))
+            throw component::ConfigInvalid("FastWT: preconditioning needs positive row_threshold: and col_threshold:");
+    }
+    // FastWT.cpp:48: beta = stabilization_factor * |log10(average_size)|
+    const double factor = ploader->getDouble("stabilization_factor:", -1.0);
+    const double avg = ploader->getDouble("average_size:", -1.0);
+    if (!(factor >= 0) || !(avg > 0))
+        throw component::ConfigInvalid("FastWT: stabilization_factor: (>= 0) and average_size: (> 0) must be given");
+    beta_ = factor * std::fabs(std::log10(avg));
+    box_threshold_ = ploader->getDouble("mlfma_box_threshold:", 0.5);
+    if (!(iter_threshold_ > 0) || max_iter_ < 1 || !(box_threshold_ > 0))
+        throw component::ConfigInvalid("FastWT: iteration_threshold, max_iteration_number and mlfma_box_threshold "
+                                       "must be positive");
+    std::string ls = ploader->getString("linear_solver:", "BICGSTAB");
+    if (ls != "BICGSTAB" && ls != "GMRES") throw component::ConfigInvalid("FastWT: linear_solver must be BICGSTAB or GMRES");
+    gmres_ = ls == "GMRES";
+    gmres_m_ = ploader->getInt("gmres_restart:", 30);
+    std::string cm = ploader->getString("cfie_self_term:", "reference");
+    if (cm != "reference" && cm != "accurate")
+        throw component::ConfigInvalid("cfie_self_term must be reference or accurate");
+    cfie_mode_ = cm == "accurate" ? 1 : 0;
+    device_ = ploader->getInt("device:", 0);
+    k_ = 2.0 * kPi * incidence_.freq / kC0;
+    logger_.open(dir_ + "/sca_parallel_runtime.log");
+    logger_ << ploader->getReport();
+    reportInfo(logger_);
+    timeLog("init");
+}
+
+void FastWT::reportInfo(std::ostream &strm) const {
+    mesh_.reportInfo(strm);
+    int conformal = 0;
+    for (int u = 0; u < unknowns_; ++u) conformal += hrwg_[6 * (size_t)u + 5] == 0;
+    // ContourEdge::reportInfo (ContourEdge.cpp:103-117)
+    strm << "ContourEdge Information\n"
+         << "  -> Total half-RWGs:" << unknowns_ << '\n'
+         << "     conformal: " << conformal << '\n'
+         << "     nonconformal: " << n_nc_ << '\n'
+         << "FastWT Information\n"
+         << "  -> Stabilization parameter:" << beta_ << '\n'
+         << "  -> box threshold: " << box_threshold_ << " wavelengths\n"
+         << "  -> " << (gmres_ ? "GMRES(" + std::to_string(gmres_m_) + ")" : std::string("BICGSTAB")) << " to "
+         << iter_threshold_ << ", at most " << max_iter_ << " iterations\n"
+         << "  -> preconditioning: "
+         << (precond_ ? "sparse approximate inverse, row/col thresholds " + std::to_string(row_threshold_) + " / " +
+                            std::to_string(col_threshold_) + " leaf edges"
+                      : std::string("off"))
+         << '\n';
+}
+
+// FastWT::reportLevelDetail (FastWT.cpp:1295-1318): per level and box its
+// near and far boxes and unknowns (the reference's child count is not kept by
+// the device operator)
+void FastWT::reportLevelDetail(emfa_mlfma *op) const {
+    emfa_mlfma_info info{};
+    emfa_mlfma_get_info(op, &info);
+    std::ofstream f(dir_ + "/level_detail_info.txt");
+    f << "MLFMA Level Information\n  -> Number of level: " << info.levels + 1 << "\n  -> Number of valid boxes:\n";
+    for (int l = 1; l <= info.levels; ++l) f << "    => level " << std::setw(2) << l << std::setw(15) << ':' << info.boxes[l] << '\n';
+    f << "Detail of level box: \n";
+    for (int l = 1; l <= info.levels; ++l) {
+        std::vector<int> nn(info.boxes[l]), nf(info.boxes[l]), nu(info.boxes[l]);
+        emfa_mlfma_boxes(op, l, nn.data(), nf.data(), nu.data());
+        f << "  -> level " << l << " :\n";
+        for (int b = 0; b < info.boxes[l]; ++b)
+            f << "    => box " << std::setw(8) << b + 1 << ":    near: " << std::setw(3) << nn[b]
+              << "    far: " << std::setw(3) << nf[b] << "    unknown: " << nu[b] << '\n';
+    }
+}
+
+void FastWT::solve() {
+    std::cout << "Solve Information\n";
+    ctx_ = device_context(device_);
+    const auto &xyz = mesh_.xyz();
+    const auto &tri = mesh_.tri();
+    if (ce_.getCommonEdgeNum() <= 0) throw component::FileError("FastWT: the mesh has no shared edge");
+    check(emfa_mesh_upload(ctx_, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3),
+                           ce_.flat().data(), ce_.getCommonEdgeNum(), &dmesh_),
+          "mesh upload");
+    const int n = unknowns_;
+    emfa_mat *B = nullptr;
+    try {
+        emfa_mlfma_params mp{EMFA_CFIE, k_, alpha_, box_threshold_, cfie_mode_};
+        check(emfa_fastwt_create(ctx_, dmesh_, n, hrwg_.data(), hlen_.data(), n_nc_, line_ptr_.data(),
+                                 line_tri_.data(), line_seg_.data(), &mp, beta_, &op_),
+              "FastWT setup");
+        emfa_mlfma_info info{};
+        emfa_mlfma_get_info(op_, &info);
+        logger_ << "FastWT operator\n  -> levels: " << info.levels << ", near entries: " << info.near_nnz
+                << ", device bytes: " << info.bytes << '\n';
+        for (int l = 2; l <= info.levels; ++l)
+            logger_ << "  -> level " << l << ": " << info.boxes[l] << " boxes, L = " << info.multipoles[l] << '\n';
+        reportLevelDetail(op_);
+        timeLog("FastWT setup");
+        if (precond_) {
+            long long entries = 0;
+            check(emfa_mlfma_precondition(ctx_, op_, dmesh_, row_threshold_, col_threshold_, &entries),
+                  "Preconditioner");
+            logger_ << "  -> Number of precondition: " << entries << '\n';
+            timeLog("Preconditioner");
+        }
+        std::vector<double> inc(9);
+        for (int c = 0; c < 3; ++c) {
+            inc[c] = inc_k_[c];
+            inc[3 + c] = inc_e_[c];
+            inc[6 + c] = inc_h_[c];
+        }
+        check(emfa_mat_create(ctx_, n, 1, EMFA_F64, &B), "excitation");
+        check(emfa_fastwt_rhs(ctx_, op_, dmesh_, B, 1, inc.data()), "fillVbyBox");
+        timeLog("fillVbyBox");
+        check(emfa_mat_create(ctx_, n, 1, EMFA_F64, &X_), "solution");
+        int iters = 0;
+        double relres = 0;
+        emfa_status st;
+        if (gmres_) {
+            const int m = std::max(1, gmres_m_);
+            emfa_gmres_report rep{};
+            st = emfa_mlfma_gmres(ctx_, op_, B, X_, m, iter_threshold_, (max_iter_ + m - 1) / m, &rep);
+            iters = rep.iterations;
+            relres = rep.relres;
+        } else {
+            emfa_bicgstab_report rep{};
+            st = emfa_mlfma_bicgstab(ctx_, op_, B, X_, iter_threshold_, max_iter_, 0, &rep);
+            iters = rep.iterations;
+            relres = rep.relres;
+        }
+        if (st == EMFA_E_NOCONV)
+            logger_ << "  -> WARNING: Iteration cannot converge\n";
+        else
+            check(st, gmres_ ? "GMRES" : "BICGSTAB");
+        logger_ << "  -> Iterative number: " << iters << ", relative residual: " << relres << '\n';
+        std::cout << "  -> Iterative number: " << iters << std::endl;
+        I_.assign(n, {0, 0});
+        check(emfa_mat_download(X_, reinterpret_cast<double *>(I_.data())), "solution download");
+        timeLog("iterativeSolve");
+    } catch (...) {
+        if (B) emfa_mat_destroy(B);
+        throw;
+    }
+    emfa_mat_destroy(B);
+    std::cout << "success" << std::endl;
+}
+
+void FastWT::output() {
+    std::cout << "Result Information\n";
+    // FastWT::output (FastWT.cpp:107-124): bistatic RCS, then the current
+    component::result::writeBistatic(dir_, scattering_, [&](int nd, const double *dirs, double *db) {
+        check(emfa_fastwt_rcs(ctx_, op_, dmesh_, X_, 0, nd, dirs, db), "RCS");
+    });
+    timeLog("getBistaticRCS");
+    std::vector<component::CurrentData> cur;
+    calculateSurfaceCurrent(&cur);
+    component::result::writeCurrent(dir_, cur);
+    timeLog("getSurfaceCurrent");
+}
+
+// FastWT::calculateSurfaceCurrent (FastWT.cpp:1426-1472): per triangle, its
+// half-RWGs in list order (at most three), centre and corner magnitudes from
+// l / (2 A) (r - v_free).  The corner vertices, lengths and magnitudes persist
+// from the previous triangle where a triangle has fewer than three half-RWGs,
+// as the reference declares them outside its loop.
+void FastWT::calculateSurfaceCurrent(std::vector<component::CurrentData> *currents) const {
+    const size_t nt = mesh_.getTriangleNum();
+    const auto &xyz = mesh_.xyz();
+    std::vector<std::vector<int>> tris(nt);
+    for (int u = 0; u < unknowns_; ++u) tris[hrwg_[6 * (size_t)u + 3]].push_back(u);
+    auto V = [&](int i) { return component::Vec3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}; };
+    auto sub = [](component::Vec3 a, component::Vec3 b) { return component::Vec3{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto dist = [&](component::Vec3 a, component::Vec3 b) {
+        auto d = sub(a, b);
+        return std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    };
+    currents->resize(nt);
+    component::Vec3 vx[3];
+    double len[3] = {0, 0, 0}, mg[3] = {0, 0, 0};
+    for (size_t t = 0; t < nt; ++t) {
+        const auto &unks = tris[t];
+        const size_t m = std::min<size_t>(unks.size(), 3);
+        for (size_t i = 0; i < m; ++i) {
+            vx[i] = V(hrwg_[6 * (size_t)unks[i] + 2]);
+            len[i] = hlen_[unks[i]];
+        }
+        const component::Vec3 cen{(vx[0].x + vx[1].x + vx[2].x) / 3, (vx[0].y + vx[1].y + vx[2].y) / 3,
+                                  (vx[0].z + vx[1].z + vx[2].z) / 3};
+        const double la = dist(vx[0], vx[1]), lb = dist(vx[1], vx[2]), lc = dist(vx[2], vx[0]);
+        const double s = 0.5 * (la + lb + lc);
+        const double dom = 2 * std::sqrt(std::max(0.0, s * (s - la) * (s - lb) * (s - lc)));
+        std::complex<double> cc[3] = {0, 0, 0};
+        for (size_t i = 0; i < m; ++i) {
+            const std::complex<double> f = I_[unks[i]] * (len[i] / dom);
+            const auto dv = sub(cen, vx[i]);
+            cc[0] += f * dv.x;
+            cc[1] += f * dv.y;
+            cc[2] += f * dv.z;
+            std::complex<double> nc[3] = {0, 0, 0};
+            for (size_t j = 0; j < m; ++j) {
+                if (i == j) continue;
+                const std::complex<double> fj = I_[unks[j]] * (len[j] / dom);
+                const auto d2 = sub(vx[i], vx[j]);
+                nc[0] += fj * d2.x;
+                nc[1] += fj * d2.y;
+                nc[2] += fj * d2.z;
+            }
+            mg[i] = std::sqrt(std::norm(nc[0]) + std::norm(nc[1]) + std::norm(nc[2]));
+        }
+        auto &d = (*currents)[t];
+        d.v1 = vx[0];
+        d.v2 = vx[1];
+        d.v3 = vx[2];
+        d.magnc = std::sqrt(std::norm(cc[0]) + std::norm(cc[1]) + std::norm(cc[2]));
+        d.magn1 = mg[0];
+        d.magn2 = mg[1];
+        d.magn3 = mg[2];
+    }
+}
+
+void FastWT::clear() {
+    if (X_) emfa_mat_destroy(X_);
+    if (op_) emfa_mlfma_destroy(op_);
+    if (dmesh_) emfa_mesh_destroy(dmesh_);
+    X_ = nullptr;
+    op_ = nullptr;
+    dmesh_ = nullptr;
+    mesh_.clear();
+    ce_.clear();
+}
+
 }  // namespace mom
 
 namespace solver {
@@ -627,6 +897,8 @@ void loadSolver(const std::string &conf_file) {
         invoke_solver<mom::MLFMA>(&loader);
     else if (solver_type == "FMM")
         invoke_solver<mom::FMM>(&loader);
+    else if (solver_type == "FastWT")
+        invoke_solver<mom::FastWT>(&loader);
     else
         throw component::ConfigInvalid("undefined EM solver type (" + solver_type + ')');
 }

@@ -201,6 +201,53 @@ class FMM : public MLFMA {
     const char *operatorName() const override { return "FMM"; }
 };
 
+// EMSolver/FastWT.cpp: the IP-SIE (interior-penalty surface integral
+// equation) fast solver on half-RWGs.  The half-RWG list with its contour
+// lines is emfa_contour_build's (ContourEdge with policy::HALF_RWG); the
+// operator, excitation and far field run on the device (emfa_fastwt_*), the
+// solve is BiCGSTAB (FastWT::iterativeSolve, FastWT.cpp:916-962) to
+// iteration_threshold: in at most max_iteration_number: steps, or
+// GMRES(gmres_restart) with linear_solver: GMRES; preconditioning: true
+// applies the sparse approximate inverse (FastWT.cpp:808-914).  Keys as
+// FastWT::init (FastWT.cpp:26-70): alpha:, mlfma_box_threshold:,
+// stabilization_factor: and average_size: (beta = factor |log10(size)|),
+// row_threshold: / col_threshold: with preconditioning.  Output folder
+// <mesh>_FastWT_<alpha>, log sca_parallel_runtime.log, the bistatic RCS files
+// and current.sc of FastWT::output (FastWT.cpp:107-124).
+class FastWT : public EM {
+  public:
+    ~FastWT() override;
+    void init(component::ConfigLoader *ploader) override;
+    void solve() override;
+    void output() override;
+    void clear() override;
+    void reportInfo(std::ostream &strm) const override;
+
+    int unknowns() const { return unknowns_; }
+    const std::vector<std::complex<double>> &current() const { return I_; }  // half-RWG order
+    const std::vector<int> &halfRWGs() const { return hrwg_; }             // n x 6
+    double wavenumber() const { return k_; }
+
+  protected:
+    void check(emfa_status st, const char *what) const;
+    void calculateSurfaceCurrent(std::vector<component::CurrentData> *currents) const;
+    void reportLevelDetail(emfa_mlfma *op) const;
+
+    component::Mesh mesh_;
+    component::CommonEdge ce_;  // RWG list of the mesh (the device mesh needs one)
+    std::vector<int> hrwg_, line_ptr_, line_tri_;
+    std::vector<double> hlen_, line_seg_;
+    int unknowns_ = 0, n_nc_ = 0, max_iter_ = 1000, gmres_m_ = 30, cfie_mode_ = 0, device_ = 0;
+    double k_ = 0, alpha_ = 0.5, beta_ = 0, box_threshold_ = 0.5, iter_threshold_ = 1e-3;
+    double row_threshold_ = 0.9, col_threshold_ = 0.87;
+    bool precond_ = false, gmres_ = false;
+    std::vector<std::complex<double>> I_;
+    emfa_ctx *ctx_ = nullptr;
+    emfa_mesh *dmesh_ = nullptr;
+    emfa_mlfma *op_ = nullptr;
+    emfa_mat *X_ = nullptr;
+};
+
 }  // namespace mom
 
 namespace solver {

@@ -27,7 +27,7 @@ EXPORTS = [
     "emfa_prof_enable", "emfa_prof_get", "emfa_prof_reset", "emfa_mlfma_create", "emfa_mlfma_destroy",
     "emfa_mlfma_get_info", "emfa_mlfma_matvec", "emfa_mlfma_gmres", "emfa_lu_set_params", "emfa_lu_get_pivoting", "emfa_lu_autotune",
     "emfa_lu_set_update_mode", "emfa_bicgstab", "emfa_mlfma_bicgstab", "emfa_bicgstab_noalloc",
-    "emfa_mlfma_boxes", "emfa_fmm_create", "emfa_ctx_set_graphs", "emfa_ctx_graph_stats",
+    "emfa_mlfma_boxes", "emfa_fmm_create", "emfa_fastwt_create", "emfa_fastwt_rhs", "emfa_fastwt_rcs", "emfa_ctx_set_graphs", "emfa_ctx_graph_stats",
     "emfa_mlfma_precondition",
 ]
 
@@ -520,6 +520,40 @@ class FMM(MLFMA):
 
     def __init__(self, ctx, mesh, k, alpha=0.5, box_length=0.4, kind=CFIE, cfie_mode=0):
         super().__init__(ctx, mesh, k, alpha, box_length, kind, cfie_mode)
+
+
+class FastWT(MLFMA):
+    """IP-SIE operator on half-RWGs (emfa_fastwt_create); `contour` is
+    contour_build's result for the mesh, beta the stabilization factor times
+    |log10(average size)|.  Vectors are in half-RWG order."""
+
+    def __init__(self, ctx, mesh, contour, k, alpha=0.5, box_threshold=0.25, beta=0.1 * 1.1248, cfie_mode=0):
+        self.ctx = ctx
+        self.h = _vp()
+        self.mesh = mesh
+        c = contour
+        p = MlfmaParams(CFIE, float(k), float(alpha), float(box_threshold), int(cfie_mode))
+        h = np.ascontiguousarray(c.hrwg, dtype=np.int32)
+        ln = np.ascontiguousarray(c.length, dtype=np.float64)
+        ptr = np.ascontiguousarray(c.line_ptr, dtype=np.int32)
+        lt = np.ascontiguousarray(c.line_tri, dtype=np.int32)
+        seg = np.ascontiguousarray(c.line_seg, dtype=np.float64)
+        nnc = len(ptr) - 1
+        _check(lib().emfa_fastwt_create(ctx.h, mesh.h, h.shape[0], _pi(h), _pd(ln), nnc, _pi(ptr) if nnc else None,
+                                        _pi(lt) if nnc else None, _pd(seg) if nnc else None, ctypes.byref(p),
+                                        ctypes.c_double(beta), ctypes.byref(self.h)))
+        self.n = h.shape[0]
+
+    def rhs(self, B, inc):
+        inc = np.ascontiguousarray(inc, dtype=np.float64).reshape(-1, 9)
+        _check(lib().emfa_fastwt_rhs(self.ctx.h, self.h, self.mesh.h, B.h, inc.shape[0], _pd(inc)))
+
+    def rcs(self, X, dirs, col=0):
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(dirs.shape[0])
+        _check(lib().emfa_fastwt_rcs(self.ctx.h, self.h, self.mesh.h, X.h, int(col), dirs.shape[0], _pd(dirs),
+                                     _pd(out)))
+        return out
 
 
 def rcs(ctx, mesh, k, X, dirs, cols=None):

@@ -84,6 +84,32 @@ def mixed_stream(count, seed=SEED, repeat=0.3):
     return out
 
 
+def nonconformal_cube(res, size=1.0):
+    """Closed cube surface of edge `size` centred at the origin, its six faces
+    (+x, -x, +y, -y, +z, -z) meshed separately with res[f] x res[f] cells,
+    each face with its own vertices, triangles counter-clockwise seen from
+    outside: faces of unequal resolution meet nonconformally, equal ones with
+    coincident vertices under distinct ids (the IP-SIE solver's workload)."""
+    h = size / 2
+    s = 2 * h
+    faces = [((h, -h, -h), (0, s, 0), (0, 0, s)), ((-h, -h, -h), (0, 0, s), (0, s, 0)),
+             ((-h, h, -h), (0, 0, s), (s, 0, 0)), ((-h, -h, -h), (s, 0, 0), (0, 0, s)),
+             ((-h, -h, h), (s, 0, 0), (0, s, 0)), ((-h, -h, -h), (0, s, 0), (s, 0, 0))]
+    xyz, tri, off = [], [], 0
+    for r, (o, a, b) in zip(res, faces):
+        o, a, b = (np.asarray(v, dtype=np.float64) for v in (o, a, b))
+        i, j = np.meshgrid(np.arange(r + 1), np.arange(r + 1))
+        pts = o + (i.ravel()[:, None] / r) * a + (j.ravel()[:, None] / r) * b
+        jj, ii = np.meshgrid(np.arange(r), np.arange(r), indexing="ij")
+        v00 = (jj * (r + 1) + ii).ravel()
+        v10, v01 = v00 + 1, v00 + r + 1
+        t = np.stack([np.stack([v00, v10, v01 + 1], 1), np.stack([v00, v01 + 1, v01], 1)], 1).reshape(-1, 3)
+        xyz.append(np.round(pts, 9))
+        tri.append(t + off)
+        off += len(pts)
+    return np.concatenate(xyz), np.concatenate(tri).astype(np.int32)
+
+
 def unknowns(r):
     return serve.unknowns(r["kind"], r["subdiv"])
 

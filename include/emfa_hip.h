@@ -335,6 +335,36 @@ emfa_status emfa_mlfma_boxes(const emfa_mlfma *m, int level, int *nnear, int *nf
  * emfa_mlfma_get_info (levels = 1, the grid at index 1) and
  * emfa_mlfma_destroy apply to it. */
 emfa_status emfa_fmm_create(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_mlfma_params *p, emfa_mlfma **out);
+
+/* IP-SIE fast operator on half-RWGs: replaces FastWT::init's box set-up and
+ * FastWT::solve's set-up stages (EMSolver/FastWT.cpp:26-70,167-400 tree and
+ * samples as the MLFMA's, calculateRadAndRecv :583-653, calculateNearZ
+ * :689-784).  The half-RWG list is emfa_contour_build's (n entries of hrwg /
+ * length, the contour lines of its n_nc nonconformal edges); the mesh is the
+ * one it was built from (its RWG list is not used).  Near entry of field f,
+ * source s: l_f l_s K(f, s) + alpha (line + ip), K the CFIE triangle-pair
+ * kernel, line the consistency terms of both edges against the other's
+ * triangle, ip the interior penalty 2 beta Z0 / k times a length (the edge's
+ * own, or that of f's contour line on s's triangle); beta =
+ * stabilization_factor |log10(average_size)| (FastWT.cpp:48).  Patterns carry
+ * the scalar line channel besides the vector ones (far part of
+ * FastWT::matrixVectorMultiply, :1006-1267).  p->kind must be EMFA_CFIE.  The
+ * handle is an emfa_mlfma: emfa_mlfma_matvec, emfa_mlfma_bicgstab (variant 0
+ * = FastWT::iterativeSolve, :916-962), emfa_mlfma_gmres and
+ * emfa_mlfma_precondition (FastWT::calculatePreconditioner, :808-914) apply;
+ * vectors are in half-RWG order. */
+emfa_status emfa_fastwt_create(emfa_ctx *ctx, const emfa_mesh *mesh, int n, const int *hrwg, const double *length,
+                               int n_nc, const int *line_ptr, const int *line_tri, const double *line_seg,
+                               const emfa_mlfma_params *p, double beta, emfa_mlfma **out);
+/* Excitation of every half-RWG (FastWT::fillVbyBox / cVKernel,
+ * FastWT.cpp:786-806,1475-1492; 3-point rule): B is F64 n x nrhs,
+ * inc nrhs x 9 {k, e, h} as for emfa_rhs_fill. */
+emfa_status emfa_fastwt_rhs(emfa_ctx *ctx, const emfa_mlfma *m, const emfa_mesh *mesh, emfa_mat *B, int nrhs,
+                            const double *inc);
+/* Bistatic RCS in dB of column col of X (half-RWG order) at ndir unit
+ * directions (FastWT::getBiRCS, FastWT.cpp:1393-1420; 3-point rule). */
+emfa_status emfa_fastwt_rcs(emfa_ctx *ctx, const emfa_mlfma *m, const emfa_mesh *mesh, const emfa_mat *X, int col,
+                            int ndir, const double *dirs, double *rcs_db);
 /* Y = Z_mlfma X for F64 n x ncols matrices (column by column). */
 emfa_status emfa_mlfma_matvec(emfa_ctx *ctx, emfa_mlfma *m, const emfa_mat *X, emfa_mat *Y);
 /* GMRES(restart) with FP64 orthogonalisation on the MLFMA operator; X holds

@@ -135,42 +135,78 @@ def entries(xyz, tri, C, k, alpha, beta_, pairs):
     return np.array(out)
 
 
-def dense_parts(xyz, tri, C, k, beta_, alpha=1.0, mode=0):
-    """The three parts of the near-entry formula for every pair at once,
-    vectorised: (Zs, line, ip) with entry = Zs + alpha (line + ip); Zs is
-    the surface part at the given alpha."""
-    n = C.n
-    Zs = O.fill_hrwg(xyz, tri, C.hrwg, C.length, k, alpha, 3, mode)
-    g = np.einsum("qa,uac->uqc", R3, xyz[tri[C.hrwg[:, 3]]])          # n x 3 Gauss points
-    a, b = xyz[C.hrwg[:, 0]], xyz[C.hrwg[:, 1]]
-    e = a[:, None, :] + GL3[None, :, None] * (b - a)[:, None, :]       # n x 3 edge points
-    # I[t, s] = sum_p sum_q glw3[p] w3[q] G(e_s[p], g_t[q])
-    r = np.linalg.norm(e[None, :, :, None, :] - g[:, None, None, :, :], axis=4)   # t, s, p, q
-    Ig = np.einsum("p,tspq->ts", GLW3 / 3.0, np.exp(-1j * k * r) / r)
-    elen = np.linalg.norm(a - b, axis=1)
-    cons = Z0 / (k * 4.0 * np.pi * 1j) * Ig * elen[None, :]                  # C(T_t; edge_s)
-    L = C.length
-    line = -(L[:, None] * cons + L[None, :] * cons.T)
-    ip = np.zeros((n, n))
+def _ip_matrix(C, k, beta_):
+    """Interior-penalty part for every pair, as a dict {(f, s): value}."""
     stab = lambda ln: stabilization(ln, beta_, k)
-    ip[np.arange(n), np.arange(n)] = stab(L)
-    conf = np.where(C.hrwg[:, 5] == 0)[0]
+    L = C.length
+    ip = {(u, u): stab(L[u]) for u in range(C.n)}
     by_edge = {}
-    for u in conf:
-        by_edge.setdefault(int(C.hrwg[u, 4]), []).append(u)
+    for u in np.where(C.hrwg[:, 5] == 0)[0]:
+        by_edge.setdefault(int(C.hrwg[u, 4]), []).append(int(u))
     for us in by_edge.values():
         for f in us:
             for s in us:
                 if C.hrwg[f, 3] != C.hrwg[s, 3]:
-                    ip[f, s] = stab(L[f])
-    nc = np.where(C.hrwg[:, 5] == 1)[0]
+                    ip[(f, s)] = stab(L[f])
+    nc = [int(u) for u in np.where(C.hrwg[:, 5] == 1)[0]]
     for f in nc:
         for s in nc:
             if C.hrwg[f, 3] != C.hrwg[s, 3]:
                 ln = C.line_length(f, C.hrwg[s, 3])
                 if ln is not None:
-                    ip[f, s] = stab(ln)
-    return Zs, line, ip
+                    ip[(f, s)] = stab(ln)
+    return ip
+
+
+def _gauss_and_edge_points(xyz, tri, C):
+    g = np.einsum("qa,uac->uqc", R3, xyz[tri[C.hrwg[:, 3]]])          # n x 3 Gauss points
+    a, b = xyz[C.hrwg[:, 0]], xyz[C.hrwg[:, 1]]
+    e = a[:, None, :] + GL3[None, :, None] * (b - a)[:, None, :]       # n x 3 edge points
+    return g, e, np.linalg.norm(a - b, axis=1)
+
+
+def _pot(g, e, elen, t, s, k):
+    """|edge s| sum_p sum_q glw3[p] w3[q] G(e_s[p], g_t[q]) for index arrays t, s."""
+    r = np.linalg.norm(e[s][:, :, None, :] - g[t][:, None, :, :], axis=3)   # pair, p, q
+    return elen[s] * np.einsum("p,ipq->i", GLW3 / 3.0, np.exp(-1j * k * r) / r)
+
+
+def parts_at(xyz, tri, C, k, beta_, rows, cols, alpha=1.0, mode=0):
+    """(Zs, line, ip) of the near-entry formula at the pairs (rows[i], cols[i]),
+    vectorised in chunks."""
+    rows, cols = np.asarray(rows, dtype=np.int64), np.asarray(cols, dtype=np.int64)
+    Zd = O.fill_hrwg(xyz, tri, C.hrwg, C.length, k, alpha, 3, mode)
+    zs = Zd[rows, cols]
+    del Zd
+    g, e, elen = _gauss_and_edge_points(xyz, tri, C)
+    cst = Z0 / (k * 4.0 * np.pi * 1j)
+    L = C.length
+    line = np.zeros(len(rows), dtype=complex)
+    for i0 in range(0, len(rows), 200000):
+        f, s = rows[i0:i0 + 200000], cols[i0:i0 + 200000]
+        line[i0:i0 + 200000] = -cst * (L[f] * _pot(g, e, elen, f, s, k) + L[s] * _pot(g, e, elen, s, f, k))
+    ipd = _ip_matrix(C, k, beta_)
+    ip = np.array([ipd.get((int(f), int(s)), 0.0) for f, s in zip(rows, cols)])
+    return zs, line, ip
+
+
+def dense_parts(xyz, tri, C, k, beta_, alpha=1.0, mode=0):
+    """The three parts of the near-entry formula for every pair (n x n each):
+    (Zs, line, ip) with entry = Zs + alpha (line + ip); Zs is the surface
+    part at the given alpha."""
+    n = C.n
+    f, s = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    zs, line, ip = parts_at(xyz, tri, C, k, beta_, f.ravel(), s.ravel(), alpha, mode)
+    return zs.reshape(n, n), line.reshape(n, n), ip.reshape(n, n)
+
+
+def near_entries(xyz, tri, C, T, k, alpha, beta_, mode=0):
+    """The operator's near matrix (dense n x n, zero off the near pattern)."""
+    rows, cols = T.near_pairs()
+    zs, line, ip = parts_at(xyz, tri, C, k, beta_, rows, cols, alpha, mode)
+    Z = np.zeros((C.n, C.n), dtype=complex)
+    Z[rows, cols] = zs + alpha * (line + ip)
+    return Z
 
 
 def dense(xyz, tri, C, k, alpha, beta_, mode=0):
@@ -285,3 +321,40 @@ def expand(C, rwg):
         e = C.hrwg[u, 4]
         P[u, e] = 1.0 if C.hrwg[u, 3] == rwg[e, 4] else -1.0
     return P
+
+
+def current(xyz, tri, C, I):
+    """Per-triangle current magnitudes of FastWT::calculateSurfaceCurrent
+    (FastWT.cpp:1426-1472): the triangle's half-RWGs in list order (at most
+    three); corner vertices, lengths and corner magnitudes carry over from
+    the previous triangle where fewer than three are present (the reference
+    declares them outside its loop).  Returns (nt x 4: magn1, magn2, magn3,
+    magnc) and (nt x 9 corner vertices)."""
+    nt = len(tri)
+    per = [[] for _ in range(nt)]
+    for u in range(C.n):
+        per[C.hrwg[u, 3]].append(u)
+    vx = np.zeros((3, 3))
+    ln = np.zeros(3)
+    mag = np.zeros(3)
+    out, vout = np.zeros((nt, 4)), np.zeros((nt, 9))
+    for t in range(nt):
+        us = per[t][:3]
+        for i, u in enumerate(us):
+            vx[i] = xyz[C.hrwg[u, 2]]
+            ln[i] = C.length[u]
+        cen = vx.mean(0)
+        la, lb, lc = (np.linalg.norm(vx[0] - vx[1]), np.linalg.norm(vx[1] - vx[2]), np.linalg.norm(vx[2] - vx[0]))
+        sp = 0.5 * (la + lb + lc)
+        dom = 2 * np.sqrt(max(0.0, sp * (sp - la) * (sp - lb) * (sp - lc)))
+        cc = np.zeros(3, dtype=complex)
+        for i, u in enumerate(us):
+            cc += I[u] * (ln[i] / dom) * (cen - vx[i])
+            nc = np.zeros(3, dtype=complex)
+            for j, w in enumerate(us):
+                if i != j:
+                    nc += I[w] * (ln[j] / dom) * (vx[i] - vx[j])
+            mag[i] = np.sqrt(np.sum(np.abs(nc) ** 2))
+        out[t] = [mag[0], mag[1], mag[2], np.sqrt(np.sum(np.abs(cc) ** 2))]
+        vout[t] = vx.ravel()
+    return out, vout

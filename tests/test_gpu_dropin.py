@@ -50,9 +50,10 @@ BASE = """    EMSolver: {solver}
 """
 
 
-def _run(tmp_path, solver, rtype, extra="", freq="3e8", radius=0.5, subdiv=2, tto=360, tdelta=5):
-    xyz, tri = meshgen.icosphere(subdiv, radius)
-    mesh = str(tmp_path / "sphere.nas")
+def _run(tmp_path, solver, rtype, extra="", freq="3e8", radius=0.5, subdiv=2, tto=360, tdelta=5, geometry=None,
+         name="sphere"):
+    xyz, tri = meshgen.icosphere(subdiv, radius) if geometry is None else geometry
+    mesh = str(tmp_path / f"{name}.nas")
     write_nas(mesh, xyz, tri)
     cfg = tmp_path / "CONFIG.yml"
     cfg.write_text("# drop-in test\n" + BASE.format(solver=solver, rtype=rtype, mesh=mesh, freq=freq, tto=tto,
@@ -264,3 +265,43 @@ def test_launcher_beside_a_process_that_launched_cooperatively(tmp_path):
     assert np.array_equal(th, np.arange(0, 61, 10.0))
     ref = O.rcs(xyz, tri, rwg, k, X, _dirs(th, 0))
     assert np.abs(val - ref).max() <= 0.05 + 1e-4
+
+
+def test_fastwt_solver_files(tmp_path):
+    """EMSolver: FastWT on a cube whose faces meet nonconformally (4 and 5
+    cells per edge): the files of FastWT::output, the RCS against the
+    restated IP-SIE operator's own answer (GMRES on oracle/fastwt.py's
+    operator) and current.sc against the restated current of that answer."""
+    from emfa_amd import hip
+    from oracle import fastwt as W
+    from scipy.sparse.linalg import LinearOperator, gmres
+    geo = meshgen.cube_faces([4, 5, 4, 5, 4, 5], 0.6)
+    xyz, tri = _run(tmp_path, "FastWT", "BistaticRCS", freq="3e8", tto=180, tdelta=10, geometry=geo, name="cube",
+                    extra="    iteration_threshold: 1e-6\n    max_iteration_number: 2000\n"
+                          "    mlfma_box_threshold: 0.2\n    stabilization_factor: 0.1\n    average_size: 0.07503\n")
+    out = tmp_path / "cube_FastWT_0.5"
+    for f in ["bistatic_rcs_phi_0.dat", "bistatic_rcs_phi_90.dat", "bistatic_rcs_theta_90.dat", "current.sc",
+              "sca_parallel_runtime.log", "level_detail_info.txt"]:
+        assert (out / f).exists(), f
+    log = open(out / "sca_parallel_runtime.log").read()
+    assert "nonconformal" in log and "Iterative number" in log and "cannot converge" not in log
+    c = hip.contour_build(xyz, tri)
+    C = W.Contour(c.hrwg, c.length, c.line_ptr, c.line_tri, c.line_seg)
+    assert (C.hrwg[:, 5] == 1).sum() > 50
+    f, k = 3e8, O.wavenumber(3e8)
+    T = W.tree(xyz, tri, C, f, 0.2)
+    Zn = W.near_entries(xyz, tri, C, T, k, 0.5, W.beta(0.1, 0.07503))
+    rad, rec = W.patterns(T, xyz, tri, C, 0.5)
+    A = LinearOperator((C.n, C.n), matvec=lambda x: W.matvec(T, Zn, rad, rec, x), dtype=complex)
+    B = W.rhs(xyz, tri, C, k, 0.5, O.inc_table([(0, 0, 0)]))[:, 0]
+    x, info = gmres(A, B, rtol=1e-8, restart=300, maxiter=5)
+    assert info == 0
+    th, val = _read_rcs(out / "bistatic_rcs_phi_0.dat")
+    ref = W.rcs(xyz, tri, C, k, x, _dirs(th, 0))
+    lit = ref > ref.max() - 15
+    assert np.abs(val - ref)[lit].max() <= 0.05
+    lines = open(out / "current.sc").read().splitlines()
+    assert lines[0] == f"triangles: {tri.shape[0]}" and len(lines) == tri.shape[0] + 2
+    mags, _ = W.current(xyz, tri, C, x)
+    vals = np.array([[float(v) for v in ln.split()[1:]] for ln in lines[2:]])
+    assert np.allclose(vals[:, 9:13], mags, rtol=2e-3, atol=1e-6 * mags.max())
