@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precondition", action="store_true",
+                    help="fastwt: sparse approximate inverse (preconditioning: true, row/col 0.9 / 0.87)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-stream summary of the default line")
@@ -759,7 +761,7 @@ def fastwt_cpu_baseline(near_nnz, freq, alpha, beta, reps=3, sample_res=(4, 5, 4
 
 
 def main_fastwt(a, res=(40, 48, 40, 48, 40, 48), size=1.0, freq=1.5e9, alpha=0.5, threshold=0.25, tol=1e-4,
-                stab_factor=0.1, avg_size=0.07503):
+                stab_factor=0.1, avg_size=0.07503, row_threshold=0.9, col_threshold=0.87):
     """fastwt: one step = contour edges + IP-SIE operator set-up (tree,
     tables, near matrix, patterns) + excitation + BiCGSTAB to 1e-4 + bistatic
     RCS at 361 angles, on a cube whose faces are meshed separately with 40
@@ -780,6 +782,8 @@ def main_fastwt(a, res=(40, 48, 40, 48, 40, 48), size=1.0, freq=1.5e9, alpha=0.5
         t0 = time.perf_counter()
         c = hip.contour_build(xyz, tri)
         op = hip.FastWT(ctx, mesh, c, k, alpha, threshold, beta)
+        if a.precondition:
+            out["precond_nnz"] = op.precondition(mesh, row_threshold, col_threshold)
         ctx.sync()
         t1 = time.perf_counter()
         n = op.n
@@ -832,13 +836,16 @@ def main_fastwt(a, res=(40, 48, 40, 48, 40, 48), size=1.0, freq=1.5e9, alpha=0.5
         "data": "synthetic nonconformal cube mesh",
         "config": {"workload": f"FastWT IP-SIE alpha={alpha} PEC cube {size} m, faces of {res[0]} / {res[1]} cells "
                                f"per edge meshed separately, f={freq / 1e9:.1f} GHz, box threshold {threshold} "
-                               f"lambda, BiCGSTAB to {tol}",
+                               f"lambda, BiCGSTAB to {tol}" + (f", SAI preconditioner (row / col thresholds "
+                                                              f"{row_threshold} / {col_threshold} leaf edges)"
+                                                              if a.precondition else ""),
                    "unknowns": n, "nonconformal": out["nonconformal"], "triangles": int(tri.shape[0]),
                    "parallelism": "single GPU"},
         "fastwt": {"levels": info["levels"], "boxes": info["boxes"], "multipoles": info["multipoles"],
                    "near_nnz": info["near_nnz"], "device_gb": info["bytes"] / 2**30,
                    "setup_s": out["setup_s"], "solve_s": out["solve_s"], "bicgstab_iterations": out["iterations"],
-                   "relres": out["relres"], "converged": out["converged"], "matvec_ms": mv_ms / max(matvecs, 1)},
+                   "relres": out["relres"], "converged": out["converged"], "matvec_ms": mv_ms / max(matvecs, 1),
+                   "precond_nnz": out.get("precond_nnz")},
         "roofline": {"bound": "hbm", "kernel": "mlfma_down (k_ml_down<4>: far-box translation + anterpolation)",
                      "achieved": down_gbs, "peak": PEAK["hbm"], "unit": "GB/s", "frac": down_gbs / PEAK["hbm"],
                      "traffic": None, "avg_launch_ms": dn["ms_per_step"] / max(dn["launches_per_step"], 1),
