@@ -571,124 +571,6 @@ __global__ __launch_bounds__(256) void k_ml_down(const int *__restrict__ far_ptr
     for (int c = 0; c < NC; ++c) o[c] = a[c];
 }
 
-// Translation at level l >= 3 grouped by parent, staged through LDS: one
-// workgroup per (parent P, 32 samples), one 32-thread slot per child of P
-// (at most eight).  The candidate source boxes are the children of P's near
-// boxes, in the order of P's near list and of each near box's children --
-// the order in which every child's far list was built -- and they go through
-// LDS 32 at a time (patterns of the workgroup's 32 samples, loaded once for
-// all children, the next chunk held in registers while this one is used).
-// Each child adds the candidates that are not its grid neighbours, so its
-// sums run over its far list in order and the results are k_ml_down's bit
-// for bit.  Then the anterpolation from P, as k_ml_down.
-template <int NC>
-__global__ __launch_bounds__(256) void k_ml_down_grp(
-    const int *__restrict__ pnear_ptr, const int *__restrict__ pnear_idx, const int *__restrict__ pch_ptr,
-    const int *__restrict__ pch_idx, int nqc, const int *__restrict__ ix, const int *__restrict__ oct,
-    const cf *__restrict__ trans, const cf *__restrict__ up, int K, const int *__restrict__ an_ptr,
-    const int *__restrict__ an_kp, const float *__restrict__ an_w, const cf *__restrict__ php, int Kp,
-    const cf *__restrict__ downp, cf *__restrict__ down) {
-    constexpr int QC = 32, CB = 32, PER = QC * NC / 8;  // cf per thread and chunk (256 threads)
-    __shared__ __attribute__((aligned(16))) cf pat[2][CB][QC * NC];
-    __shared__ int cix[2][CB][4];
-    __shared__ int cand[216];
-    __shared__ int ncand;
-    const int tid = threadIdx.x;
-    const int P = blockIdx.x / nqc, q0 = (blockIdx.x % nqc) * QC;
-    const int slot = tid >> 5, qi = tid & 31, q = q0 + qi;
-    const int c0 = pch_ptr[P], nch = pch_ptr[P + 1] - c0;
-    // candidate list: children of P's near boxes in order
-    if (tid == 0) {
-        int m = 0;
-        for (int e = pnear_ptr[P]; e < pnear_ptr[P + 1]; ++e) {
-            const int pn = pnear_idx[e];
-            for (int f = pch_ptr[pn]; f < pch_ptr[pn + 1] && m < 216; ++f) cand[m++] = pch_idx[f];
-        }
-        ncand = m;
-    }
-    __syncthreads();
-    const int nc_all = ncand;
-    const bool mine = slot < nch && q < K;
-    const int b = slot < nch ? pch_idx[c0 + slot] : 0;
-    const int bx = ix[3 * b], by = ix[3 * b + 1], bz = ix[3 * b + 2];
-    cf a[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) a[c] = {0, 0};
-    // this thread's share of a chunk: candidate tid / 8, cf [8 * PER ... ) of its QC * NC
-    const int lc = tid >> 3, part = tid & 7;
-    float4 stg[PER / 2];
-    int sbox = 0, sx = 0, sy = 0, sz = 0;
-    auto gload = [&](int ch) {
-        const int j = ch * CB + lc;
-        if (j < nc_all) {
-            sbox = cand[j];
-            const cf *src = up + ((size_t)sbox * K + q0) * NC + part * PER;
-            const int first = q0 * NC + part * PER;  // cf index within the box's K * NC
-#pragma unroll
-            for (int i = 0; i < PER / 2; ++i) {
-                if (first + 2 * i + 1 < K * NC)
-                    stg[i] = *(const float4 *)(src + 2 * i);
-                else
-                    stg[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            if (part == 0) {
-                sx = ix[3 * sbox];
-                sy = ix[3 * sbox + 1];
-                sz = ix[3 * sbox + 2];
-            }
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < PER / 2; ++i) *(float4 *)&pat[buf][lc][part * PER + 2 * i] = stg[i];
-        if (part == 0) {
-            cix[buf][lc][0] = sx;
-            cix[buf][lc][1] = sy;
-            cix[buf][lc][2] = sz;
-        }
-    };
-    const int nchunk = (nc_all + CB - 1) / CB;
-    gload(0);
-    lstore(0);
-    if (nchunk > 1) gload(1);
-    __syncthreads();
-    for (int ch = 0; ch < nchunk; ++ch) {
-        const int buf = ch & 1, jn = min(CB, nc_all - ch * CB);
-        if (mine) {
-            for (int j = 0; j < jn; ++j) {
-                const int dx = bx - cix[buf][j][0], dy = by - cix[buf][j][1], dz = bz - cix[buf][j][2];
-                if (max(abs(dx), max(abs(dy), abs(dz))) <= 1) continue;
-                const cf t = trans[(size_t)((dx + 3) * 49 + (dy + 3) * 7 + (dz + 3)) * K + q];
-                const cf *u = &pat[buf][j][qi * NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], t, u[c]);
-            }
-        }
-        if (ch + 1 < nchunk) {
-            lstore(buf ^ 1);  // last read in step ch - 1, before the previous barrier
-            if (ch + 2 < nchunk) gload(ch + 2);
-        }
-        __syncthreads();
-    }
-    if (!mine) return;
-    {
-        const int p = P, o = oct[b];
-        const cf *dp = downp + (size_t)p * Kp * NC;
-        const cf *phase = php + (size_t)o * Kp;
-        for (int g = an_ptr[q]; g < an_ptr[q + 1]; ++g) {
-            const int kq = an_kp[g];
-            const cf f0 = phase[kq];
-            const cf f = {an_w[g] * f0.re, -an_w[g] * f0.im};
-            const cf *v = dp + (size_t)kq * NC;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], f, v[c]);
-        }
-    }
-    cf *o = down + ((size_t)b * K + q) * NC;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) o[c] = a[c];
-}
-
 // Downward pass at level l >= 3 grouped by parent: the children of parent P
 // share their candidate source boxes (children of P's near boxes), so each
 // candidate's pattern is read once and applied to every sibling for which it
@@ -1810,11 +1692,6 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
         const char *e = getenv("EMFA_MLFMA_DOWN_UNROLL");
         return e && atoi(e) == 1 ? 1 : 4;
     }();
-    // levels >= 3: translation grouped by parent through LDS (EMFA_MLFMA_DOWN_GRP=1)
-    static const bool down_grp = [] {
-        const char *e = getenv("EMFA_MLFMA_DOWN_GRP");
-        return e && atoi(e) == 1;
-    }();
     for (int l = 2; l <= lvl; ++l) {
         MlLevel &C = M->lv[l];
         MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;
@@ -1826,19 +1703,6 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
         const double ws = (double)C.nbox * C.K * pb * 2 + (double)C.offsets_used * C.K * 8.0 +
                           (P ? (double)P->nbox * P->K * pb + (double)C.K * 9 * 12.0 : 0.0);
         ProfScope ps(ctx, "mlfma_down", ws);
-        if (P && down_grp) {
-            const int nqc = (C.K + 31) / 32;
-            const unsigned gg = (unsigned)((long long)P->nbox * nqc);
-            if (M->nc == 4)
-                k_ml_down_grp<4><<<gg, 256, 0, s>>>(P->near_ptr_d, P->near_idx_d, P->ch_ptr_d, P->ch_idx_d, nqc,
-                                                    C.ix_d, C.oct_d, C.trans, C.up, C.K, C.an_ptr, C.an_kp, C.an_w,
-                                                    P->ph, P->K, P->down, C.down);
-            else
-                k_ml_down_grp<3><<<gg, 256, 0, s>>>(P->near_ptr_d, P->near_idx_d, P->ch_ptr_d, P->ch_idx_d, nqc,
-                                                    C.ix_d, C.oct_d, C.trans, C.up, C.K, C.an_ptr, C.an_kp, C.an_w,
-                                                    P->ph, P->K, P->down, C.down);
-            continue;
-        }
         if (P && sib) {
             k_ml_down_sib<<<(unsigned)(((long long)P->nbox * C.K + 255) / 256), 256, 0, s>>>(
                 P->near_ptr_d, P->near_idx_d, P->ch_ptr_d, P->ch_idx_d, P->nbox, C.ix_d, C.oct_d, C.trans, C.up,
