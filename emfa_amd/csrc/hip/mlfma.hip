@@ -418,8 +418,9 @@ __global__ __launch_bounds__(256) void k_ml_near(const long long *__restrict__ r
 
 // Patterns carry NC channels per sample: the three vector components (and,
 // for the IP-SIE operator, a fourth scalar line channel), stored together.
-// leaf aggregation: up[b][q] = sum_{i in b} x_i rad[i][q]
-template <int NC>
+// leaf aggregation: up[b][q] = sum_{i in b} x_i rad[i][q], U unknowns per
+// loop step with their loads issued together (sums in unknown order for any U)
+template <int NC, int U>
 __global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, const int *__restrict__ uend,
                                                 const cf *__restrict__ rad, const cf *__restrict__ xb, int K,
                                                 int nbox, cf *__restrict__ up) {
@@ -429,7 +430,23 @@ __global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, co
     cf a[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) a[c] = {0, 0};
-    for (int i = ubeg[b]; i < uend[b]; ++i) {
+    int i = ubeg[b];
+    const int i_end = uend[b];
+    for (; i + U <= i_end; i += U) {
+        cf x[U], r[U][NC];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            x[j] = xb[i + j];
+            const cf *src = rad + ((size_t)(i + j) * K + q) * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) r[j][c] = src[c];
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], x[j], r[j][c]);
+    }
+    for (; i < i_end; ++i) {
         const cf x = xb[i];
         const cf *r = rad + ((size_t)i * K + q) * NC;
 #pragma unroll
@@ -440,8 +457,9 @@ __global__ __launch_bounds__(256) void k_ml_agg(const int *__restrict__ ubeg, co
     for (int c = 0; c < NC; ++c) o[c] = a[c];
 }
 
-// upward: up_p[q] = sum_children ph[oct][q] * sum_9 w * up_c[col]
-template <int NC>
+// upward: up_p[q] = sum_children ph[oct][q] * sum_9 w * up_c[col]; U children
+// per loop step with their loads issued together (sums in child order)
+template <int NC, int U>
 __global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, const int *__restrict__ ch_idx,
                                                const int *__restrict__ ch_oct, const int *__restrict__ ip_col,
                                                const float *__restrict__ ip_w, const cf *__restrict__ ph, int K,
@@ -459,24 +477,36 @@ __global__ __launch_bounds__(256) void k_ml_up(const int *__restrict__ ch_ptr, c
     cf a[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) a[c] = {0, 0};
-    for (int e = ch_ptr[p]; e < ch_ptr[p + 1]; ++e) {
-        const int ch = ch_idx[e];
-        const cf *src = upc + (size_t)ch * Kc * NC;
-        cf sm[NC];
+    int e = ch_ptr[p];
+    const int e_end = ch_ptr[p + 1];
+    for (; e < e_end; e += U) {
+        const int nu = min(U, e_end - e);
+        cf sm[U][NC], f[U];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c] = {0, 0};
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const cf *v = src + (size_t)cols[j] * NC;
+            for (int c = 0; c < NC; ++c) sm[u][c] = {0, 0};
+            f[u] = {0, 0};
+            if (u < nu) {
+                const int ch = ch_idx[e + u];
+                const cf *src = upc + (size_t)ch * Kc * NC;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const cf vc = v[c];
-                sm[c] = {sm[c].re + ws[j] * vc.re, sm[c].im + ws[j] * vc.im};
+                for (int j = 0; j < 9; ++j) {
+                    const cf *v = src + (size_t)cols[j] * NC;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const cf vc = v[c];
+                        sm[u][c] = {sm[u][c].re + ws[j] * vc.re, sm[u][c].im + ws[j] * vc.im};
+                    }
+                }
+                f[u] = ph[ch_oct[ch] * K + q];
             }
         }
-        const cf f = ph[ch_oct[ch] * K + q];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], f, sm[c]);
+        for (int u = 0; u < U; ++u)
+            if (u < nu)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) a[c] = cfma(a[c], f[u], sm[u][c]);
     }
     cf *o = upp + ((size_t)p * K + q) * NC;
 #pragma unroll
@@ -1656,6 +1686,11 @@ static emfa_status fmm_build(emfa_ctx *ctx, const emfa_mesh *mesh, const emfa_ml
 
 static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, double2 *y) {
     hipStream_t s = ctx->stream;
+    // loads in flight per thread: aggregation and translation take four
+    // unknowns / far boxes per loop step, the upward pass two children
+    // (EMFA_MLFMA_UNROLL=1: one per step; the sums are the same either way)
+    const char *unroll_env = getenv("EMFA_MLFMA_UNROLL");
+    const int unroll = unroll_env && atoi(unroll_env) == 1 ? 1 : 4;
     const int n = M->n, lvl = M->levels;
     MlLevel &F = M->lv[lvl];
     k_ml_gather<<<(n + 255) / 256, 256, 0, s>>>(x, M->perm, n, M->xb);
@@ -1666,32 +1701,45 @@ static emfa_status ml_apply(emfa_ctx *ctx, emfa_mlfma *M, const double2 *x, doub
     {
         ProfScope ps(ctx, "mlfma_agg", 0);
         const unsigned g = (unsigned)(((long long)F.nbox * F.K + 255) / 256);
-        if (M->nc == 4)
-            k_ml_agg<4><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
-        else
-            k_ml_agg<3><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
+        if (M->nc == 4) {
+            if (unroll == 4)
+                k_ml_agg<4, 4><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
+            else
+                k_ml_agg<4, 1><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
+        } else {
+            if (unroll == 4)
+                k_ml_agg<3, 4><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
+            else
+                k_ml_agg<3, 1><<<g, 256, 0, s>>>(F.ubeg_d, F.uend_d, M->rad, M->xb, F.K, F.nbox, F.up);
+        }
     }
     for (int l = lvl - 1; l >= 2; --l) {
         MlLevel &C = M->lv[l], &N = M->lv[l + 1];
         ProfScope ps(ctx, "mlfma_up", 0);
         const unsigned g = (unsigned)(((long long)C.nbox * C.K + 255) / 256);
-        if (M->nc == 4)
-            k_ml_up<4><<<g, 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox,
-                                         N.up, C.up);
-        else
-            k_ml_up<3><<<g, 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox,
-                                         N.up, C.up);
+#define ML_UP(NC, U)                                                                                               \
+    k_ml_up<NC, U><<<g, 256, 0, s>>>(C.ch_ptr_d, C.ch_idx_d, N.oct_d, C.ip_col, C.ip_w, C.ph, C.K, N.K, C.nbox, N.up, \
+                                     C.up)
+        if (M->nc == 4) {
+            if (unroll == 4)
+                ML_UP(4, 2);
+            else
+                ML_UP(4, 1);
+        } else {
+            if (unroll == 4)
+                ML_UP(3, 2);
+            else
+                ML_UP(3, 1);
+        }
+#undef ML_UP
     }
     // sibling-grouped translation measured slower than one thread per (box, sample): opt-in
     const char *sib_env = getenv("EMFA_MLFMA_SIBLING");
     const bool sib = sib_env && atoi(sib_env) != 0 && M->nc == 3;
-    // far-list entries per step of the translation loop (1 or 4; same sums).
-    // Four: C5's translations 47.8 -> 32.4 ms per solve, the IP-SIE cube's
-    // 358 -> 241 ms; eight measured slower than four.
-    static const int down_unroll = [] {
-        const char *e = getenv("EMFA_MLFMA_DOWN_UNROLL");
-        return e && atoi(e) == 1 ? 1 : 4;
-    }();
+    // far-list entries per step of the translation loop (1 or 4).  Four: C5's
+    // translations 47.8 -> 32.4 ms per solve, the IP-SIE cube's 358 -> 241 ms;
+    // eight measured slower than four.
+    const int down_unroll = unroll;
     for (int l = 2; l <= lvl; ++l) {
         MlLevel &C = M->lv[l];
         MlLevel *P = l > 2 ? &M->lv[l - 1] : nullptr;

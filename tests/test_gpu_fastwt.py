@@ -115,6 +115,16 @@ def test_matvec_is_repeatable(ctx, built):
     assert np.array_equal(_apply(ctx, v["op"], x), _apply(ctx, v["op"], x))
 
 
+def test_multi_item_loops_give_the_same_matvec(ctx, built, monkeypatch):
+    # four channels: several unknowns / children / far boxes per loop step
+    v = built["cube_nonconformal"]
+    x = _vec(v["C"].n, 8)
+    y4 = _apply(ctx, v["op"], x)
+    monkeypatch.setenv("EMFA_MLFMA_UNROLL", "1")
+    y1 = _apply(ctx, v["op"], x)
+    assert np.array_equal(y4, y1)
+
+
 def test_conformal_halves_recombine_into_the_rwg_operator(ctx, built):
     """On the conformal sphere the half-RWG tree equals the RWG tree and the
     line and penalty parts cancel: P^T A_wt P = A_rwg (FP32 operators)."""

@@ -190,6 +190,20 @@ def test_sibling_grouped_translation_gives_the_same_matvec(ctx, built, monkeypat
     assert np.array_equal(y0, y1)
 
 
+def test_multi_item_loops_give_the_same_matvec(ctx, built, monkeypatch):
+    # aggregation, upward pass and translation take several unknowns /
+    # children / far boxes per loop step with their loads issued together,
+    # summed in list order: the bits of the one-item loops
+    for name in CASES:
+        v = built[name]
+        x = _vec(v["rwg"].shape[0], 10)
+        y4 = _apply(ctx, v["op"], x)
+        monkeypatch.setenv("EMFA_MLFMA_UNROLL", "1")
+        y1 = _apply(ctx, v["op"], x)
+        monkeypatch.delenv("EMFA_MLFMA_UNROLL")
+        assert np.array_equal(y4, y1), name
+
+
 def test_preconditioned_operator_matches_oracle_and_speeds_bicgstab(ctx):
     """MLFMA::calculatePreconditioner (MLFMA.cpp:1294-1391) on the device:
     P's nonzero count equals the restatement's, the preconditioned matvec
