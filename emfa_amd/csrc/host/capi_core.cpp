@@ -146,8 +146,10 @@ long long emfa_serve_submit(emfa_server *s, const emfa_serve_request *r) {
     if (!s || !r || r->nrhs < 1 || !r->angles) return fail(-1, "emfa_serve_submit: bad argument");
     if (r->mesh_kind != EMFA_MESH_ICOSPHERE && r->mesh_kind != EMFA_MESH_PLATE && r->mesh_kind != EMFA_MESH_USER)
         return fail(-1, "emfa_serve_submit: unknown mesh kind");
-    if (r->solver != EMFA_SERVE_DENSE && r->solver != EMFA_SERVE_MLFMA)
-        return fail(-1, "emfa_serve_submit: solver must be EMFA_SERVE_DENSE or EMFA_SERVE_MLFMA");
+    if (r->solver != EMFA_SERVE_DENSE && r->solver != EMFA_SERVE_MLFMA && r->solver != EMFA_SERVE_FASTWT)
+        return fail(-1, "emfa_serve_submit: solver must be EMFA_SERVE_DENSE, EMFA_SERVE_MLFMA or EMFA_SERVE_FASTWT");
+    if (r->solver == EMFA_SERVE_FASTWT && r->formulation != EMFA_CFIE)
+        return fail(-1, "emfa_serve_submit: the IP-SIE operator is the combined-field one (formulation CFIE)");
     if (r->formulation != EMFA_EFIE && r->formulation != EMFA_CFIE)
         return fail(-1, "emfa_serve_submit: formulation must be EFIE or CFIE");
     if (r->precision != EMFA_F32 && r->precision != EMFA_F64) return fail(-1, "emfa_serve_submit: bad precision");
@@ -166,11 +168,12 @@ long long emfa_serve_submit(emfa_server *s, const emfa_serve_request *r) {
     q.quad = r->quad;
     q.angles.assign(r->angles, r->angles + 3 * (size_t)r->nrhs);
     q.solver = r->solver;
-    if (r->solver == EMFA_SERVE_MLFMA) {
+    if (r->solver == EMFA_SERVE_MLFMA || r->solver == EMFA_SERVE_FASTWT) {
         q.mlfma_box = r->mlfma_box > 0 ? r->mlfma_box : 0.25;
         q.iter_tol = r->iter_tol > 0 ? r->iter_tol : 1e-4;
         q.iter_max = r->iter_max > 0 ? r->iter_max : 600;
     }
+    if (r->solver == EMFA_SERVE_FASTWT && r->fastwt_beta > 0) q.fastwt_beta = r->fastwt_beta;
     if (r->mesh_kind == EMFA_MESH_USER) {
         if (r->nv < 3 || r->nt < 1 || !r->xyz || !r->tri) return fail(-1, "emfa_serve_submit: empty caller mesh");
         auto xyz = std::make_shared<std::vector<double>>(r->xyz, r->xyz + 3 * (size_t)r->nv);

@@ -48,6 +48,7 @@ struct Server::Factor {
     emfa_mat *Z = nullptr;
     emfa_lu *lu = nullptr;
     emfa_mlfma *op = nullptr;
+    bool fastwt = false;  // op is the IP-SIE operator; vectors are in half-RWG order
     int n = 0, prec = EMFA_F64;
     size_t bytes = 0;
     double k = 0;
@@ -289,6 +290,7 @@ static std::string factor_key(const Request &r, int prec) {
     ks.precision(17);
     ks << r.mesh.key() << '|' << r.freq << '|' << r.formulation << '|' << r.alpha << '|' << prec << '|' << r.quad;
     if (r.solver == MLFMA_OP) ks << "|mlfma|" << r.mlfma_box;
+    if (r.solver == FASTWT_OP) ks << "|fastwt|" << r.mlfma_box << '|' << r.fastwt_beta;
     return ks.str();
 }
 
@@ -548,6 +550,30 @@ std::shared_ptr<Server::Factor> Server::make_factor(emfa_ctx *ctx, const Request
     check(emfa_mesh_upload(ctx, xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3), rwg.data(), n,
                            &nf->mesh),
           "mesh");
+    if (r.solver == FASTWT_OP) {
+        // FastWT::init .. calculateNearZ (EMSolver/FastWT.cpp:26-70,167-784): the
+        // half-RWG list of the mesh, then the IP-SIE operator on the device
+        const int nv = (int)(xyz.size() / 3), nt = (int)(tri.size() / 3);
+        int nh = 0, n_nc = 0, n_lines = 0;
+        check(emfa_contour_build(nv, xyz.data(), nt, tri.data(), nullptr, nullptr, 0, &nh, nullptr, nullptr, nullptr,
+                                 0, &n_nc, &n_lines),
+              "contour edges");
+        std::vector<int> h(6 * (size_t)nh), lptr(n_nc + 1), ltri(std::max(1, n_lines));
+        std::vector<double> len(nh), seg(6 * (size_t)std::max(1, n_lines));
+        check(emfa_contour_build(nv, xyz.data(), nt, tri.data(), h.data(), len.data(), nh, &nh, lptr.data(),
+                                 ltri.data(), seg.data(), (int)ltri.size(), &n_nc, &n_lines),
+              "contour edges");
+        emfa_mlfma_params mp{EMFA_CFIE, k, r.alpha, r.mlfma_box, 0};
+        check(emfa_fastwt_create(ctx, nf->mesh, nh, h.data(), len.data(), n_nc, lptr.data(), ltri.data(), seg.data(),
+                                 &mp, r.fastwt_beta, &nf->op),
+              "FastWT setup");
+        emfa_mlfma_info info{};
+        check(emfa_mlfma_get_info(nf->op, &info), "FastWT info");
+        nf->bytes = info.bytes;
+        nf->n = nh;
+        nf->fastwt = true;
+        return nf;
+    }
     if (r.solver == MLFMA_OP) {
         // MLFMA::init .. calculateNearZ (EMSolver/MLFMA.cpp:23-65, 688-1422) on the device
         emfa_mlfma_params mp{r.formulation, k, r.alpha, r.mlfma_box, 0};
@@ -605,8 +631,10 @@ std::shared_ptr<Server::Factor> Server::acquire(emfa_ctx *ctx, const Request &r,
         n = f->n;
     else
         build_mesh(r.mesh, xyz, tri, rwg, n);
-    const bool mlfma = r.solver == MLFMA_OP;
-    const size_t need = mlfma ? (size_t)n * nrhs * 16 * 4 + (f ? 0 : mlfma_estimate(n))
+    const bool mlfma = r.solver == MLFMA_OP || r.solver == FASTWT_OP;
+    // the IP-SIE operator has up to three half-RWGs per triangle
+    const int nvec = r.solver == FASTWT_OP && !f ? 3 * (int)(tri.size() / 3) : n;
+    const size_t need = mlfma ? (size_t)nvec * nrhs * 16 * 4 + (f ? 0 : mlfma_estimate(nvec))
                               : footprint(n, nrhs, prec, (bool)f);
     {
         mu_.lock();
@@ -702,7 +730,10 @@ void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result
         }
     } mg{&B, &X};
     check(emfa_mat_create(ctx, n, nrhs, EMFA_F64, &B), "B");
-    check(emfa_rhs_fill(ctx, f->mesh, r.formulation, k, r.alpha, nrhs, inc.data(), B), "rhs");
+    if (f->fastwt)
+        check(emfa_fastwt_rhs(ctx, f->op, f->mesh, B, nrhs, inc.data()), "rhs");
+    else
+        check(emfa_rhs_fill(ctx, f->mesh, r.formulation, k, r.alpha, nrhs, inc.data(), B), "rhs");
     check(emfa_mat_create(ctx, n, nrhs, EMFA_F64, &X), "X");
     int iters = 0;
     double relres = 0;
@@ -714,7 +745,7 @@ void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result
         emfa_status st = emfa_mlfma_gmres(ctx, f->op, B, X, 30, r.iter_tol, restarts, &rep);
         iters = rep.iterations;
         relres = rep.relres;
-        check(st, "MLFMA GMRES");
+        check(st, f->fastwt ? "FastWT GMRES" : "MLFMA GMRES");
     } else if (r.precision == EMFA_F64) {
         check(emfa_mat_copy(X, B), "copy");
         check(emfa_lu_solve(ctx, f->lu, X), "solve");
@@ -735,7 +766,12 @@ void Server::run(emfa_ctx *ctx, const std::vector<Job> &jobs, std::vector<Result
         }
     }
     std::vector<double> rcs(nrhs);
-    check(emfa_rcs(ctx, f->mesh, k, X, nrhs, dirs.data(), cols.data(), rcs.data()), "rcs");
+    if (f->fastwt) {
+        for (int j = 0; j < nrhs; ++j)
+            check(emfa_fastwt_rcs(ctx, f->op, f->mesh, X, j, 1, dirs.data() + 3 * (size_t)j, rcs.data() + j), "rcs");
+    } else {
+        check(emfa_rcs(ctx, f->mesh, k, X, nrhs, dirs.data(), cols.data(), rcs.data()), "rcs");
+    }
     std::vector<double> cur;
     if (keep_current_) {
         cur.resize(2 * (size_t)n * nrhs);

@@ -28,7 +28,7 @@
 namespace serve {
 
 enum MeshKind { ICOSPHERE = 0, PLATE = 1, USER = 2 };
-enum SolverKind { DENSE = 0, MLFMA_OP = 1 };
+enum SolverKind { DENSE = 0, MLFMA_OP = 1, FASTWT_OP = 2 };
 
 struct MeshSpec {
     int kind = ICOSPHERE;
@@ -54,12 +54,16 @@ struct Request {
     int precision = EMFA_F64;     // EMFA_F64: FP64 LU; EMFA_F32: FP32 LU + FP64 refinement
     int quad = 3;
     std::vector<double> angles;   // nrhs x 3 (theta, phi, pole) degrees; monostatic RCS per column
-    // MLFMA_OP: the multilevel fast multipole operator stays resident instead
-    // of Z and its LU; each request is solved by GMRES(30) on it
+    // MLFMA_OP / FASTWT_OP: the multilevel fast multipole operator (or the
+    // IP-SIE operator on half-RWGs) stays resident instead of Z and its LU;
+    // each request is solved by GMRES(30) on it
     int solver = DENSE;
     double mlfma_box = 0.25;      // finest box edge, wavelengths
     double iter_tol = 1e-4;       // GMRES relative residual
     int iter_max = 600;           // GMRES iterations at most
+    // FASTWT_OP: the IP-SIE operator on the mesh's half-RWGs stays resident;
+    // beta = stabilization factor x |log10(average size)|
+    double fastwt_beta = 0.1 * 1.124764;
 };
 
 struct Result {

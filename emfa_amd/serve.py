@@ -24,7 +24,7 @@ EXPORTS = [
 ]
 
 ICOSPHERE, PLATE, USER = 0, 1, 2
-DENSE, MLFMA = 0, 1
+DENSE, MLFMA, FASTWT = 0, 1, 2
 
 
 class Request(ctypes.Structure):
@@ -34,7 +34,7 @@ class Request(ctypes.Structure):
                 ("quad", ctypes.c_int), ("nrhs", ctypes.c_int), ("angles", ctypes.POINTER(ctypes.c_double)),
                 ("nv", ctypes.c_int), ("nt", ctypes.c_int), ("xyz", ctypes.POINTER(ctypes.c_double)),
                 ("tri", ctypes.POINTER(ctypes.c_int)), ("solver", ctypes.c_int), ("mlfma_box", ctypes.c_double),
-                ("iter_tol", ctypes.c_double), ("iter_max", ctypes.c_int)]
+                ("iter_tol", ctypes.c_double), ("iter_max", ctypes.c_int), ("fastwt_beta", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -121,15 +121,19 @@ class Server:
 
     def submit(self, angles, kind=ICOSPHERE, subdiv=2, axes=(1.0, 1.0, 1.0), formulation=hip.EFIE,
                alpha=1.0, freq=3e8, precision=hip.F64, quad=3, mesh=None, solver=DENSE, mlfma_box=0.25,
-               iter_tol=1e-4, iter_max=600):
+               iter_tol=1e-4, iter_max=600, fastwt_beta=0.0):
         """angles: (nrhs, 3) array of (theta, phi, pole) in degrees.  mesh:
         (xyz (nv, 3), tri (nt, 3) 0-based) of the caller's own mesh (kind
         becomes USER).  solver=MLFMA keeps the multipole operator resident and
-        solves by GMRES(30) to iter_tol."""
+        solves by GMRES(30) to iter_tol; solver=FASTWT does the same with the
+        IP-SIE operator on the mesh's half-RWGs (CFIE; fastwt_beta 0: the
+        default stabilization 0.1 |log10(0.07503)|), whose solutions are in
+        half-RWG order."""
         a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64).reshape(-1, 3))
         r = Request(kind, subdiv, axes[0], axes[1], axes[2], formulation, alpha, freq, precision, quad, a.shape[0],
                     a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         r.solver, r.mlfma_box, r.iter_tol, r.iter_max = solver, mlfma_box, iter_tol, iter_max
+        r.fastwt_beta = fastwt_beta
         n = unknowns(kind, subdiv)
         if mesh is not None:
             xyz = np.ascontiguousarray(mesh[0], dtype=np.float64).reshape(-1, 3)
@@ -137,7 +141,7 @@ class Server:
             r.mesh_kind, r.nv, r.nt = USER, xyz.shape[0], tri.shape[0]
             r.xyz = xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             r.tri = tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-            n = hip.rwg_build(tri)[0].shape[0]
+            n = hip.contour_build(xyz, tri).n if solver == FASTWT else hip.rwg_build(tri)[0].shape[0]
         with self._lock:
             rid = core().emfa_serve_submit(self.h, ctypes.byref(r))
             if rid < 0:

@@ -49,7 +49,7 @@ int emfa_core_run(const char *config_path);
 typedef struct emfa_server emfa_server;
 
 enum { EMFA_MESH_ICOSPHERE = 0, EMFA_MESH_PLATE = 1, EMFA_MESH_USER = 2 };
-enum { EMFA_SERVE_DENSE = 0, EMFA_SERVE_MLFMA = 1 };
+enum { EMFA_SERVE_DENSE = 0, EMFA_SERVE_MLFMA = 1, EMFA_SERVE_FASTWT = 2 };
 
 typedef struct {
     int mesh_kind;      /* EMFA_MESH_ICOSPHERE (ellipsoid), EMFA_MESH_PLATE or EMFA_MESH_USER */
@@ -71,11 +71,14 @@ typedef struct {
     /* EMFA_SERVE_DENSE: Z and its LU stay resident; EMFA_SERVE_MLFMA: the
      * multilevel fast multipole operator stays resident and each request is
      * solved by GMRES(30) on it (EMSolver/MLFMA.cpp; quad and precision are
-     * not used) */
+     * not used); EMFA_SERVE_FASTWT: the IP-SIE operator on the mesh's
+     * half-RWGs (EMSolver/FastWT.cpp; CFIE only, nonconformal meshes
+     * allowed) stays resident, solved the same way */
     int solver;
     double mlfma_box;   /* finest box edge in wavelengths (0: 0.25) */
     double iter_tol;    /* GMRES relative residual (0: 1e-4) */
     int iter_max;       /* GMRES iterations at most (0: 600) */
+    double fastwt_beta; /* EMFA_SERVE_FASTWT: stabilization factor x |log10(average size)| (0: 0.1 x |log10(0.07503)|) */
 } emfa_serve_request;
 
 typedef struct {

@@ -247,3 +247,44 @@ def test_mlfma_operator_resident_and_same_alone_and_in_the_mix():
     assert np.array_equal(alone["rcs_db"], direct)
     assert np.array_equal(alone["current"], xd)
     assert np.array_equal(got[tid]["rcs_db"], alone["rcs_db"])
+
+
+def test_fastwt_operator_resident_and_same_as_the_direct_solve():
+    """solver=FASTWT keeps the IP-SIE operator of a nonconformal caller mesh
+    resident: the answer equals GMRES(30) on the same operator built
+    directly (hip.FastWT) bit for bit, with RCS from its half-RWG far field,
+    and a repeat reuses the operator."""
+    xyz, tri = workload.nonconformal_cube((4, 5, 4, 5, 4, 5), 0.6)
+    freq, alpha, box, tol = 3e8, 0.5, 0.2, 1e-6
+    beta = 0.1 * abs(np.log10(0.07503))
+    ang = np.array([[0.0, 0.0, 0.0], [70.0, 20.0, 90.0]])
+    ctx = hip.Context(0)
+    try:
+        rwg, _ = hip.rwg_build(tri)
+        mesh = hip.Mesh(ctx, xyz, tri, rwg)
+        k = hip.wavenumber(freq)
+        c = hip.contour_build(xyz, tri)
+        op = hip.FastWT(ctx, mesh, c, k, alpha, box, beta)
+        inc = hip.inc_table(ang)
+        B = hip.Mat(ctx, c.n, 2, hip.F64)
+        op.rhs(B, inc)
+        X = hip.Mat(ctx, c.n, 2, hip.F64)
+        rep, ok = op.gmres(B, X, m=30, tol=tol, max_restart=20)
+        assert ok
+        direct = np.array([op.rcs(X, -inc[j:j + 1, 0:3], col=j)[0] for j in range(2)])
+        xd = X.numpy()
+        for m in (X, B, op, mesh):
+            m.close()
+    finally:
+        ctx.close()
+    kw = dict(formulation=hip.CFIE, alpha=alpha, freq=freq, mesh=(xyz, tri), solver=serve.FASTWT, mlfma_box=box,
+              iter_tol=tol, iter_max=600, fastwt_beta=beta)
+    with serve.Server(workers=2, hbm_budget=16 * GIB, keep_current=True) as srv:
+        first = srv.wait(srv.submit(ang, **kw))
+        again = srv.wait(srv.submit(ang, **kw))
+    assert first["status"] == 0, first["error"]
+    assert not first["cache_hit"] and again["cache_hit"]
+    assert first["unknowns"] == c.n
+    assert np.array_equal(first["rcs_db"], direct)
+    assert np.array_equal(first["current"], xd)
+    assert np.array_equal(again["rcs_db"], first["rcs_db"])
