@@ -152,7 +152,14 @@ Server::Server(int device, int workers, size_t hbm_budget, bool keep_current)
         // FP32 factors are only ever used under FP64 refinement: factor them
         // without interchanges (redone with pivoting when the multipliers
         // grow) and with the bf16-split trailing update; FP64 factors pivot
-        if (env_flag("EMFA_SERVE_NOPIV", true)) emfa_lu_set_params(c, 0, 0, 2);
+        // outer block 512 for every size (mixed 64-request stream, medians of
+        // three passes: 120.1 / 118.1 solves/s with 256, 123.3 / 122.6 with
+        // 512, 120.7-120.9 with 512 only from 2048 or 8192 unknowns up; the
+        // 256-problem batch of at most 7680 unknowns: 224.3 with 256, 222.5
+        // with 512)
+        int nb = 512;
+        if (const char *e = std::getenv("EMFA_SERVE_LU_NB")) nb = std::max(0, std::atoi(e));
+        emfa_lu_set_params(c, nb, 0, env_flag("EMFA_SERVE_NOPIV", true) ? 2 : -1);
         if (const char *e = std::getenv("EMFA_SERVE_REFINE_TOL")) refine_tol_ = std::max(1e-14, std::atof(e));
         if (const char *e = std::getenv("EMFA_SERVE_BATCH_RHS")) batch_max_rhs_ = std::max(0, std::atoi(e));
         if (const char *e = std::getenv("EMFA_SERVE_FACTOR_BATCH")) factor_batch_max_ = std::max(0, std::atoi(e));
