@@ -235,6 +235,12 @@ int emfa_serve_drain(emfa_server *s) {
     return 0;
 }
 
+int emfa_serve_hold(emfa_server *s, int on) {
+    if (!s) return fail(1, "emfa_serve_hold: bad argument");
+    s->srv->hold(on != 0);
+    return 0;
+}
+
 int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out) {
     if (!s || !out) return fail(1, "emfa_serve_get_stats: bad argument");
     serve::Stats t = s->srv->stats();

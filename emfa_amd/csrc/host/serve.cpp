@@ -238,7 +238,22 @@ void Server::rebalance_locked() {
 
 void Server::drain() {
     std::unique_lock<std::mutex> l(mu_);
+    if (held_) {
+        held_ = false;
+        cv_work_.notify_all();
+    }
     cv_done_.wait(l, [&] { return queue_.empty() && busy_ == 0; });
+}
+
+// A batch submitted while held is ordered as a whole when the hold ends: the
+// workers then take the largest queued work first instead of whatever
+// arrived first.
+void Server::hold(bool on) {
+    {
+        std::lock_guard<std::mutex> l(mu_);
+        held_ = on;
+    }
+    if (!on) cv_work_.notify_all();
 }
 
 size_t Server::cached_factors() {
@@ -324,7 +339,7 @@ void Server::worker(int w) {
         bool factor_batch = false;
         {
             std::unique_lock<std::mutex> l(mu_);
-            cv_work_.wait(l, [&] { return stop_ || !queue_.empty(); });
+            cv_work_.wait(l, [&] { return stop_ || (!held_ && !queue_.empty()); });
             if (stop_) return;
             std::vector<Job> &jobs = groups[0];
             // the queued request with the most work first (a new dense factor

@@ -103,7 +103,8 @@ class Server {
     int64_t submit(const Request &r);
     // blocks until request `id` is done (id < 0: any); false if none pending
     bool wait(int64_t id, Result *out);
-    void drain();  // wait until the queue is empty and all workers are idle
+    void drain();  // wait until the queue is empty and all workers are idle (releases a hold)
+    void hold(bool on);  // while on, workers take no queued request
     Stats stats();
     void reset_stats();
     size_t cached_factors();
@@ -161,6 +162,7 @@ class Server {
     double max_wait_ms_ = 1000.0;
     void rebalance_locked();  // CU shares of all contexts from busy_workers_
     bool stop_ = false;
+    bool held_ = false;
     size_t reserved_ = 0, peak_ = 0;
     // factor cache (LRU)
     std::map<std::string, std::shared_ptr<Factor>> cache_;

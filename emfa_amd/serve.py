@@ -19,6 +19,7 @@ LIB_CORE = os.path.join(HERE, "libemfa_core.so")
 EXPORTS = [
     "emfa_core_last_error", "emfa_core_config_get", "emfa_core_read_mesh", "emfa_core_run", "emfa_serve_create",
     "emfa_serve_destroy", "emfa_serve_submit", "emfa_serve_wait", "emfa_serve_fetch", "emfa_serve_drain",
+    "emfa_serve_hold",
     "emfa_serve_get_stats", "emfa_serve_reset_stats", "emfa_serve_cached_factors", "emfa_serve_footprint",
     "emfa_core_write_formats",
 ]
@@ -78,6 +79,7 @@ def core():
                      "emfa_serve_cached_factors"):
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.emfa_serve_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.emfa_serve_hold.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _CORE = L
     return _CORE
 
@@ -172,6 +174,11 @@ class Server:
 
     def drain(self):
         core().emfa_serve_drain(self.h)
+
+    def hold(self, on=True):
+        """While held, workers take no queued request; requests submitted
+        together are then scheduled as one queue when the hold ends."""
+        core().emfa_serve_hold(self.h, 1 if on else 0)
 
     def stats(self):
         s = Stats()

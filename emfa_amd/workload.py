@@ -174,13 +174,22 @@ def run_served(srv, reqs, precision, rate=None, seed=SEED):
 
     def feed():
         t0 = time.perf_counter()
-        for i, r in enumerate(reqs):
-            if arrivals is not None:
-                dt = arrivals[i] - (time.perf_counter() - t0)
-                if dt > 0:
-                    time.sleep(dt)
-            ids[i] = srv.submit(r["angles"], kind=r["kind"], subdiv=r["subdiv"], axes=r["axes"],
-                                formulation=r["formulation"], alpha=r["alpha"], freq=r["freq"], precision=precision)
+        # all at t=0: the whole stream is queued before any worker takes a
+        # request, so that the server orders it as one queue
+        if arrivals is None:
+            srv.hold(True)
+        try:
+            for i, r in enumerate(reqs):
+                if arrivals is not None:
+                    dt = arrivals[i] - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                ids[i] = srv.submit(r["angles"], kind=r["kind"], subdiv=r["subdiv"], axes=r["axes"],
+                                    formulation=r["formulation"], alpha=r["alpha"], freq=r["freq"],
+                                    precision=precision)
+        finally:
+            if arrivals is None:
+                srv.hold(False)
 
     th = threading.Thread(target=feed)
     th.start()

@@ -120,6 +120,11 @@ int emfa_serve_wait(emfa_server *s, long long id, emfa_serve_result *res);
  * buffer too small). */
 int emfa_serve_fetch(emfa_server *s, long long id, double *rcs_db, int rcs_cap, double *current, long long current_cap);
 int emfa_serve_drain(emfa_server *s);
+/* No reference counterpart.  on = 1: workers take no queued request until
+ * emfa_serve_hold(s, 0) (or emfa_serve_drain), so that requests submitted
+ * together are scheduled as one queue (largest work first) rather than in
+ * the order they reached idle workers.  Returns 0, or 1 on a bad argument. */
+int emfa_serve_hold(emfa_server *s, int on);
 int emfa_serve_get_stats(emfa_server *s, emfa_serve_stats *out);
 int emfa_serve_reset_stats(emfa_server *s);
 int emfa_serve_cached_factors(emfa_server *s);

@@ -133,6 +133,27 @@ def test_mixed_stream_sample_served_matches_plain_fp64(ctx):
         assert np.abs(p - s["rcs_db"]).max() < 0.1
 
 
+def test_hold_queues_a_batch_that_is_then_served_largest_first():
+    import time
+    small = _req(serve.ICOSPHERE, 2, (0.5, 0.5, 0.5), hip.EFIE, 3.0e8, 3)
+    large = _req(serve.ICOSPHERE, 3, (0.7, 0.5, 0.5), hip.EFIE, 3.0e8, 3)
+    with serve.Server(workers=1, hbm_budget=16 * GIB) as srv:
+        srv.hold(True)
+        ids = [_submit(srv, small, hip.F32), _submit(srv, large, hip.F32)]
+        time.sleep(0.3)
+        assert srv.stats()["completed"] == 0
+        srv.hold(False)
+        res = {i: srv.wait(i) for i in ids}
+        assert all(r["status"] == 0 for r in res.values())
+        # one worker: the larger new factor first although it was submitted second
+        assert res[ids[1]]["queue_ms"] < res[ids[0]]["queue_ms"]
+        # a hold is released by drain as well
+        srv.hold(True)
+        rid = _submit(srv, small, hip.F32)
+        srv.drain()
+        assert srv.wait(rid)["status"] == 0
+
+
 def test_queued_requests_of_one_factor_are_batched_and_match_alone():
     # continuous batching: requests queued behind a busy worker that share a
     # factor are solved together (one stacked refinement); each answer stays
