@@ -11,6 +11,8 @@ that do not depend on size:
   1.5 GHz, IP-SIE + BiCGSTAB: the backscatter at normal incidence is the
   same on a 40-cell face and on the opposite 48-cell face, and near the
   physical-optics value 4 pi A^2 / lambda^2.
+* The largest requests of the serving stream (plates near 15000 unknowns):
+  served FP32 + refinement answers equal the plain FP64 path's.
 """
 import numpy as np
 import pytest
@@ -130,3 +132,22 @@ def test_fastwt_cube_backscatter_same_on_40_and_48_cell_faces(ctx):
     # measured: 24.931 / 24.926 dB, physical optics 24.978 dB
     assert abs(out[0] - out[1]) < 0.05
     assert max(abs(out[0] - po), abs(out[1] - po)) < 0.2
+
+
+def test_c4_largest_requests_served_match_the_plain_fp64_path(ctx):
+    """The two largest requests of the 64-request serving stream (1 x 1 m
+    plates near 15000 unknowns) through the solve server (FP32 fill,
+    pivot-free split-update LU, FP64 refinement) against the plain FP64 path
+    (FP64 fill, pivoted FP64 LU): the same monostatic RCS."""
+    from emfa_amd import serve
+    reqs = sorted(workload.mixed_stream(64), key=workload.unknowns)[-2:]
+    _say(f"c4 largest: {[workload.unknowns(r) for r in reqs]} unknowns, "
+         f"{[len(r['angles']) for r in reqs]} right-hand sides")
+    plain, _, _ = workload.run_plain(ctx, reqs)
+    with serve.Server(workers=2, hbm_budget=96 * (1 << 30)) as srv:
+        served, st = workload.run_served(srv, reqs, hip.F32)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    err = max(float(np.abs(p - s["rcs_db"]).max()) for p, s in zip(plain, served))
+    _say(f"c4 largest: served vs plain FP64 {err:.2e} dB, refinement steps "
+         f"{[s['refine_iterations'] for s in served]}")
+    assert err < 1e-3
