@@ -490,6 +490,30 @@ def main():
         ms, cnt, work = ctx.prof_get(f)
         fams[f] = {"ms_per_step": ms / a.steps, "launches_per_step": cnt / a.steps, "work_per_step": work / a.steps}
     ctx.prof(False)
+    # the FP32 fill: k_fill_pairs (T^2 Q^2 Green's evaluations, writing the
+    # per-triangle-pair scratch of 9 complex values per pair) and k_fill_gather
+    # (per side of every RWG: two scratch reads and a read-modify-write of Z
+    # per entry), beside the N^2 complex entries of Z itself
+    T = int(tri.shape[0])
+    pairs_ms, gather_ms = fams["fill_pairs"]["ms_per_step"], fams["fill_gather"]["ms_per_step"]
+    scratch_b = 9.0 * T * T * 8
+    gather_b = 2 * (2 * 8.0 + 2 * 8.0) * n * n
+    z_b = 8.0 * n * n
+    fill_roof = {
+        "pairs": {"ms": pairs_ms, "green_evals": fams["fill_pairs"]["work_per_step"],
+                  "evals_per_s": fams["fill_pairs"]["work_per_step"] / (pairs_ms * 1e-3) if pairs_ms > 0 else 0.0,
+                  "scratch_write_bytes": scratch_b,
+                  "write_gb_s": scratch_b / (pairs_ms * 1e-3) / 1e9 if pairs_ms > 0 else 0.0,
+                  "bound": "valu (sqrt, reciprocal, sin/cos per evaluation)"},
+        "gather": {"ms": gather_ms, "bytes": gather_b,
+                   "gb_s": gather_b / (gather_ms * 1e-3) / 1e9 if gather_ms > 0 else 0.0,
+                   "frac_hbm": gather_b / (gather_ms * 1e-3) / 1e9 / PEAK["hbm"] if gather_ms > 0 else 0.0,
+                   "bound": "hbm"},
+        "z_bytes": z_b,
+        "traffic_over_z": (scratch_b + gather_b + z_b) / z_b,
+        "note": "bytes as the kernels move them (scratch write, two scratch reads and a read-modify-write of Z "
+                "per entry and side, the zeroing of Z), not counter readings",
+    }
     # the dominant kernel: the trailing update of the outer panels -- bf16-split
     # planes (k_gemm_bf3r, bf16 MFMA) when the tuner chose them, else FP32 MFMA
     split = fams["lu_gemm_split"]["launches_per_step"] > 0
@@ -611,6 +635,7 @@ def main():
         "roofline": roof,
         "roofline_fp32_update": fp32_update,
         "phases_ms_per_step": {f: round(v["ms_per_step"], 3) for f, v in fams.items()},
+        "fill": fill_roof,
         "lu_tuning": tuning,
         "refine": {"iterations": reports[-1][0], "relres_first": reports[-1][1], "relres_final": reports[-1][2],
                    "converged": reports[-1][3]},
